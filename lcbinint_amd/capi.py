@@ -1,0 +1,117 @@
+"""ctypes binding of the C ABI declared in include/lcbinint_amd.h.
+
+The library must have been built (``python -m lcbinint_amd.build``); importing
+this module raises if it is missing, so no code path can silently fall back to
+a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "liblcbinint_amd.so"
+HEADER = PKG.parent / "include" / "lcbinint_amd.h"
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `python -m lcbinint_amd.build` (needs hipcc for gfx950)"
+    )
+
+lib = ctypes.CDLL(str(LIB_PATH))
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
+
+STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "numerical error", 3: "unsupported",
+                4: "device error"}
+
+
+class Curve(ctypes.Structure):
+    """lcbg_curve: one light curve's model in VBM coordinates."""
+    _fields_ = [(name, ctypes.c_double) for name in
+                ("t0", "tE", "u0", "alpha", "s", "q", "rho", "ld_c", "ld_d")]
+
+
+class Options(ctypes.Structure):
+    """lcbg_options: the finite-source options the plain method reads."""
+    _fields_ = [
+        ("source_bins", ctypes.c_int), ("caustic_bins", ctypes.c_int),
+        ("mode", ctypes.c_int), ("polar_source_bins", ctypes.c_int),
+        ("grid_ratio", ctypes.c_double), ("polar_grid_ratio", ctypes.c_double),
+        ("point_source_threshold", ctypes.c_double),
+        ("hexadecapole_threshold", ctypes.c_double),
+        ("adaptive_hex_threshold", ctypes.c_double),
+    ]
+
+
+def _sig(name, restype, *argtypes):
+    fn = getattr(lib, name)
+    fn.restype = restype
+    fn.argtypes = list(argtypes)
+    return fn
+
+
+lcbg_default_options = _sig("lcbg_default_options", Options)
+lcbg_status_string = _sig("lcbg_status_string", ctypes.c_char_p, ctypes.c_int)
+lcbg_binary_mag0_device = _sig(
+    "lcbg_binary_mag0_device", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+lcbg_batch_create = _sig(
+    "lcbg_batch_create", ctypes.c_int, ctypes.POINTER(Curve), ctypes.c_int32,
+    ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_void_p))
+lcbg_batch_destroy = _sig("lcbg_batch_destroy", ctypes.c_int, ctypes.c_void_p)
+lcbg_batch_point_source_device = _sig(
+    "lcbg_batch_point_source_device", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+lcbg_light_curve_point_source = _sig(
+    "lcbg_light_curve_point_source", ctypes.c_int, ctypes.POINTER(Curve), c_double_p,
+    ctypes.c_int64, c_double_p)
+lcbg_last_kernel_ms = _sig("lcbg_last_kernel_ms", ctypes.c_double)
+
+
+class LcbgError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}")
+        self.status = status
+
+
+def check(status: int, where: str) -> None:
+    if status != 0:
+        raise LcbgError(status, where)
+
+
+def header_functions() -> list[str]:
+    """Names of every function the public header declares."""
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lcbg_[a-z0-9_]+)\s*\(", text)))
+
+
+class Batch:
+    """Owns an lcbg_batch handle (per-curve records resident on the device)."""
+
+    def __init__(self, curves, options: Options | None = None):
+        arr = (Curve * len(curves))(*curves)
+        handle = ctypes.c_void_p()
+        opts = ctypes.byref(options) if options is not None else None
+        check(lcbg_batch_create(arr, len(curves), opts, ctypes.byref(handle)), "lcbg_batch_create")
+        self.handle = handle
+        self.n_curves = len(curves)
+
+    def point_source(self, offsets_ptr: int, times_ptr: int, mag_ptr: int, stream: int = 0):
+        check(lcbg_batch_point_source_device(self.handle, offsets_ptr, times_ptr, mag_ptr, stream),
+              "lcbg_batch_point_source_device")
+
+    def close(self):
+        if self.handle:
+            lcbg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

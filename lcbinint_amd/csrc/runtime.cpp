@@ -1,0 +1,139 @@
+// Host-side runtime: error mapping, kernel timing, batch handles.
+#include "runtime.hpp"
+
+#include <cmath>
+#include <new>
+#include <vector>
+
+namespace lcbg {
+
+lcbg_status map_hip(hipError_t e)
+{
+    if (e == hipSuccess) return LCBG_OK;
+    if (e == hipErrorInvalidValue) return LCBG_INVALID_ARGUMENT;
+    return LCBG_DEVICE_ERROR;
+}
+
+lcbg_status check_launch() { return map_hip(hipGetLastError()); }
+
+CurveRec make_curve_rec(const lcbg_curve& cv)
+{
+    CurveRec r;
+    r.t0 = cv.t0;
+    r.inverse_tE = 1.0 / cv.tE;
+    r.u0 = cv.u0;
+    r.cos_alpha = std::cos(cv.alpha);
+    r.sin_alpha = std::sin(cv.alpha);
+    r.s = cv.s;
+    r.q_eff = cv.q != 0.0 ? 1.0 / cv.q : cv.q;
+    r.rho = std::abs(cv.rho);
+    r.ld_c = cv.ld_c;
+    r.ld_d = cv.ld_d;
+    return r;
+}
+
+namespace {
+
+struct ThreadTiming {
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+    bool pending = false;
+};
+
+thread_local ThreadTiming g_timing;
+
+}  // namespace
+
+KernelTimer::KernelTimer(hipStream_t stream) : stream_(stream)
+{
+    if (!g_timing.start) {
+        (void)hipEventCreate(&g_timing.start);
+        (void)hipEventCreate(&g_timing.stop);
+    }
+    (void)hipEventRecord(g_timing.start, stream_);
+}
+
+void KernelTimer::stop()
+{
+    (void)hipEventRecord(g_timing.stop, stream_);
+    g_timing.pending = true;
+}
+
+}  // namespace lcbg
+
+using namespace lcbg;
+
+extern "C" double lcbg_last_kernel_ms(void)
+{
+    if (!g_timing.pending) return -1.0;
+    if (hipEventSynchronize(g_timing.stop) != hipSuccess) return -1.0;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, g_timing.start, g_timing.stop) != hipSuccess) return -1.0;
+    return static_cast<double>(ms);
+}
+
+extern "C" const char* lcbg_status_string(lcbg_status status)
+{
+    switch (status) {
+    case LCBG_OK: return "ok";
+    case LCBG_INVALID_ARGUMENT: return "invalid argument";
+    case LCBG_NUMERICAL_ERROR: return "numerical error";
+    case LCBG_UNSUPPORTED: return "unsupported";
+    case LCBG_DEVICE_ERROR: return "device error";
+    }
+    return "unknown status";
+}
+
+extern "C" lcbg_options lcbg_default_options(void)
+{
+    lcbg_options o;
+    o.source_bins = 50;
+    o.caustic_bins = 1400;
+    o.mode = 4;
+    o.polar_source_bins = 0;
+    o.grid_ratio = 4.0;
+    o.polar_grid_ratio = 0.0;
+    o.point_source_threshold = 20.0;
+    o.hexadecapole_threshold = 3.0;
+    o.adaptive_hex_threshold = 0.001;
+    return o;
+}
+
+extern "C" lcbg_status lcbg_batch_create(const lcbg_curve* curves, int32_t n_curves,
+                                         const lcbg_options* options, lcbg_batch** out)
+{
+    if (!out || n_curves < 0 || (n_curves > 0 && !curves)) return LCBG_INVALID_ARGUMENT;
+    *out = nullptr;
+    for (int32_t c = 0; c < n_curves; ++c) {
+        if (!std::isfinite(curves[c].tE) || curves[c].tE == 0.0 || !std::isfinite(curves[c].q) ||
+            !std::isfinite(curves[c].s))
+            return LCBG_INVALID_ARGUMENT;  // LensParameters::is_valid (lens_parameters.cpp:7-11)
+    }
+    lcbg_batch* b = new (std::nothrow) lcbg_batch();
+    if (!b) return LCBG_DEVICE_ERROR;
+    b->n_curves = n_curves;
+    b->options = options ? *options : lcbg_default_options();
+    lcbg_status st = map_hip(hipGetDevice(&b->device));
+    if (st == LCBG_OK && n_curves > 0) {
+        std::vector<CurveRec> recs(static_cast<std::size_t>(n_curves));
+        for (int32_t c = 0; c < n_curves; ++c) recs[static_cast<std::size_t>(c)] = make_curve_rec(curves[c]);
+        st = map_hip(hipMalloc(reinterpret_cast<void**>(&b->d_curves), recs.size() * sizeof(CurveRec)));
+        if (st == LCBG_OK)
+            st = map_hip(hipMemcpy(b->d_curves, recs.data(), recs.size() * sizeof(CurveRec),
+                                   hipMemcpyHostToDevice));
+    }
+    if (st != LCBG_OK) {
+        lcbg_batch_destroy(b);
+        return st;
+    }
+    *out = b;
+    return LCBG_OK;
+}
+
+extern "C" lcbg_status lcbg_batch_destroy(lcbg_batch* batch)
+{
+    if (!batch) return LCBG_OK;
+    if (batch->d_curves) (void)hipFree(batch->d_curves);
+    delete batch;
+    return LCBG_OK;
+}

@@ -1,0 +1,46 @@
+// Host-side runtime shared by the entry points: HIP error mapping, per-launch
+// device timing, and the batch handle with its device-resident records.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "lcbinint_amd.h"
+
+namespace lcbg {
+
+// Per-curve constants as the kernels read them.  cos/sin(alpha) are rounded by
+// the host libm, as the reference's std::cos/std::sin
+// (python/lcbinint_pybind.cpp:1122-1123) are, because the device libm rounds a
+// small fraction of arguments differently.  q_eff is the VBM flip 1/q
+// (python/lcbinint_pybind.cpp:1125).
+struct CurveRec {
+    double t0, inverse_tE, u0, cos_alpha, sin_alpha, s, q_eff, rho, ld_c, ld_d;
+};
+
+CurveRec make_curve_rec(const lcbg_curve& cv);
+
+lcbg_status map_hip(hipError_t e);
+lcbg_status check_launch();
+
+// Records HIP events around one kernel on its launch stream; the elapsed time
+// is kept per host thread and read back by lcbg_last_kernel_ms().
+class KernelTimer {
+public:
+    explicit KernelTimer(hipStream_t stream);
+    void stop();
+
+private:
+    hipStream_t stream_;
+};
+
+}  // namespace lcbg
+
+struct lcbg_batch {
+    int device = 0;
+    int32_t n_curves = 0;
+    lcbg_options options{};
+    lcbg::CurveRec* d_curves = nullptr;  // n_curves records
+};

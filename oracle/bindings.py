@@ -1,0 +1,201 @@
+"""ctypes bindings of the two CPU checkers.
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.
+"""
+import ctypes
+
+import numpy as np
+
+D = ctypes.c_double
+P = ctypes.POINTER
+I32 = ctypes.c_int
+
+
+def _f(lib, name, res, *args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class RefSettings(ctypes.Structure):
+    """Layout of RefSettings in oracle/ref_shim.cpp (= FiniteSourceSettings fields)."""
+    _fields_ = [
+        ("source_bins", ctypes.c_int), ("caustic_bins", ctypes.c_int),
+        ("grid_ratio", D), ("polar_source_bins", ctypes.c_int), ("polar_grid_ratio", D),
+        ("finite_mode", ctypes.c_int), ("kinji_threshold", D), ("hex_threshold", D),
+        ("adaptive_hex_threshold", D), ("limb_darkening_c", D), ("limb_darkening_d", D),
+    ]
+
+
+def default_settings(**kw):
+    s = RefSettings(50, 1400, 4.0, 0, 0.0, 4, 20.0, 3.0, 0.001, 0.0, 0.0)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+class Oracle:
+    def __init__(self, path):
+        L = ctypes.CDLL(str(path))
+        self.lib = L
+        V = ctypes.c_void_p
+        self._mag0 = _f(L, "lcbo_binary_mag0", D, D, D, D, D, P(I32))
+        self._mag0_many = _f(L, "lcbo_binary_mag0_many", None, V, V, V, V, ctypes.c_long, V, V)
+        self._mag0_der = _f(L, "lcbo_binary_mag0_with_derivatives", D, D, D, D, D, P(I32), P(D))
+        self._cands = _f(L, "lcbo_binary_image_candidates", I32, D, D, D, D, V)
+        self._roots = _f(L, "lcbo_cmplx_roots_gen", None, V, I32, I32, I32, V)
+        self._lc_ps = _f(L, "lcbo_light_curve_ps", None, V, ctypes.c_long, D, D, D, D, D, D, V)
+        self._ld_table = _f(L, "lcbo_ld_table", None, D, D, V)
+        self._ld_lookup = _f(L, "lcbo_ld_lookup", D, V, D)
+        self._hex = _f(L, "lcbo_hexadecapole", D, D, D, D, D, D, D, D, D, P(D))
+        self.flop_count = _f(L, "lcbo_flop_count", D)
+        self.flop_reset = _f(L, "lcbo_flop_reset", None)
+
+    def binary_mag0(self, s, q, x, y):
+        n = I32()
+        a = self._mag0(s, q, x, y, ctypes.byref(n))
+        return a, n.value
+
+    def binary_mag0_many(self, s, q, x, y):
+        s, q, x, y = (np.ascontiguousarray(v, dtype=np.float64) for v in (s, q, x, y))
+        out = np.empty(len(s))
+        ic = np.empty(len(s), dtype=np.int32)
+        self._mag0_many(ptr(s), ptr(q), ptr(x), ptr(y), len(s), ptr(out), ptr(ic))
+        return out, ic
+
+    def binary_mag0_with_derivatives(self, s, q, x, y):
+        n = I32()
+        ind = D()
+        a = self._mag0_der(s, q, x, y, ctypes.byref(n), ctypes.byref(ind))
+        return a, n.value, ind.value
+
+    def image_candidates(self, s, q, x, y):
+        out = np.zeros((5, 5))
+        k = self._cands(s, q, x, y, ptr(out))
+        return out[:k]
+
+    def roots_gen(self, coeffs, polish=True, start=None):
+        coeffs = np.asarray(coeffs, dtype=np.complex128)
+        deg = len(coeffs) - 1
+        c = np.ascontiguousarray(np.stack([coeffs.real, coeffs.imag], axis=1).ravel())
+        r = np.zeros(2 * deg) if start is None else np.ascontiguousarray(
+            np.stack([np.real(start), np.imag(start)], axis=1).ravel())
+        self._roots(ptr(c), deg, int(polish), int(start is not None), ptr(r))
+        return r[0::2] + 1j * r[1::2]
+
+    def light_curve_ps(self, times, t0, tE, u0, alpha, s, q):
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        out = np.empty(len(t))
+        self._lc_ps(ptr(t), len(t), t0, tE, u0, alpha, s, q, ptr(out))
+        return out
+
+    def ld_table(self, c, d=0.0):
+        t = np.empty(20001)
+        self._ld_table(c, d, ptr(t))
+        return t
+
+    def ld_lookup(self, table, r2):
+        return self._ld_lookup(ptr(table), r2)
+
+    def hexadecapole(self, s, q, x, y, rho, c, d, a0):
+        e = D()
+        m = self._hex(s, q, x, y, rho, c, d, a0, ctypes.byref(e))
+        return m, e.value
+
+
+class Ref:
+    def __init__(self, path):
+        L = ctypes.CDLL(str(path))
+        self.lib = L
+        V = ctypes.c_void_p
+        self._mag0 = _f(L, "ref_binary_mag0", D, D, D, D, D, P(I32))
+        self._mag0_many = _f(L, "ref_binary_mag0_many", None, V, V, V, V, ctypes.c_long, V, V)
+        self._mag0_der = _f(L, "ref_binary_mag0_with_derivatives", D, D, D, D, D, P(I32), P(D))
+        self._cands = _f(L, "ref_binary_image_candidates", I32, D, D, D, D, V)
+        self._roots = _f(L, "ref_cmplx_roots_gen", None, V, I32, I32, I32, V)
+        self._poly = _f(L, "ref_poly_solve", I32, V, I32, V)
+        self._lens_eq = _f(L, "ref_binary_lens_equation", None, D, D, D, D, V)
+        self._fs = _f(L, "ref_fs_binary_mag", D, P(RefSettings), D, D, D, D, D, D,
+                      P(I32), P(D), P(I32))
+        self._lc = _f(L, "ref_light_curve_static", I32, P(RefSettings), V, ctypes.c_long,
+                      D, D, D, D, D, D, D, V, V)
+        self._ir = _f(L, "ref_binary_inverse_ray", D, P(RefSettings), D, D, D, D, D, P(I32))
+
+    def binary_mag0(self, s, q, x, y):
+        n = I32()
+        a = self._mag0(s, q, x, y, ctypes.byref(n))
+        return a, n.value
+
+    def binary_mag0_many(self, s, q, x, y):
+        s, q, x, y = (np.ascontiguousarray(v, dtype=np.float64) for v in (s, q, x, y))
+        out = np.empty(len(s))
+        ic = np.empty(len(s), dtype=np.int32)
+        self._mag0_many(ptr(s), ptr(q), ptr(x), ptr(y), len(s), ptr(out), ptr(ic))
+        return out, ic
+
+    def binary_mag0_with_derivatives(self, s, q, x, y):
+        n = I32()
+        ind = D()
+        a = self._mag0_der(s, q, x, y, ctypes.byref(n), ctypes.byref(ind))
+        return a, n.value, ind.value
+
+    def image_candidates(self, s, q, x, y):
+        out = np.zeros((5, 5))
+        k = self._cands(s, q, x, y, ptr(out))
+        return out[:k]
+
+    def roots_gen(self, coeffs, polish=True, start=None):
+        coeffs = np.asarray(coeffs, dtype=np.complex128)
+        deg = len(coeffs) - 1
+        c = np.ascontiguousarray(np.stack([coeffs.real, coeffs.imag], axis=1).ravel())
+        r = np.zeros(2 * deg) if start is None else np.ascontiguousarray(
+            np.stack([np.real(start), np.imag(start)], axis=1).ravel())
+        self._roots(ptr(c), deg, int(polish), int(start is not None), ptr(r))
+        return r[0::2] + 1j * r[1::2]
+
+    def poly_solve(self, coeffs):
+        coeffs = np.asarray(coeffs, dtype=np.complex128)
+        c = np.ascontiguousarray(np.stack([coeffs.real, coeffs.imag], axis=1).ravel())
+        r = np.zeros(2 * len(coeffs))
+        k = self._poly(ptr(c), len(coeffs), ptr(r))
+        return (r[0::2] + 1j * r[1::2])[:k]
+
+    def lens_equation(self, s, q, zx, zy):
+        out = np.zeros(2)
+        self._lens_eq(s, q, zx, zy, ptr(out))
+        return out
+
+    def fs_binary_mag(self, settings, s, q, x, y, rho, a_ps):
+        m = I32()
+        e = D()
+        c = I32()
+        a = self._fs(ctypes.byref(settings), s, q, x, y, rho, a_ps, ctypes.byref(m),
+                     ctypes.byref(e), ctypes.byref(c))
+        return a, m.value, e.value, c.value
+
+    def light_curve(self, settings, times, t0, tE, u0, alpha, s, q, rho):
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        out = np.empty(len(t))
+        meth = np.empty(len(t), dtype=np.int32)
+        st = self._lc(ctypes.byref(settings), ptr(t), len(t), t0, tE, u0, alpha, s, q, rho,
+                      ptr(out), ptr(meth))
+        return out, meth, st
+
+    def inverse_ray(self, settings, s, q, x, y, rho):
+        m = I32()
+        a = self._ir(ctypes.byref(settings), s, q, x, y, rho, ctypes.byref(m))
+        return a, m.value
+
+
+def load_oracle(path):
+    return Oracle(path)
+
+
+def load_ref(path):
+    return Ref(path)

@@ -1,0 +1,248 @@
+// C-ABI shim over the reference lcbinint C++ core, compiled from the reference
+// sources (never copied) by oracle/Makefile into oracle/_ref/liblcbinint_ref.so.
+//
+// Test infrastructure only: the parity tests, the golden-vector generator and
+// bench.py's cpu_baseline leg load this library through ctypes as the checker.
+// Every entry point mirrors one reference call path so that a test can compare
+// the HIP product against the reference on identical inputs.
+#include "lcbinint/lcbinint.h"
+#include "lcbinint/magnification/finite_source_magnifier.hpp"
+#include "lcbinint/magnification/point_source_magnifier.hpp"
+#include "lcbinint/math/polynomial_roots.hpp"
+#include "lcbinint/model/lens_model.hpp"
+#include "lcbinint/model/lens_parameters.hpp"
+#include "lcbinint/model/orbital_motion.hpp"
+#include "SkowronGould.h"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+using lcbinint::SourcePosition;
+using lcbinint::magnification::FiniteSourceMagnifier;
+using lcbinint::magnification::FiniteSourceSettings;
+using lcbinint::magnification::PointSourceMagnifier;
+
+namespace {
+
+// Flat settings record shared with the Python side (ctypes Structure of the
+// same layout): the fields of FiniteSourceSettings (finite_source_magnifier.hpp:23-39).
+struct RefSettings {
+    int source_bins;
+    int caustic_bins;
+    double grid_ratio;
+    int polar_source_bins;
+    double polar_grid_ratio;
+    int finite_mode;
+    double kinji_threshold;
+    double hex_threshold;
+    double adaptive_hex_threshold;
+    double limb_darkening_c;
+    double limb_darkening_d;
+};
+
+FiniteSourceSettings to_settings(const RefSettings& s)
+{
+    FiniteSourceSettings out;
+    out.source_bins = s.source_bins;
+    out.caustic_bins = s.caustic_bins;
+    out.grid_ratio = s.grid_ratio;
+    out.polar_source_bins = s.polar_source_bins;
+    out.polar_grid_ratio = s.polar_grid_ratio;
+    out.finite_mode = s.finite_mode;
+    out.kinji_threshold = s.kinji_threshold;
+    out.hex_threshold = s.hex_threshold;
+    out.adaptive_hex_threshold = s.adaptive_hex_threshold;
+    out.limb_darkening_c = s.limb_darkening_c;
+    out.limb_darkening_d = s.limb_darkening_d;
+    return out;
+}
+
+} // namespace
+
+extern "C" {
+
+// PointSourceMagnifier::binary_mag0 (point_source_magnifier.cpp:567), uncached,
+// VBM convention (s, q as passed).
+double ref_binary_mag0(double s, double q, double x, double y, int* image_count)
+{
+    PointSourceMagnifier pm;
+    const auto r = pm.binary_mag0(s, q, {x, y});
+    if (image_count) *image_count = r.image_count;
+    return r.magnification;
+}
+
+// Batch of binary_mag0 calls; one (s, q) per call row.
+void ref_binary_mag0_many(const double* s, const double* q, const double* x, const double* y,
+                          long n, double* out, int* image_count)
+{
+    PointSourceMagnifier pm;
+    for (long i = 0; i < n; ++i) {
+        const auto r = pm.binary_mag0(s[i], q[i], {x[i], y[i]});
+        out[i] = r.magnification;
+        if (image_count) image_count[i] = r.image_count;
+    }
+}
+
+// binary_mag0_with_derivatives (point_source_magnifier.cpp:583).
+double ref_binary_mag0_with_derivatives(double s, double q, double x, double y,
+                                        int* image_count, double* indicator)
+{
+    PointSourceMagnifier pm;
+    const auto r = pm.binary_mag0_with_derivatives(s, q, {x, y});
+    if (image_count) *image_count = r.image_count;
+    if (indicator) *indicator = r.derivative_error_indicator;
+    return r.magnification;
+}
+
+// binary_image_candidates (point_source_magnifier.cpp:739): out rows of
+// (x, y, jacobian, residual, physical).
+int ref_binary_image_candidates(double s, double q, double x, double y, double* out)
+{
+    PointSourceMagnifier pm;
+    const auto c = pm.binary_image_candidates(s, q, {x, y});
+    for (std::size_t i = 0; i < c.size(); ++i) {
+        out[5 * i + 0] = c[i].position.x;
+        out[5 * i + 1] = c[i].position.y;
+        out[5 * i + 2] = c[i].jacobian_determinant;
+        out[5 * i + 3] = c[i].residual;
+        out[5 * i + 4] = c[i].physical ? 1.0 : 0.0;
+    }
+    return static_cast<int>(c.size());
+}
+
+// binary_lens_equation (point_source_magnifier.cpp:780).
+void ref_binary_lens_equation(double s, double q, double zx, double zy, double* out)
+{
+    PointSourceMagnifier pm;
+    const auto w = pm.binary_lens_equation(s, q, {zx, zy});
+    out[0] = w.x;
+    out[1] = w.y;
+}
+
+// cmplx_roots_gen (SkowronGould.cpp:79) on interleaved (re, im) coefficients.
+void ref_cmplx_roots_gen(const double* coeffs, int degree, int polish, int use_start,
+                         double* roots_inout)
+{
+    std::vector<complex> poly(static_cast<std::size_t>(degree) + 1);
+    std::vector<complex> roots(static_cast<std::size_t>(degree));
+    for (int i = 0; i <= degree; ++i) poly[i] = complex(coeffs[2 * i], coeffs[2 * i + 1]);
+    for (int i = 0; i < degree; ++i) roots[i] = complex(roots_inout[2 * i], roots_inout[2 * i + 1]);
+    cmplx_roots_gen(roots.data(), poly.data(), degree, polish != 0, use_start != 0);
+    for (int i = 0; i < degree; ++i) {
+        roots_inout[2 * i] = roots[i].re;
+        roots_inout[2 * i + 1] = roots[i].im;
+    }
+}
+
+// PolynomialRootSolver::solve (polynomial_roots.cpp:35) with default options.
+int ref_poly_solve(const double* coeffs, int ncoeff, double* roots)
+{
+    std::vector<lcbinint::Complex> c(static_cast<std::size_t>(ncoeff));
+    for (int i = 0; i < ncoeff; ++i) c[i] = {coeffs[2 * i], coeffs[2 * i + 1]};
+    lcbinint::math::PolynomialRootSolver solver;
+    const auto r = solver.solve(c);
+    if (r.status != lcbinint::math::RootSolverStatus::ok) return -1;
+    for (std::size_t i = 0; i < r.roots.size(); ++i) {
+        roots[2 * i] = r.roots[i].real();
+        roots[2 * i + 1] = r.roots[i].imag();
+    }
+    return static_cast<int>(r.roots.size());
+}
+
+// FiniteSourceMagnifier::binary_mag (finite_source_magnifier.cpp:4633) with a
+// magnifier kept per settings, as the pybind layer does
+// (lcbinint_pybind.cpp:927-943).  Returns the magnification; method and error
+// estimate are written through the pointers.
+double ref_fs_binary_mag(const RefSettings* rs, double s, double q, double x, double y,
+                         double rho, double a_ps, int* method, double* error, int* converged)
+{
+    thread_local std::unique_ptr<FiniteSourceMagnifier> cached;
+    thread_local RefSettings cached_settings {};
+    thread_local bool valid = false;
+    if (!valid || std::memcmp(&cached_settings, rs, sizeof(RefSettings)) != 0) {
+        cached = std::make_unique<FiniteSourceMagnifier>(to_settings(*rs));
+        cached_settings = *rs;
+        valid = true;
+    }
+    PointSourceMagnifier pm;
+    const auto r = cached->binary_mag(s, q, {x, y}, rho, a_ps, nullptr, true);
+    (void)pm;
+    if (method) *method = static_cast<int>(r.decision.method);
+    if (error) *error = r.error_estimate;
+    if (converged) *converged = r.converged ? 1 : 0;
+    return r.magnification;
+}
+
+// The static VBM light-curve loop of evaluate_binary_light_curve_numpy
+// (lcbinint_pybind.cpp:1120-1158): source from (t0, tE, u0, alpha), the uncached
+// point-source solve, then binary_mag with effective q = 1/q.  Per-epoch
+// method codes go to `method` when non-null.  Returns 0, or 2 on a non-finite
+// magnification (the reference raises RuntimeError("numerical error")).
+int ref_light_curve_static(const RefSettings* rs, const double* times, long n, double t0,
+                           double tE, double u0, double alpha, double s, double q, double rho,
+                           double* out, int* method)
+{
+    PointSourceMagnifier pm;
+    const double cos_alpha = std::cos(alpha);
+    const double sin_alpha = std::sin(alpha);
+    const double inverse_tE = 1.0 / tE;
+    const double effective_q = q != 0.0 ? 1.0 / q : q;
+    if (rho == 0.0) {
+        for (long i = 0; i < n; ++i) {
+            const double tau = (times[i] - t0) * inverse_tE;
+            const SourcePosition src {tau * cos_alpha - u0 * sin_alpha,
+                                      tau * sin_alpha + u0 * cos_alpha};
+            out[i] = pm.binary_mag0(s, effective_q, src).magnification;
+            if (method) method[i] = 0;
+        }
+        return 0;
+    }
+    FiniteSourceMagnifier fm(to_settings(*rs));
+    for (long i = 0; i < n; ++i) {
+        const double tau = (times[i] - t0) * inverse_tE;
+        const SourcePosition src {tau * cos_alpha - u0 * sin_alpha,
+                                  tau * sin_alpha + u0 * cos_alpha};
+        const auto point = pm.binary_mag0(s, effective_q, src);
+        const auto fs = fm.binary_mag(s, effective_q, src, std::abs(rho), point.magnification,
+                                      nullptr, true);
+        out[i] = fs.magnification;
+        if (method) method[i] = static_cast<int>(fs.decision.method);
+        if (!std::isfinite(fs.magnification)) return 2;
+    }
+    return 0;
+}
+
+// Forced inverse-ray, as lcbinint.binary_inverse_ray (lcbinint_pybind.cpp:3826-3847):
+// bbox exit, derivative check and hexadecapole are switched off.
+double ref_binary_inverse_ray(const RefSettings* rs, double s, double q, double x, double y,
+                              double rho, int* method)
+{
+    FiniteSourceSettings settings = to_settings(*rs);
+    settings.kinji_threshold = 1e18;
+    settings.hex_threshold = 0.0;
+    settings.adaptive_hex_threshold = 0.0;
+    PointSourceMagnifier pm;
+    FiniteSourceMagnifier fm(settings);
+    const auto point = pm.binary_mag0(s, q, {x, y});
+    const auto r = fm.binary_mag(s, q, {x, y}, std::abs(rho), point.magnification, nullptr, true);
+    if (method) *method = static_cast<int>(r.decision.method);
+    return r.magnification;
+}
+
+// Orbital state (orbital_motion.cpp:183): separation and angle at `time`.
+void ref_orbital_state(const lcbi_params* params, double time, double* out)
+{
+    const auto p = lcbinint::model::from_c_params(*params);
+    const auto st = lcbinint::model::orbital_state(p, time);
+    out[0] = st.separation;
+    out[1] = st.angle;
+    out[2] = st.line_of_sight_separation;
+}
+
+int ref_sizeof_params(void) { return static_cast<int>(sizeof(lcbi_params)); }
+int ref_sizeof_options(void) { return static_cast<int>(sizeof(lcbi_options)); }
+int ref_sizeof_result(void) { return static_cast<int>(sizeof(lcbi_result)); }
+
+} // extern "C"
