@@ -50,10 +50,22 @@ def _run(cmd: list[str]) -> None:
         sys.stderr.write(err + "\n")
 
 
+# Host-only sources are compiled by g++ so that their libm calls (cos/sin of
+# the trajectory angle, the fixed probe and phase tables) are glibc's, as in the
+# reference build; hipcc's host pass resolves std::cos/std::sin to a different
+# implementation that rounds some arguments one ulp away.
+HOST_FLAGS = [
+    "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+    f"-I{ROCM / 'include'}", f"-I{INCLUDE}", f"-I{CSRC}",
+]
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
-    lang = ["-x", "hip"] if src.suffix == ".hip" else []
-    _run([HIPCC, *HIP_FLAGS, *lang, "-c", str(src), "-o", str(obj)])
+    if src.suffix == ".hip":
+        _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(src), "-o", str(obj)])
+    else:
+        _run(["g++", *HOST_FLAGS, "-c", str(src), "-o", str(obj)])
     return obj
 
 

@@ -2,7 +2,7 @@
 // device timing, and the batch handle with its device-resident records.
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 
 #include <cstddef>
 #include <cstdint>
