@@ -104,6 +104,35 @@ lcbg_status lcbg_batch_point_source_device(lcbg_batch* batch, const int64_t* off
 lcbg_status lcbg_light_curve_point_source(const lcbg_curve* curve, const double* times,
                                           int64_t n, double* mag);
 
+/* Finite-source light curves of every curve of the batch: the plain method of
+ * FiniteSourceMagnifier::binary_mag (finite_source_magnifier.cpp:4633-4843) as
+ * the static VBM epoch loop of evaluate_binary_light_curve_numpy
+ * (python/lcbinint_pybind.cpp:1141-1158) runs it: point source, hexadecapole,
+ * or inverse-ray shooting on the Cartesian or polar lattice with limb
+ * darkening.  Device pointers; n_epochs = offsets[n_curves].  method[i]
+ * receives the lcbg_method of each epoch.  The first call on a batch builds the
+ * per-curve caustic caches; later calls reuse them and the workspace. */
+lcbg_status lcbg_batch_finite_source_device(lcbg_batch* batch, const int64_t* offsets,
+                                            const double* times, int64_t n_epochs, double* mag,
+                                            int32_t* method, void* stream);
+
+/* lcbg_batch_finite_source_device, then waits and reports the device time of
+ * its three stages (stage_ms[3]: point source + bounding box, caustic distance
+ * + hexadecapole, inverse ray), the queue sizes (queue_counts[2]: epochs past
+ * the bounding box, inverse-ray epochs) and the overflow flags (error_flags). */
+lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const int64_t* offsets,
+                                             const double* times, int64_t n_epochs, double* mag,
+                                             int32_t* method, void* stream, float* stage_ms,
+                                             uint32_t* queue_counts, uint32_t* error_flags);
+
+/* Host-pointer light curve of one curve with the given options (NULL =
+ * defaults): the drop-in path behind LightCurve(...)(times, params).  Returns
+ * LCBG_NUMERICAL_ERROR if any magnification is not finite (the reference raises
+ * RuntimeError("numerical error"), python/lcbinint_pybind.cpp:1153-1155) and
+ * LCBG_UNSUPPORTED if a workspace capacity was exceeded. */
+lcbg_status lcbg_light_curve(const lcbg_curve* curve, const lcbg_options* options,
+                             const double* times, int64_t n, double* mag, int32_t* method);
+
 /* Device time in milliseconds of the last kernel launched by the calling host
  * thread through a *_device entry point, measured with HIP events recorded on
  * the launch stream around that kernel.  Returns a negative value if none. */

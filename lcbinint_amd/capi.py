@@ -115,3 +115,34 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+lcbg_batch_finite_source_device = _sig(
+    "lcbg_batch_finite_source_device", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+lcbg_batch_finite_source_profile = _sig(
+    "lcbg_batch_finite_source_profile", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint32),
+    ctypes.POINTER(ctypes.c_uint32))
+lcbg_light_curve = _sig(
+    "lcbg_light_curve", ctypes.c_int, ctypes.POINTER(Curve), ctypes.POINTER(Options), c_double_p,
+    ctypes.c_int64, c_double_p, c_int32_p)
+
+
+def finite_source(batch: Batch, offsets_ptr: int, times_ptr: int, n_epochs: int, mag_ptr: int,
+                  method_ptr: int, stream: int = 0, profile: bool = False):
+    """Runs the finite-source pipeline; with profile=True returns
+    (stage_ms[3], queue_counts[2], error_flags)."""
+    if not profile:
+        check(lcbg_batch_finite_source_device(batch.handle, offsets_ptr, times_ptr, n_epochs,
+                                              mag_ptr, method_ptr, stream),
+              "lcbg_batch_finite_source_device")
+        return None
+    ms = (ctypes.c_float * 3)()
+    qc = (ctypes.c_uint32 * 2)()
+    fl = ctypes.c_uint32()
+    check(lcbg_batch_finite_source_profile(batch.handle, offsets_ptr, times_ptr, n_epochs, mag_ptr,
+                                           method_ptr, stream, ms, qc, ctypes.byref(fl)),
+          "lcbg_batch_finite_source_profile")
+    return list(ms), list(qc), fl.value

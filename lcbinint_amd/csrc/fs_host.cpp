@@ -1,0 +1,353 @@
+// Host orchestration of the finite-source pipeline: per-batch device state
+// (caustic caches, trig tables, epoch workspace, inverse-ray scratch) and the
+// finite-source entry points of the C ABI.
+//
+// Compiled by g++: the phase and probe angle tables are rounded by glibc's
+// cos/sin exactly as the reference's std::polar / std::cos / std::sin calls are
+// (finite_source_magnifier.cpp:553, 1551-1553, 1579-1583, 1606-1610, 1748, 4025).
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "fs_launch.hpp"
+
+namespace lcbg {
+
+constexpr double kPiHost = 3.14159265358979323846;
+
+struct FsState {
+    FsCache cache{};
+    FsTables tab{};
+    FsOptions opt{};
+    void* table_mem = nullptr;
+    void* cache_mem = nullptr;
+    // epoch workspace
+    int64_t epoch_cap = 0;
+    void* epoch_mem = nullptr;
+    unsigned* counters = nullptr;
+    int stage2_waves = 2048;
+};
+
+namespace {
+
+template <class T>
+T* carve(char*& p, std::size_t count)
+{
+    const std::size_t align = 256;
+    std::size_t off = reinterpret_cast<std::uintptr_t>(p) % align;
+    if (off) p += align - off;
+    T* out = reinterpret_cast<T*>(p);
+    p += count * sizeof(T);
+    return out;
+}
+
+std::size_t carve_size(std::size_t bytes) { return bytes + 256; }
+
+lcbg_status ensure_epochs(FsState* fs, int64_t n)
+{
+    if (n <= fs->epoch_cap && fs->epoch_mem) return LCBG_OK;
+    if (fs->epoch_mem) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(fs->epoch_mem);
+        fs->epoch_mem = nullptr;
+    }
+    int64_t cap = 1024;
+    while (cap < n) cap *= 2;
+    const std::size_t bytes = carve_size(cap * sizeof(int32_t)) + carve_size(cap * sizeof(Pt2)) +
+                              carve_size(cap * sizeof(double)) * 2 +
+                              carve_size(cap * 10 * sizeof(double)) +
+                              carve_size(cap * sizeof(int64_t)) * 2 + carve_size(64 * sizeof(unsigned));
+    lcbg_status st = map_hip(hipMalloc(&fs->epoch_mem, bytes));
+    if (st != LCBG_OK) {
+        fs->epoch_mem = nullptr;
+        return st;
+    }
+    fs->epoch_cap = cap;
+    return LCBG_OK;
+}
+
+FsEpochs epoch_view(FsState* fs, int64_t n, const int64_t* offsets, const double* times, double* mag,
+                    int32_t* method)
+{
+    char* p = static_cast<char*>(fs->epoch_mem);
+    const int64_t cap = fs->epoch_cap;
+    FsEpochs ep{};
+    ep.n = n;
+    ep.offsets = offsets;
+    ep.times = times;
+    ep.mag = mag;
+    ep.method = method;
+    ep.curve_of = carve<int32_t>(p, cap);
+    ep.src = carve<Pt2>(p, cap);
+    ep.a_ps = carve<double>(p, cap);
+    ep.dist = carve<double>(p, cap);
+    ep.roots = carve<double>(p, cap * 10);
+    ep.near_list = carve<int64_t>(p, cap);
+    ep.ir_list = carve<int64_t>(p, cap);
+    ep.counters = carve<unsigned>(p, 64);
+    return ep;
+}
+
+// Inverse-ray scratch, one pool per device shared by every batch: its use is
+// serialised by a mutex held while a run is enqueued and by an event that the
+// next run waits for.
+struct ScratchPool {
+    std::mutex mu;
+    IrScratch scr{};
+    void* mem = nullptr;
+    hipEvent_t last_use = nullptr;
+};
+
+constexpr int kMaxDevices = 64;
+ScratchPool g_pool[kMaxDevices];
+
+lcbg_status ensure_scratch(ScratchPool& pool)
+{
+    if (pool.mem) return LCBG_OK;
+    const int slots = 512;
+    const std::size_t per_slot = carve_size(kSeedCap * sizeof(Pt2)) + carve_size(kSeedCap * sizeof(double)) +
+                                 carve_size(kSeedCap * sizeof(int)) +
+                                 carve_size(static_cast<std::size_t>(kRowCap) * 5 * sizeof(double)) +
+                                 carve_size(static_cast<std::size_t>(kRowCap) * 2 * sizeof(int)) +
+                                 carve_size(static_cast<std::size_t>(kPhiCap) * sizeof(int)) +
+                                 carve_size(static_cast<std::size_t>(kRunCap) * 4 * sizeof(int));
+    const std::size_t bytes = per_slot * slots;
+    lcbg_status st = map_hip(hipMalloc(&pool.mem, bytes));
+    if (st != LCBG_OK) {
+        pool.mem = nullptr;
+        return st;
+    }
+    char* p = static_cast<char*>(pool.mem);
+    IrScratch& s = pool.scr;
+    s.n_slots = slots;
+    s.seeds = carve<Pt2>(p, static_cast<std::size_t>(slots) * kSeedCap);
+    s.areaimage = carve<double>(p, static_cast<std::size_t>(slots) * kSeedCap);
+    s.overlap = carve<int>(p, static_cast<std::size_t>(slots) * kSeedCap);
+    s.rows = carve<double>(p, static_cast<std::size_t>(slots) * kRowCap * 5);
+    s.row_key = carve<int>(p, static_cast<std::size_t>(slots) * kRowCap * 2);
+    s.phi_head = carve<int>(p, static_cast<std::size_t>(slots) * kPhiCap);
+    s.runs = carve<int>(p, static_cast<std::size_t>(slots) * kRunCap * 4);
+    st = map_hip(hipEventCreateWithFlags(&pool.last_use, hipEventDisableTiming));
+    if (st != LCBG_OK) return st;
+    // empty visited-run lists (-1 heads); the kernel restores them per epoch
+    st = map_hip(hipMemset(s.phi_head, 0xff, static_cast<std::size_t>(slots) * kPhiCap * sizeof(int)));
+    if (st != LCBG_OK) return st;
+    return map_hip(hipDeviceSynchronize());
+}
+
+}  // namespace
+
+void fs_destroy(FsState* fs)
+{
+    if (!fs) return;
+    if (fs->table_mem) (void)hipFree(fs->table_mem);
+    if (fs->cache_mem) (void)hipFree(fs->cache_mem);
+    if (fs->epoch_mem) (void)hipFree(fs->epoch_mem);
+    delete fs;
+}
+
+// Builds the trig tables and the per-curve caches of a batch.
+lcbg_status fs_prepare(lcbg_batch* b)
+{
+    if (b->fs) return LCBG_OK;
+    FsState* fs = new (std::nothrow) FsState();
+    if (!fs) return LCBG_DEVICE_ERROR;
+    b->fs = fs;
+    const lcbg_options& o = b->options;
+    fs->opt.source_bins = o.source_bins;
+    fs->opt.caustic_bins = o.caustic_bins;
+    fs->opt.mode = o.mode;
+    fs->opt.polar_source_bins = o.polar_source_bins;
+    fs->opt.grid_ratio = o.grid_ratio;
+    fs->opt.polar_grid_ratio = o.polar_grid_ratio;
+    fs->opt.kinji = o.point_source_threshold;
+    fs->opt.hex_threshold = o.hexadecapole_threshold;
+    fs->opt.adaptive_hex = o.adaptive_hex_threshold;
+    const int bins = o.caustic_bins > 32 ? o.caustic_bins : 32;
+
+    // angle tables, each expression as the reference writes it
+    std::vector<Pt2> host;
+    const std::size_t n_caus = static_cast<std::size_t>(bins);
+    host.reserve(n_caus + kSeedPhases + kBoundaryProbes + kAngularProbes + kInteriorProbes);
+    for (int i = 0; i < bins; ++i) {
+        const double phase = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(bins);
+        host.push_back({std::cos(phase), std::sin(phase)});
+    }
+    const double phase_step = 2.0 * kPiHost / static_cast<double>(kSeedPhases);
+    for (int k = 0; k < kSeedPhases; ++k) {
+        const double phi = phase_step * static_cast<double>(k);
+        host.push_back({std::cos(phi), std::sin(phi)});
+    }
+    for (int i = 0; i < kBoundaryProbes; ++i) {
+        const double phi = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(kBoundaryProbes);
+        host.push_back({std::cos(phi), std::sin(phi)});
+    }
+    for (int i = 0; i < kAngularProbes; ++i) {
+        const double th = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(kAngularProbes);
+        host.push_back({std::cos(th), std::sin(th)});
+    }
+    for (int i = 0; i < kInteriorProbes; ++i) {
+        const double phi = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(kInteriorProbes);
+        host.push_back({std::cos(phi), std::sin(phi)});
+    }
+    lcbg_status st = map_hip(hipMalloc(&fs->table_mem, host.size() * sizeof(Pt2)));
+    if (st != LCBG_OK) return st;
+    st = map_hip(hipMemcpy(fs->table_mem, host.data(), host.size() * sizeof(Pt2), hipMemcpyHostToDevice));
+    if (st != LCBG_OK) return st;
+    const Pt2* t = static_cast<const Pt2*>(fs->table_mem);
+    fs->tab.caus_trig = t;
+    fs->tab.seed_trig = t + n_caus;
+    fs->tab.boundary_trig = fs->tab.seed_trig + kSeedPhases;
+    fs->tab.angular_trig = fs->tab.boundary_trig + kBoundaryProbes;
+    fs->tab.interior_trig = fs->tab.angular_trig + kAngularProbes;
+
+    const int nc = b->n_curves > 0 ? b->n_curves : 1;
+    const std::size_t per = static_cast<std::size_t>(nc);
+    const std::size_t bytes = carve_size(per * bins * 4 * sizeof(Pt2)) * 2 +
+                              carve_size(per * bins * 4 * sizeof(int)) * 2 +
+                              carve_size(per * kSeedPhases * 4 * sizeof(Pt2)) +
+                              carve_size(per * kSeedPhases * 4) + carve_size(per * sizeof(CausticScalars));
+    st = map_hip(hipMalloc(&fs->cache_mem, bytes));
+    if (st != LCBG_OK) return st;
+    char* p = static_cast<char*>(fs->cache_mem);
+    fs->cache.bins = bins;
+    fs->cache.pts = carve<Pt2>(p, per * bins * 4);
+    fs->cache.br = carve<Pt2>(p, per * bins * 4);
+    fs->cache.pt_cell = carve<int>(p, per * bins * 4);
+    fs->cache.br_cell = carve<int>(p, per * bins * 4);
+    fs->cache.crit = carve<Pt2>(p, per * kSeedPhases * 4);
+    fs->cache.assign = carve<uint8_t>(p, per * kSeedPhases * 4);
+    fs->cache.sc = carve<CausticScalars>(p, per);
+    st = launch_curve_prep(b->d_curves, b->n_curves, fs->cache, fs->tab, nullptr);
+    if (st != LCBG_OK) return st;
+    return map_hip(hipDeviceSynchronize());
+}
+
+lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, int64_t n_epochs,
+                   double* mag, int32_t* method, hipStream_t stream, float* stage_ms)
+{
+    lcbg_status st = fs_prepare(b);
+    if (st != LCBG_OK) return st;
+    FsState* fs = b->fs;
+    st = ensure_epochs(fs, n_epochs > 0 ? n_epochs : 1);
+    if (st != LCBG_OK) return st;
+    const FsEpochs ep = epoch_view(fs, n_epochs, offsets, times, mag, method);
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (stage_ms) {
+        for (auto& e : ev) (void)hipEventCreate(&e);
+        (void)hipEventRecord(ev[0], stream);
+    }
+    ScratchPool& pool = g_pool[b->device % kMaxDevices];
+    std::lock_guard<std::mutex> lock(pool.mu);
+    st = ensure_scratch(pool);
+    if (st != LCBG_OK) return st;
+    st = map_hip(hipStreamWaitEvent(stream, pool.last_use, 0));
+    if (st == LCBG_OK) st = launch_reset_counters(ep.counters, 8, stream);
+    if (st == LCBG_OK) st = launch_fs_stage1(b->d_curves, b->n_curves, fs->cache, fs->opt, ep, stream);
+    if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[1], stream);
+    if (st == LCBG_OK) st = launch_fs_stage2(b->d_curves, fs->cache, fs->opt, ep, fs->stage2_waves, stream);
+    if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[2], stream);
+    if (st == LCBG_OK) st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, pool.scr, stream);
+    if (st == LCBG_OK) st = map_hip(hipEventRecord(pool.last_use, stream));
+    if (stage_ms && st == LCBG_OK) {
+        (void)hipEventRecord(ev[3], stream);
+        (void)hipEventSynchronize(ev[3]);
+        for (int k = 0; k < 3; ++k) (void)hipEventElapsedTime(&stage_ms[k], ev[k], ev[k + 1]);
+    }
+    if (stage_ms) {
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+    return st;
+}
+
+lcbg_status fs_error_flags(lcbg_batch* b, unsigned* flags, unsigned* counts)
+{
+    if (!b->fs || !b->fs->epoch_mem) return LCBG_INVALID_ARGUMENT;
+    const FsEpochs ep = epoch_view(b->fs, 0, nullptr, nullptr, nullptr, nullptr);
+    unsigned h[8];
+    lcbg_status st = map_hip(hipMemcpy(h, ep.counters, sizeof(h), hipMemcpyDeviceToHost));
+    if (st != LCBG_OK) return st;
+    if (flags) *flags = h[4];
+    if (counts) {
+        counts[0] = h[0];
+        counts[1] = h[2];
+    }
+    return LCBG_OK;
+}
+
+}  // namespace lcbg
+
+using namespace lcbg;
+
+extern "C" lcbg_status lcbg_batch_finite_source_device(lcbg_batch* batch, const int64_t* offsets,
+                                                       const double* times, int64_t n_epochs,
+                                                       double* mag, int32_t* method, void* stream)
+{
+    if (!batch || n_epochs < 0) return LCBG_INVALID_ARGUMENT;
+    if (batch->n_curves == 0 || n_epochs == 0) return LCBG_OK;
+    if (!offsets || !times || !mag || !method) return LCBG_INVALID_ARGUMENT;
+    return fs_run(batch, offsets, times, n_epochs, mag, method, static_cast<hipStream_t>(stream),
+                  nullptr);
+}
+
+extern "C" lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const int64_t* offsets,
+                                                        const double* times, int64_t n_epochs,
+                                                        double* mag, int32_t* method, void* stream,
+                                                        float* stage_ms, uint32_t* queue_counts,
+                                                        uint32_t* error_flags)
+{
+    if (!batch || n_epochs < 0 || !stage_ms) return LCBG_INVALID_ARGUMENT;
+    if (batch->n_curves == 0 || n_epochs == 0) return LCBG_OK;
+    lcbg_status st = fs_run(batch, offsets, times, n_epochs, mag, method,
+                            static_cast<hipStream_t>(stream), stage_ms);
+    if (st != LCBG_OK) return st;
+    return fs_error_flags(batch, error_flags, queue_counts);
+}
+
+extern "C" lcbg_status lcbg_light_curve(const lcbg_curve* curve, const lcbg_options* options,
+                                        const double* times, int64_t n, double* mag,
+                                        int32_t* method)
+{
+    if (!curve || n < 0 || (n > 0 && (!times || !mag))) return LCBG_INVALID_ARGUMENT;
+    if (n == 0) return LCBG_OK;
+    lcbg_batch* b = nullptr;
+    lcbg_status st = lcbg_batch_create(curve, 1, options, &b);
+    if (st != LCBG_OK) return st;
+    double* d_times = nullptr;
+    double* d_mag = nullptr;
+    int32_t* d_method = nullptr;
+    int64_t* d_off = nullptr;
+    const int64_t off[2] = {0, n};
+    const std::size_t bytes = static_cast<std::size_t>(n) * sizeof(double);
+    st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_times), bytes));
+    if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_mag), bytes));
+    if (st == LCBG_OK)
+        st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_method), static_cast<std::size_t>(n) * sizeof(int32_t)));
+    if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_off), sizeof(off)));
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(d_times, times, bytes, hipMemcpyHostToDevice));
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(d_off, off, sizeof(off), hipMemcpyHostToDevice));
+    if (st == LCBG_OK) st = lcbg_batch_finite_source_device(b, d_off, d_times, n, d_mag, d_method, nullptr);
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(mag, d_mag, bytes, hipMemcpyDeviceToHost));
+    if (st == LCBG_OK && method)
+        st = map_hip(hipMemcpy(method, d_method, static_cast<std::size_t>(n) * sizeof(int32_t),
+                               hipMemcpyDeviceToHost));
+    unsigned flags = 0;
+    if (st == LCBG_OK) st = fs_error_flags(b, &flags, nullptr);
+    if (st == LCBG_OK && flags) st = LCBG_UNSUPPORTED;
+    if (st == LCBG_OK) {
+        for (int64_t i = 0; i < n; ++i) {
+            if (!std::isfinite(mag[i])) {
+                st = LCBG_NUMERICAL_ERROR;  // python/lcbinint_pybind.cpp:1153-1155
+                break;
+            }
+        }
+    }
+    if (d_times) (void)hipFree(d_times);
+    if (d_mag) (void)hipFree(d_mag);
+    if (d_method) (void)hipFree(d_method);
+    if (d_off) (void)hipFree(d_off);
+    lcbg_batch_destroy(b);
+    return st;
+}

@@ -1,0 +1,1159 @@
+// Inverse-ray shooting for the epochs the dispatcher could not settle:
+// fixed_inverse_ray_binary (finite_source_magnifier.cpp:3042-3172) without the
+// adaptive refinement, i.e. image seeds, then the Cartesian row scan or the
+// polar flood fill on the reference's own lattice.
+//
+// One 64-lane wave per epoch, resident waves pulling epochs from a queue.  The
+// epoch's control program (seed bookkeeping, row records, gap repair, overlap
+// rules, the flood-fill queue) runs uniformly on all lanes; the lanes share the
+// data-parallel work:
+//   * seed search: critical sources come precomputed per curve; probe batches
+//     (8 caustic probes, 80 angular, 400 boundary, 192 interior) are solved one
+//     probe per lane, then merged in the reference's order;
+//   * Cartesian rows: 64 consecutive lattice cells per step, the cell
+//     coordinates reproducing the reference's repeated `x += incr` rounding
+//     exactly (see seq_coord), the row end found by ballot;
+//   * polar: radial runs and neighbour rows evaluated 64 cells at a time.
+#include "fs_launch.hpp"
+#include "fs_math.hpp"
+#include "fs_wave.hpp"
+
+namespace lcbg {
+
+// ------------------------------------------------------------------ context
+
+struct Ctx {
+    Lens L;
+    double s, q;   // separation and effective mass ratio as binary_mag receives them
+    Pt src;
+    double rho, r2, inv_r2;
+    double ld_c, ld_d;
+    bool uniform;
+    int lane;
+    Pt* seeds;
+    int n_seeds;
+    bool overflow;
+};
+
+// ------------------------------------------------------- probe image search
+
+// binary_image_candidates (point_source_magnifier.cpp:739-778) restricted to
+// selected_point_images (finite_source_magnifier.cpp:1873-1889): roots sorted
+// by residual, kept while residual^2 < 1e-12.
+__device__ __forceinline__ int probe_r3_images(const Ctx& C, Pt probe, Pt* out)
+{
+    if (C.s == 0.0 || C.q <= 0.0) return 0;
+    const Geom g = make_geom(C.s, C.q, probe.x, probe.y);
+    cx roots[5];
+    solve_quintic(g, roots);
+    double res[5];
+    int ord[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        res[i] = __builtin_sqrt(residual_squared(g, roots[i]));
+        ord[i] = i;
+    }
+    // stable insertion sort by residual (std::sort of 5 elements)
+    for (int i = 1; i < 5; ++i) {
+        const int v = ord[i];
+        const double rv = res[v];
+        int j = i;
+        while (j > 0 && rv < res[ord[j - 1]]) {
+            ord[j] = ord[j - 1];
+            --j;
+        }
+        ord[j] = v;
+    }
+    int n = 0;
+    for (int i = 0; i < 5; ++i) {
+        const int k = ord[i];
+        if (res[k] * res[k] < 1.0e-12) out[n++] = Pt{roots[k].re, roots[k].im};
+    }
+    return n;
+}
+
+__device__ __forceinline__ void push_seed(Ctx& C, Pt p)
+{
+    if (C.n_seeds >= kSeedCap) {
+        C.overflow = true;
+        return;
+    }
+    if (C.lane == 0) C.seeds[C.n_seeds] = p;
+    C.n_seeds += 1;
+    wave_sync();
+}
+
+// Index of the first seed in [0, n) within 0.25 rho of `img`, or -1.
+__device__ __forceinline__ int first_near_seed(const Ctx& C, Pt img, int n)
+{
+    const double lim = 0.0625 * C.r2;
+    for (int base = 0; base < n; base += kWave) {
+        const int k = base + C.lane;
+        bool hit = false;
+        if (k < n) hit = dist2(img, C.seeds[k]) < lim;
+        const uint64_t m = ballot(hit);
+        if (m) return base + first_set(m);
+    }
+    return -1;
+}
+
+// One probe's contribution, append_valid_probe_image_seeds
+// (finite_source_magnifier.cpp:1449-1498), replayed uniformly from the images a
+// lane found.  valid = probe inside the disk and more than three images.
+__device__ void merge_probe(Ctx& C, int n_img, const Pt* img, const double* md)
+{
+    const int n_before = C.n_seeds;
+    const double lim = C.r2 * (1.0 + 1.0e-8);
+    for (int i = 0; i < n_img; ++i) {
+        if (md[i] > lim) continue;
+        const int si = first_near_seed(C, img[i], n_before);
+        if (si >= 0) {
+            const double em = mapped_dist2(C.L, C.seeds[si].x, C.seeds[si].y, C.src);
+            if (md[i] + 1.0e-16 < em) {
+                if (C.lane == 0) C.seeds[si] = img[i];
+                wave_sync();
+            }
+        } else {
+            push_seed(C, img[i]);
+        }
+    }
+}
+
+// Per-lane probe evaluation: inside-disk test, R3 images, mapped distances.
+struct ProbeResult {
+    int n;
+    Pt img[5];
+    double md[5];
+};
+
+__device__ __forceinline__ ProbeResult eval_probe(const Ctx& C, Pt probe)
+{
+    ProbeResult r;
+    r.n = 0;
+    if (!(C.rho > 0.0) || dist2(probe, C.src) >= C.r2 * (1.0 + 1.0e-10)) return r;
+    Pt im[5];
+    const int n = probe_r3_images(C, probe, im);
+    if (n <= 3) return r;
+    r.n = n;
+    for (int i = 0; i < n; ++i) {
+        r.img[i] = im[i];
+        r.md[i] = mapped_dist2(C.L, im[i].x, im[i].y, C.src);
+    }
+    return r;
+}
+
+// Replays one lane's probe result uniformly.
+__device__ __forceinline__ void merge_lane(Ctx& C, const ProbeResult& mine, int src_lane)
+{
+    const int n = shfl(mine.n, src_lane);
+    if (n <= 3) return;
+    Pt img[5];
+    double md[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        img[i].x = shfl(mine.img[i].x, src_lane);
+        img[i].y = shfl(mine.img[i].y, src_lane);
+        md[i] = shfl(mine.md[i], src_lane);
+    }
+    merge_probe(C, n, img, md);
+}
+
+// append_caustic_probe_image_seeds (finite_source_magnifier.cpp:1500-1563)
+__device__ void caustic_probes(Ctx& C, Pt cs, const FsTables& tab)
+{
+    const double dx = cs.x - C.src.x;
+    const double dy = cs.y - C.src.y;
+    const double distance = hypot_cr(dx, dy);
+    if (distance <= 0.0 || distance >= C.rho) return;
+    const double ux = dx / distance;
+    const double uy = dy / distance;
+    {
+        // 4 steps x 2 directions in the reference's order, one probe per lane
+        const double steps[4] = {0.02 * C.rho, 0.05 * C.rho, 0.15 * C.rho, 0.35 * C.rho};
+        ProbeResult mine;
+        mine.n = 0;
+        if (C.lane < 8) {
+            const double st = steps[C.lane >> 1];
+            const Pt p = (C.lane & 1) ? Pt{cs.x - ux * st, cs.y - uy * st}
+                                      : Pt{cs.x + ux * st, cs.y + uy * st};
+            mine = eval_probe(C, p);
+        }
+        for (int k = 0; k < 8; ++k) merge_lane(C, mine, k);
+    }
+    if (C.n_seeds <= 3) {
+        const double radial[5] = {0.02 * C.rho, 0.05 * C.rho, 0.10 * C.rho, 0.20 * C.rho,
+                                  0.35 * C.rho};
+        for (int base = 0; base < 5 * kAngularProbes; base += kWave) {
+            const int k = base + C.lane;
+            ProbeResult mine;
+            mine.n = 0;
+            if (k < 5 * kAngularProbes) {
+                const double st = radial[k / kAngularProbes];
+                const Pt t = tab.angular_trig[k % kAngularProbes];
+                mine = eval_probe(C, Pt{cs.x + st * t.x, cs.y + st * t.y});
+            }
+            const int cnt = (5 * kAngularProbes - base) < kWave ? (5 * kAngularProbes - base) : kWave;
+            for (int j = 0; j < cnt; ++j) {
+                merge_lane(C, mine, j);
+                if (C.n_seeds > 3) return;
+            }
+        }
+    }
+}
+
+// nearest_critical_source_distance2 (finite_source_magnifier.cpp:1643-1664) at
+// an arbitrary phase; the phase's cos/sin come from the device libm.
+__device__ double nearest_critical(const Ctx& C, double phase, Pt* nearest)
+{
+    double sn, cs;
+    sincos(phase, &sn, &cs);
+    cx roots[4];
+    critical_roots(C.L, cx{cs, sn}, roots);
+    double best = __builtin_inf();
+    for (int r = 0; r < 4; ++r) {
+        const Pt p = map_real(C.L, roots[r].re, roots[r].im);
+        const double d2 = dist2(p, C.src);
+        if (d2 < best) {
+            best = d2;
+            *nearest = p;
+        }
+    }
+    return best;
+}
+
+// refine_nearest_critical_source (finite_source_magnifier.cpp:1666-1707)
+__device__ Pt refine_critical(const Ctx& C, double center, double step)
+{
+    double left = center - step;
+    double right = center + step;
+    const double golden = 0.61803398874989484820;
+    double x1 = right - golden * (right - left);
+    double x2 = left + golden * (right - left);
+    Pt n1{0.0, 0.0}, n2{0.0, 0.0};
+    double f1 = nearest_critical(C, x1, &n1);
+    double f2 = nearest_critical(C, x2, &n2);
+    for (int iter = 0; iter < 32; ++iter) {
+        if (f1 > f2) {
+            left = x1;
+            x1 = x2;
+            f1 = f2;
+            n1 = n2;
+            x2 = left + golden * (right - left);
+            f2 = nearest_critical(C, x2, &n2);
+        } else {
+            right = x2;
+            x2 = x1;
+            f2 = f1;
+            n2 = n1;
+            x1 = right - golden * (right - left);
+            f1 = nearest_critical(C, x1, &n1);
+        }
+    }
+    return f1 < f2 ? n1 : n2;
+}
+
+// augmented_image_seeds (finite_source_magnifier.cpp:1709-1871) without seed
+// hints, then augment_seeds_from_caustic_branches (:4845-4926).
+__device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const Pt* br,
+                            const int* br_cell, const CausticScalars& sc, int bins,
+                            const FsTables& tab, const double* roots_ps)
+{
+    // Phase 0: binary_images (point_source_magnifier.cpp:723-778) of the source
+    {
+        const Geom g = make_geom(C.s, C.q, C.src.x, C.src.y);
+        cx roots[5];
+        double res[5];
+        int ord[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            roots[i] = cx{roots_ps[2 * i], roots_ps[2 * i + 1]};
+            res[i] = __builtin_sqrt(residual_squared(g, roots[i]));
+            ord[i] = i;
+        }
+        for (int i = 1; i < 5; ++i) {
+            const int v = ord[i];
+            const double rv = res[v];
+            int j = i;
+            while (j > 0 && rv < res[ord[j - 1]]) {
+                ord[j] = ord[j - 1];
+                --j;
+            }
+            ord[j] = v;
+        }
+        const int physical = (res[ord[3]] * 1.0e-4 > res[ord[2]] + 1.0e-12) ? 3 : 5;
+        for (int i = 0; i < physical; ++i) push_seed(C, Pt{roots[ord[i]].re, roots[ord[i]].im});
+    }
+    if (!(C.rho > 0.0)) return;
+    const double phase_step = 2.0 * kPi / static_cast<double>(kSeedPhases);
+
+    // Phase 1: strided scan (sample = j*40 + k; k outer, j inner), stopping at
+    // the first phase whose caustic probes add a seed.
+    bool found = false;
+    double best_d2 = __builtin_inf();
+    int best_ord = 0x7fffffff;
+    constexpr int kOrd = kSeedPhases * 4;  // scan positions: (k*35 + j)*4 + root
+    for (int base = 0; base < kOrd && !found; base += kWave) {
+        const int o = base + C.lane;
+        double d2 = __builtin_inf();
+        bool inside = false;
+        if (o < kOrd) {
+            const int ph = o >> 2;
+            const int r = o & 3;
+            const int kphi = ph / 35;
+            const int jphi = ph - kphi * 35;
+            const int sample = jphi * 40 + kphi;
+            d2 = dist2(crit[4 * sample + r], C.src);
+            inside = d2 < C.r2 && d2 > 0.0;
+        }
+        // running best in scan order (ties keep the earliest)
+        {
+            double v = d2;
+            int idx = o < kOrd ? o : 0x7fffffff;
+            wave_argmin(v, idx);
+            if (v < best_d2) {
+                best_d2 = v;
+                best_ord = idx;
+            }
+        }
+        uint64_t m = ballot(inside);
+        int cur_phase = -1;
+        bool phase_found = false;
+        while (m) {
+            const int b = first_set(m);
+            m &= m - 1;
+            const int o2 = base + b;
+            const int ph = o2 >> 2;
+            if (ph != cur_phase) {
+                if (cur_phase >= 0 && phase_found) break;
+                cur_phase = ph;
+                phase_found = false;
+            }
+            const int kphi = ph / 35;
+            const int jphi = ph - kphi * 35;
+            const int sample = jphi * 40 + kphi;
+            const int before = C.n_seeds;
+            caustic_probes(C, crit[4 * sample + (o2 & 3)], tab);
+            phase_found = C.n_seeds > before;
+        }
+        if (cur_phase >= 0 && phase_found) {
+            found = true;
+            // the scan stops here; best_d2 is not used once a crossing is found
+        }
+    }
+    if (!found && best_d2 < C.r2 && best_d2 > 0.0) {
+        const int ph = best_ord >> 2;
+        const int kphi = ph / 35;
+        const int jphi = ph - kphi * 35;
+        const int sample = jphi * 40 + kphi;
+        const double best_phase = phase_step * static_cast<double>(sample);
+        const Pt cs = refine_critical(C, best_phase, phase_step);
+        caustic_probes(C, cs, tab);
+    }
+
+    // Phase 2: branch-tracked scan, probes on outside->inside transitions and
+    // every 20th inside sample for large sources.
+    if (C.n_seeds >= 5) {
+        const bool arc = C.rho >= 2.0e-2;
+        for (int base = 0; base < kOrd; base += kWave) {
+            const int o = base + C.lane;
+            bool fire = false;
+            if (o < kOrd) {
+                const int sample = o >> 2;
+                const int bi = o & 3;
+                const int j = assign[4 * sample + bi];
+                const double d2 = dist2(crit[4 * sample + j], C.src);
+                const bool now = d2 < C.r2 && d2 > 0.0;
+                bool was = false;
+                if (sample > 0) {
+                    const int jp = assign[4 * (sample - 1) + bi];
+                    const double dp = dist2(crit[4 * (sample - 1) + jp], C.src);
+                    was = dp < C.r2 && dp > 0.0;
+                }
+                fire = (now && !was) || (arc && now && (sample % 20 == 0));
+            }
+            uint64_t m = ballot(fire);
+            while (m) {
+                const int b = first_set(m);
+                m &= m - 1;
+                const int o2 = base + b;
+                const int sample = o2 >> 2;
+                const int j = assign[4 * sample + (o2 & 3)];
+                caustic_probes(C, crit[4 * sample + j], tab);
+            }
+        }
+    }
+
+    // boundary probes at 0.98 rho (:1565-1591), stopping at 128 seeds
+    if (C.n_seeds < 128) {
+        const double pr = C.rho * (1.0 - 0.02);
+        for (int base = 0; base < kBoundaryProbes && C.n_seeds < 128; base += kWave) {
+            const int k = base + C.lane;
+            ProbeResult mine;
+            mine.n = 0;
+            if (k < kBoundaryProbes) {
+                const Pt t = tab.boundary_trig[k];
+                mine = eval_probe(C, Pt{C.src.x + pr * t.x, C.src.y + pr * t.y});
+            }
+            const int cnt = (kBoundaryProbes - base) < kWave ? (kBoundaryProbes - base) : kWave;
+            for (int j = 0; j < cnt && C.n_seeds < 128; ++j) merge_lane(C, mine, j);
+        }
+    }
+    // interior probes (:1593-1620) for rho >= 0.02
+    if (C.rho >= 2.0e-2) {
+        const double fr[3] = {0.25, 0.5, 0.75};
+        for (int ri = 0; ri < 3; ++ri) {
+            const double pr = C.rho * fr[ri];
+            const Pt t = tab.interior_trig[C.lane];
+            const ProbeResult mine = eval_probe(C, Pt{C.src.x + pr * t.x, C.src.y + pr * t.y});
+            for (int j = 0; j < kInteriorProbes; ++j) merge_lane(C, mine, j);
+        }
+    }
+
+    // Phase 3 (:4845-4926), only with fewer than five seeds: probe past the
+    // nearest caustic segment; a longer image list replaces the seeds.
+    if (C.n_seeds < 5 && sc.n >= 2) {
+        const double seg_search = C.rho + sc.max_seg_len;
+        const int ix0 = iclamp(static_cast<int>((C.src.x - seg_search - sc.min_x) / sc.step_x), 0, kGrid - 1);
+        const int ix1 = iclamp(static_cast<int>((C.src.x + seg_search - sc.min_x) / sc.step_x), 0, kGrid - 1);
+        const int iy0 = iclamp(static_cast<int>((C.src.y - seg_search - sc.min_y) / sc.step_y), 0, kGrid - 1);
+        const int iy1 = iclamp(static_cast<int>((C.src.y + seg_search - sc.min_y) / sc.step_y), 0, kGrid - 1);
+        const int n = sc.n;
+        // Candidates are visited in (cell, branch, position) order; the seed
+        // count is tested only when the cell changes, as in the reference.
+        int last_key = -1;
+        long long cursor = -1;  // last processed composite key
+        for (;;) {
+            // find the smallest composite key greater than cursor among candidates
+            long long best = 0x7fffffffffffffffLL;
+            for (int k = C.lane; k < 4 * bins; k += kWave) {
+                const int cell = br_cell[k];
+                const int iy = cell / kGrid;
+                const int ix = cell - iy * kGrid;
+                if (ix < ix0 || ix > ix1 || iy < iy0 || iy > iy1) continue;
+                const long long key = static_cast<long long>(cell) * (4LL * bins) + k;
+                if (key <= cursor || key >= best) continue;
+                const int b = k / bins;
+                const int pos = k - b * bins;
+                const int next = (pos + 1) % n;
+                if (next == 0) continue;
+                const Pt* bb = br + static_cast<size_t>(b) * bins;
+                if (point_segment_distance(C.src, bb[pos], bb[next]) >= C.rho) continue;
+                best = key;
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const long long o = __shfl_xor(best, off, kWave);
+                best = o < best ? o : best;
+            }
+            if (best == 0x7fffffffffffffffLL) break;
+            cursor = best;
+            const int cell = static_cast<int>(best / (4LL * bins));
+            if (cell != last_key) {
+                if (C.n_seeds >= 5) break;
+                last_key = cell;
+            }
+            const int k = static_cast<int>(best - static_cast<long long>(cell) * (4LL * bins));
+            const int b = k / bins;
+            const int pos = k - b * bins;
+            const int next = (pos + 1) % n;
+            const Pt* bb = br + static_cast<size_t>(b) * bins;
+            const Pt p0 = bb[pos], p1 = bb[next];
+            const double seg_dx = p1.x - p0.x;
+            const double seg_dy = p1.y - p0.y;
+            const double seg_len2 = seg_dx * seg_dx + seg_dy * seg_dy;
+            const double t = seg_len2 > 0.0
+                                 ? dclamp(((C.src.x - p0.x) * seg_dx + (C.src.y - p0.y) * seg_dy) / seg_len2,
+                                          0.0, 1.0)
+                                 : 0.0;
+            const Pt nearest{p0.x + t * seg_dx, p0.y + t * seg_dy};
+            const double distance = __builtin_sqrt(dist2(nearest, C.src));
+            if (distance <= 0.0 || distance >= C.rho) continue;
+            const double step = C.rho * 0.05 / distance;
+            const Pt probe{nearest.x + (nearest.x - C.src.x) * step, nearest.y + (nearest.y - C.src.y) * step};
+            if (dist2(probe, C.src) >= C.rho * C.rho) continue;
+            Pt im[5];
+            const int ni = probe_r3_images(C, probe, im);
+            if (ni > C.n_seeds) {
+                for (int i = 0; i < ni; ++i) {
+                    if (C.lane == 0) C.seeds[i] = im[i];
+                }
+                C.n_seeds = ni;
+                wave_sync();
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ Cartesian scan
+
+// x after k sequential additions of `step` to x0 (the reference advances
+// lattice cells with `x += incr`, finite_source_magnifier.cpp:2004-2010).
+// Within one binade, with no rounding tie, every addition adds the same
+// rounded increment d, so x_k = x0 + k d exactly; otherwise the lanes replay
+// the additions one by one.
+__device__ __forceinline__ double seq_coord(double x0, double step, int k)
+{
+    bool fast = false;
+    double d = 0.0;
+    if (x0 != 0.0 && isfinite(x0)) {
+        const double a0 = fabs(x0);
+        int e;
+        frexp(a0, &e);  // a0 in [2^(e-1), 2^e)
+        const double lo = ldexp(1.0, e - 1);
+        const double hi = ldexp(1.0, e);
+        const double ulp = ldexp(1.0, e - 53);
+        const double x1 = x0 + step;
+        d = x1 - x0;
+        const double xe = x0 + 64.0 * d;
+        const double as = fabs(step);
+        const double ae = fabs(xe);
+        const bool same_sign = (x0 > 0.0) == (xe > 0.0);
+        const bool in_binade = a0 - as >= lo && a0 + as < hi && ae - as >= lo && ae + as < hi;
+        const bool no_tie = fabs(step - d) != 0.5 * ulp;
+        const bool exact = fabs(d) * 64.0 < 0x1p53 * ulp;
+        fast = same_sign && in_binade && no_tie && exact && lo >= 0x1p-1000;
+    }
+    if (fast) return x0 + static_cast<double>(k) * d;
+    double x = x0;
+    for (int j = 0; j < kWave; ++j) {
+        if (j < k) x = x + step;
+    }
+    return x;
+}
+
+struct Rows {
+    double* r;     // [row*5 + field]
+    int* key;      // pushed row keys
+    int* idx;      // pushed row indices (stored after the key array)
+    int size;      // rows allocated (ensure)
+    int pushed;
+};
+
+enum { XMIN = 0, XMAX = 1, AX = 2, YY = 3, DYS = 4 };
+
+struct Cart {
+    double incr, inv_incr;
+    int jac_sign;
+    long long max_steps;
+};
+
+__device__ __forceinline__ bool rows_ensure(Ctx& C, Rows& R, int index)
+{
+    if (index < R.size) return true;
+    if (index >= kRowCap) {
+        C.overflow = true;
+        return false;
+    }
+    const int from = R.size * 5;
+    const int to = (index + 1) * 5;
+    for (int k = from + C.lane; k < to; k += kWave) R.r[k] = 0.0;
+    R.size = index + 1;
+    wave_sync();
+    return true;
+}
+
+__device__ __forceinline__ void rset(Ctx& C, Rows& R, int row, int f, double v)
+{
+    if (C.lane == 0) R.r[row * 5 + f] = v;
+    wave_sync();
+}
+
+__device__ __forceinline__ double rget(const Rows& R, int row, int f) { return R.r[row * 5 + f]; }
+
+// One lattice cell: inside flag and brightness weight
+// (finite_source_magnifier.cpp:1922-1958).
+__device__ __forceinline__ bool cart_cell(const Ctx& C, const Cart& K, double x, double y, double* w)
+{
+    double md, dz2;
+    if (K.jac_sign == 0) {
+        md = mapped_dist2(C.L, x, y, C.src);
+        dz2 = md;
+    } else {
+        double jac;
+        md = mapped_dist2_jac(C.L, x, y, C.src, &jac);
+        const int es = sign_of(jac);
+        dz2 = (es != -K.jac_sign) ? md : C.r2 + 1.0;
+    }
+    const bool in = dz2 <= C.r2;
+    *w = in ? (C.uniform ? 1.0 : ld_brightness(md * C.inv_r2, C.ld_c, C.ld_d)) : 0.0;
+    return in;
+}
+
+// cartesian_image_area_impl (finite_source_magnifier.cpp:1892-2014).
+__device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi)
+{
+    const double incr = K.incr;
+    double countx = 0.0;
+    double countall = 0.0;
+    bool last_in = 99999999.9 <= C.r2;
+    int dir = 1;
+    double x = seed.x;
+    double y = seed.y;
+    double x0 = seed.x;
+    long long guard = 0;
+    const long long budget = K.max_steps - 1;
+    for (;;) {
+        if (!rows_ensure(C, R, yi)) return __builtin_nan("");
+        const long long remaining = budget - guard;
+        if (remaining <= 0) return countall;
+        const int n_eval = remaining < kWave ? static_cast<int>(remaining) : kWave;
+        const double step = dir > 0 ? incr : -incr;
+        const double xk = seq_coord(x, step, C.lane);
+        double w = 0.0;
+        bool in = false;
+        if (C.lane < n_eval) in = cart_cell(C, K, xk, y, &w);
+        const bool valid = C.lane < n_eval;
+        const uint64_t in_m = ballot(in && valid);
+        if (dir > 0) {
+            const int s = first_set(ballot(!in && valid));
+            if (s < 0) {
+                countx += wave_sum(w);
+                guard += n_eval;
+                last_in = true;
+                x = shfl(xk, n_eval - 1) + step;
+                continue;
+            }
+            countx += wave_sum(C.lane < s ? w : 0.0);
+            guard += s + 1;
+            const double xs = shfl(xk, s);
+            const bool prev_in = s > 0 ? true : last_in;
+            if (prev_in) rset(C, R, yi, XMAX, xs);
+            dir = -1;
+            x = x0;
+            rset(C, R, yi, XMIN, x + (-incr));
+            last_in = false;
+            x = x + (-incr);
+            continue;
+        }
+        // left scan
+        const bool cb = countx != 0.0;
+        double thr = 0.0;
+        if (yi != 0) thr = rget(R, yi - 1, XMIN) - (-incr);
+        const bool cond = (yi != 0) && (xk >= thr);
+        const int f = first_set(in_m);
+        const bool stop = valid && !in && (cb || (f >= 0 && C.lane > f) || yi == 0 || !cond);
+        const int s = first_set(ballot(stop));
+        const int limit = s >= 0 ? s : n_eval;
+        const double add = wave_sum((C.lane < limit && in) ? w : 0.0);
+        if (!cb && f >= 0 && f < limit) rset(C, R, yi, XMAX, shfl(xk, f) - (-incr));
+        if (s < 0) {
+            countx += add;
+            guard += n_eval;
+            last_in = ((in_m >> (n_eval - 1)) & 1ull) != 0;
+            x = shfl(xk, n_eval - 1) + (-incr);
+            continue;
+        }
+        countx += add;
+        guard += s + 1;
+        const double xs = shfl(xk, s);
+        const bool prev_in = s > 0 ? (((in_m >> (s - 1)) & 1ull) != 0) : last_in;
+        if (prev_in) rset(C, R, yi, XMIN, xs);
+        // close the row
+        countall += countx;
+        rset(C, R, yi, AX, countx);
+        rset(C, R, yi, YY, y);
+        rset(C, R, yi, DYS, dy);
+        if (countx == 0.0) {
+            rset(C, R, yi, DYS, -dy);
+            return countall;
+        }
+        const int key = static_cast<int>(y * K.inv_incr + 2000000);
+        {
+            const double xmin_y = rget(R, yi, XMIN);
+            const double xmax_y = rget(R, yi, XMAX);
+            bool dup = false;
+            for (int base = 0; base < R.pushed && !dup; base += kWave) {
+                const int k = base + C.lane;
+                bool hit = false;
+                if (k < R.pushed && R.key[k] == key) {
+                    const int e = R.idx[k];
+                    hit = xmin_y + incr < rget(R, e, XMAX) && xmax_y - incr > rget(R, e, XMIN);
+                }
+                dup = ballot(hit) != 0;
+            }
+            if (dup) return countall - countx;
+        }
+        if (R.pushed >= kRowCap) {
+            C.overflow = true;
+            return __builtin_nan("");
+        }
+        if (C.lane == 0) {
+            R.key[R.pushed] = key;
+            R.idx[R.pushed] = yi;
+        }
+        R.pushed += 1;
+        wave_sync();
+        ++yi;
+        if (!rows_ensure(C, R, yi)) return __builtin_nan("");
+        dir = 1;
+        x0 = rget(R, yi - 1, XMAX);
+        x = x0 - incr;
+        y = y + dy;
+        countx = 0.0;
+        last_in = false;
+        x = x + incr;
+    }
+}
+
+// inverse_ray_cartesian_binary_mag (finite_source_magnifier.cpp:2470-2768)
+__device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* rows_key,
+                                double* areaimage, int* flags)
+{
+    const int n_img = C.n_seeds;
+    if (n_img == 0 || !(C.rho > 0.0)) return __builtin_nan("");
+    const double nbin = static_cast<double>(source_bins > 1 ? source_bins : 1);
+    Cart K;
+    K.incr = C.rho / nbin;
+    K.inv_incr = 1.0 / K.incr;
+    const int sb = source_bins;
+    K.max_steps = (100000LL > static_cast<long long>(sb) * sb * 2000LL) ? 100000LL
+                                                                         : static_cast<long long>(sb) * sb * 2000LL;
+    const double incr = K.incr;
+    const double m = 0.5 * incr * 1.01;
+    for (int k = C.lane; k < n_img; k += kWave) {
+        areaimage[k] = 0.0;
+        flags[k] = 0;  // bit 0: overlap, bit 1: subtracted
+    }
+    wave_sync();
+    double area = 0.0;
+    for (int i = 0; i < n_img; ++i) {
+        if (flags[i] & 1) continue;
+        Rows R;
+        R.r = rows_mem;
+        R.key = rows_key;
+        R.idx = rows_key + kRowCap;
+        R.size = 0;
+        R.pushed = 0;
+        if (!rows_ensure(C, R, 1)) return __builtin_nan("");
+        double area0 = 0.0, areai = 0.0;
+        double dy = incr;
+        int yi = 0;
+        const Pt seed = C.seeds[i];
+        const double J_seed = jacobian_real(C.L, seed.x, seed.y);
+        K.jac_sign = fabs(J_seed) < 0.02 ? sign_of(J_seed) : 0;
+        rset(C, R, 0, XMIN, seed.x);
+        rset(C, R, 0, XMAX, seed.x);
+        areai = cart_scan(C, K, R, seed, dy, yi);
+        if (C.overflow) return __builtin_nan("");
+        dy = -incr;
+        if (!rows_ensure(C, R, yi)) return __builtin_nan("");
+        const Pt lower{rget(R, 0, XMAX), seed.y + dy};
+        rset(C, R, yi, XMIN, rget(R, 0, XMIN));
+        rset(C, R, yi, XMAX, rget(R, 0, XMAX));
+        rset(C, R, yi, YY, rget(R, 0, YY));
+        rset(C, R, yi, DYS, dy);
+        ++yi;
+        areai += cart_scan(C, K, R, lower, dy, yi);
+        if (C.overflow) return __builtin_nan("");
+        int nyi = yi;
+        double areabound = 0.0;
+        const double g = 1.1 * incr;
+        for (int row = 0; row < nyi; ++row) {
+            if (!rows_ensure(C, R, row + 1)) return __builtin_nan("");
+            const double dxmax = rget(R, row + 1, XMAX) - rget(R, row, XMAX);
+            const double dxmin = rget(R, row + 1, XMIN) - rget(R, row, XMIN);
+            if (rget(R, row + 1, AX) > 0.0) {
+                if (dxmax > g) {
+                    const Pt extra{rget(R, row + 1, XMAX), rget(R, row, YY)};
+                    if (!rows_ensure(C, R, yi)) return __builtin_nan("");
+                    rset(C, R, yi, XMIN, rget(R, row, XMAX));
+                    rset(C, R, yi, XMAX, rget(R, row + 1, XMAX));
+                    dy = -rget(R, row, DYS);
+                    rset(C, R, yi, DYS, dy);
+                    ++yi;
+                    area0 = cart_scan(C, K, R, extra, dy, yi);
+                    areai += area0;
+                    areabound += area0;
+                    if (area0 <= 0.0) --yi;
+                }
+                if (dxmin > g) {
+                    const Pt extra{rget(R, row + 1, XMIN) - incr, rget(R, row + 1, YY)};
+                    if (!rows_ensure(C, R, yi)) return __builtin_nan("");
+                    rset(C, R, yi, XMIN, rget(R, row, XMIN));
+                    rset(C, R, yi, XMAX, rget(R, row + 1, XMIN));
+                    dy = rget(R, row, DYS);
+                    rset(C, R, yi, DYS, dy);
+                    ++yi;
+                    area0 = cart_scan(C, K, R, extra, dy, yi);
+                    areai += area0;
+                    areabound += area0;
+                    if (area0 <= 0.0) --yi;
+                }
+                if (dxmin < -g) {
+                    const Pt extra{rget(R, row, XMIN) - incr, rget(R, row, YY)};
+                    if (!rows_ensure(C, R, yi)) return __builtin_nan("");
+                    rset(C, R, yi, XMIN, rget(R, row + 1, XMIN));
+                    rset(C, R, yi, XMAX, rget(R, row, XMIN));
+                    dy = -rget(R, row, DYS);
+                    rset(C, R, yi, DYS, dy);
+                    ++yi;
+                    area0 = cart_scan(C, K, R, extra, dy, yi);
+                    areai += area0;
+                    areabound += area0;
+                    if (area0 <= 0.0) --yi;
+                }
+                if (dxmax < -g) {
+                    const Pt extra{rget(R, row + 1, XMAX) + incr, rget(R, row + 1, YY)};
+                    if (!rows_ensure(C, R, yi)) return __builtin_nan("");
+                    rset(C, R, yi, XMIN, rget(R, row + 1, XMAX));
+                    rset(C, R, yi, XMAX, rget(R, row, XMAX));
+                    dy = rget(R, row, DYS);
+                    rset(C, R, yi, DYS, dy);
+                    ++yi;
+                    area0 = cart_scan(C, K, R, extra, dy, yi);
+                    areai += area0;
+                    areabound += area0;
+                    if (area0 <= 0.0) --yi;
+                }
+                if (C.overflow) return __builtin_nan("");
+            }
+            if (row == nyi - 1 && areabound > 0.0 && yi > nyi) nyi = yi;
+        }
+        area += areai;
+        if (C.lane == 0) areaimage[i] = areai;
+        wave_sync();
+        // overlap bookkeeping (:2681-2739)
+        for (int other = 0; other < n_img; ++other) {
+            if (other == i) continue;
+            const Pt pos = C.seeds[other];
+            if (K.jac_sign != 0) {
+                const double Jo = jacobian_real(C.L, pos.x, pos.y);
+                if (sign_of(Jo) == -K.jac_sign) continue;
+                if (C.rho >= 4.0e-3 && source_bins >= 35 && other > i && fabs(J_seed) < 1.0e-3 &&
+                    fabs(Jo) < 0.02)
+                    continue;
+            }
+            bool hit = false;
+            for (int base = 0; base < nyi && !hit; base += kWave) {
+                const int row = base + C.lane;
+                bool h = false;
+                if (row < nyi && rget(R, row, AX) > 0.0) {
+                    const double yy = rget(R, row, YY);
+                    h = pos.y >= yy - m && pos.y <= yy + m && pos.x >= rget(R, row, XMIN) - m &&
+                        pos.x <= rget(R, row, XMAX) + m;
+                }
+                hit = ballot(h) != 0;
+            }
+            if (hit) {
+                if (other < i) {
+                    const int fl = flags[other];
+                    if (!(fl & 2)) {
+                        area -= areaimage[other];
+                        if (C.lane == 0) flags[other] = fl | 2;
+                    }
+                } else {
+                    if (C.lane == 0) flags[other] = flags[other] | 1;
+                }
+                wave_sync();
+            }
+        }
+    }
+    const double scale = source_flux(C.rho, C.ld_c, C.ld_d) / (C.rho * C.rho) * nbin * nbin;
+    return area / scale;
+}
+
+// ---------------------------------------------------------------- polar fill
+
+struct Polar {
+    double dr, dphi;
+    int phi_bins;
+    int* head;   // [iphi] -> first run index, -1 if none
+    int* runs;   // [run*4]: iphi, left, right, next
+    int n_runs;
+};
+
+__device__ __forceinline__ bool polar_visited(const Polar& P, int ir, int iphi)
+{
+    if (ir < 0) return true;
+    for (int r = P.head[iphi]; r >= 0; r = P.runs[4 * r + 3]) {
+        if (ir >= P.runs[4 * r + 1] && ir <= P.runs[4 * r + 2]) return true;
+    }
+    return false;
+}
+
+// cell_inside of inverse_ray_polar_boundary_binary (:1241-1254); the angle's
+// cos/sin come from the device libm.
+__device__ __forceinline__ bool polar_inside(const Ctx& C, const Polar& P, int ir, double cph,
+                                             double sph, double* dz2)
+{
+    if (ir < 0) return false;
+    const double radius = (static_cast<double>(ir) + 0.5) * P.dr;
+    const Pt w = map_real(C.L, radius * cph, radius * sph);
+    const double d = dist2(w, C.src);
+    *dz2 = d;
+    return d <= C.r2;
+}
+
+__device__ __forceinline__ void phi_trig(const Polar& P, int iphi, double* c, double* s)
+{
+    const double phi = (static_cast<double>(iphi) + 0.5) * P.dphi;
+    sincos(phi, s, c);
+}
+
+// Extends an inside, unvisited cell to its radial run (:1290-1297), records
+// it and returns its brightness-weighted sum of radii.
+__device__ double polar_add_run(Ctx& C, Polar& P, int ir, int iphi, double cph, double sph)
+{
+    // left
+    int left = ir;
+    for (;;) {
+        const int c = left - 1 - C.lane;
+        bool ok = false;
+        double dz2;
+        if (c >= 0) ok = !polar_visited(P, c, iphi) && polar_inside(C, P, c, cph, sph, &dz2);
+        const uint64_t m = ballot(ok);
+        const int stop = first_set(~m);
+        if (stop < 0) {
+            left -= kWave;
+            continue;
+        }
+        left -= stop;
+        break;
+    }
+    int right = ir;
+    for (;;) {
+        const int c = right + 1 + C.lane;
+        double dz2;
+        const bool ok = !polar_visited(P, c, iphi) && polar_inside(C, P, c, cph, sph, &dz2);
+        const uint64_t m = ballot(ok);
+        const int stop = first_set(~m);
+        if (stop < 0) {
+            right += kWave;
+            continue;
+        }
+        right += stop;
+        break;
+    }
+    if (P.n_runs >= kRunCap) {
+        C.overflow = true;
+        return 0.0;
+    }
+    if (C.lane == 0) {
+        const int r = P.n_runs;
+        P.runs[4 * r + 0] = iphi;
+        P.runs[4 * r + 1] = left;
+        P.runs[4 * r + 2] = right;
+        P.runs[4 * r + 3] = P.head[iphi];
+        P.head[iphi] = r;
+    }
+    P.n_runs += 1;
+    wave_sync();
+    double sum = 0.0;
+    for (int c = left + C.lane; c <= right; c += kWave) {
+        double dz2 = 0.0;
+        polar_inside(C, P, c, cph, sph, &dz2);
+        const double radius = (static_cast<double>(c) + 0.5) * P.dr;
+        const double b = C.uniform ? 1.0 : ld_brightness(dz2 * C.inv_r2, C.ld_c, C.ld_d);
+        sum += b * radius;
+    }
+    return wave_sum(sum);
+}
+
+__device__ __forceinline__ double wrap_angle(double angle)
+{
+    while (angle < 0.0) angle += 2.0 * kPi;
+    while (angle >= 2.0 * kPi) angle -= 2.0 * kPi;
+    return angle;
+}
+
+// inverse_ray_polar_boundary_binary (:1156-1319): the flood fill sums every
+// cell of the 4-connected inside components that contain a start cell; here
+// the components are grown run by run (scan-line fill over neighbour rows).
+__device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* head, int* runs)
+{
+    const int n_img = C.n_seeds;
+    if (n_img == 0) return __builtin_nan("");
+    Polar P;
+    P.dr = C.rho / static_cast<double>(source_bins);
+    double max_r = 1.0;
+    for (int i = 0; i < n_img; ++i) {
+        const Pt s = C.seeds[i];
+        max_r = dmax(max_r, hypot_cr(s.x, s.y) + 4.0 * C.rho);
+    }
+    const int pb = static_cast<int>(ceil(2.0 * kPi * max_r / (P.dr * grid_ratio)));
+    P.phi_bins = pb > 16 ? pb : 16;
+    if (P.phi_bins > kPhiCap) {
+        C.overflow = true;
+        return __builtin_nan("");
+    }
+    P.dphi = 2.0 * kPi / static_cast<double>(P.phi_bins);
+    const double flux = source_flux(C.rho, C.ld_c, C.ld_d);
+    if (!isfinite(flux)) return __builtin_nan("");
+    P.head = head;
+    P.runs = runs;
+    P.n_runs = 0;
+    double total = 0.0;
+
+    // start cells (find_polar_inside_start, :1321-1364): 242 candidates in
+    // the reference's shell / direction / radial order, evaluated in parallel
+    for (int i = 0; i < n_img && !C.overflow; ++i) {
+        const Pt seed = C.seeds[i];
+        const double seed_radius = hypot_cr(seed.x, seed.y);
+        const double seed_phi = wrap_angle(atan2(seed.y, seed.x));
+        const int seed_ir = static_cast<int>(floor(seed_radius / P.dr));
+        const int seed_iphi = iclamp(static_cast<int>(seed_phi / P.dphi), 0, P.phi_bins - 1);
+        int found_ord = -1;
+        for (int base = 0; base < 242 && found_ord < 0; base += kWave) {
+            const int o = base + C.lane;
+            bool in = false;
+            if (o < 242) {
+                // decode o -> (shell, dir, d_ir); shell s has 2*(2s+1) entries
+                int shell = 0, rem = o;
+                while (rem >= 2 * (2 * shell + 1)) {
+                    rem -= 2 * (2 * shell + 1);
+                    ++shell;
+                }
+                const int dir = rem < (2 * shell + 1) ? -1 : 1;
+                const int d_ir = (rem % (2 * shell + 1)) - shell;
+                const int d_iphi = (shell - abs(d_ir)) * dir;
+                const int ir = seed_ir + d_ir;
+                if (ir >= 0) {
+                    int iphi = (seed_iphi + d_iphi) % P.phi_bins;
+                    if (iphi < 0) iphi += P.phi_bins;
+                    double cph, sph, dz2;
+                    phi_trig(P, iphi, &cph, &sph);
+                    in = polar_inside(C, P, ir, cph, sph, &dz2);
+                }
+            }
+            const uint64_t m = ballot(in);
+            if (m) found_ord = base + first_set(m);
+        }
+        if (found_ord < 0) continue;
+        int shell = 0, rem = found_ord;
+        while (rem >= 2 * (2 * shell + 1)) {
+            rem -= 2 * (2 * shell + 1);
+            ++shell;
+        }
+        const int dir = rem < (2 * shell + 1) ? -1 : 1;
+        const int d_ir = (rem % (2 * shell + 1)) - shell;
+        const int d_iphi = (shell - abs(d_ir)) * dir;
+        const int sir = seed_ir + d_ir;
+        int siphi = (seed_iphi + d_iphi) % P.phi_bins;
+        if (siphi < 0) siphi += P.phi_bins;
+        const double grid_radius = (static_cast<double>(sir) + 0.5) * P.dr;
+        const double image_phi = (static_cast<double>(siphi) + 0.5) * P.dphi;
+        const int fl = static_cast<int>(floor(grid_radius / P.dr));
+        const int ir = fl > 0 ? fl : 0;
+        const int iphi = iclamp(static_cast<int>(image_phi / P.dphi), 0, P.phi_bins - 1);
+        double cph, sph, dz2;
+        phi_trig(P, iphi, &cph, &sph);
+        if (polar_visited(P, ir, iphi) || !polar_inside(C, P, ir, cph, sph, &dz2)) continue;
+        total += polar_add_run(C, P, ir, iphi, cph, sph);
+    }
+    // grow: each recorded run visits its two angular neighbour rows
+    for (int q = 0; q < P.n_runs && !C.overflow; ++q) {
+        const int p = runs[4 * q + 0];
+        const int L = runs[4 * q + 1];
+        const int Rr = runs[4 * q + 2];
+        for (int side = 0; side < 2; ++side) {
+            int nb = side == 0 ? p - 1 : p + 1;
+            if (nb < 0) nb += P.phi_bins;
+            if (nb >= P.phi_bins) nb -= P.phi_bins;
+            double cph, sph;
+            phi_trig(P, nb, &cph, &sph);
+            int c = L;
+            while (c <= Rr && !C.overflow) {
+                const int cc = c + C.lane;
+                bool ok = false;
+                if (cc <= Rr) {
+                    double dz2;
+                    ok = !polar_visited(P, cc, nb) && polar_inside(C, P, cc, cph, sph, &dz2);
+                }
+                const uint64_t m = ballot(ok);
+                if (!m) {
+                    c += kWave;
+                    continue;
+                }
+                const int start = c + first_set(m);
+                total += polar_add_run(C, P, start, nb, cph, sph);
+                c = runs[4 * (P.n_runs - 1) + 2] + 1;
+            }
+        }
+    }
+    // reset the rows this epoch touched
+    for (int q = C.lane; q < P.n_runs; q += kWave) head[runs[4 * q]] = -1;
+    wave_sync();
+    if (C.overflow) return __builtin_nan("");
+    const double image_flux = total * P.dr * P.dphi;
+    return image_flux / flux;
+}
+
+// ------------------------------------------------------------------- kernel
+
+__global__ void __launch_bounds__(kWave) k_fs_ir(const CurveRec* __restrict__ curves, FsCache cache,
+                                                  FsTables tab, FsOptions opt, FsEpochs ep,
+                                                  IrScratch scr)
+{
+    const int lane = lane_id();
+    const int slot = blockIdx.x;
+    __shared__ int s_item;
+    Pt* seeds = scr.seeds + static_cast<size_t>(slot) * kSeedCap;
+    double* areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
+    int* flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
+    double* rows = scr.rows + static_cast<size_t>(slot) * kRowCap * 5;
+    int* row_key = scr.row_key + static_cast<size_t>(slot) * kRowCap * 2;
+    int* head = scr.phi_head + static_cast<size_t>(slot) * kPhiCap;
+    int* runs = scr.runs + static_cast<size_t>(slot) * kRunCap * 4;
+    for (;;) {
+        if (lane == 0) s_item = static_cast<int>(atomicAdd(&ep.counters[3], 1u));
+        wave_sync();
+        const int item = s_item;
+        wave_sync();
+        if (item >= static_cast<int>(ep.counters[2])) break;
+        const int64_t e = ep.ir_list[item];
+        const int c = ep.curve_of[e];
+        const CurveRec cv = curves[c];
+        const int bins = cache.bins;
+        Ctx C;
+        C.s = cv.s;
+        C.q = cv.q_eff;
+        C.L = make_lens(cv.s, cv.q_eff);
+        C.src = ep.src[e];
+        C.rho = cv.rho;
+        C.r2 = cv.rho * cv.rho;
+        C.inv_r2 = 1.0 / C.r2;
+        C.ld_c = cv.ld_c;
+        C.ld_d = cv.ld_d;
+        C.uniform = cv.ld_c == 0.0 && cv.ld_d == 0.0;
+        C.lane = lane;
+        C.seeds = seeds;
+        C.n_seeds = 0;
+        C.overflow = false;
+        const CausticScalars sc = cache.sc[c];
+        image_seeds(C, cache.crit + static_cast<size_t>(c) * kSeedPhases * 4,
+                    cache.assign + static_cast<size_t>(c) * kSeedPhases * 4,
+                    cache.br + static_cast<size_t>(c) * bins * 4,
+                    cache.br_cell + static_cast<size_t>(c) * bins * 4, sc, bins, tab,
+                    ep.roots + 10 * e);
+        double mag = __builtin_nan("");
+        const int method = ep.method[e];
+        if (!C.overflow) {
+            if (method == LCBG_METHOD_INVERSE_RAY_POLAR) {
+                const int pbins = opt.polar_source_bins > 0 ? opt.polar_source_bins : opt.source_bins;
+                const int psb = pbins > 1 ? pbins : 1;
+                double ratio = opt.polar_grid_ratio > 0.0 ? opt.polar_grid_ratio : opt.grid_ratio;
+                ratio = dmax(ratio, 1.0e-12);
+                if (opt.mode == 4) ratio = dmax(ratio, 12.0);
+                mag = polar_mag(C, psb, ratio, head, runs);
+            } else {
+                mag = cartesian_mag(C, opt.source_bins, rows, row_key, areaimage, flags);
+            }
+        }
+        if (lane == 0) {
+            ep.mag[e] = mag;
+            if (C.overflow) atomicOr(&ep.counters[4], kErrSeedCap);
+        }
+        wave_sync();
+    }
+}
+
+lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
+                         const FsOptions& opt, const FsEpochs& ep, const IrScratch& scr,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fs_ir, dim3(static_cast<unsigned>(scr.n_slots)), dim3(kWave), 0, st,
+                       d_curves, cache, tab, opt, ep, scr);
+    return check_launch();
+}
+
+}  // namespace lcbg

@@ -1,0 +1,34 @@
+// Host-side launchers of the finite-source kernels.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "fs_types.hpp"
+#include "runtime.hpp"
+
+namespace lcbg {
+
+lcbg_status launch_curve_prep(const CurveRec* d_curves, int n_curves, const FsCache& cache,
+                              const FsTables& tab, hipStream_t st);
+
+// Stage 1 (one workgroup per curve): trajectory, point-source solve, the
+// bounding-box exit; queues the remaining epochs for stage 2.
+lcbg_status launch_fs_stage1(const CurveRec* d_curves, int n_curves, const FsCache& cache,
+                             const FsOptions& opt, const FsEpochs& ep, hipStream_t st);
+
+// Stage 2 (persistent waves over the queue): caustic distances, the
+// point-source derivative test and the hexadecapole test; queues the epochs
+// that need inverse-ray shooting.
+lcbg_status launch_fs_stage2(const CurveRec* d_curves, const FsCache& cache, const FsOptions& opt,
+                             const FsEpochs& ep, int n_waves, hipStream_t st);
+
+// Stage 3 (persistent waves over the queue): image seeds, then the Cartesian
+// row scan or the polar flood fill.
+lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
+                         const FsOptions& opt, const FsEpochs& ep, const IrScratch& scr,
+                         hipStream_t st);
+
+// Zeroes the queue counters in-stream with a kernel (graph-capturable).
+lcbg_status launch_reset_counters(unsigned* counters, int n, hipStream_t st);
+
+}  // namespace lcbg

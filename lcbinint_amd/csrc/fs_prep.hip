@@ -1,0 +1,230 @@
+// Per-curve precomputation for the finite-source path: everything that depends
+// only on the lens (s, q) and not on the epoch.
+//
+//  * the caustic cache of ensure_binary_caustic_cache
+//    (finite_source_magnifier.cpp:4014-4105): critical-curve quartic at `bins`
+//    phases, caustic points through the complex lens equation, branches tracked
+//    by the minimum-cost permutation (append_tracked_caustic_points :569-611),
+//    bounding box, 128x128 grid cell of every point, longest non-wrap segment;
+//  * the critical sources of the 1400-phase seed scan (critical_sources_at_phase
+//    :1622-1641) and the greedy branch matching of seed Phase 2 (:1813-1840),
+//    which the reference recomputes for every inverse-ray epoch although they
+//    depend on (s, q) only.
+//
+// One 256-thread workgroup per curve; root solves are one thread each, the
+// sequential branch tracking runs on wave 0 with the 24 permutations across lanes.
+#include "fs_launch.hpp"
+#include "fs_math.hpp"
+#include "fs_wave.hpp"
+
+namespace lcbg {
+
+constexpr int kPrepBlock = 256;
+
+__constant__ unsigned char c_perm[24][4] = {
+    {0, 1, 2, 3}, {0, 1, 3, 2}, {0, 2, 1, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {0, 3, 2, 1},
+    {1, 0, 2, 3}, {1, 0, 3, 2}, {1, 2, 0, 3}, {1, 2, 3, 0}, {1, 3, 0, 2}, {1, 3, 2, 0},
+    {2, 0, 1, 3}, {2, 0, 3, 1}, {2, 1, 0, 3}, {2, 1, 3, 0}, {2, 3, 0, 1}, {2, 3, 1, 0},
+    {3, 0, 1, 2}, {3, 0, 2, 1}, {3, 1, 0, 2}, {3, 1, 2, 0}, {3, 2, 0, 1}, {3, 2, 1, 0}};
+
+__device__ __forceinline__ int grid_index(double v, double lo, double step)
+{
+    return iclamp(static_cast<int>((v - lo) / step), 0, kGrid - 1);
+}
+
+__global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __restrict__ curves,
+                                                           FsCache cache, FsTables tab)
+{
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const CurveRec cv = curves[c];
+    const int bins = cache.bins;
+    Pt* pts = cache.pts + static_cast<size_t>(c) * bins * 4;
+    Pt* br = cache.br + static_cast<size_t>(c) * bins * 4;
+    int* pt_cell = cache.pt_cell + static_cast<size_t>(c) * bins * 4;
+    int* br_cell = cache.br_cell + static_cast<size_t>(c) * bins * 4;
+    Pt* crit = cache.crit + static_cast<size_t>(c) * kSeedPhases * 4;
+    uint8_t* assign = cache.assign + static_cast<size_t>(c) * kSeedPhases * 4;
+    CausticScalars* sc = cache.sc + c;
+
+    __shared__ double red[4][kPrepBlock];
+
+    if (!(cv.rho > 0.0) || cv.s == 0.0 || !(cv.q_eff > 0.0)) {
+        if (tid == 0) sc->n = 0;
+        return;
+    }
+    const Lens L = make_lens(cv.s, cv.q_eff);
+
+    for (int i = tid; i < bins; i += kPrepBlock) {
+        const Pt ph = tab.caus_trig[i];
+        cx roots[4];
+        critical_roots(L, cx{ph.x, ph.y}, roots);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pts[4 * i + r] = lens_equation_complex(L, roots[r]);
+    }
+    for (int k = tid; k < kSeedPhases; k += kPrepBlock) {
+        const Pt ph = tab.seed_trig[k];
+        cx roots[4];
+        critical_roots(L, cx{ph.x, ph.y}, roots);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) crit[4 * k + r] = map_real(L, roots[r].re, roots[r].im);
+    }
+    __syncthreads();
+
+    // bounding box (finite_source_magnifier.cpp:4034-4041); min/max are order-free
+    double mnx = __builtin_inf(), mxx = -__builtin_inf(), mny = __builtin_inf(), mxy = -__builtin_inf();
+    for (int i = tid; i < bins * 4; i += kPrepBlock) {
+        const Pt p = pts[i];
+        mnx = dmin(mnx, p.x);
+        mxx = dmax(mxx, p.x);
+        mny = dmin(mny, p.y);
+        mxy = dmax(mxy, p.y);
+    }
+    red[0][tid] = mnx;
+    red[1][tid] = mxx;
+    red[2][tid] = mny;
+    red[3][tid] = mxy;
+    __syncthreads();
+    for (int off = kPrepBlock / 2; off >= 1; off >>= 1) {
+        if (tid < off) {
+            red[0][tid] = dmin(red[0][tid], red[0][tid + off]);
+            red[1][tid] = dmax(red[1][tid], red[1][tid + off]);
+            red[2][tid] = dmin(red[2][tid], red[2][tid + off]);
+            red[3][tid] = dmax(red[3][tid], red[3][tid + off]);
+        }
+        __syncthreads();
+    }
+    const double min_x = red[0][0], max_x = red[1][0], min_y = red[2][0], max_y = red[3][0];
+    const double width = dmax(max_x - min_x, 1.0e-12);
+    const double height = dmax(max_y - min_y, 1.0e-12);
+    const double step_x = width / static_cast<double>(kGrid);
+    const double step_y = height / static_cast<double>(kGrid);
+    for (int i = tid; i < bins * 4; i += kPrepBlock) {
+        const Pt p = pts[i];
+        pt_cell[i] = grid_index(p.y, min_y, step_y) * kGrid + grid_index(p.x, min_x, step_x);
+    }
+    __syncthreads();
+
+    // branch tracking on wave 0 (append_tracked_caustic_points :569-611)
+    if (tid < kWave) {
+        const int lane = tid;
+        Pt prev[4];
+        {
+            // first phase: order by atan2 (std::sort of 4 elements = insertion sort)
+            Pt p[4];
+            double key[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                p[r] = pts[r];
+                key[r] = atan2(p[r].y, p[r].x);
+            }
+            for (int i = 1; i < 4; ++i) {
+                const Pt vp = p[i];
+                const double vk = key[i];
+                int j = i;
+                while (j > 0 && vk < key[j - 1]) {
+                    p[j] = p[j - 1];
+                    key[j] = key[j - 1];
+                    --j;
+                }
+                p[j] = vp;
+                key[j] = vk;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) prev[r] = p[r];
+            if (lane == 0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) br[r * bins] = prev[r];
+            }
+        }
+        const int pl = lane < 24 ? lane : 0;
+        const int p0 = c_perm[pl][0], p1 = c_perm[pl][1], p2 = c_perm[pl][2], p3 = c_perm[pl][3];
+        for (int i = 1; i < bins; ++i) {
+            const Pt* cur = pts + 4 * i;
+            double cost = 0.0;
+            cost += dist2(prev[0], cur[p0]);
+            cost += dist2(prev[1], cur[p1]);
+            cost += dist2(prev[2], cur[p2]);
+            cost += dist2(prev[3], cur[p3]);
+            if (lane >= 24) cost = __builtin_inf();
+            int idx = lane;
+            wave_argmin(cost, idx);
+            // the reference keeps the identity when no cost is below +inf
+            if (!(cost < __builtin_inf())) idx = 0;
+            const int b0 = c_perm[idx][0], b1 = c_perm[idx][1], b2 = c_perm[idx][2], b3 = c_perm[idx][3];
+            prev[0] = cur[b0];
+            prev[1] = cur[b1];
+            prev[2] = cur[b2];
+            prev[3] = cur[b3];
+            if (lane == 0) {
+                br[0 * bins + i] = prev[0];
+                br[1 * bins + i] = prev[1];
+                br[2 * bins + i] = prev[2];
+                br[3 * bins + i] = prev[3];
+            }
+        }
+    }
+    __syncthreads();
+
+    // branch grid cells and the longest non-wrap segment (:4065-4095)
+    double seg2 = 0.0;
+    for (int k = tid; k < 4 * bins; k += kPrepBlock) {
+        const int b = k / bins;
+        const int j = k - b * bins;
+        const Pt pt = br[k];
+        if (j < bins - 1) seg2 = dmax(seg2, dist2(pt, br[k + 1]));
+        br_cell[k] = grid_index(pt.y, min_y, step_y) * kGrid + grid_index(pt.x, min_x, step_x);
+    }
+    red[0][tid] = seg2;
+    __syncthreads();
+    for (int off = kPrepBlock / 2; off >= 1; off >>= 1) {
+        if (tid < off) red[0][tid] = dmax(red[0][tid], red[0][tid + off]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        sc->min_x = min_x;
+        sc->max_x = max_x;
+        sc->min_y = min_y;
+        sc->max_y = max_y;
+        sc->step_x = step_x;
+        sc->step_y = step_y;
+        sc->max_seg_len = __builtin_sqrt(red[0][0]);
+        sc->n = bins;
+        // seed Phase-2 greedy nearest-neighbour matching (:1813-1840)
+        Pt prevp[4] = {{1.0e30, 1.0e30}, {1.0e30, 1.0e30}, {1.0e30, 1.0e30}, {1.0e30, 1.0e30}};
+        for (int k = 0; k < kSeedPhases; ++k) {
+            const Pt* cs = crit + 4 * k;
+            bool used[4] = {false, false, false, false};
+            int asg[4];
+            for (int bi = 0; bi < 4; ++bi) {
+                double best_d2 = 1.0e60;
+                int best_j = -1;
+                for (int j = 0; j < 4; ++j) {
+                    if (used[j]) continue;
+                    const double d2 = dist2(cs[j], prevp[bi]);
+                    if (d2 < best_d2) {
+                        best_d2 = d2;
+                        best_j = j;
+                    }
+                }
+                asg[bi] = best_j;
+                if (best_j >= 0) used[best_j] = true;
+            }
+            for (int bi = 0; bi < 4; ++bi) {
+                assign[4 * k + bi] = static_cast<uint8_t>(asg[bi] < 0 ? 255 : asg[bi]);
+                if (asg[bi] >= 0) prevp[bi] = cs[asg[bi]];
+            }
+        }
+    }
+}
+
+lcbg_status launch_curve_prep(const CurveRec* d_curves, int n_curves, const FsCache& cache,
+                              const FsTables& tab, hipStream_t st)
+{
+    if (n_curves <= 0) return LCBG_OK;
+    hipLaunchKernelGGL(k_curve_prep, dim3(static_cast<unsigned>(n_curves)), dim3(kPrepBlock), 0,
+                       st, d_curves, cache, tab);
+    return check_launch();
+}
+
+}  // namespace lcbg

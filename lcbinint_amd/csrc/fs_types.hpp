@@ -1,0 +1,104 @@
+// Device-resident data of the finite-source pipeline, shared by the host
+// runtime (which allocates it) and the kernels.
+//
+// Layout in HBM, per batch:
+//   per curve   - caustic cache (caustic_points_at_phase points, tracked
+//                 branches, grid cells, bounding box), the critical sources of
+//                 the 1400-phase seed scan and their branch assignment;
+//   per epoch   - source position, point-source magnification and roots,
+//                 output magnification and method;
+//   per slot    - scratch of one resident inverse-ray wave: seeds, Cartesian
+//                 row records, polar visited runs.
+#pragma once
+
+#include <cstdint>
+
+namespace lcbg {
+
+struct Pt2 {
+    double x, y;
+};
+
+constexpr int kSeedPhases = 1400;       // augmented_image_seeds samples (finite_source_magnifier.cpp:1742)
+constexpr int kGrid = 128;              // caustic_cache_grid_size_ (finite_source_magnifier.hpp:129)
+constexpr int kBoundaryProbes = 400;    // finite_source_magnifier.cpp:1577
+constexpr int kAngularProbes = 16;      // finite_source_magnifier.cpp:1538
+constexpr int kInteriorProbes = 64;     // finite_source_magnifier.cpp:1604
+
+// Capacities of the per-slot scratch; exceeding one marks the epoch unsupported.
+constexpr int kSeedCap = 1024;
+constexpr int kRowCap = 16384;
+constexpr int kPhiCap = 1 << 19;
+constexpr int kRunCap = 1 << 17;
+
+struct CausticScalars {
+    double min_x, max_x, min_y, max_y, step_x, step_y, max_seg_len;
+    int n;  // points per branch (0: cache empty)
+    int pad;
+};
+
+struct FsTables {
+    const Pt2* caus_trig;     // bins: cos, sin of 2 pi i / bins
+    const Pt2* seed_trig;     // 1400: cos, sin of (2 pi / 1400) i
+    const Pt2* boundary_trig; // 400: cos, sin of 2 pi i / 400
+    const Pt2* angular_trig;  // 16
+    const Pt2* interior_trig; // 64
+};
+
+struct FsCache {
+    int bins;
+    Pt2* pts;          // [curve][bins*4]   caustic points in root order
+    Pt2* br;           // [curve][4*bins]   tracked branches
+    int* pt_cell;      // [curve][bins*4]   grid cell (iy*128+ix) of each point
+    int* br_cell;      // [curve][4*bins]
+    Pt2* crit;         // [curve][1400*4]   critical sources of the seed scan
+    uint8_t* assign;   // [curve][1400*4]   Phase-2 branch -> root index
+    CausticScalars* sc;// [curve]
+};
+
+struct FsOptions {
+    int source_bins;
+    int caustic_bins;
+    int mode;
+    int polar_source_bins;
+    double grid_ratio;
+    double polar_grid_ratio;
+    double kinji;
+    double hex_threshold;
+    double adaptive_hex;
+};
+
+struct FsEpochs {
+    int64_t n;
+    const int64_t* offsets;
+    const double* times;
+    double* mag;
+    int32_t* method;
+    int32_t* curve_of;  // [epoch]
+    Pt2* src;           // [epoch]
+    double* a_ps;       // [epoch]
+    double* roots;      // [epoch][10]
+    double* dist;       // [epoch] refined caustic distance (IR epochs)
+    int64_t* near_list;
+    int64_t* ir_list;
+    unsigned* counters; // [0] near count, [1] near cursor, [2] ir count, [3] ir cursor, [4] error flags
+};
+
+struct IrScratch {
+    int n_slots;
+    Pt2* seeds;        // [slot][kSeedCap]
+    double* areaimage; // [slot][kSeedCap]
+    int* overlap;      // [slot][kSeedCap]
+    double* rows;      // [slot][kRowCap*5]: xmin, xmax, ax, y, dys
+    int* row_key;      // [slot][kRowCap]
+    int* phi_head;     // [slot][kPhiCap]
+    int* runs;         // [slot][kRunCap*4]: iphi, left, right, next
+};
+
+// error flags in counters[4]
+constexpr unsigned kErrSeedCap = 1u;
+constexpr unsigned kErrRowCap = 2u;
+constexpr unsigned kErrPolarCap = 4u;
+constexpr unsigned kErrUnsupported = 8u;
+
+}  // namespace lcbg

@@ -1,0 +1,334 @@
+// pybind11 drop-in for the reference Python API on the finite-source hot path:
+//
+//   lcbinint.LightCurve(options=Options(coordinates=..., nbin=...),
+//                       limb_darkening=LimbDarkening.linear(u))(times, params)
+//
+// Same class names, keyword arguments, defaults, aliases and error types as
+// python/lcbinint_pybind.cpp (LimbDarkening :2828-2847, Options :2985-3112,
+// LightCurve :3114-3181, dict keys :145-179).  The light curve is evaluated by
+// the HIP library through lcbg_light_curve (include/lcbinint_amd.h) with the GIL
+// released; the static binary lens in VBM coordinates is the supported model,
+// other models raise as unsupported.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lcbinint_amd.h"
+
+namespace py = pybind11;
+
+namespace {
+
+constexpr double kNaN = std::numeric_limits<double>::quiet_NaN();
+
+// lcbi_orbital_motion_mode (include/lcbinint/lcbinint.h:15-19)
+enum OrbitalMotionMode { STATIC = 0, CIRCULAR = 1, KEPLER = 2 };
+
+struct PyLimbDarkening {
+    double c = 0.0;
+    double d = 0.0;
+};
+
+// The public option set (lcbi_options of the reference, python defaults of
+// public_default_options, lcbinint_pybind.cpp:213-220).
+struct PyOptions {
+    lcbg_options o = lcbg_default_options();
+    int vbm_compatible = 1;
+    int center_of_mass = 0;
+    int adaptive_source_bins = 0;
+    int max_source_bins = 400;
+    double finite_source_tol = 0.0;
+    double finite_source_reltol = 0.0;
+    std::string xallarap = "none";
+};
+
+void apply_param_type(PyOptions& p, const std::string& t)
+{
+    if (t == "auto" || t.empty() || t == "vbm" || t == "vbbl" || t == "standard") {
+        p.center_of_mass = 0;
+        p.vbm_compatible = 1;
+    } else if (t == "lcbinint" || t == "original" || t == "legacy") {
+        p.center_of_mass = 0;
+        p.vbm_compatible = 0;
+    } else if (t == "center_of_mass") {
+        p.center_of_mass = 1;
+        p.vbm_compatible = 0;
+    } else if (t == "vbm_center_of_mass") {
+        p.center_of_mass = 1;
+        p.vbm_compatible = 1;
+    } else {
+        throw std::invalid_argument(
+            "param_type must be 'vbm', 'lcbinint', 'original', or 'center_of_mass'");
+    }
+}
+
+std::string coordinate_name(const PyOptions& p)
+{
+    if (p.vbm_compatible != 0 && p.center_of_mass == 0) return "vbm";
+    if (p.center_of_mass != 0 && p.vbm_compatible != 0) return "vbm_center_of_mass";
+    if (p.center_of_mass != 0) return "center_of_mass";
+    return "lcbinint";
+}
+
+void apply_grid(PyOptions& p, const std::string& g)
+{
+    if (g == "auto" || g.empty()) {
+        p.o.mode = 4;
+    } else if (g == "cartesian" || g == "inverse_ray_cartesian") {
+        p.o.mode = 1;
+    } else if (g == "polar" || g == "inverse_ray_polar") {
+        p.o.mode = 2;
+    } else {
+        throw std::invalid_argument("inverse_ray_grid must be 'auto', 'cartesian', or 'polar'");
+    }
+}
+
+std::string grid_name(const PyOptions& p)
+{
+    if (p.o.mode == 4) return "auto";
+    if (p.o.mode == 1) return "cartesian";
+    if (p.o.mode == 2) return "polar";
+    return "custom";
+}
+
+int int_or(const py::object& v, int d) { return v.is_none() ? d : py::cast<int>(v); }
+double double_or(const py::object& v, double d) { return v.is_none() ? d : py::cast<double>(v); }
+py::object positive_int(int v) { return v <= 0 ? py::object(py::none()) : py::object(py::int_(v)); }
+py::object positive_double(double v)
+{
+    return v <= 0.0 ? py::object(py::none()) : py::object(py::float_(v));
+}
+
+double dict_get(const py::dict& d, const char* k, double dflt)
+{
+    const py::str key(k);
+    return d.contains(key) ? py::cast<double>(d[key]) : dflt;
+}
+
+void raise_status(lcbg_status st)
+{
+    switch (st) {
+    case LCBG_OK: return;
+    case LCBG_NUMERICAL_ERROR: throw std::runtime_error("numerical error");
+    case LCBG_UNSUPPORTED: throw std::runtime_error("unsupported");
+    case LCBG_INVALID_ARGUMENT: throw std::invalid_argument("invalid argument");
+    default: throw std::runtime_error(lcbg_status_string(st));
+    }
+}
+
+class PyLightCurve {
+public:
+    PyLightCurve(std::string lens, std::string source, const PyOptions& options,
+                 const PyLimbDarkening& ld, int orbital_motion_mode, bool parallax,
+                 bool terrestrial_parallax)
+        : lens_(std::move(lens)), source_(std::move(source)), options_(options), ld_(ld),
+          orbit_(orbital_motion_mode), parallax_(parallax), terrestrial_(terrestrial_parallax)
+    {
+        if (lens_ != "binary_lens" && lens_ != "triple_lens")
+            throw std::invalid_argument("lens must be 'binary_lens' or 'triple_lens'");
+        if (source_ != "single" && source_ != "binary")
+            throw std::invalid_argument("source must be 'single' or 'binary'");
+    }
+
+    py::array_t<double> evaluate(py::array_t<double, py::array::c_style | py::array::forcecast> times,
+                                 double t0, double tE, double u0, double alpha, double s, double q,
+                                 double rho, double piEN, double piEE, double g1, double g2,
+                                 double g3) const
+    {
+        const py::buffer_info in = times.request();
+        if (in.ndim != 1) throw std::invalid_argument("times must be one-dimensional");
+        const auto n = static_cast<py::ssize_t>(in.shape[0]);
+        std::vector<double> t(static_cast<const double*>(in.ptr), static_cast<const double*>(in.ptr) + n);
+        auto out = py::array_t<double>({n}, {static_cast<py::ssize_t>(sizeof(double))});
+        // models beyond the static binary lens in VBM coordinates are outside this library
+        const bool supported = lens_ == "binary_lens" && source_ == "single" && orbit_ == 0 &&
+                               (!parallax_ || (piEN == 0.0 && piEE == 0.0)) && !terrestrial_ &&
+                               g1 == 0.0 && g2 == 0.0 && g3 == 0.0 && options_.vbm_compatible != 0 &&
+                               options_.center_of_mass == 0 && options_.xallarap == "none" &&
+                               (options_.adaptive_source_bins == 0 ||
+                                (options_.finite_source_tol <= 0.0 && options_.finite_source_reltol <= 0.0));
+        if (!supported) throw std::runtime_error("unsupported");
+        if (!std::isfinite(tE) || tE == 0.0 || !std::isfinite(q) || !std::isfinite(s))
+            throw std::runtime_error("numerical error");
+        lcbg_curve cv{t0, tE, u0, alpha, s, q, rho, ld_.c, ld_.d};
+        lcbg_status st;
+        {
+            py::gil_scoped_release release;
+            st = lcbg_light_curve(&cv, &options_.o, t.data(), static_cast<int64_t>(n),
+                                  out.mutable_data(), nullptr);
+        }
+        raise_status(st);
+        return out;
+    }
+
+    py::array_t<double> from_dict(py::array_t<double, py::array::c_style | py::array::forcecast> times,
+                                  const py::dict& p) const
+    {
+        return evaluate(times, dict_get(p, "t0", 0.0), dict_get(p, "tE", 1.0), dict_get(p, "u0", 0.0),
+                        dict_get(p, "alpha", 0.0), dict_get(p, "s", 1.0), dict_get(p, "q", 1.0),
+                        dict_get(p, "rho", 0.0), parallax_ ? dict_get(p, "piEN", 0.0) : 0.0,
+                        parallax_ ? dict_get(p, "piEE", 0.0) : 0.0, dict_get(p, "g1", 0.0),
+                        dict_get(p, "g2", 0.0), dict_get(p, "g3", 0.0));
+    }
+
+    const std::string& lens() const { return lens_; }
+    const std::string& source() const { return source_; }
+    const PyOptions& options() const { return options_; }
+    const PyLimbDarkening& limb_darkening() const { return ld_; }
+    bool parallax() const { return parallax_; }
+    bool terrestrial_parallax() const { return terrestrial_; }
+    int orbital_motion_mode() const { return orbit_; }
+
+private:
+    std::string lens_, source_;
+    PyOptions options_;
+    PyLimbDarkening ld_;
+    int orbit_;
+    bool parallax_, terrestrial_;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_lcbinint, m)
+{
+    m.doc() = "MI355X (gfx950) binary-lens light curves behind the lcbinint Python API";
+
+    py::enum_<lcbg_method>(m, "FiniteSourceMethod")
+        .value("point_source", LCBG_METHOD_POINT_SOURCE)
+        .value("hexadecapole", LCBG_METHOD_HEXADECAPOLE)
+        .value("inverse_ray_cartesian", LCBG_METHOD_INVERSE_RAY_CARTESIAN)
+        .value("inverse_ray_polar", LCBG_METHOD_INVERSE_RAY_POLAR);
+
+    py::enum_<OrbitalMotionMode>(m, "OrbitalMotionMode")
+        .value("STATIC", STATIC)
+        .value("CIRCULAR", CIRCULAR)
+        .value("KEPLER", KEPLER)
+        .export_values();
+
+    py::class_<PyLimbDarkening>(m, "LimbDarkening")
+        .def("__repr__", [](const PyLimbDarkening& ld) {
+            return "LimbDarkening(c=" + std::to_string(ld.c) + ", d=" + std::to_string(ld.d) + ")";
+        })
+        .def(py::init([](double c, double d) { return PyLimbDarkening{c, d}; }), py::arg("c") = 0.0,
+             py::arg("d") = 0.0)
+        .def_static("none", []() { return PyLimbDarkening{}; })
+        .def_static("linear", [](double c) { return PyLimbDarkening{c, 0.0}; }, py::arg("c"))
+        .def_static("quadratic", [](double c, double d) { return PyLimbDarkening{c, d}; }, py::arg("c"),
+                    py::arg("d"))
+        .def_readwrite("c", &PyLimbDarkening::c)
+        .def_readwrite("d", &PyLimbDarkening::d);
+
+    py::class_<PyOptions>(m, "Options")
+        .def(py::init([](int source_bins, const py::object& nbin, const std::string& inverse_ray_grid,
+                         int caustic_bins, double grid_ratio, const py::object& polar_nbin,
+                         const py::object& polar_source_bins, const py::object& polar_grid_ratio,
+                         double point_source_threshold, double hexadecapole_threshold,
+                         double adaptive_hex_threshold, int adaptive_source_bins, int max_source_bins,
+                         double finite_source_tol, double finite_source_reltol, double tol,
+                         double reltol, const std::string& coordinates, double hex_tol,
+                         const std::string& param_type, const std::string& xallarap_param_type) {
+                 PyOptions p;
+                 if (!param_type.empty() && param_type != "auto")
+                     apply_param_type(p, param_type);
+                 else
+                     apply_param_type(p, coordinates);
+                 if (xallarap_param_type != "none" && xallarap_param_type != "angular_velocity" &&
+                     xallarap_param_type != "orbital_elements" && !xallarap_param_type.empty())
+                     throw std::invalid_argument(
+                         "xallarap_param_type must be 'none', 'angular_velocity', or 'orbital_elements'");
+                 p.xallarap = xallarap_param_type.empty() ? "none" : xallarap_param_type;
+                 apply_grid(p, inverse_ray_grid);
+                 p.o.source_bins = int_or(nbin, source_bins);
+                 p.o.caustic_bins = caustic_bins;
+                 p.o.grid_ratio = grid_ratio;
+                 p.o.polar_source_bins = int_or(polar_nbin, int_or(polar_source_bins, 0));
+                 p.o.polar_grid_ratio = double_or(polar_grid_ratio, 0.0);
+                 p.o.point_source_threshold = point_source_threshold;
+                 p.o.hexadecapole_threshold = hexadecapole_threshold;
+                 p.o.adaptive_hex_threshold = std::isnan(hex_tol) ? adaptive_hex_threshold : hex_tol;
+                 p.max_source_bins = max_source_bins;
+                 p.finite_source_tol = std::isnan(tol) ? finite_source_tol : tol;
+                 p.finite_source_reltol = std::isnan(reltol) ? finite_source_reltol : reltol;
+                 p.adaptive_source_bins = adaptive_source_bins < 0 ? 0 : adaptive_source_bins;
+                 return p;
+             }),
+             py::arg("source_bins") = 50, py::arg("nbin") = py::none(),
+             py::arg("inverse_ray_grid") = "auto", py::arg("caustic_bins") = 1400,
+             py::arg("grid_ratio") = 4.0, py::arg("polar_nbin") = py::none(),
+             py::arg("polar_source_bins") = py::none(), py::arg("polar_grid_ratio") = py::none(),
+             py::arg("point_source_threshold") = 20.0, py::arg("hexadecapole_threshold") = 3.0,
+             py::arg("adaptive_hex_threshold") = 0.001, py::arg("adaptive_source_bins") = -1,
+             py::arg("max_source_bins") = 400, py::arg("finite_source_tol") = 0.0,
+             py::arg("finite_source_reltol") = 0.0, py::arg("tol") = kNaN, py::arg("reltol") = kNaN,
+             py::arg("coordinates") = "auto", py::arg("hex_tol") = kNaN,
+             py::arg("param_type") = "auto", py::arg("xallarap_param_type") = "none")
+        .def_property("source_bins", [](const PyOptions& p) { return p.o.source_bins; },
+                      [](PyOptions& p, int v) { p.o.source_bins = v; })
+        .def_property("nbin", [](const PyOptions& p) { return p.o.source_bins; },
+                      [](PyOptions& p, int v) { p.o.source_bins = v; })
+        .def_property("inverse_ray_grid", &grid_name, &apply_grid)
+        .def_property_readonly("_mode", [](const PyOptions& p) { return p.o.mode; })
+        .def_property("caustic_bins", [](const PyOptions& p) { return p.o.caustic_bins; },
+                      [](PyOptions& p, int v) { p.o.caustic_bins = v; })
+        .def_property("grid_ratio", [](const PyOptions& p) { return p.o.grid_ratio; },
+                      [](PyOptions& p, double v) { p.o.grid_ratio = v; })
+        .def_property("polar_nbin", [](const PyOptions& p) { return positive_int(p.o.polar_source_bins); },
+                      [](PyOptions& p, const py::object& v) { p.o.polar_source_bins = int_or(v, 0); })
+        .def_property("polar_source_bins",
+                      [](const PyOptions& p) { return positive_int(p.o.polar_source_bins); },
+                      [](PyOptions& p, const py::object& v) { p.o.polar_source_bins = int_or(v, 0); })
+        .def_property("polar_grid_ratio",
+                      [](const PyOptions& p) { return positive_double(p.o.polar_grid_ratio); },
+                      [](PyOptions& p, const py::object& v) { p.o.polar_grid_ratio = double_or(v, 0.0); })
+        .def_property("point_source_threshold", [](const PyOptions& p) { return p.o.point_source_threshold; },
+                      [](PyOptions& p, double v) { p.o.point_source_threshold = v; })
+        .def_property("hexadecapole_threshold", [](const PyOptions& p) { return p.o.hexadecapole_threshold; },
+                      [](PyOptions& p, double v) { p.o.hexadecapole_threshold = v; })
+        .def_property("adaptive_hex_threshold", [](const PyOptions& p) { return p.o.adaptive_hex_threshold; },
+                      [](PyOptions& p, double v) { p.o.adaptive_hex_threshold = v; })
+        .def_property("hex_tol", [](const PyOptions& p) { return p.o.adaptive_hex_threshold; },
+                      [](PyOptions& p, double v) { p.o.adaptive_hex_threshold = v; })
+        .def_readwrite("adaptive_source_bins", &PyOptions::adaptive_source_bins)
+        .def_readwrite("max_source_bins", &PyOptions::max_source_bins)
+        .def_readwrite("finite_source_tol", &PyOptions::finite_source_tol)
+        .def_readwrite("finite_source_reltol", &PyOptions::finite_source_reltol)
+        .def_property("tol", [](const PyOptions& p) { return p.finite_source_tol; },
+                      [](PyOptions& p, double v) { p.finite_source_tol = v; })
+        .def_property("reltol", [](const PyOptions& p) { return p.finite_source_reltol; },
+                      [](PyOptions& p, double v) { p.finite_source_reltol = v; })
+        .def_property("coordinates", &coordinate_name, &apply_param_type)
+        .def_property("param_type", &coordinate_name, &apply_param_type)
+        .def_property_readonly("xallarap_param_type", [](const PyOptions& p) { return p.xallarap; });
+
+    py::class_<PyLightCurve>(m, "LightCurve")
+        .def(py::init([](const std::string& lens, const std::string& source, const py::object& event,
+                         const PyOptions& options, const PyLimbDarkening& ld, OrbitalMotionMode orbital_motion_mode,
+                         bool parallax, bool terrestrial_parallax) {
+                 (void)event;
+                 return PyLightCurve(lens, source, options, ld, static_cast<int>(orbital_motion_mode), parallax,
+                                     terrestrial_parallax);
+             }),
+             py::kw_only(), py::arg("lens") = "binary_lens", py::arg("source") = "single",
+             py::arg("event") = py::none(), py::arg("options") = PyOptions{},
+             py::arg("limb_darkening") = PyLimbDarkening{}, py::arg("orbital_motion_mode") = STATIC,
+             py::arg("parallax") = false, py::arg("terrestrial_parallax") = false)
+        .def_property_readonly("lens", &PyLightCurve::lens)
+        .def_property_readonly("source", &PyLightCurve::source)
+        .def_property_readonly("options", &PyLightCurve::options)
+        .def_property_readonly("limb_darkening", &PyLightCurve::limb_darkening)
+        .def_property_readonly("orbital_motion_mode", &PyLightCurve::orbital_motion_mode)
+        .def_property_readonly("parallax", &PyLightCurve::parallax)
+        .def_property_readonly("terrestrial_parallax", &PyLightCurve::terrestrial_parallax)
+        .def("__call__", &PyLightCurve::from_dict, py::arg("times"), py::arg("params"))
+        .def("__call__", &PyLightCurve::evaluate, py::arg("times"), py::kw_only(),
+             py::arg("t0") = 0.0, py::arg("tE") = 1.0, py::arg("u0") = 0.0, py::arg("alpha") = 0.0,
+             py::arg("s") = 1.0, py::arg("q") = 1.0, py::arg("rho") = 0.0, py::arg("piEN") = 0.0,
+             py::arg("piEE") = 0.0, py::arg("g1") = 0.0, py::arg("g2") = 0.0, py::arg("g3") = 0.0)
+        .def("light_curve", &PyLightCurve::from_dict, py::arg("times"), py::arg("params"));
+}

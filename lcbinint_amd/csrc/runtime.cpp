@@ -133,6 +133,7 @@ extern "C" lcbg_status lcbg_batch_create(const lcbg_curve* curves, int32_t n_cur
 extern "C" lcbg_status lcbg_batch_destroy(lcbg_batch* batch)
 {
     if (!batch) return LCBG_OK;
+    if (batch->fs) fs_destroy(batch->fs);
     if (batch->d_curves) (void)hipFree(batch->d_curves);
     delete batch;
     return LCBG_OK;

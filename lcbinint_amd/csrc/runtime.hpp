@@ -36,6 +36,9 @@ private:
     hipStream_t stream_;
 };
 
+struct FsState;
+void fs_destroy(FsState* fs);
+
 }  // namespace lcbg
 
 struct lcbg_batch {
@@ -43,4 +46,5 @@ struct lcbg_batch {
     int32_t n_curves = 0;
     lcbg_options options{};
     lcbg::CurveRec* d_curves = nullptr;  // n_curves records
+    lcbg::FsState* fs = nullptr;         // finite-source state, built on first use
 };

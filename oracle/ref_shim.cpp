@@ -14,6 +14,11 @@
 #include "lcbinint/model/orbital_motion.hpp"
 #include "SkowronGould.h"
 
+// The reference's finite-source translation unit, compiled here from where it
+// lies (the include path points into the reference tree) so that its internal
+// stage functions can be exposed below.
+#include "lcbinint/magnification/finite_source_magnifier.cpp"
+
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -239,6 +244,105 @@ void ref_orbital_state(const lcbi_params* params, double time, double* out)
     out[0] = st.separation;
     out[1] = st.angle;
     out[2] = st.line_of_sight_separation;
+}
+
+
+// ---------------------------------------------------------------- stages
+// Stage accessors over functions internal to finite_source_magnifier.cpp.
+namespace fsm = lcbinint::magnification;
+
+// caustic_points_at_phase (finite_source_magnifier.cpp:547-567): 4 source-plane
+// points (x, y interleaved) of the critical curve at one phase.
+int ref_caustic_points_at_phase(double s, double q, double phase, double* out)
+{
+    PointSourceMagnifier pm;
+    const auto pts = fsm::caustic_points_at_phase(pm, s, q, phase);
+    for (std::size_t i = 0; i < pts.size(); ++i) {
+        out[2 * i] = pts[i].x;
+        out[2 * i + 1] = pts[i].y;
+    }
+    return static_cast<int>(pts.size());
+}
+
+// critical_sources_at_phase (finite_source_magnifier.cpp:1622-1641).
+int ref_critical_sources_at_phase(double s, double q, double phase, double* out)
+{
+    const auto mapper = fsm::make_binary_lens_mapper(s, q);
+    const auto pts = fsm::critical_sources_at_phase(mapper, s, q, phase);
+    for (std::size_t i = 0; i < pts.size(); ++i) {
+        out[2 * i] = pts[i].x;
+        out[2 * i + 1] = pts[i].y;
+    }
+    return static_cast<int>(pts.size());
+}
+
+// augmented_image_seeds (finite_source_magnifier.cpp:1709-1871) followed by
+// augment_seeds_from_caustic_branches (:4845-4926), as fixed_inverse_ray_binary
+// runs them (:3055-3063).  Writes up to `cap` seeds; returns the seed count.
+int ref_image_seeds(const RefSettings* rs, double s, double q, double x, double y, double rho,
+                    double* out, int cap)
+{
+    PointSourceMagnifier pm;
+    FiniteSourceMagnifier fm(to_settings(*rs));
+    const auto mapper = fsm::make_binary_lens_mapper(s, q);
+    auto seeds = fsm::augmented_image_seeds(pm, mapper, s, q, {x, y}, rho);
+    fm.augment_seeds_from_caustic_branches(s, q, {x, y}, rho, seeds);
+    for (std::size_t i = 0; i < seeds.size() && static_cast<int>(i) < cap; ++i) {
+        out[2 * i] = seeds[i].x;
+        out[2 * i + 1] = seeds[i].y;
+    }
+    return static_cast<int>(seeds.size());
+}
+
+// inverse_ray_cartesian_binary_mag (finite_source_magnifier.cpp:2470-2768) on a
+// given seed list; diagnostics as (seed_count, processed, fold, rows, gaps,
+// overlaps, max_jump, estimated_error).
+double ref_cartesian_area(const RefSettings* rs, double s, double q, double x, double y,
+                          double rho, const double* seeds_xy, int n_seeds, double* diag)
+{
+    PointSourceMagnifier pm;
+    FiniteSourceMagnifier fm(to_settings(*rs));
+    std::vector<SourcePosition> seeds(static_cast<std::size_t>(n_seeds));
+    for (int i = 0; i < n_seeds; ++i) seeds[i] = {seeds_xy[2 * i], seeds_xy[2 * i + 1]};
+    fsm::LegacyAreaDiagnostics d;
+    const double a = fsm::inverse_ray_cartesian_binary_mag(pm, s, q, {x, y}, rho, fm.settings(),
+                                                           &fm, &seeds, &d, nullptr);
+    if (diag) {
+        diag[0] = d.seed_count;
+        diag[1] = d.processed_images;
+        diag[2] = d.fold_seed_count;
+        diag[3] = d.boundary_rows;
+        diag[4] = d.gap_repairs;
+        diag[5] = d.overlaps;
+        diag[6] = d.max_jump_cells;
+        diag[7] = d.estimated_error;
+    }
+    return a;
+}
+
+// inverse_ray_polar_boundary_binary (finite_source_magnifier.cpp:1156-1319) on a
+// given seed list, with the settings as passed (callers raise polar_grid_ratio
+// to 12 for the automatic mode as binary_mag does, :4813-4821).
+double ref_polar_area(const RefSettings* rs, double s, double q, double x, double y, double rho,
+                      const double* seeds_xy, int n_seeds)
+{
+    PointSourceMagnifier pm;
+    FiniteSourceMagnifier fm(to_settings(*rs));
+    std::vector<SourcePosition> seeds(static_cast<std::size_t>(n_seeds));
+    for (int i = 0; i < n_seeds; ++i) seeds[i] = {seeds_xy[2 * i], seeds_xy[2 * i + 1]};
+    return fsm::inverse_ray_polar_boundary_binary(pm, s, q, {x, y}, rho, fm.settings(), &fm,
+                                                  nullptr, &seeds);
+}
+
+// hexadecapole_binary (finite_source_magnifier.cpp:792-847) with known A0.
+double ref_hexadecapole(const RefSettings* rs, double s, double q, double x, double y,
+                        double rho, double a0, double* rel_err)
+{
+    PointSourceMagnifier pm;
+    const auto settings = to_settings(*rs);
+    const auto h = fsm::hexadecapole_binary(pm, s, q, {x, y}, rho, settings, &a0);
+    if (rel_err) *rel_err = h.relative_error;
+    return h.magnification;
 }
 
 int ref_sizeof_params(void) { return static_cast<int>(sizeof(lcbi_params)); }

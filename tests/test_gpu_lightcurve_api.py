@@ -1,0 +1,40 @@
+"""The Python drop-in (LightCurve / Options / LimbDarkening) on the GPU,
+against the reference compiled from source, through the dict and keyword paths."""
+import numpy as np
+import pytest
+
+from oracle.bindings import default_settings
+
+pytestmark = pytest.mark.gpu
+
+
+def test_readme_light_curve(gpu, ref):
+    import lcbinint_amd as lcbinint
+    t = np.linspace(-0.5, 0.5, 200)
+    params = {"t0": 0.0, "tE": 1.0, "u0": -0.01, "alpha": 0.5, "s": 0.95, "q": 1e-2, "rho": 5e-3}
+    lc = lcbinint.LightCurve(options=lcbinint.Options(coordinates="vbm", nbin=50),
+                             limb_darkening=lcbinint.LimbDarkening.linear(0.5))
+    got = lc(t, params)
+    assert got.dtype == np.float64 and got.shape == (200,)
+    exp, meth, st = ref.light_curve(default_settings(limb_darkening_c=0.5), t, 0.0, 1.0, -0.01, 0.5,
+                                    0.95, 1e-2, 5e-3)
+    ps = meth <= 1
+    assert np.array_equal(got[ps], exp[ps])
+    assert np.max(np.abs(got - exp) / exp) <= 1e-9
+    kw = lc(t, t0=0.0, tE=1.0, u0=-0.01, alpha=0.5, s=0.95, q=1e-2, rho=5e-3)
+    assert np.array_equal(kw, got)
+
+
+def test_point_source_curve_and_errors(gpu, ref):
+    import lcbinint_amd as lcbinint
+    t = np.linspace(-1.0, 1.0, 101)
+    lc = lcbinint.LightCurve()
+    got = lc(t, {"u0": 0.1, "alpha": 1.0, "s": 1.2, "q": 0.3})
+    exp, _, _ = ref.light_curve(default_settings(), t, 0.0, 1.0, 0.1, 1.0, 1.2, 0.3, 0.0)
+    assert np.array_equal(got, exp)
+    with pytest.raises(ValueError):
+        lc(np.zeros((3, 3)), {})
+    with pytest.raises(RuntimeError, match="unsupported"):
+        lcbinint.LightCurve(orbital_motion_mode=lcbinint.OrbitalMotionMode.CIRCULAR)(t, {"g1": 0.1})
+    with pytest.raises(RuntimeError, match="numerical error"):
+        lc(t, {"tE": 0.0})
