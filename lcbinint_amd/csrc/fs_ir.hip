@@ -163,7 +163,7 @@ __device__ void caustic_probes(Ctx& C, Pt cs, const FsTables& tab)
 {
     const double dx = cs.x - C.src.x;
     const double dy = cs.y - C.src.y;
-    const double distance = hypot_cr(dx, dy);
+    const double distance = hypot_glibc(dx, dy);
     if (distance <= 0.0 || distance >= C.rho) return;
     const double ux = dx / distance;
     const double uy = dy / distance;
@@ -968,7 +968,7 @@ __device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* hea
     double max_r = 1.0;
     for (int i = 0; i < n_img; ++i) {
         const Pt s = C.seeds[i];
-        max_r = dmax(max_r, hypot_cr(s.x, s.y) + 4.0 * C.rho);
+        max_r = dmax(max_r, hypot_glibc(s.x, s.y) + 4.0 * C.rho);
     }
     const int pb = static_cast<int>(ceil(2.0 * kPi * max_r / (P.dr * grid_ratio)));
     P.phi_bins = pb > 16 ? pb : 16;
@@ -988,7 +988,7 @@ __device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* hea
     // the reference's shell / direction / radial order, evaluated in parallel
     for (int i = 0; i < n_img && !C.overflow; ++i) {
         const Pt seed = C.seeds[i];
-        const double seed_radius = hypot_cr(seed.x, seed.y);
+        const double seed_radius = hypot_glibc(seed.x, seed.y);
         const double seed_phi = wrap_angle(atan2(seed.y, seed.x));
         const int seed_ir = static_cast<int>(floor(seed_radius / P.dr));
         const int seed_iphi = iclamp(static_cast<int>(seed_phi / P.dphi), 0, P.phi_bins - 1);

@@ -10,7 +10,7 @@
 //     :65-80 and :278-291 divide, :121-153 multiplies by reciprocals, :196-209
 //     divides by squared denominators);
 //   - limb-darkening nodes (finite_source_magnifier.cpp:369-375, 3979-4012);
-//   - a correctly rounded hypot (glibc's hypot is correctly rounded).
+//   - glibc's hypot algorithm.
 #pragma once
 
 #include "binary_lens.hpp"
@@ -209,62 +209,50 @@ __device__ __forceinline__ double source_flux(double rho, double c, double d)
     return kPi * rho * rho * ff;
 }
 
-// Correctly rounded hypot(x, y).  The sum of squares is formed exactly as a
-// double-double, its square root rounded, and the result corrected against the
-// exact midpoint test, so the value equals glibc's (correctly rounded) hypot.
-__device__ __forceinline__ double hypot_cr(double x, double y)
+// hypot(x, y) as glibc computes it (the reference calls std::hypot): the
+// square root of the sum of squares corrected by one Newton-type step, the
+// non-FMA kernel of Borges, "An Improved Algorithm for hypot(a,b)" (2019),
+// with the scaling thresholds of glibc's implementation.  Bitwise equal to the
+// host libm on 2e7 random pairs, including nearly equal arguments where a
+// correctly rounded hypot would differ.
+__device__ __forceinline__ double hypot_kernel(double ax, double ay)
 {
+    double t1, t2;
+    double h = __builtin_sqrt(ax * ax + ay * ay);
+    if (h <= 2.0 * ay) {
+        const double delta = h - ay;
+        t1 = ax * (2.0 * delta - ax);
+        t2 = (delta - 2.0 * (ax - ay)) * delta;
+    } else {
+        const double delta = h - ax;
+        t1 = 2.0 * delta * (ax - 2.0 * ay);
+        t2 = (4.0 * delta - ay) * ay + delta * delta;
+    }
+    h -= (t1 + t2) / (2.0 * h);
+    return h;
+}
+
+__device__ __forceinline__ double hypot_glibc(double x, double y)
+{
+    if (!isfinite(x) || !isfinite(y)) {
+        if (isinf(x) || isinf(y)) return __builtin_inf();
+        return x + y;
+    }
     x = fabs(x);
     y = fabs(y);
-    if (x < y) {
-        const double t = x;
-        x = y;
-        y = t;
+    const double ax = x < y ? y : x;
+    const double ay = x < y ? x : y;
+    const double kScale = 0x1p-600, kLarge = 0x1p+511, kTiny = 0x1p-511, kEps = 0x1p-54;
+    if (ax > kLarge) {
+        if (ay <= ax * kEps) return ax + ay;
+        return hypot_kernel(ax * kScale, ay * kScale) / kScale;
     }
-    if (!(x <= 0x1.fffffffffffffp+1023)) return (x == __builtin_inf() || y == __builtin_inf()) ? __builtin_inf() : x + y;
-    if (y == 0.0) return x;
-    // scale into a safe range by a power of two (exact)
-    double scale = 1.0;
-    if (x > 0x1p+500) {
-        scale = 0x1p+600;
-        x *= 0x1p-600;
-        y *= 0x1p-600;
-    } else if (x < 0x1p-500) {
-        scale = 0x1p-600;
-        x *= 0x1p+600;
-        y *= 0x1p+600;
+    if (ay < kTiny) {
+        if (ax >= ay / kEps) return ax + ay;
+        return hypot_kernel(ax / kScale, ay / kScale) * kScale;
     }
-    const double xx = x * x;
-    const double xx_lo = __builtin_fma(x, x, -xx);
-    const double yy = y * y;
-    const double yy_lo = __builtin_fma(y, y, -yy);
-    // two-sum of the high parts
-    const double sh = xx + yy;
-    const double bb = sh - xx;
-    const double sl0 = (xx - (sh - bb)) + (yy - bb);
-    const double sl = sl0 + xx_lo + yy_lo;
-    const double s_hi = sh + sl;
-    const double s_lo = sl - (s_hi - sh);
-    double r = __builtin_sqrt(s_hi);
-    // exact residual S - r^2 with S = s_hi + s_lo
-    for (int it = 0; it < 2; ++it) {
-        const double rr = r * r;
-        const double rr_lo = __builtin_fma(r, r, -rr);
-        const double e = ((s_hi - rr) - rr_lo) + s_lo;  // S - r^2 (to ~1e-32 relative)
-        // midpoints r +- ulp/2; (r + h)^2 - r^2 = 2 r h + h^2
-        const double up = __longlong_as_double(__double_as_longlong(r) + 1);
-        const double dn = __longlong_as_double(__double_as_longlong(r) - 1);
-        const double hu = 0.5 * (up - r);
-        const double hd = 0.5 * (r - dn);
-        if (e > 0.0 && e > 2.0 * r * hu + hu * hu) {
-            r = up;
-        } else if (e < 0.0 && -e > 2.0 * r * hd - hd * hd) {
-            r = dn;
-        } else {
-            break;
-        }
-    }
-    return r * scale;
+    if (ay <= ax * kEps) return ax + ay;
+    return hypot_kernel(ax, ay);
 }
 
 // point_segment_distance (finite_source_magnifier.cpp:330-342)
@@ -275,7 +263,7 @@ __device__ __forceinline__ double point_segment_distance(Pt p, Pt a, Pt b)
     const double length2 = dx * dx + dy * dy;
     if (length2 == 0.0) return __builtin_sqrt(dist2(p, a));
     const double t = dclamp(((p.x - a.x) * dx + (p.y - a.y) * dy) / length2, 0.0, 1.0);
-    return hypot_cr(p.x - (a.x + t * dx), p.y - (a.y + t * dy));
+    return hypot_glibc(p.x - (a.x + t * dx), p.y - (a.y + t * dy));
 }
 
 // Quartic of the critical curve at one phase, solved as PolynomialRootSolver::solve
