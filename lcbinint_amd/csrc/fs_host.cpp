@@ -6,6 +6,8 @@
 // cos/sin exactly as the reference's std::polar / std::cos / std::sin calls are
 // (finite_source_magnifier.cpp:553, 1551-1553, 1579-1583, 1606-1610, 1748, 4025).
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -131,6 +133,9 @@ lcbg_status ensure_scratch(ScratchPool& pool)
     s.runs = carve<int>(p, static_cast<std::size_t>(slots) * kRunCap * 4);
     st = map_hip(hipEventCreateWithFlags(&pool.last_use, hipEventDisableTiming));
     if (st != LCBG_OK) return st;
+    // record once so that the first run's stream wait has a completed event
+    st = map_hip(hipEventRecord(pool.last_use, nullptr));
+    if (st != LCBG_OK) return st;
     // empty visited-run lists (-1 heads); the kernel restores them per epoch
     st = map_hip(hipMemset(s.phi_head, 0xff, static_cast<std::size_t>(slots) * kPhiCap * sizeof(int)));
     if (st != LCBG_OK) return st;
@@ -243,13 +248,28 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     std::lock_guard<std::mutex> lock(pool.mu);
     st = ensure_scratch(pool);
     if (st != LCBG_OK) return st;
+    // LCBG_STAGE_SYNC=1: wait after every stage and report it on stderr
+    // (a diagnostic in the spirit of the reference's LCBININT_AREA_DIAGNOSTICS)
+    const bool stage_sync = std::getenv("LCBG_STAGE_SYNC") != nullptr;
+    auto stage_done = [&](const char* name) {
+        if (!stage_sync || st != LCBG_OK) return;
+        st = map_hip(hipStreamSynchronize(stream));
+        unsigned h[8] = {0};
+        (void)hipMemcpy(h, ep.counters, sizeof(h), hipMemcpyDeviceToHost);
+        std::fprintf(stderr, "[lcbg] %s done (status %d, queued %u / %u, flags %u)\n", name,
+                     static_cast<int>(st), h[0], h[2], h[4]);
+        std::fflush(stderr);
+    };
     st = map_hip(hipStreamWaitEvent(stream, pool.last_use, 0));
     if (st == LCBG_OK) st = launch_reset_counters(ep.counters, 8, stream);
     if (st == LCBG_OK) st = launch_fs_stage1(b->d_curves, b->n_curves, fs->cache, fs->opt, ep, stream);
+    stage_done("stage 1 (point source, bounding box)");
     if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[1], stream);
     if (st == LCBG_OK) st = launch_fs_stage2(b->d_curves, fs->cache, fs->opt, ep, fs->stage2_waves, stream);
+    stage_done("stage 2 (caustic distance, hexadecapole)");
     if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[2], stream);
     if (st == LCBG_OK) st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, pool.scr, stream);
+    stage_done("stage 3 (inverse ray)");
     if (st == LCBG_OK) st = map_hip(hipEventRecord(pool.last_use, stream));
     if (stage_ms && st == LCBG_OK) {
         (void)hipEventRecord(ev[3], stream);

@@ -33,7 +33,15 @@ struct Ctx {
     Pt* seeds;
     int n_seeds;
     bool overflow;
+    long long steps;  // wave steps spent on this epoch
 };
+
+// Counts one wave step; past the budget the epoch is abandoned with an error.
+__device__ __forceinline__ bool step_ok(Ctx& C)
+{
+    if (++C.steps > kStepBudget) C.overflow = true;
+    return !C.overflow;
+}
 
 // ------------------------------------------------------- probe image search
 
@@ -78,6 +86,7 @@ __device__ __forceinline__ void push_seed(Ctx& C, Pt p)
         C.overflow = true;
         return;
     }
+    wave_sync();
     if (C.lane == 0) C.seeds[C.n_seeds] = p;
     C.n_seeds += 1;
     wave_sync();
@@ -102,6 +111,7 @@ __device__ __forceinline__ int first_near_seed(const Ctx& C, Pt img, int n)
 // lane found.  valid = probe inside the disk and more than three images.
 __device__ void merge_probe(Ctx& C, int n_img, const Pt* img, const double* md)
 {
+    if (!step_ok(C)) return;
     const int n_before = C.n_seeds;
     const double lim = C.r2 * (1.0 + 1.0e-8);
     for (int i = 0; i < n_img; ++i) {
@@ -110,6 +120,7 @@ __device__ void merge_probe(Ctx& C, int n_img, const Pt* img, const double* md)
         if (si >= 0) {
             const double em = mapped_dist2(C.L, C.seeds[si].x, C.seeds[si].y, C.src);
             if (md[i] + 1.0e-16 < em) {
+                wave_sync();  // every lane has read seeds[si] before it changes
                 if (C.lane == 0) C.seeds[si] = img[i];
                 wave_sync();
             }
@@ -474,6 +485,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             Pt im[5];
             const int ni = probe_r3_images(C, probe, im);
             if (ni > C.n_seeds) {
+                wave_sync();
                 for (int i = 0; i < ni; ++i) {
                     if (C.lane == 0) C.seeds[i] = im[i];
                 }
@@ -546,14 +558,18 @@ __device__ __forceinline__ bool rows_ensure(Ctx& C, Rows& R, int index)
     }
     const int from = R.size * 5;
     const int to = (index + 1) * 5;
+    wave_sync();
     for (int k = from + C.lane; k < to; k += kWave) R.r[k] = 0.0;
     R.size = index + 1;
     wave_sync();
     return true;
 }
 
+// Lane 0 stores a row field; the barriers order it after every lane's earlier
+// reads of the row records and before any later read.
 __device__ __forceinline__ void rset(Ctx& C, Rows& R, int row, int f, double v)
 {
+    wave_sync();
     if (C.lane == 0) R.r[row * 5 + f] = v;
     wave_sync();
 }
@@ -593,6 +609,7 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
     long long guard = 0;
     const long long budget = K.max_steps - 1;
     for (;;) {
+        if (!step_ok(C)) return __builtin_nan("");
         if (!rows_ensure(C, R, yi)) return __builtin_nan("");
         const long long remaining = budget - guard;
         if (remaining <= 0) return countall;
@@ -677,6 +694,7 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
             C.overflow = true;
             return __builtin_nan("");
         }
+        wave_sync();
         if (C.lane == 0) {
             R.key[R.pushed] = key;
             R.idx[R.pushed] = yi;
@@ -710,6 +728,7 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* 
                                                                          : static_cast<long long>(sb) * sb * 2000LL;
     const double incr = K.incr;
     const double m = 0.5 * incr * 1.01;
+    wave_sync();
     for (int k = C.lane; k < n_img; k += kWave) {
         areaimage[k] = 0.0;
         flags[k] = 0;  // bit 0: overlap, bit 1: subtracted
@@ -810,6 +829,7 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* 
             if (row == nyi - 1 && areabound > 0.0 && yi > nyi) nyi = yi;
         }
         area += areai;
+        wave_sync();
         if (C.lane == 0) areaimage[i] = areai;
         wave_sync();
         // overlap bookkeeping (:2681-2739)
@@ -835,14 +855,15 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* 
                 hit = ballot(h) != 0;
             }
             if (hit) {
+                const int fl = flags[other];
+                wave_sync();
                 if (other < i) {
-                    const int fl = flags[other];
                     if (!(fl & 2)) {
                         area -= areaimage[other];
                         if (C.lane == 0) flags[other] = fl | 2;
                     }
                 } else {
-                    if (C.lane == 0) flags[other] = flags[other] | 1;
+                    if (C.lane == 0) flags[other] = fl | 1;
                 }
                 wave_sync();
             }
@@ -897,6 +918,7 @@ __device__ double polar_add_run(Ctx& C, Polar& P, int ir, int iphi, double cph, 
     // left
     int left = ir;
     for (;;) {
+        if (!step_ok(C)) return 0.0;
         const int c = left - 1 - C.lane;
         bool ok = false;
         double dz2;
@@ -912,6 +934,7 @@ __device__ double polar_add_run(Ctx& C, Polar& P, int ir, int iphi, double cph, 
     }
     int right = ir;
     for (;;) {
+        if (!step_ok(C)) return 0.0;
         const int c = right + 1 + C.lane;
         double dz2;
         const bool ok = !polar_visited(P, c, iphi) && polar_inside(C, P, c, cph, sph, &dz2);
@@ -928,12 +951,14 @@ __device__ double polar_add_run(Ctx& C, Polar& P, int ir, int iphi, double cph, 
         C.overflow = true;
         return 0.0;
     }
+    const int old_head = P.head[iphi];
+    wave_sync();
     if (C.lane == 0) {
         const int r = P.n_runs;
         P.runs[4 * r + 0] = iphi;
         P.runs[4 * r + 1] = left;
         P.runs[4 * r + 2] = right;
-        P.runs[4 * r + 3] = P.head[iphi];
+        P.runs[4 * r + 3] = old_head;
         P.head[iphi] = r;
     }
     P.n_runs += 1;
@@ -1052,7 +1077,7 @@ __device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* hea
             double cph, sph;
             phi_trig(P, nb, &cph, &sph);
             int c = L;
-            while (c <= Rr && !C.overflow) {
+            while (c <= Rr && step_ok(C)) {
                 const int cc = c + C.lane;
                 bool ok = false;
                 if (cc <= Rr) {
@@ -1071,6 +1096,7 @@ __device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* hea
         }
     }
     // reset the rows this epoch touched
+    wave_sync();
     for (int q = C.lane; q < P.n_runs; q += kWave) head[runs[4 * q]] = -1;
     wave_sync();
     if (C.overflow) return __builtin_nan("");
@@ -1119,6 +1145,7 @@ __global__ void __launch_bounds__(kWave) k_fs_ir(const CurveRec* __restrict__ cu
         C.seeds = seeds;
         C.n_seeds = 0;
         C.overflow = false;
+        C.steps = 0;
         const CausticScalars sc = cache.sc[c];
         image_seeds(C, cache.crit + static_cast<size_t>(c) * kSeedPhases * 4,
                     cache.assign + static_cast<size_t>(c) * kSeedPhases * 4,

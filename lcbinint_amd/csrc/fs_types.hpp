@@ -100,5 +100,11 @@ constexpr unsigned kErrSeedCap = 1u;
 constexpr unsigned kErrRowCap = 2u;
 constexpr unsigned kErrPolarCap = 4u;
 constexpr unsigned kErrUnsupported = 8u;
+constexpr unsigned kErrBudget = 16u;
+
+// Work budget of one inverse-ray epoch in wave steps (64-cell chunks, probe
+// merges, polar runs); an epoch that exceeds it ends with an error flag, so
+// every resident wave finishes in bounded time.
+constexpr long long kStepBudget = 1ll << 22;
 
 }  // namespace lcbg
