@@ -54,6 +54,7 @@ class Oracle:
         self._ld_table = _f(L, "lcbo_ld_table", None, D, D, V)
         self._ld_lookup = _f(L, "lcbo_ld_lookup", D, V, D)
         self._hex = _f(L, "lcbo_hexadecapole", D, D, D, D, D, D, D, D, D, P(D))
+        self._crit = _f(L, "lcbo_critical_curve", None, D, D, D, D, V, V)
         self.flop_count = _f(L, "lcbo_flop_count", D)
         self.flop_reset = _f(L, "lcbo_flop_reset", None)
 
@@ -103,6 +104,12 @@ class Oracle:
     def ld_lookup(self, table, r2):
         return self._ld_lookup(ptr(table), r2)
 
+    def critical_curve(self, s, q, cos_phi, sin_phi):
+        ca = np.zeros(8)
+        cr = np.zeros(8)
+        self._crit(s, q, cos_phi, sin_phi, ptr(ca), ptr(cr))
+        return ca.reshape(4, 2), cr.reshape(4, 2)
+
     def hexadecapole(self, s, q, x, y, rho, c, d, a0):
         e = D()
         m = self._hex(s, q, x, y, rho, c, d, a0, ctypes.byref(e))
@@ -126,6 +133,12 @@ class Ref:
         self._lc = _f(L, "ref_light_curve_static", I32, P(RefSettings), V, ctypes.c_long,
                       D, D, D, D, D, D, D, V, V)
         self._ir = _f(L, "ref_binary_inverse_ray", D, P(RefSettings), D, D, D, D, D, P(I32))
+        self._caus = _f(L, "ref_caustic_points_at_phase", I32, D, D, D, V)
+        self._crits = _f(L, "ref_critical_sources_at_phase", I32, D, D, D, V)
+        self._seeds = _f(L, "ref_image_seeds", I32, P(RefSettings), D, D, D, D, D, V, I32)
+        self._cart = _f(L, "ref_cartesian_area", D, P(RefSettings), D, D, D, D, D, V, I32, V)
+        self._polar = _f(L, "ref_polar_area", D, P(RefSettings), D, D, D, D, D, V, I32)
+        self._hex = _f(L, "ref_hexadecapole", D, P(RefSettings), D, D, D, D, D, D, P(D))
 
     def binary_mag0(self, s, q, x, y):
         n = I32()
@@ -186,6 +199,36 @@ class Ref:
         st = self._lc(ctypes.byref(settings), ptr(t), len(t), t0, tE, u0, alpha, s, q, rho,
                       ptr(out), ptr(meth))
         return out, meth, st
+
+    def caustic_points(self, s, q, phase):
+        out = np.zeros(8)
+        k = self._caus(s, q, phase, ptr(out))
+        return out[:2 * k].reshape(-1, 2)
+
+    def critical_sources(self, s, q, phase):
+        out = np.zeros(8)
+        k = self._crits(s, q, phase, ptr(out))
+        return out[:2 * k].reshape(-1, 2)
+
+    def image_seeds(self, settings, s, q, x, y, rho, cap=4096):
+        out = np.zeros(2 * cap)
+        k = self._seeds(ctypes.byref(settings), s, q, x, y, rho, ptr(out), cap)
+        return out[:2 * min(k, cap)].reshape(-1, 2)
+
+    def cartesian_area(self, settings, s, q, x, y, rho, seeds):
+        sd = np.ascontiguousarray(seeds, dtype=np.float64).ravel()
+        diag = np.zeros(8)
+        a = self._cart(ctypes.byref(settings), s, q, x, y, rho, ptr(sd), len(sd) // 2, ptr(diag))
+        return a, diag
+
+    def polar_area(self, settings, s, q, x, y, rho, seeds):
+        sd = np.ascontiguousarray(seeds, dtype=np.float64).ravel()
+        return self._polar(ctypes.byref(settings), s, q, x, y, rho, ptr(sd), len(sd) // 2)
+
+    def hexadecapole(self, settings, s, q, x, y, rho, a0):
+        e = D()
+        m = self._hex(ctypes.byref(settings), s, q, x, y, rho, a0, ctypes.byref(e))
+        return m, e.value
 
     def inverse_ray(self, settings, s, q, x, y, rho):
         m = I32()

@@ -892,3 +892,75 @@ double lcbo_hexadecapole(double s, double q, double x, double y, double rho, dou
     if (rel_err) *rel_err = fabs(hex) / (am < 1.0e-10 ? 1.0e-10 : am); /* std::max */
     return mag;
 }
+
+/* ------------------------------------------------------ critical curve */
+
+/* libgcc __divdc3 (GCC 12): Smith's method; the power-of-two prescaling of
+ * the library is exact and omitted, the subnormal-ratio branch is kept. */
+static cx std_cdiv(double a, double b, double c, double d)
+{
+    const double RMIN = 0x1p-1022;
+    double ratio, denom, x, y;
+    if (fabs(c) < fabs(d)) {
+        ratio = c / d;
+        denom = (c * ratio) + d;
+        if (fabs(ratio) > RMIN) {
+            x = ((a * ratio) + b) / denom;
+            y = ((b * ratio) - a) / denom;
+        } else {
+            x = ((c * (a / d)) + b) / denom;
+            y = ((c * (b / d)) - a) / denom;
+        }
+    } else {
+        ratio = d / c;
+        denom = (d * ratio) + c;
+        if (fabs(ratio) > RMIN) {
+            x = ((b * ratio) + a) / denom;
+            y = (b - (a * ratio)) / denom;
+        } else {
+            x = (a + (d * (b / c))) / denom;
+            y = (b - (d * (a / c))) / denom;
+        }
+    }
+    return C(x, y);
+}
+
+void lcbo_critical_curve(double s, double q, double cos_phi, double sin_phi, double* caustic,
+                         double* critical)
+{
+    const double sa = fabs(s);
+    const double q_input = fabs(q);
+    const double qq = q_input < 1.0 ? q_input : 1.0 / q_input;
+    const double a_re = q_input < 1.0 ? -sa : sa;
+    const double m1r = 1.0 / (1.0 + qq);
+    const double m2r = qq * m1r;
+    /* std::complex products with the zero imaginary parts carried along */
+    const cx a = C(a_re, 0.0), m1 = C(m1r, 0.0), m2 = C(m2r, qq * 0.0), ph = C(cos_phi, sin_phi);
+    const cx a2 = cmul(a, a);
+    cx c[5];
+    c[0] = cmul(m2, a2);
+    c[1] = cmul(cscale(-2.0, m2), a);
+    c[2] = csub(cadd(m1, m2), cmul(ph, a2));
+    c[3] = cmul(cscale(2.0, ph), a);
+    c[4] = C(-ph.re, -ph.im);
+    cx roots[4];
+    sg_roots_gen(roots, c, 4, 1, 0);
+    for (int r = 0; r < 4; r++) {
+        const cx z = roots[r];
+        /* binary_lens_equation: z - m1/(conj(z) - a) - m2/conj(z) - a*m1 */
+        const cx zc = cconj(z);
+        const cx t1 = std_cdiv(m1.re, m1.im, zc.re - a.re, zc.im - a.im);
+        const cx t2 = std_cdiv(m2.re, m2.im, zc.re, zc.im);
+        const cx sh = C((z.re - t1.re) - t2.re, (z.im - t1.im) - t2.im);
+        const cx am1 = cmul(a, m1);
+        caustic[2 * r] = sh.re - am1.re;
+        caustic[2 * r + 1] = sh.im - am1.im;
+        /* map_binary_lens_real */
+        const double x = z.re, y = z.im;
+        const double xa = x - a_re;
+        const double den1 = xa * xa + y * y;
+        const double den2 = x * x + y * y;
+        critical[2 * r] = x - m1r * xa / den1 - m2r * x / den2 - a_re * m1r;
+        critical[2 * r + 1] = y - m1r * y / den1 - m2r * y / den2;
+    }
+}

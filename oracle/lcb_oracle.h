@@ -60,6 +60,17 @@ double lcbo_ld_lookup(const double* table, double normalized_radius2);
 double lcbo_hexadecapole(double s, double q, double x, double y, double rho, double c,
                          double d, double a0, double* rel_err);
 
+/* Critical curve at phase (cos_phi, sin_phi): the quartic of
+ * critical_curve_polynomial_coefficients (finite_source_magnifier.cpp:528-545)
+ * solved as PolynomialRootSolver::solve (polynomial_roots.cpp:35-73), then
+ * mapped to the source plane by the complex lens equation binary_lens_equation
+ * (point_source_magnifier.cpp:780-792, as caustic_points_at_phase :547-567 does)
+ * into caustic[8], and by the real map map_binary_lens_real
+ * (finite_source_magnifier.cpp:278-291, as critical_sources_at_phase
+ * :1622-1641 does) into critical[8]; interleaved (x, y). */
+void lcbo_critical_curve(double s, double q, double cos_phi, double sin_phi, double* caustic,
+                         double* critical);
+
 /* Number of floating-point operations counted by the restatement since the last
  * reset (+,-,*,/,sqrt = 1 each; compares 0).  Used to state algorithmic work. */
 double lcbo_flop_count(void);
