@@ -81,13 +81,10 @@ __global__ void __launch_bounds__(kWave) k_fs_stage2(const CurveRec* __restrict_
                                                      FsCache cache, FsOptions opt, FsEpochs ep)
 {
     const int lane = lane_id();
-    __shared__ int s_item;
+    const int count = uni(static_cast<int>(ep.counters[0]));
     for (;;) {
-        if (lane == 0) s_item = static_cast<int>(atomicAdd(&ep.counters[1], 1u));
-        wave_sync();
-        const int item = s_item;
-        wave_sync();
-        if (item >= static_cast<int>(ep.counters[0])) break;
+        const int item = next_item(&ep.counters[1]);
+        if (item >= count) break;
         const int64_t e = ep.near_list[item];
         const int c = ep.curve_of[e];
         const CurveRec cv = curves[c];
@@ -120,13 +117,13 @@ __global__ void __launch_bounds__(kWave) k_fs_stage2(const CurveRec* __restrict_
             if (ix >= ix0 && ix <= ix1 && iy >= iy0 && iy <= iy1 && cand < d2) d2 = cand;
             if (cand < d2all) d2all = cand;
         }
-        d2 = wave_min(d2);
+        d2 = uni(wave_min(d2));
         const double sampled = (d2 == __builtin_inf()) ? d2 : __builtin_sqrt(d2);
 
         // binary_caustic_distance (:4107-4181): segments next to branch points
         // in the cells within nearest + max_seg_len
         double nearest = sampled;
-        if (!isfinite(nearest)) nearest = __builtin_sqrt(wave_min(d2all));
+        if (!isfinite(nearest)) nearest = uni(__builtin_sqrt(wave_min(d2all)));
         const double seg_radius = nearest + sc.max_seg_len;
         const int jx0 = cell_coord(src.x - seg_radius, sc.min_x, sc.step_x);
         const int jx1 = cell_coord(src.x + seg_radius, sc.min_x, sc.step_x);
@@ -149,7 +146,7 @@ __global__ void __launch_bounds__(kWave) k_fs_stage2(const CurveRec* __restrict_
                 dseg = dmin(dseg, point_segment_distance(src, bb[pos], bb[next]));
             }
         }
-        const double refined = wave_min(dseg);
+        const double refined = uni(wave_min(dseg));
         const bool near_caustic = refined < opt.hex_threshold * rho;
 
         int method = -1;
@@ -204,9 +201,9 @@ __global__ void __launch_bounds__(kWave) k_fs_stage2(const CurveRec* __restrict_
                 }
                 double a1p = 0.0, a2p = 0.0, a1x = 0.0;
                 for (int i = 0; i < 4; ++i) {
-                    a1p += shfl(my, 3 * i);
-                    a2p += shfl(my, 3 * i + 1);
-                    a1x += shfl(my, 3 * i + 2);
+                    a1p += uni(shfl(my, 3 * i));
+                    a2p += uni(shfl(my, 3 * i + 1));
+                    a1x += uni(shfl(my, 3 * i + 2));
                 }
                 a1p = a1p / 4.0 - a_ps;
                 a2p = a2p / 4.0 - a_ps;
@@ -226,23 +223,21 @@ __global__ void __launch_bounds__(kWave) k_fs_stage2(const CurveRec* __restrict_
                 }
             }
         }
-        if (lane == 0) {
-            if (method >= 0) {
-                ep.mag[e] = mag;
-                ep.method[e] = method;
+        if (method >= 0) {
+            wave_store(&ep.mag[e], mag);
+            wave_store(&ep.method[e], method);
+        } else {
+            const bool auto_polar = opt.mode == 4 && fabs(a_ps) >= 100.0;
+            if (opt.mode == 3) {
+                wave_store(&ep.mag[e], __builtin_nan(""));
+                wave_store(&ep.method[e], -1);
+                atomicOr(&ep.counters[4], kErrUnsupported);
             } else {
-                const bool auto_polar = opt.mode == 4 && fabs(a_ps) >= 100.0;
-                if (opt.mode == 3) {
-                    ep.mag[e] = __builtin_nan("");
-                    ep.method[e] = -1;
-                    atomicOr(&ep.counters[4], kErrUnsupported);
-                } else {
-                    ep.method[e] = (opt.mode == 2 || auto_polar) ? LCBG_METHOD_INVERSE_RAY_POLAR
-                                                                 : LCBG_METHOD_INVERSE_RAY_CARTESIAN;
-                    ep.dist[e] = refined;
-                    const unsigned slot = atomicAdd(&ep.counters[2], 1u);
-                    ep.ir_list[slot] = e;
-                }
+                wave_store(&ep.method[e], (opt.mode == 2 || auto_polar) ? LCBG_METHOD_INVERSE_RAY_POLAR
+                                                                       : LCBG_METHOD_INVERSE_RAY_CARTESIAN);
+                wave_store(&ep.dist[e], refined);
+                const int slot = wave_fetch_inc(&ep.counters[2]);
+                wave_store(&ep.ir_list[slot], e);
             }
         }
     }

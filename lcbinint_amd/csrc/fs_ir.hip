@@ -39,7 +39,8 @@ struct Ctx {
 // Counts one wave step; past the budget the epoch is abandoned with an error.
 __device__ __forceinline__ bool step_ok(Ctx& C)
 {
-    if (++C.steps > kStepBudget) C.overflow = true;
+    C.steps = uni(C.steps + 1);
+    if (C.steps > kStepBudget) C.overflow = true;
     return !C.overflow;
 }
 
@@ -87,8 +88,8 @@ __device__ __forceinline__ void push_seed(Ctx& C, Pt p)
         return;
     }
     wave_sync();
-    if (C.lane == 0) C.seeds[C.n_seeds] = p;
-    C.n_seeds += 1;
+    wave_store(&C.seeds[C.n_seeds], p);
+    C.n_seeds = uni(C.n_seeds + 1);
     wave_sync();
 }
 
@@ -119,9 +120,9 @@ __device__ void merge_probe(Ctx& C, int n_img, const Pt* img, const double* md)
         const int si = first_near_seed(C, img[i], n_before);
         if (si >= 0) {
             const double em = mapped_dist2(C.L, C.seeds[si].x, C.seeds[si].y, C.src);
-            if (md[i] + 1.0e-16 < em) {
+            if (uni(md[i] + 1.0e-16 < em)) {
                 wave_sync();  // every lane has read seeds[si] before it changes
-                if (C.lane == 0) C.seeds[si] = img[i];
+                wave_store(&C.seeds[si], img[i]);
                 wave_sync();
             }
         } else {
@@ -156,15 +157,15 @@ __device__ __forceinline__ ProbeResult eval_probe(const Ctx& C, Pt probe)
 // Replays one lane's probe result uniformly.
 __device__ __forceinline__ void merge_lane(Ctx& C, const ProbeResult& mine, int src_lane)
 {
-    const int n = shfl(mine.n, src_lane);
+    const int n = uni(shfl(mine.n, src_lane));
     if (n <= 3) return;
     Pt img[5];
     double md[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-        img[i].x = shfl(mine.img[i].x, src_lane);
-        img[i].y = shfl(mine.img[i].y, src_lane);
-        md[i] = shfl(mine.md[i], src_lane);
+        img[i].x = uni(shfl(mine.img[i].x, src_lane));
+        img[i].y = uni(shfl(mine.img[i].y, src_lane));
+        md[i] = uni(shfl(mine.md[i], src_lane));
     }
     merge_probe(C, n, img, md);
 }
@@ -174,7 +175,7 @@ __device__ void caustic_probes(Ctx& C, Pt cs, const FsTables& tab)
 {
     const double dx = cs.x - C.src.x;
     const double dy = cs.y - C.src.y;
-    const double distance = hypot_glibc(dx, dy);
+    const double distance = uni(hypot_glibc(dx, dy));
     if (distance <= 0.0 || distance >= C.rho) return;
     const double ux = dx / distance;
     const double uy = dy / distance;
@@ -241,8 +242,8 @@ __device__ Pt refine_critical(const Ctx& C, double center, double step)
     double x1 = right - golden * (right - left);
     double x2 = left + golden * (right - left);
     Pt n1{0.0, 0.0}, n2{0.0, 0.0};
-    double f1 = nearest_critical(C, x1, &n1);
-    double f2 = nearest_critical(C, x2, &n2);
+    double f1 = uni(nearest_critical(C, x1, &n1));
+    double f2 = uni(nearest_critical(C, x2, &n2));
     for (int iter = 0; iter < 32; ++iter) {
         if (f1 > f2) {
             left = x1;
@@ -250,14 +251,14 @@ __device__ Pt refine_critical(const Ctx& C, double center, double step)
             f1 = f2;
             n1 = n2;
             x2 = left + golden * (right - left);
-            f2 = nearest_critical(C, x2, &n2);
+            f2 = uni(nearest_critical(C, x2, &n2));
         } else {
             right = x2;
             x2 = x1;
             f2 = f1;
             n2 = n1;
             x1 = right - golden * (right - left);
-            f1 = nearest_critical(C, x1, &n1);
+            f1 = uni(nearest_critical(C, x1, &n1));
         }
     }
     return f1 < f2 ? n1 : n2;
@@ -321,6 +322,8 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             double v = d2;
             int idx = o < kOrd ? o : 0x7fffffff;
             wave_argmin(v, idx);
+            v = uni(v);
+            idx = uni(idx);
             if (v < best_d2) {
                 best_d2 = v;
                 best_ord = idx;
@@ -344,7 +347,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             const int sample = jphi * 40 + kphi;
             const int before = C.n_seeds;
             caustic_probes(C, crit[4 * sample + (o2 & 3)], tab);
-            phase_found = C.n_seeds > before;
+            phase_found = uni(C.n_seeds > before);
         }
         if (cur_phase >= 0 && phase_found) {
             found = true;
@@ -456,6 +459,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
                 const long long o = __shfl_xor(best, off, kWave);
                 best = o < best ? o : best;
             }
+            best = uni(best);
             if (best == 0x7fffffffffffffffLL) break;
             cursor = best;
             const int cell = static_cast<int>(best / (4LL * bins));
@@ -486,9 +490,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             const int ni = probe_r3_images(C, probe, im);
             if (ni > C.n_seeds) {
                 wave_sync();
-                for (int i = 0; i < ni; ++i) {
-                    if (C.lane == 0) C.seeds[i] = im[i];
-                }
+                for (int i = 0; i < ni; ++i) wave_store(&C.seeds[i], im[i]);
                 C.n_seeds = ni;
                 wave_sync();
             }
@@ -565,12 +567,12 @@ __device__ __forceinline__ bool rows_ensure(Ctx& C, Rows& R, int index)
     return true;
 }
 
-// Lane 0 stores a row field; the barriers order it after every lane's earlier
-// reads of the row records and before any later read.
+// Stores a row field; the barriers order it after every lane's earlier reads of
+// the row records and before any later read.
 __device__ __forceinline__ void rset(Ctx& C, Rows& R, int row, int f, double v)
 {
     wave_sync();
-    if (C.lane == 0) R.r[row * 5 + f] = v;
+    wave_store(&R.r[row * 5 + f], v);
     wave_sync();
 }
 
@@ -624,15 +626,15 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
         if (dir > 0) {
             const int s = first_set(ballot(!in && valid));
             if (s < 0) {
-                countx += wave_sum(w);
+                countx = uni(countx + wave_sum(w));
                 guard += n_eval;
                 last_in = true;
-                x = shfl(xk, n_eval - 1) + step;
+                x = uni(shfl(xk, n_eval - 1) + step);
                 continue;
             }
-            countx += wave_sum(C.lane < s ? w : 0.0);
+            countx = uni(countx + wave_sum(C.lane < s ? w : 0.0));
             guard += s + 1;
-            const double xs = shfl(xk, s);
+            const double xs = uni(shfl(xk, s));
             const bool prev_in = s > 0 ? true : last_in;
             if (prev_in) rset(C, R, yi, XMAX, xs);
             dir = -1;
@@ -651,18 +653,18 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
         const bool stop = valid && !in && (cb || (f >= 0 && C.lane > f) || yi == 0 || !cond);
         const int s = first_set(ballot(stop));
         const int limit = s >= 0 ? s : n_eval;
-        const double add = wave_sum((C.lane < limit && in) ? w : 0.0);
-        if (!cb && f >= 0 && f < limit) rset(C, R, yi, XMAX, shfl(xk, f) - (-incr));
+        const double add = uni(wave_sum((C.lane < limit && in) ? w : 0.0));
+        if (!cb && f >= 0 && f < limit) rset(C, R, yi, XMAX, uni(shfl(xk, f)) - (-incr));
         if (s < 0) {
-            countx += add;
+            countx = uni(countx + add);
             guard += n_eval;
             last_in = ((in_m >> (n_eval - 1)) & 1ull) != 0;
-            x = shfl(xk, n_eval - 1) + (-incr);
+            x = uni(shfl(xk, n_eval - 1) + (-incr));
             continue;
         }
-        countx += add;
+        countx = uni(countx + add);
         guard += s + 1;
-        const double xs = shfl(xk, s);
+        const double xs = uni(shfl(xk, s));
         const bool prev_in = s > 0 ? (((in_m >> (s - 1)) & 1ull) != 0) : last_in;
         if (prev_in) rset(C, R, yi, XMIN, xs);
         // close the row
@@ -695,11 +697,9 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
             return __builtin_nan("");
         }
         wave_sync();
-        if (C.lane == 0) {
-            R.key[R.pushed] = key;
-            R.idx[R.pushed] = yi;
-        }
-        R.pushed += 1;
+        wave_store(&R.key[R.pushed], key);
+        wave_store(&R.idx[R.pushed], yi);
+        R.pushed = uni(R.pushed + 1);
         wave_sync();
         ++yi;
         if (!rows_ensure(C, R, yi)) return __builtin_nan("");
@@ -830,7 +830,7 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* 
         }
         area += areai;
         wave_sync();
-        if (C.lane == 0) areaimage[i] = areai;
+        wave_store(&areaimage[i], areai);
         wave_sync();
         // overlap bookkeeping (:2681-2739)
         for (int other = 0; other < n_img; ++other) {
@@ -860,10 +860,10 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* 
                 if (other < i) {
                     if (!(fl & 2)) {
                         area -= areaimage[other];
-                        if (C.lane == 0) flags[other] = fl | 2;
+                        wave_store(&flags[other], fl | 2);
                     }
                 } else {
-                    if (C.lane == 0) flags[other] = fl | 1;
+                    wave_store(&flags[other], fl | 1);
                 }
                 wave_sync();
             }
@@ -953,15 +953,15 @@ __device__ double polar_add_run(Ctx& C, Polar& P, int ir, int iphi, double cph, 
     }
     const int old_head = P.head[iphi];
     wave_sync();
-    if (C.lane == 0) {
+    {
         const int r = P.n_runs;
-        P.runs[4 * r + 0] = iphi;
-        P.runs[4 * r + 1] = left;
-        P.runs[4 * r + 2] = right;
-        P.runs[4 * r + 3] = old_head;
-        P.head[iphi] = r;
+        wave_store(&P.runs[4 * r + 0], iphi);
+        wave_store(&P.runs[4 * r + 1], left);
+        wave_store(&P.runs[4 * r + 2], right);
+        wave_store(&P.runs[4 * r + 3], old_head);
+        wave_store(&P.head[iphi], r);
     }
-    P.n_runs += 1;
+    P.n_runs = uni(P.n_runs + 1);
     wave_sync();
     double sum = 0.0;
     for (int c = left + C.lane; c <= right; c += kWave) {
@@ -971,7 +971,7 @@ __device__ double polar_add_run(Ctx& C, Polar& P, int ir, int iphi, double cph, 
         const double b = C.uniform ? 1.0 : ld_brightness(dz2 * C.inv_r2, C.ld_c, C.ld_d);
         sum += b * radius;
     }
-    return wave_sum(sum);
+    return uni(wave_sum(sum));
 }
 
 __device__ __forceinline__ double wrap_angle(double angle)
@@ -1112,7 +1112,6 @@ __global__ void __launch_bounds__(kWave) k_fs_ir(const CurveRec* __restrict__ cu
 {
     const int lane = lane_id();
     const int slot = blockIdx.x;
-    __shared__ int s_item;
     Pt* seeds = scr.seeds + static_cast<size_t>(slot) * kSeedCap;
     double* areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
     int* flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
@@ -1120,12 +1119,10 @@ __global__ void __launch_bounds__(kWave) k_fs_ir(const CurveRec* __restrict__ cu
     int* row_key = scr.row_key + static_cast<size_t>(slot) * kRowCap * 2;
     int* head = scr.phi_head + static_cast<size_t>(slot) * kPhiCap;
     int* runs = scr.runs + static_cast<size_t>(slot) * kRunCap * 4;
+    const int count = uni(static_cast<int>(ep.counters[2]));
     for (;;) {
-        if (lane == 0) s_item = static_cast<int>(atomicAdd(&ep.counters[3], 1u));
-        wave_sync();
-        const int item = s_item;
-        wave_sync();
-        if (item >= static_cast<int>(ep.counters[2])) break;
+        const int item = next_item(&ep.counters[3]);
+        if (item >= count) break;
         const int64_t e = ep.ir_list[item];
         const int c = ep.curve_of[e];
         const CurveRec cv = curves[c];
@@ -1166,10 +1163,8 @@ __global__ void __launch_bounds__(kWave) k_fs_ir(const CurveRec* __restrict__ cu
                 mag = cartesian_mag(C, opt.source_bins, rows, row_key, areaimage, flags);
             }
         }
-        if (lane == 0) {
-            ep.mag[e] = mag;
-            if (C.overflow) atomicOr(&ep.counters[4], kErrSeedCap);
-        }
+        wave_store(&ep.mag[e], mag);
+        if (C.overflow) atomicOr(&ep.counters[4], kErrSeedCap);
         wave_sync();
     }
 }

@@ -132,10 +132,8 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r) prev[r] = p[r];
-            if (lane == 0) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) br[r * bins] = prev[r];
-            }
+            for (int r = 0; r < 4; ++r) wave_store(&br[r * bins], prev[r]);
         }
         const int pl = lane < 24 ? lane : 0;
         const int p0 = c_perm[pl][0], p1 = c_perm[pl][1], p2 = c_perm[pl][2], p3 = c_perm[pl][3];
@@ -149,6 +147,8 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
             if (lane >= 24) cost = __builtin_inf();
             int idx = lane;
             wave_argmin(cost, idx);
+            cost = uni(cost);
+            idx = uni(idx);
             // the reference keeps the identity when no cost is below +inf
             if (!(cost < __builtin_inf())) idx = 0;
             const int b0 = c_perm[idx][0], b1 = c_perm[idx][1], b2 = c_perm[idx][2], b3 = c_perm[idx][3];
@@ -156,12 +156,10 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
             prev[1] = cur[b1];
             prev[2] = cur[b2];
             prev[3] = cur[b3];
-            if (lane == 0) {
-                br[0 * bins + i] = prev[0];
-                br[1 * bins + i] = prev[1];
-                br[2 * bins + i] = prev[2];
-                br[3 * bins + i] = prev[3];
-            }
+            wave_store(&br[0 * bins + i], prev[0]);
+            wave_store(&br[1 * bins + i], prev[1]);
+            wave_store(&br[2 * bins + i], prev[2]);
+            wave_store(&br[3 * bins + i], prev[3]);
         }
     }
     __syncthreads();

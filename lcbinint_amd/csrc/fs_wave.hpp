@@ -8,6 +8,8 @@
 
 #include <cstdint>
 
+#include "fs_types.hpp"
+
 namespace lcbg {
 
 constexpr int kWave = 64;
@@ -68,5 +70,56 @@ __device__ __forceinline__ void wave_argmin(double& v, int& idx)
 }
 
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+// Values of the wave's control program are the same on every lane.  uni()
+// hands such a value to the compiler as a scalar (lane 0's copy, which equals
+// every other lane's), so the branches that depend on it are scalar branches:
+// the control program then stays converged as written, instead of being
+// restructured into per-lane loops around the lane-0 stores it relies on.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ unsigned uni(unsigned v)
+{
+    return static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+__device__ __forceinline__ bool uni(bool v) { return __builtin_amdgcn_readfirstlane(static_cast<int>(v)) != 0; }
+__device__ __forceinline__ long long uni(long long v)
+{
+    const unsigned long long u = static_cast<unsigned long long>(v);
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(u & 0xffffffffull)));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(u >> 32)));
+    return static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ long uni(long v) { return static_cast<long>(uni(static_cast<long long>(v))); }
+__device__ __forceinline__ double uni(double v) { return __longlong_as_double(uni(__double_as_longlong(v))); }
+
+__device__ __forceinline__ Pt2 uni(Pt2 p) { return Pt2{uni(p.x), uni(p.y)}; }
+
+// Side effects of the control program carry no `lane == 0` branch.  Such a
+// branch is loop-invariant, and the compiler may thread it across a loop's back
+// edge, which splits the wave into a lane-0 loop and a loop of the other lanes
+// that never meet again.  Instead every lane performs the operation: a store
+// writes lane 0's value from every lane, and a counter is advanced by every lane
+// with an increment that is zero except on lane 0.
+
+// *p = v, v being uniform.
+template <class T>
+struct same_type {
+    using type = T;
+};
+template <class T>
+__device__ __forceinline__ void wave_store(T* p, typename same_type<T>::type v)
+{
+    *p = uni(v);
+}
+
+// atomicAdd(counter, 1) on behalf of the wave; every lane receives the old value.
+__device__ __forceinline__ int wave_fetch_inc(unsigned* counter)
+{
+    const unsigned inc = lane_id() == 0 ? 1u : 0u;
+    return __builtin_amdgcn_readfirstlane(static_cast<int>(atomicAdd(counter, inc)));
+}
+
+// Next item of a work queue, as a scalar.
+__device__ __forceinline__ int next_item(unsigned* cursor) { return wave_fetch_inc(cursor); }
 
 }  // namespace lcbg
