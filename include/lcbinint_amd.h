@@ -91,6 +91,14 @@ lcbg_status lcbg_batch_create(const lcbg_curve* curves, int32_t n_curves,
                               const lcbg_options* options, lcbg_batch** out);
 lcbg_status lcbg_batch_destroy(lcbg_batch* batch);
 
+/* Per-curve inverse-ray resolution: source_bins[c] (host array of n_curves)
+ * replaces options.source_bins for curve c; 0 keeps the batch option.  The
+ * reference reads one FiniteSourceSettings::source_bins per magnifier
+ * (src/lcbinint/magnification/finite_source_magnifier.hpp:24, Options.nbin at
+ * python/lcbinint_pybind.cpp:2985-3058); a batch of curves with different
+ * nbin runs here in one launch instead of one LightCurve per nbin. */
+lcbg_status lcbg_batch_set_source_bins(lcbg_batch* batch, const int32_t* source_bins);
+
 /* Point-source light curves of every curve of the batch in one launch.
  * Curve c owns epochs [offsets[c], offsets[c+1]) of `times` and `mag` (device
  * pointers, offsets has n_curves+1 entries).  Replaces the rho == 0 epoch loop
@@ -124,6 +132,40 @@ lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const int64_t* o
                                              const double* times, int64_t n_epochs, double* mag,
                                              int32_t* method, void* stream, float* stage_ms,
                                              uint32_t* queue_counts, uint32_t* error_flags);
+
+/* Diagnostics with no counterpart in the reference (its closest relative is the
+ * LCBININT_AREA_DIAGNOSTICS switch of finite_source_magnifier.cpp): after this
+ * call every finite-source run of the batch writes, for each inverse-ray epoch
+ * e, the device wall-clock ticks that epoch took to cost[e] (device pointer,
+ * n_epochs entries; other epochs are left untouched).  NULL turns it off.
+ * lcbg_wall_clock_rate_khz() gives the tick rate of the current device. */
+lcbg_status lcbg_batch_set_epoch_cost(lcbg_batch* batch, int64_t* cost);
+int64_t lcbg_wall_clock_rate_khz(void);
+
+/* Stage accessor for one epoch of the batch's last finite-source run (host
+ * pointers, synchronous, allocates a few KB; a diagnostic, not for hot loops).
+ * With n_in < 0 it runs the device seed search of that epoch (the stage of
+ * augmented_image_seeds + augment_seeds_from_caustic_branches,
+ * finite_source_magnifier.cpp:1709-1871, 4845-4926) up to seed phase
+ * last_phase (0 point-source images, 1 first caustic crossing, 2 branch scan,
+ * 3 boundary probes, 4 interior probes, 5 caustic-segment augment; -1 = all)
+ * and writes up to
+ * `capacity` seeds as (x, y) pairs to `seeds`, *n_out = the seed count (-1 if
+ * a capacity was exceeded).  With n_in >= 0 it uses the n_in seeds given.
+ * Either way *mag receives the inverse-ray magnification computed from those
+ * seeds with `method` (LCBG_METHOD_INVERSE_RAY_CARTESIAN: the row scan of
+ * inverse_ray_cartesian_binary_mag, :2470-2768; _POLAR: the flood fill of
+ * inverse_ray_polar_boundary_binary, :1156-1319). */
+lcbg_status lcbg_batch_debug_epoch(lcbg_batch* batch, int64_t epoch, int32_t method,
+                                   int32_t last_phase, double* seeds, int32_t capacity,
+                                   int32_t n_in, int32_t* n_out, double* mag);
+
+/* The per-curve critical sources of the 1400-phase seed scan (phase k, root r
+ * at out[2*(4k+r)], out[2*(4k+r)+1]: 11200 doubles), which the reference
+ * recomputes per inverse-ray epoch with critical_sources_at_phase
+ * (finite_source_magnifier.cpp:1622-1641).  Builds the batch's caches if
+ * needed; synchronous (diagnostics). */
+lcbg_status lcbg_batch_debug_critical_sources(lcbg_batch* batch, int32_t curve, double* out);
 
 /* Host-pointer light curve of one curve with the given options (NULL =
  * defaults): the drop-in path behind LightCurve(...)(times, params).  Returns

@@ -63,6 +63,8 @@ lcbg_batch_create = _sig(
     "lcbg_batch_create", ctypes.c_int, ctypes.POINTER(Curve), ctypes.c_int32,
     ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_void_p))
 lcbg_batch_destroy = _sig("lcbg_batch_destroy", ctypes.c_int, ctypes.c_void_p)
+lcbg_batch_set_source_bins = _sig("lcbg_batch_set_source_bins", ctypes.c_int, ctypes.c_void_p,
+                                  c_int32_p)
 lcbg_batch_point_source_device = _sig(
     "lcbg_batch_point_source_device", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
@@ -70,6 +72,9 @@ lcbg_light_curve_point_source = _sig(
     "lcbg_light_curve_point_source", ctypes.c_int, ctypes.POINTER(Curve), c_double_p,
     ctypes.c_int64, c_double_p)
 lcbg_last_kernel_ms = _sig("lcbg_last_kernel_ms", ctypes.c_double)
+lcbg_batch_set_epoch_cost = _sig("lcbg_batch_set_epoch_cost", ctypes.c_int, ctypes.c_void_p,
+                                 ctypes.c_void_p)
+lcbg_wall_clock_rate_khz = _sig("lcbg_wall_clock_rate_khz", ctypes.c_int64)
 
 
 class LcbgError(RuntimeError):
@@ -100,6 +105,14 @@ class Batch:
         check(lcbg_batch_create(arr, len(curves), opts, ctypes.byref(handle)), "lcbg_batch_create")
         self.handle = handle
         self.n_curves = len(curves)
+
+    def set_source_bins(self, source_bins):
+        import numpy as np
+        nb = np.ascontiguousarray(source_bins, dtype=np.int32)
+        if nb.shape != (self.n_curves,):
+            raise ValueError(f"source_bins needs {self.n_curves} entries, got {nb.shape}")
+        check(lcbg_batch_set_source_bins(self.handle, nb.ctypes.data_as(c_int32_p)),
+              "lcbg_batch_set_source_bins")
 
     def point_source(self, offsets_ptr: int, times_ptr: int, mag_ptr: int, stream: int = 0):
         check(lcbg_batch_point_source_device(self.handle, offsets_ptr, times_ptr, mag_ptr, stream),
@@ -146,3 +159,44 @@ def finite_source(batch: Batch, offsets_ptr: int, times_ptr: int, n_epochs: int,
                                            method_ptr, stream, ms, qc, ctypes.byref(fl)),
           "lcbg_batch_finite_source_profile")
     return list(ms), list(qc), fl.value
+
+
+lcbg_batch_debug_epoch = _sig(
+    "lcbg_batch_debug_epoch", ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+    ctypes.c_int32, c_double_p, ctypes.c_int32, ctypes.c_int32, c_int32_p, c_double_p)
+lcbg_batch_debug_critical_sources = _sig(
+    "lcbg_batch_debug_critical_sources", ctypes.c_int, ctypes.c_void_p, ctypes.c_int32, c_double_p)
+
+
+def debug_critical_sources(batch: Batch, curve: int):
+    """The cached critical sources of the seed scan of one curve, (1400, 4, 2)."""
+    import numpy as np
+    out = np.zeros(1400 * 4 * 2)
+    check(lcbg_batch_debug_critical_sources(batch.handle, curve, out.ctypes.data_as(c_double_p)),
+          "lcbg_batch_debug_critical_sources")
+    return out.reshape(1400, 4, 2)
+
+
+def debug_epoch(batch: Batch, epoch: int, method: int, seeds=None, capacity: int = 1024,
+                last_phase: int = -1):
+    """Seeds and inverse-ray magnification of one epoch of the batch's last
+    finite-source run.  seeds=None searches them on the device (seed phases up
+    to last_phase, -1 = all); otherwise the given (n, 2) seeds are used.
+    Returns (seeds (n, 2), magnification)."""
+    import numpy as np
+    n_out = ctypes.c_int32()
+    mag = ctypes.c_double()
+    if seeds is None:
+        buf = np.zeros(2 * capacity)
+        n_in = -1
+    else:
+        buf = np.ascontiguousarray(np.asarray(seeds, dtype=np.float64).ravel())
+        n_in = len(buf) // 2
+        capacity = n_in
+    check(lcbg_batch_debug_epoch(batch.handle, epoch, method, last_phase,
+                                 buf.ctypes.data_as(c_double_p),
+                                 capacity, n_in, ctypes.byref(n_out), ctypes.byref(mag)),
+          "lcbg_batch_debug_epoch")
+    n = n_out.value
+    out = buf[:2 * max(min(n, capacity), 0)].reshape(-1, 2).copy()
+    return out, mag.value

@@ -30,6 +30,7 @@ struct FsState {
     void* epoch_mem = nullptr;
     unsigned* counters = nullptr;
     int stage2_waves = 2048;
+    int64_t last_n = 0;  // epochs of the last run (their workspace is kept)
 };
 
 namespace {
@@ -108,12 +109,19 @@ ScratchPool g_pool[kMaxDevices];
 lcbg_status ensure_scratch(ScratchPool& pool)
 {
     if (pool.mem) return LCBG_OK;
-    const int slots = 512;
+    // k_fs_ir holds one wave per SIMD (its register use allows no more), so one
+    // slot per SIMD of the device: 4 SIMDs per CU
+    int cus = 0;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const int slots = 4 * cus;
     const std::size_t per_slot = carve_size(kSeedCap * sizeof(Pt2)) + carve_size(kSeedCap * sizeof(double)) +
                                  carve_size(kSeedCap * sizeof(int)) +
                                  carve_size(static_cast<std::size_t>(kRowCap) * 5 * sizeof(double)) +
-                                 carve_size(static_cast<std::size_t>(kRowCap) * 2 * sizeof(int)) +
-                                 carve_size(static_cast<std::size_t>(kPhiCap) * sizeof(int)) +
+                                 carve_size(static_cast<std::size_t>(kRowIntsPerSlot) * sizeof(int)) +
+                                 carve_size(static_cast<std::size_t>(kPhiHash) * sizeof(int)) +
                                  carve_size(static_cast<std::size_t>(kRunCap) * 4 * sizeof(int));
     const std::size_t bytes = per_slot * slots;
     lcbg_status st = map_hip(hipMalloc(&pool.mem, bytes));
@@ -128,16 +136,19 @@ lcbg_status ensure_scratch(ScratchPool& pool)
     s.areaimage = carve<double>(p, static_cast<std::size_t>(slots) * kSeedCap);
     s.overlap = carve<int>(p, static_cast<std::size_t>(slots) * kSeedCap);
     s.rows = carve<double>(p, static_cast<std::size_t>(slots) * kRowCap * 5);
-    s.row_key = carve<int>(p, static_cast<std::size_t>(slots) * kRowCap * 2);
-    s.phi_head = carve<int>(p, static_cast<std::size_t>(slots) * kPhiCap);
+    s.row_key = carve<int>(p, static_cast<std::size_t>(slots) * kRowIntsPerSlot);
+    s.phi_head = carve<int>(p, static_cast<std::size_t>(slots) * kPhiHash);
     s.runs = carve<int>(p, static_cast<std::size_t>(slots) * kRunCap * 4);
     st = map_hip(hipEventCreateWithFlags(&pool.last_use, hipEventDisableTiming));
     if (st != LCBG_OK) return st;
     // record once so that the first run's stream wait has a completed event
     st = map_hip(hipEventRecord(pool.last_use, nullptr));
     if (st != LCBG_OK) return st;
-    // empty visited-run lists (-1 heads); the kernel restores them per epoch
-    st = map_hip(hipMemset(s.phi_head, 0xff, static_cast<std::size_t>(slots) * kPhiCap * sizeof(int)));
+    // empty visited-run lists and row-key chains (-1 heads, dirty count -1 =
+    // nothing to clear); the kernel restores them per epoch and per seed
+    st = map_hip(hipMemset(s.phi_head, 0xff, static_cast<std::size_t>(slots) * kPhiHash * sizeof(int)));
+    if (st != LCBG_OK) return st;
+    st = map_hip(hipMemset(s.row_key, 0xff, static_cast<std::size_t>(slots) * kRowIntsPerSlot * sizeof(int)));
     if (st != LCBG_OK) return st;
     return map_hip(hipDeviceSynchronize());
 }
@@ -238,7 +249,9 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     FsState* fs = b->fs;
     st = ensure_epochs(fs, n_epochs > 0 ? n_epochs : 1);
     if (st != LCBG_OK) return st;
-    const FsEpochs ep = epoch_view(fs, n_epochs, offsets, times, mag, method);
+    FsEpochs ep = epoch_view(fs, n_epochs, offsets, times, mag, method);
+    ep.cost = b->d_cost;
+    fs->last_n = n_epochs;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if (stage_ms) {
         for (auto& e : ev) (void)hipEventCreate(&e);
@@ -324,6 +337,79 @@ extern "C" lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const
                             static_cast<hipStream_t>(stream), stage_ms);
     if (st != LCBG_OK) return st;
     return fs_error_flags(batch, error_flags, queue_counts);
+}
+
+extern "C" lcbg_status lcbg_batch_debug_epoch(lcbg_batch* batch, int64_t epoch, int32_t method,
+                                              int32_t last_phase, double* seeds, int32_t capacity,
+                                              int32_t n_in, int32_t* n_out, double* mag)
+{
+    if (!batch || !batch->fs || !batch->fs->epoch_mem || !seeds || !n_out || !mag)
+        return LCBG_INVALID_ARGUMENT;
+    FsState* fs = batch->fs;
+    if (epoch < 0 || epoch >= fs->last_n || capacity < 0 || n_in > capacity || n_in > kSeedCap)
+        return LCBG_INVALID_ARGUMENT;
+    if (method != LCBG_METHOD_INVERSE_RAY_CARTESIAN && method != LCBG_METHOD_INVERSE_RAY_POLAR)
+        return LCBG_INVALID_ARGUMENT;
+    ScratchPool& pool = g_pool[batch->device % kMaxDevices];
+    std::lock_guard<std::mutex> lock(pool.mu);
+    lcbg_status st = ensure_scratch(pool);
+    if (st != LCBG_OK) return st;
+    st = map_hip(hipDeviceSynchronize());
+    if (st != LCBG_OK) return st;
+    FsEpochs ep = epoch_view(fs, fs->last_n, nullptr, nullptr, nullptr, nullptr);
+    char* mem = nullptr;
+    const std::size_t seed_bytes = static_cast<std::size_t>(kSeedCap) * sizeof(Pt2);
+    st = map_hip(hipMalloc(reinterpret_cast<void**>(&mem), seed_bytes + 256));
+    if (st != LCBG_OK) return st;
+    Pt2* d_seeds = reinterpret_cast<Pt2*>(mem);
+    int* d_n = reinterpret_cast<int*>(mem + seed_bytes);
+    double* d_mag = reinterpret_cast<double*>(mem + seed_bytes + 64);
+    if (n_in > 0) st = map_hip(hipMemcpy(d_seeds, seeds, static_cast<std::size_t>(n_in) * sizeof(Pt2),
+                                         hipMemcpyHostToDevice));
+    if (st == LCBG_OK)
+        st = launch_ir_epoch_debug(batch->d_curves, fs->cache, fs->tab, fs->opt, ep, pool.scr, epoch,
+                                   method, last_phase, d_seeds, n_in, d_n, d_mag, nullptr);
+    if (st == LCBG_OK) st = map_hip(hipDeviceSynchronize());
+    int n = 0;
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(&n, d_n, sizeof(int), hipMemcpyDeviceToHost));
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(mag, d_mag, sizeof(double), hipMemcpyDeviceToHost));
+    if (st == LCBG_OK && n_in < 0 && n > 0) {
+        const int k = n < capacity ? n : capacity;
+        st = map_hip(hipMemcpy(seeds, d_seeds, static_cast<std::size_t>(k) * sizeof(Pt2),
+                               hipMemcpyDeviceToHost));
+    }
+    *n_out = n;
+    (void)hipFree(mem);
+    return st;
+}
+
+extern "C" lcbg_status lcbg_batch_debug_critical_sources(lcbg_batch* batch, int32_t curve,
+                                                         double* out)
+{
+    if (!batch || !out || curve < 0 || curve >= batch->n_curves) return LCBG_INVALID_ARGUMENT;
+    lcbg_status st = fs_prepare(batch);
+    if (st != LCBG_OK) return st;
+    st = map_hip(hipDeviceSynchronize());
+    if (st != LCBG_OK) return st;
+    const std::size_t n = static_cast<std::size_t>(kSeedPhases) * 4;
+    return map_hip(hipMemcpy(out, batch->fs->cache.crit + static_cast<std::size_t>(curve) * n,
+                             n * sizeof(Pt2), hipMemcpyDeviceToHost));
+}
+
+extern "C" lcbg_status lcbg_batch_set_epoch_cost(lcbg_batch* batch, int64_t* cost)
+{
+    if (!batch) return LCBG_INVALID_ARGUMENT;
+    batch->d_cost = cost;
+    return LCBG_OK;
+}
+
+extern "C" int64_t lcbg_wall_clock_rate_khz(void)
+{
+    int dev = 0;
+    int khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return -1;
+    return khz;
 }
 
 extern "C" lcbg_status lcbg_light_curve(const lcbg_curve* curve, const lcbg_options* options,

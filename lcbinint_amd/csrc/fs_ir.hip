@@ -34,6 +34,8 @@ struct Ctx {
     int n_seeds;
     bool overflow;
     long long steps;  // wave steps spent on this epoch
+    int last_phase;   // seed phases run: 0 images, 1 first crossing, 2 branches,
+                      // 3 boundary, 4 interior, 5 caustic segments (diagnostics)
 };
 
 // Counts one wave step; past the budget the epoch is abandoned with an error.
@@ -295,7 +297,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
         const int physical = (res[ord[3]] * 1.0e-4 > res[ord[2]] + 1.0e-12) ? 3 : 5;
         for (int i = 0; i < physical; ++i) push_seed(C, Pt{roots[ord[i]].re, roots[ord[i]].im});
     }
-    if (!(C.rho > 0.0)) return;
+    if (!(C.rho > 0.0) || C.last_phase < 1) return;
     const double phase_step = 2.0 * kPi / static_cast<double>(kSeedPhases);
 
     // Phase 1: strided scan (sample = j*40 + k; k outer, j inner), stopping at
@@ -364,9 +366,16 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
         caustic_probes(C, cs, tab);
     }
 
+    if (C.last_phase < 2) return;
     // Phase 2: branch-tracked scan, probes on outside->inside transitions and
-    // every 20th inside sample for large sources.
-    if (C.n_seeds >= 5) {
+    // every 20th inside sample for large sources.  A branch the greedy matching
+    // left unassigned (255) is outside (:1843-1846).  The reference starts the
+    // matching from positions at (1e30, 1e30) with a best distance of 1e60, and
+    // every squared distance to them is 2e60, so no branch is ever assigned and
+    // the phase adds nothing; the whole scan is skipped when phase 0 has no
+    // assignment, since the unmatched positions then never change.
+    const bool any_assigned = uni(static_cast<int>(assign[0] & assign[1] & assign[2] & assign[3]) != 255);
+    if (C.n_seeds >= 5 && any_assigned) {
         const bool arc = C.rho >= 2.0e-2;
         for (int base = 0; base < kOrd; base += kWave) {
             const int o = base + C.lane;
@@ -375,13 +384,18 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
                 const int sample = o >> 2;
                 const int bi = o & 3;
                 const int j = assign[4 * sample + bi];
-                const double d2 = dist2(crit[4 * sample + j], C.src);
-                const bool now = d2 < C.r2 && d2 > 0.0;
+                bool now = false;
+                if (j != 255) {
+                    const double d2 = dist2(crit[4 * sample + j], C.src);
+                    now = d2 < C.r2 && d2 > 0.0;
+                }
                 bool was = false;
                 if (sample > 0) {
                     const int jp = assign[4 * (sample - 1) + bi];
-                    const double dp = dist2(crit[4 * (sample - 1) + jp], C.src);
-                    was = dp < C.r2 && dp > 0.0;
+                    if (jp != 255) {
+                        const double dp = dist2(crit[4 * (sample - 1) + jp], C.src);
+                        was = dp < C.r2 && dp > 0.0;
+                    }
                 }
                 fire = (now && !was) || (arc && now && (sample % 20 == 0));
             }
@@ -397,6 +411,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
         }
     }
 
+    if (C.last_phase < 3) return;
     // boundary probes at 0.98 rho (:1565-1591), stopping at 128 seeds
     if (C.n_seeds < 128) {
         const double pr = C.rho * (1.0 - 0.02);
@@ -412,6 +427,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             for (int j = 0; j < cnt && C.n_seeds < 128; ++j) merge_lane(C, mine, j);
         }
     }
+    if (C.last_phase < 4) return;
     // interior probes (:1593-1620) for rho >= 0.02
     if (C.rho >= 2.0e-2) {
         const double fr[3] = {0.25, 0.5, 0.75};
@@ -423,6 +439,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
         }
     }
 
+    if (C.last_phase < 5) return;
     // Phase 3 (:4845-4926), only with fewer than five seeds: probe past the
     // nearest caustic segment; a longer image list replaces the seeds.
     if (C.n_seeds < 5 && sc.n >= 2) {
@@ -535,13 +552,33 @@ __device__ __forceinline__ double seq_coord(double x0, double step, int k)
     return x;
 }
 
+// The reference keeps, per seed, an unordered_map from row key to the rows
+// pushed with that key (finite_source_magnifier.cpp:917, 1989-1999).  Here the
+// map is a chained hash: head[key & (kRowHash-1)] -> newest row, next[] links.
+// `dirty` (in memory, so it survives an abandoned epoch) counts the keys whose
+// heads are set; they are cleared before the next seed's scan.
 struct Rows {
     double* r;     // [row*5 + field]
-    int* key;      // pushed row keys
-    int* idx;      // pushed row indices (stored after the key array)
+    int* key;      // [kRowCap] pushed row keys
+    int* idx;      // [kRowCap] pushed row indices
+    int* next;     // [kRowCap] older push with the same hash, -1 at the end
+    int* head;     // [kRowHash]
+    int* dirty;    // keys pushed since the heads were last cleared (<0: none)
     int size;      // rows allocated (ensure)
     int pushed;
 };
+
+__device__ __forceinline__ void rows_reset(Ctx& C, Rows& R)
+{
+    wave_sync();
+    const int nd = uni(*R.dirty);
+    for (int k = C.lane; k < nd; k += kWave) R.head[R.key[k] & (kRowHash - 1)] = -1;
+    wave_sync();
+    wave_store(R.dirty, 0);
+    wave_sync();
+    R.size = 0;
+    R.pushed = 0;
+}
 
 enum { XMIN = 0, XMAX = 1, AX = 2, YY = 3, DYS = 4 };
 
@@ -681,14 +718,11 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
             const double xmin_y = rget(R, yi, XMIN);
             const double xmax_y = rget(R, yi, XMAX);
             bool dup = false;
-            for (int base = 0; base < R.pushed && !dup; base += kWave) {
-                const int k = base + C.lane;
-                bool hit = false;
-                if (k < R.pushed && R.key[k] == key) {
-                    const int e = R.idx[k];
-                    hit = xmin_y + incr < rget(R, e, XMAX) && xmax_y - incr > rget(R, e, XMIN);
+            for (int p = uni(R.head[key & (kRowHash - 1)]); p >= 0 && !dup; p = uni(R.next[p])) {
+                if (uni(R.key[p]) == key) {
+                    const int e = uni(R.idx[p]);
+                    dup = xmin_y + incr < rget(R, e, XMAX) && xmax_y - incr > rget(R, e, XMIN);
                 }
-                dup = ballot(hit) != 0;
             }
             if (dup) return countall - countx;
         }
@@ -697,8 +731,15 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
             return __builtin_nan("");
         }
         wave_sync();
-        wave_store(&R.key[R.pushed], key);
-        wave_store(&R.idx[R.pushed], yi);
+        {
+            const int h = key & (kRowHash - 1);
+            const int older = uni(R.head[h]);
+            wave_store(&R.key[R.pushed], key);
+            wave_store(&R.idx[R.pushed], yi);
+            wave_store(&R.next[R.pushed], older);
+            wave_store(&R.head[h], R.pushed);
+            wave_store(R.dirty, R.pushed + 1);
+        }
         R.pushed = uni(R.pushed + 1);
         wave_sync();
         ++yi;
@@ -741,8 +782,10 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* 
         R.r = rows_mem;
         R.key = rows_key;
         R.idx = rows_key + kRowCap;
-        R.size = 0;
-        R.pushed = 0;
+        R.next = rows_key + 2 * kRowCap;
+        R.head = rows_key + 3 * kRowCap;
+        R.dirty = rows_key + 3 * kRowCap + kRowHash;
+        rows_reset(C, R);
         if (!rows_ensure(C, R, 1)) return __builtin_nan("");
         double area0 = 0.0, areai = 0.0;
         double dy = incr;
@@ -875,10 +918,13 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* 
 
 // ---------------------------------------------------------------- polar fill
 
+// Visited radial runs, the reference's per-row interval lists
+// (PolarVisitedCellIntervals, finite_source_magnifier.cpp:909) as chains hashed
+// on the phi row: head[iphi & (kPhiHash-1)] -> newest run, runs[4r+3] -> older.
 struct Polar {
     double dr, dphi;
     int phi_bins;
-    int* head;   // [iphi] -> first run index, -1 if none
+    int* head;   // [kPhiHash] -> newest run with that hash, -1 if none
     int* runs;   // [run*4]: iphi, left, right, next
     int n_runs;
 };
@@ -886,8 +932,8 @@ struct Polar {
 __device__ __forceinline__ bool polar_visited(const Polar& P, int ir, int iphi)
 {
     if (ir < 0) return true;
-    for (int r = P.head[iphi]; r >= 0; r = P.runs[4 * r + 3]) {
-        if (ir >= P.runs[4 * r + 1] && ir <= P.runs[4 * r + 2]) return true;
+    for (int r = P.head[iphi & (kPhiHash - 1)]; r >= 0; r = P.runs[4 * r + 3]) {
+        if (P.runs[4 * r] == iphi && ir >= P.runs[4 * r + 1] && ir <= P.runs[4 * r + 2]) return true;
     }
     return false;
 }
@@ -951,7 +997,7 @@ __device__ double polar_add_run(Ctx& C, Polar& P, int ir, int iphi, double cph, 
         C.overflow = true;
         return 0.0;
     }
-    const int old_head = P.head[iphi];
+    const int old_head = uni(P.head[iphi & (kPhiHash - 1)]);
     wave_sync();
     {
         const int r = P.n_runs;
@@ -959,7 +1005,7 @@ __device__ double polar_add_run(Ctx& C, Polar& P, int ir, int iphi, double cph, 
         wave_store(&P.runs[4 * r + 1], left);
         wave_store(&P.runs[4 * r + 2], right);
         wave_store(&P.runs[4 * r + 3], old_head);
-        wave_store(&P.head[iphi], r);
+        wave_store(&P.head[iphi & (kPhiHash - 1)], r);
     }
     P.n_runs = uni(P.n_runs + 1);
     wave_sync();
@@ -995,12 +1041,13 @@ __device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* hea
         const Pt s = C.seeds[i];
         max_r = dmax(max_r, hypot_glibc(s.x, s.y) + 4.0 * C.rho);
     }
-    const int pb = static_cast<int>(ceil(2.0 * kPi * max_r / (P.dr * grid_ratio)));
-    P.phi_bins = pb > 16 ? pb : 16;
-    if (P.phi_bins > kPhiCap) {
+    const double pbd = ceil(2.0 * kPi * max_r / (P.dr * grid_ratio));
+    if (!(pbd <= static_cast<double>(kPhiCap))) {
         C.overflow = true;
         return __builtin_nan("");
     }
+    const int pb = static_cast<int>(pbd);
+    P.phi_bins = pb > 16 ? pb : 16;
     P.dphi = 2.0 * kPi / static_cast<double>(P.phi_bins);
     const double flux = source_flux(C.rho, C.ld_c, C.ld_d);
     if (!isfinite(flux)) return __builtin_nan("");
@@ -1097,7 +1144,7 @@ __device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* hea
     }
     // reset the rows this epoch touched
     wave_sync();
-    for (int q = C.lane; q < P.n_runs; q += kWave) head[runs[4 * q]] = -1;
+    for (int q = C.lane; q < P.n_runs; q += kWave) head[runs[4 * q] & (kPhiHash - 1)] = -1;
     wave_sync();
     if (C.overflow) return __builtin_nan("");
     const double image_flux = total * P.dr * P.dphi;
@@ -1106,67 +1153,145 @@ __device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* hea
 
 // ------------------------------------------------------------------- kernel
 
+// Slot scratch of one resident wave.
+struct SlotMem {
+    Pt* seeds;
+    double* areaimage;
+    int* flags;
+    double* rows;
+    int* row_key;
+    int* head;
+    int* runs;
+};
+
+__device__ __forceinline__ SlotMem slot_mem(const IrScratch& scr, int slot)
+{
+    SlotMem M;
+    M.seeds = scr.seeds + static_cast<size_t>(slot) * kSeedCap;
+    M.areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
+    M.flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
+    M.rows = scr.rows + static_cast<size_t>(slot) * kRowCap * 5;
+    M.row_key = scr.row_key + static_cast<size_t>(slot) * kRowIntsPerSlot;
+    M.head = scr.phi_head + static_cast<size_t>(slot) * kPhiHash;
+    M.runs = scr.runs + static_cast<size_t>(slot) * kRunCap * 4;
+    return M;
+}
+
+__device__ __forceinline__ void init_ctx(Ctx& C, const CurveRec& cv, Pt src, Pt* seeds)
+{
+    C.s = cv.s;
+    C.q = cv.q_eff;
+    C.L = make_lens(cv.s, cv.q_eff);
+    C.src = src;
+    C.rho = cv.rho;
+    C.r2 = cv.rho * cv.rho;
+    C.inv_r2 = 1.0 / C.r2;
+    C.ld_c = cv.ld_c;
+    C.ld_d = cv.ld_d;
+    C.uniform = cv.ld_c == 0.0 && cv.ld_d == 0.0;
+    C.lane = lane_id();
+    C.seeds = seeds;
+    C.n_seeds = 0;
+    C.overflow = false;
+    C.steps = 0;
+    C.last_phase = 5;
+}
+
+// Seeds of epoch e into C (fixed_inverse_ray_binary's seed step,
+// finite_source_magnifier.cpp:3042-3100).
+__device__ __forceinline__ void epoch_seeds(Ctx& C, const FsCache& cache, const FsTables& tab,
+                                            const FsEpochs& ep, int c, int64_t e)
+{
+    const int bins = cache.bins;
+    const CausticScalars sc = cache.sc[c];
+    image_seeds(C, cache.crit + static_cast<size_t>(c) * kSeedPhases * 4,
+                cache.assign + static_cast<size_t>(c) * kSeedPhases * 4,
+                cache.br + static_cast<size_t>(c) * bins * 4,
+                cache.br_cell + static_cast<size_t>(c) * bins * 4, sc, bins, tab,
+                ep.roots + 10 * e);
+}
+
+// The inverse-ray magnification on the seeds in C: polar flood fill or the
+// Cartesian row scan, at the curve's resolution.
+__device__ __forceinline__ double epoch_area(Ctx& C, const CurveRec& cv, const FsOptions& opt,
+                                             int method, const SlotMem& M)
+{
+    if (C.overflow) return __builtin_nan("");
+    const int nbin = cv.source_bins > 0 ? cv.source_bins : opt.source_bins;
+    if (method == LCBG_METHOD_INVERSE_RAY_POLAR) {
+        const int pbins = opt.polar_source_bins > 0 ? opt.polar_source_bins : nbin;
+        const int psb = pbins > 1 ? pbins : 1;
+        double ratio = opt.polar_grid_ratio > 0.0 ? opt.polar_grid_ratio : opt.grid_ratio;
+        ratio = dmax(ratio, 1.0e-12);
+        if (opt.mode == 4) ratio = dmax(ratio, 12.0);
+        return polar_mag(C, psb, ratio, M.head, M.runs);
+    }
+    return cartesian_mag(C, nbin, M.rows, M.row_key, M.areaimage, M.flags);
+}
+
 __global__ void __launch_bounds__(kWave) k_fs_ir(const CurveRec* __restrict__ curves, FsCache cache,
                                                   FsTables tab, FsOptions opt, FsEpochs ep,
                                                   IrScratch scr)
 {
-    const int lane = lane_id();
-    const int slot = blockIdx.x;
-    Pt* seeds = scr.seeds + static_cast<size_t>(slot) * kSeedCap;
-    double* areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
-    int* flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
-    double* rows = scr.rows + static_cast<size_t>(slot) * kRowCap * 5;
-    int* row_key = scr.row_key + static_cast<size_t>(slot) * kRowCap * 2;
-    int* head = scr.phi_head + static_cast<size_t>(slot) * kPhiCap;
-    int* runs = scr.runs + static_cast<size_t>(slot) * kRunCap * 4;
+    const SlotMem M = slot_mem(scr, blockIdx.x);
     const int count = uni(static_cast<int>(ep.counters[2]));
     for (;;) {
         const int item = next_item(&ep.counters[3]);
         if (item >= count) break;
         const int64_t e = ep.ir_list[item];
+        const long long t_begin = static_cast<long long>(wall_clock64());
         const int c = ep.curve_of[e];
         const CurveRec cv = curves[c];
-        const int bins = cache.bins;
         Ctx C;
-        C.s = cv.s;
-        C.q = cv.q_eff;
-        C.L = make_lens(cv.s, cv.q_eff);
-        C.src = ep.src[e];
-        C.rho = cv.rho;
-        C.r2 = cv.rho * cv.rho;
-        C.inv_r2 = 1.0 / C.r2;
-        C.ld_c = cv.ld_c;
-        C.ld_d = cv.ld_d;
-        C.uniform = cv.ld_c == 0.0 && cv.ld_d == 0.0;
-        C.lane = lane;
-        C.seeds = seeds;
-        C.n_seeds = 0;
-        C.overflow = false;
-        C.steps = 0;
-        const CausticScalars sc = cache.sc[c];
-        image_seeds(C, cache.crit + static_cast<size_t>(c) * kSeedPhases * 4,
-                    cache.assign + static_cast<size_t>(c) * kSeedPhases * 4,
-                    cache.br + static_cast<size_t>(c) * bins * 4,
-                    cache.br_cell + static_cast<size_t>(c) * bins * 4, sc, bins, tab,
-                    ep.roots + 10 * e);
-        double mag = __builtin_nan("");
-        const int method = ep.method[e];
-        if (!C.overflow) {
-            if (method == LCBG_METHOD_INVERSE_RAY_POLAR) {
-                const int pbins = opt.polar_source_bins > 0 ? opt.polar_source_bins : opt.source_bins;
-                const int psb = pbins > 1 ? pbins : 1;
-                double ratio = opt.polar_grid_ratio > 0.0 ? opt.polar_grid_ratio : opt.grid_ratio;
-                ratio = dmax(ratio, 1.0e-12);
-                if (opt.mode == 4) ratio = dmax(ratio, 12.0);
-                mag = polar_mag(C, psb, ratio, head, runs);
-            } else {
-                mag = cartesian_mag(C, opt.source_bins, rows, row_key, areaimage, flags);
-            }
-        }
+        init_ctx(C, cv, ep.src[e], M.seeds);
+        epoch_seeds(C, cache, tab, ep, c, e);
+        const double mag = epoch_area(C, cv, opt, ep.method[e], M);
         wave_store(&ep.mag[e], mag);
+        if (ep.cost) wave_store(&ep.cost[e], static_cast<int64_t>(wall_clock64()) - t_begin);
         if (C.overflow) atomicOr(&ep.counters[4], kErrSeedCap);
         wave_sync();
     }
+}
+
+// One epoch of the last run, on slot 0: with n_in < 0 the seeds are searched
+// and written to seeds_io (*n_out = their count, up to kSeedCap); otherwise the
+// n_in seeds of seeds_io are used.  *mag_out receives the inverse-ray
+// magnification with method `method`.
+__global__ void __launch_bounds__(kWave) k_ir_epoch_debug(const CurveRec* __restrict__ curves,
+                                                          FsCache cache, FsTables tab,
+                                                          FsOptions opt, FsEpochs ep,
+                                                          IrScratch scr, int64_t e, int method,
+                                                          int last_phase, Pt* seeds_io, int n_in,
+                                                          int* n_out, double* mag_out)
+{
+    const SlotMem M = slot_mem(scr, 0);
+    const int c = ep.curve_of[e];
+    const CurveRec cv = curves[c];
+    Ctx C;
+    init_ctx(C, cv, ep.src[e], M.seeds);
+    if (last_phase >= 0) C.last_phase = last_phase;
+    if (n_in < 0) {
+        epoch_seeds(C, cache, tab, ep, c, e);
+        wave_sync();
+        for (int k = C.lane; k < C.n_seeds; k += kWave) seeds_io[k] = M.seeds[k];
+    } else {
+        for (int k = C.lane; k < n_in; k += kWave) M.seeds[k] = seeds_io[k];
+        C.n_seeds = n_in;
+    }
+    wave_sync();
+    wave_store(n_out, C.overflow ? -1 : C.n_seeds);
+    const double mag = epoch_area(C, cv, opt, method, M);
+    wave_store(mag_out, mag);
+}
+
+lcbg_status launch_ir_epoch_debug(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
+                                  const FsOptions& opt, const FsEpochs& ep, const IrScratch& scr,
+                                  int64_t e, int method, int last_phase, Pt2* seeds_io, int n_in,
+                                  int* n_out, double* mag_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ir_epoch_debug, dim3(1), dim3(kWave), 0, st, d_curves, cache, tab, opt,
+                       ep, scr, e, method, last_phase, seeds_io, n_in, n_out, mag_out);
+    return check_launch();
 }
 
 lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,

@@ -28,6 +28,13 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
                          const FsOptions& opt, const FsEpochs& ep, const IrScratch& scr,
                          hipStream_t st);
 
+// One inverse-ray epoch of the last run on slot 0 (diagnostics): seed search
+// (n_in < 0) or given seeds, then the magnification with `method`.
+lcbg_status launch_ir_epoch_debug(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
+                                  const FsOptions& opt, const FsEpochs& ep, const IrScratch& scr,
+                                  int64_t e, int method, int last_phase, Pt2* seeds_io, int n_in,
+                                  int* n_out, double* mag_out, hipStream_t st);
+
 // Zeroes the queue counters in-stream with a kernel (graph-capturable).
 lcbg_status launch_reset_counters(unsigned* counters, int n, hipStream_t st);
 

@@ -26,10 +26,21 @@ constexpr int kAngularProbes = 16;      // finite_source_magnifier.cpp:1538
 constexpr int kInteriorProbes = 64;     // finite_source_magnifier.cpp:1604
 
 // Capacities of the per-slot scratch; exceeding one marks the epoch unsupported.
+// Rows: a Cartesian epoch pushes one record per scanned row of every seed; a
+// long arc near the Einstein ring at nbin 100 and rho 0.03 scans ~1e4 rows per
+// seed.  Phi rows: the polar lattice has ceil(2 pi R / (dr g)) rows, about
+// 2.1e6 at rho 1e-4, nbin 100, g 12 and R 4.
+// Row keys and phi rows are found through hashed chains (kRowHash, kPhiHash
+// heads), so no dense per-row array is needed.
 constexpr int kSeedCap = 1024;
-constexpr int kRowCap = 16384;
-constexpr int kPhiCap = 1 << 19;
-constexpr int kRunCap = 1 << 17;
+constexpr int kRowCap = 1 << 17;
+constexpr int kRowHash = 1 << 16;
+constexpr int kPhiCap = 1 << 30;
+constexpr int kPhiHash = 1 << 16;
+constexpr int kRunCap = 1 << 18;
+// ints per slot of IrScratch::row_key: key, idx, next per row, the hash heads,
+// and the count of keys whose heads are still set (padded to 64)
+constexpr int kRowIntsPerSlot = 3 * kRowCap + kRowHash + 64;
 
 struct CausticScalars {
     double min_x, max_x, min_y, max_y, step_x, step_y, max_seg_len;
@@ -82,6 +93,7 @@ struct FsEpochs {
     int64_t* near_list;
     int64_t* ir_list;
     unsigned* counters; // [0] near count, [1] near cursor, [2] ir count, [3] ir cursor, [4] error flags
+    int64_t* cost;      // [epoch] wall-clock ticks of each inverse-ray epoch, or null
 };
 
 struct IrScratch {
@@ -90,8 +102,8 @@ struct IrScratch {
     double* areaimage; // [slot][kSeedCap]
     int* overlap;      // [slot][kSeedCap]
     double* rows;      // [slot][kRowCap*5]: xmin, xmax, ax, y, dys
-    int* row_key;      // [slot][kRowCap]
-    int* phi_head;     // [slot][kPhiCap]
+    int* row_key;      // [slot][kRowIntsPerSlot]: key, idx, next, heads, dirty count
+    int* phi_head;     // [slot][kPhiHash]: hashed run-chain heads
     int* runs;         // [slot][kRunCap*4]: iphi, left, right, next
 };
 

@@ -29,6 +29,8 @@ CurveRec make_curve_rec(const lcbg_curve& cv)
     r.rho = std::abs(cv.rho);
     r.ld_c = cv.ld_c;
     r.ld_d = cv.ld_d;
+    r.source_bins = 0;
+    r.pad = 0;
     return r;
 }
 
@@ -115,7 +117,8 @@ extern "C" lcbg_status lcbg_batch_create(const lcbg_curve* curves, int32_t n_cur
     b->options = options ? *options : lcbg_default_options();
     lcbg_status st = map_hip(hipGetDevice(&b->device));
     if (st == LCBG_OK && n_curves > 0) {
-        std::vector<CurveRec> recs(static_cast<std::size_t>(n_curves));
+        std::vector<CurveRec>& recs = b->h_curves;
+        recs.resize(static_cast<std::size_t>(n_curves));
         for (int32_t c = 0; c < n_curves; ++c) recs[static_cast<std::size_t>(c)] = make_curve_rec(curves[c]);
         st = map_hip(hipMalloc(reinterpret_cast<void**>(&b->d_curves), recs.size() * sizeof(CurveRec)));
         if (st == LCBG_OK)
@@ -137,4 +140,17 @@ extern "C" lcbg_status lcbg_batch_destroy(lcbg_batch* batch)
     if (batch->d_curves) (void)hipFree(batch->d_curves);
     delete batch;
     return LCBG_OK;
+}
+
+extern "C" lcbg_status lcbg_batch_set_source_bins(lcbg_batch* batch, const int32_t* source_bins)
+{
+    if (!batch || (batch->n_curves > 0 && !source_bins)) return LCBG_INVALID_ARGUMENT;
+    for (int32_t c = 0; c < batch->n_curves; ++c) {
+        if (source_bins[c] < 0) return LCBG_INVALID_ARGUMENT;
+    }
+    for (int32_t c = 0; c < batch->n_curves; ++c)
+        batch->h_curves[static_cast<std::size_t>(c)].source_bins = source_bins[c];
+    if (batch->n_curves == 0) return LCBG_OK;
+    return map_hip(hipMemcpy(batch->d_curves, batch->h_curves.data(),
+                             batch->h_curves.size() * sizeof(CurveRec), hipMemcpyHostToDevice));
 }

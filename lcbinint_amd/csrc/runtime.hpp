@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "lcbinint_amd.h"
 
@@ -15,9 +16,12 @@ namespace lcbg {
 // the host libm, as the reference's std::cos/std::sin
 // (python/lcbinint_pybind.cpp:1122-1123) are, because the device libm rounds a
 // small fraction of arguments differently.  q_eff is the VBM flip 1/q
-// (python/lcbinint_pybind.cpp:1125).
+// (python/lcbinint_pybind.cpp:1125).  source_bins is the curve's own inverse-ray
+// lattice resolution (0: the batch option).
 struct CurveRec {
     double t0, inverse_tE, u0, cos_alpha, sin_alpha, s, q_eff, rho, ld_c, ld_d;
+    int source_bins;
+    int pad;
 };
 
 CurveRec make_curve_rec(const lcbg_curve& cv);
@@ -46,5 +50,7 @@ struct lcbg_batch {
     int32_t n_curves = 0;
     lcbg_options options{};
     lcbg::CurveRec* d_curves = nullptr;  // n_curves records
+    std::vector<lcbg::CurveRec> h_curves; // host copy of the same records
     lcbg::FsState* fs = nullptr;         // finite-source state, built on first use
+    int64_t* d_cost = nullptr;           // optional per-epoch inverse-ray cost output (device)
 };
