@@ -1,16 +1,27 @@
 """Throughput benchmark (one JSON line on stdout).
 
-Workload (default): the point-source batch - 4096 binary-lens parameter sets x
-256 epochs, one quintic solve per epoch (1,048,576 solves), FP64, synthetic
-inputs with a fixed seed (lcbinint_amd.workloads.point_source_batch).  A step is
-one launch of lcbg_batch_point_source_device over every epoch, with the epoch
-times already resident in HBM.
+Headline workload: the batched finite-source light curves - 2048 binary-lens
+parameter sets with per-curve epoch counts floor(log-uniform[50, 2001)) (1,068,351
+epochs), rho log-uniform [1e-4, 3e-2], linear limb darkening u uniform
+[0.1, 0.9], nbin uniform in [25, 100] per curve, automatic Cartesian/polar
+choice, FP64, synthetic inputs with a fixed seed
+(lcbinint_amd.workloads.finite_source_batch).  One step re-evaluates every
+light curve from its parameters: the curve records are uploaded again (which
+marks the per-curve caustic caches stale), then one finite-source run rebuilds
+the caches and runs point source, dispatcher, hexadecapole and inverse-ray
+shooting over all epochs.  Epoch times are resident in HBM.
+
+Also reported: the plain GPU path on the single README curve (200 epochs,
+nbin 50), called once per curve and once per epoch through the host-pointer
+entry point; the 4096 x 256 point-source batch; the reference core compiled
+from source (oracle/_ref) timed on the host cores on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -23,110 +34,222 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "LD finite-source mags/s and value+grad/s, batched vs plain GPU, 1e-4 rel err"
-FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (vendor datasheet)
-# Algorithmic FP64 operations per point-source solve (+,-,*,/,sqrt = 1 each),
-# counted by the oracle's operation counter over the full point-source batch
-# (oracle/count_flops.py).
-FLOPS_PER_PS_SOLVE = 3158.2
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (vendor datasheet)
+
+# Algorithmic FP64 operations (+ - * / sqrt = 1 each) per unit of work.
+# Point-source solve: mean over the 4096 x 256 point-source batch counted by the
+# oracle's operation counter (oracle/count_flops.py).  Critical-curve solve
+# (quartic + lens equation of its 4 roots): mean over 1400 phases of 128 curves
+# of the batched workload, same counter.  Lattice cells: the lens map, distance
+# and limb-darkening weight as written in finite_source_magnifier.cpp:65-80,
+# 121-153, 1241-1311 (Cartesian 30, Jacobian-guarded Cartesian 53, polar 37).
+FLOPS_PS_SOLVE = 3158.2
+FLOPS_CRIT_SOLVE = 1694.6
+FLOPS_CART_CELL = 30.0
+FLOPS_CART_GUARDED_CELL = 53.0
+FLOPS_POLAR_CELL = 37.0
+
+# HBM bytes per k_fs_ir launch on the headline workload, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (profiles/k_fs_ir_batched_fs_pmc.txt), each scaled
+# by the same counter's reading for a device copy of known size; None until
+# measured.
+K_FS_IR_TRAFFIC_BYTES = None
 
 
-def _cpu_baseline_worker(args):
-    lib_path, kind, s, q, x, y = args
-    from oracle.bindings import load_oracle, load_ref
-    lib = load_ref(lib_path) if kind == "reference" else load_oracle(lib_path)
+def _cpu_worker(args):
+    lib_path, items = args
+    from oracle.bindings import default_settings, load_ref
+    ref = load_ref(lib_path)
     t0 = time.perf_counter()
-    lib.binary_mag0_many(s, q, x, y)
-    return time.perf_counter() - t0, len(s)
+    n = 0
+    for cv, nb, tt in items:
+        st = default_settings(source_bins=int(nb), limb_darkening_c=cv["ld_c"])
+        _, _, status = ref.light_curve(st, tt, cv["t0"], cv["tE"], cv["u0"], cv["alpha"], cv["s"],
+                                       cv["q"], cv["rho"])
+        if status != 0:
+            raise RuntimeError(f"reference light curve failed with status {status}")
+        n += len(tt)
+    return time.perf_counter() - t0, n
 
 
-def cpu_baseline(curves, offsets, times, max_solves=1 << 20):
-    """Times the reference core (oracle/_ref) on the host cores over the same
-    epochs, one process per core (the reference solver keeps static state)."""
+def cpu_baseline(curves, offsets, times, nbin, stride=8):
+    """The reference core (oracle/_ref, compiled from its own sources) on every
+    `stride`-th curve of the same workload, one process per core (the
+    reference solver keeps static state, so it is not run threaded)."""
     from multiprocessing import get_context
 
     ref = ROOT / "oracle" / "_ref" / "liblcbinint_ref.so"
-    port = ROOT / "oracle" / "liblcb_oracle.so"
-    if ref.exists():
-        lib_path, kind = ref, "reference"
-    elif port.exists():
-        lib_path, kind = port, "port"
-    else:
+    if not ref.exists():
         return None
-    n = min(len(times), max_solves)
-    s = np.empty(n)
-    q = np.empty(n)
-    x = np.empty(n)
-    y = np.empty(n)
-    for c, cv in enumerate(curves):
-        lo, hi = int(offsets[c]), min(int(offsets[c + 1]), n)
-        if lo >= n:
-            break
-        tau = (times[lo:hi] - cv["t0"]) * (1.0 / cv["tE"])
-        ca, sa = np.cos(cv["alpha"]), np.sin(cv["alpha"])
-        x[lo:hi] = tau * ca - cv["u0"] * sa
-        y[lo:hi] = tau * sa + cv["u0"] * ca
-        s[lo:hi] = cv["s"]
-        q[lo:hi] = 1.0 / cv["q"]
+    idx = np.arange(0, len(curves), stride)
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
-    chunks = np.array_split(np.arange(n), cores)
-    work = [(str(lib_path), kind, s[c], q[c], x[c], y[c]) for c in chunks]
-    t0 = time.perf_counter()
+    # longest-first round robin so the processes finish together
+    counts = np.diff(offsets)[idx]
+    order = idx[np.argsort(-counts, kind="stable")]
+    parts = [[] for _ in range(cores)]
+    for k, c in enumerate(order):
+        parts[k % cores].append((curves[c], int(nbin[c]), times[offsets[c]:offsets[c + 1]]))
     with get_context("spawn").Pool(cores) as pool:
-        res = pool.map(_cpu_baseline_worker, work)
-    wall = time.perf_counter() - t0
+        res = pool.map(_cpu_worker, [(str(ref), p) for p in parts])
     busy = max(r[0] for r in res)
+    n = sum(r[1] for r in res)
     return {
         "value": n / busy,
         "unit": "mags/s",
         "cores": cores,
-        "kind": kind,
-        "sample": f"{n} point-source solves of the same batch split over {cores} processes "
-                  f"(slowest process {busy:.2f} s, wall {wall:.2f} s incl. process start)",
+        "kind": "reference",
+        "sample": f"every {stride}th curve of the batched workload ({len(idx)} curves, {n} epochs, "
+                  f"{sum(r[0] for r in res):.1f} s of CPU work) split over {cores} processes; "
+                  f"slowest process {busy:.2f} s",
     }
+
+
+def bench_batched(capi, torch, steps, warmup):
+    from lcbinint_amd.workloads import finite_source_batch
+    curves, offsets, times, nbin = finite_source_batch()
+    n_epochs = len(times)
+    recs = (capi.Curve * len(curves))(*[capi.Curve(**c) for c in curves])
+    batch = capi.Batch(list(recs))
+    batch.set_source_bins(nbin)
+    d_off = torch.from_numpy(offsets).cuda()
+    d_t = torch.from_numpy(times).cuda()
+    d_m = torch.empty_like(d_t)
+    d_k = torch.empty(n_epochs, dtype=torch.int32, device="cuda")
+    sptr = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        capi.check(capi.lcbg_batch_update_curves(batch.handle, recs, sptr), "update_curves")
+        return capi.finite_source(batch, d_off.data_ptr(), d_t.data_ptr(), n_epochs,
+                                  d_m.data_ptr(), d_k.data_ptr(), sptr, profile=True)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    stages = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ms, qc, flags = step()
+        if flags:
+            raise RuntimeError(f"finite-source run reported capacity flags {flags}")
+        stages.append(ms)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    work = capi.work_counts(batch)
+    meth = np.bincount(d_k.cpu().numpy(), minlength=4)
+    finite = bool(torch.isfinite(d_m).all().item())
+    batch.close()
+    if not finite:
+        raise RuntimeError("non-finite magnification in the batched run")
+    stage_ms = np.mean(np.array(stages), axis=0)
+    return dict(n_curves=len(curves), n_epochs=n_epochs, wall=wall, steps=steps,
+                stage_ms=stage_ms.tolist(), queue=qc, work=work, methods=meth.tolist())
+
+
+def bench_plain(capi, calls_per_epoch=50):
+    """The unbatched drop-in path on the README curve: lcbg_light_curve with
+    host arrays, once for the whole curve and once per epoch."""
+    from lcbinint_amd.workloads import plain_curve
+    cv, t = plain_curve()
+    rec = capi.Curve(**cv)
+    opt = capi.lcbg_default_options()
+    mag = np.empty_like(t)
+    meth = np.empty(len(t), dtype=np.int32)
+    P = capi.c_double_p
+
+    def run(tt, out, mt):
+        capi.check(capi.lcbg_light_curve(ctypes.byref(rec), ctypes.byref(opt),
+                                         tt.ctypes.data_as(P), len(tt), out.ctypes.data_as(P),
+                                         mt.ctypes.data_as(capi.c_int32_p)), "lcbg_light_curve")
+
+    run(t, mag, meth)  # warm-up
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run(t, mag, meth)
+    per_curve = reps * len(t) / (time.perf_counter() - t0)
+    idx = np.linspace(0, len(t) - 1, calls_per_epoch).astype(int)
+    one = np.empty(1)
+    one_m = np.empty(1, dtype=np.int32)
+    t0 = time.perf_counter()
+    for i in idx:
+        run(t[i:i + 1], one, one_m)
+    per_epoch = len(idx) / (time.perf_counter() - t0)
+    return per_curve, per_epoch, np.bincount(meth, minlength=4).tolist()
+
+
+def bench_point_source(capi, torch, steps=10):
+    from lcbinint_amd.workloads import point_source_batch
+    curves, offsets, times = point_source_batch()
+    batch = capi.Batch([capi.Curve(**c) for c in curves])
+    d_off = torch.from_numpy(offsets).cuda()
+    d_t = torch.from_numpy(times).cuda()
+    d_m = torch.empty_like(d_t)
+    sptr = torch.cuda.current_stream().cuda_stream
+    batch.point_source(d_off.data_ptr(), d_t.data_ptr(), d_m.data_ptr(), sptr)
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.point_source(d_off.data_ptr(), d_t.data_ptr(), d_m.data_ptr(), sptr)
+        kms.append(capi.lcbg_last_kernel_ms())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    batch.close()
+    kernel_ms = float(np.mean(kms))
+    tflops = FLOPS_PS_SOLVE * len(times) / (kernel_ms * 1e-3) / 1e12
+    return {"value": len(times) * steps / wall, "unit": "mags/s", "epochs": len(times),
+            "kernel": "k_light_curves_ps", "kernel_ms": kernel_ms, "achieved_tflops": tflops,
+            "frac": tflops / FP64_PEAK_TFLOPS}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the plain-path and point-source measurements")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the batch shards over "
-              "curves with no exchange, so N GPUs run N independent shares.", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for {args.gpus}): light curves are independent, "
+              "so N GPUs would each run their own share of the curves with no exchange.",
+              file=sys.stderr)
         return 2
 
     import torch
 
     from lcbinint_amd import capi
-    from lcbinint_amd.workloads import point_source_batch
 
     torch.cuda.set_device(0)
-    curves, offsets, times = point_source_batch()
-    n_epochs = len(times)
-    batch = capi.Batch([capi.Curve(**c) for c in curves])
-    d_off = torch.from_numpy(offsets).cuda()
-    d_t = torch.from_numpy(times).cuda()
-    d_m = torch.empty_like(d_t)
-    stream = torch.cuda.current_stream()
-    sptr = stream.cuda_stream
-
-    for _ in range(args.warmup):
-        batch.point_source(d_off.data_ptr(), d_t.data_ptr(), d_m.data_ptr(), sptr)
-    torch.cuda.synchronize()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.point_source(d_off.data_ptr(), d_t.data_ptr(), d_m.data_ptr(), sptr)
-        kernel_ms.append(capi.lcbg_last_kernel_ms())
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    batch.close()
-    ms_per_step = 1e3 * wall / args.steps
-    value = n_epochs / (wall / args.steps)
-    avg_kernel_ms = float(np.mean(kernel_ms))
-    achieved = FLOPS_PER_PS_SOLVE * n_epochs / (avg_kernel_ms * 1e-3) / 1e12
+    print("[bench] batched finite-source workload", file=sys.stderr, flush=True)
+    b = bench_batched(capi, torch, args.steps, args.warmup)
+    value = b["n_epochs"] * b["steps"] / b["wall"]
+    w = b["work"]
+    ir_flops = (w["quintic"] * FLOPS_PS_SOLVE + w["quartic"] * FLOPS_CRIT_SOLVE +
+                w["cart"] * FLOPS_CART_CELL + w["cart_guarded"] * FLOPS_CART_GUARDED_CELL +
+                w["polar"] * FLOPS_POLAR_CELL)
+    ir_ms = b["stage_ms"][3]
+    achieved = ir_flops / (ir_ms * 1e-3) / 1e12
+    extras = {}
+    if not args.no_extras:
+        print("[bench] plain path and point-source batch", file=sys.stderr, flush=True)
+        per_curve, per_epoch, plain_meth = bench_plain(capi)
+        extras["plain_gpu"] = {
+            "workload": "README curve: t0=0 tE=1 u0=-0.01 alpha=0.5 s=0.95 q=0.01 rho=5e-3, "
+                        "linear LD 0.5, 200 epochs, nbin 50, auto grid",
+            "per_curve_call_mags_per_s": per_curve, "per_epoch_call_mags_per_s": per_epoch,
+            "methods": plain_meth,
+            "batched_over_per_epoch": value / per_epoch,
+            "batched_over_per_curve": value / per_curve,
+        }
+        extras["point_source_batch"] = bench_point_source(capi, torch)
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] reference CPU baseline", file=sys.stderr, flush=True)
+        from lcbinint_amd.workloads import finite_source_batch
+        curves, offsets, times, nbin = finite_source_batch()
+        cpu = cpu_baseline(curves, offsets, times, nbin)
     out = {
         "metric": METRIC,
         "value": value,
@@ -134,30 +257,39 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * b["wall"] / b["steps"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "point-source batch: 4096 binary-lens parameter sets x 256 epochs, "
-                        "one quintic solve per epoch (1,048,576 solves), seed 2",
-            "curves": len(curves),
-            "epochs": n_epochs,
-            "kernel": "k_light_curves_ps",
+            "workload": "batched finite-source light curves: 2048 curves, 1,068,351 epochs "
+                        "(per-curve counts log-uniform in [50, 2000]), rho log-uniform "
+                        "[1e-4, 3e-2], linear LD u in [0.1, 0.9], nbin 25-100 per curve, auto "
+                        "Cartesian/polar, seed 3; caustic caches rebuilt every step",
+            "curves": b["n_curves"],
+            "epochs": b["n_epochs"],
+            "methods": b["methods"],
+            "stage_ms": {"caustic_cache": b["stage_ms"][0], "point_source": b["stage_ms"][1],
+                         "dispatch_hexadecapole": b["stage_ms"][2],
+                         "inverse_ray": b["stage_ms"][3]},
+            "parallelism": "one process, one GPU",
         },
         "roofline": {
             "bound": "valu",
+            "kernel": "k_fs_ir",
             "achieved": achieved,
-            "peak": FP64_VALU_PEAK_TFLOPS,
+            "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / FP64_VALU_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel_ms": avg_kernel_ms,
-            "flops_per_unit": FLOPS_PER_PS_SOLVE,
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": K_FS_IR_TRAFFIC_BYTES,
+            "kernel_ms": ir_ms,
+            "algorithmic_flops": ir_flops,
+            "work": w,
         },
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(curves, offsets, times),
+        "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(out))
     return 0

@@ -91,6 +91,14 @@ lcbg_status lcbg_batch_create(const lcbg_curve* curves, int32_t n_curves,
                               const lcbg_options* options, lcbg_batch** out);
 lcbg_status lcbg_batch_destroy(lcbg_batch* batch);
 
+/* Replaces the models of every curve of the batch (host array of n_curves
+ * records, as for lcbg_batch_create; per-curve source_bins are kept).  The
+ * copy is ordered on `stream`, and the next finite-source run on the batch
+ * rebuilds the per-curve caustic caches in-stream, as the reference's
+ * magnifier rebuilds its cache when (s, q) change
+ * (ensure_binary_caustic_cache, finite_source_magnifier.cpp:4014-4030). */
+lcbg_status lcbg_batch_update_curves(lcbg_batch* batch, const lcbg_curve* curves, void* stream);
+
 /* Per-curve inverse-ray resolution: source_bins[c] (host array of n_curves)
  * replaces options.source_bins for curve c; 0 keeps the batch option.  The
  * reference reads one FiniteSourceSettings::source_bins per magnifier
@@ -118,15 +126,17 @@ lcbg_status lcbg_light_curve_point_source(const lcbg_curve* curve, const double*
  * (python/lcbinint_pybind.cpp:1141-1158) runs it: point source, hexadecapole,
  * or inverse-ray shooting on the Cartesian or polar lattice with limb
  * darkening.  Device pointers; n_epochs = offsets[n_curves].  method[i]
- * receives the lcbg_method of each epoch.  The first call on a batch builds the
- * per-curve caustic caches; later calls reuse them and the workspace. */
+ * receives the lcbg_method of each epoch.  The first call on a batch, and the
+ * first after lcbg_batch_update_curves, builds the per-curve caustic caches
+ * in-stream; later calls reuse them and the workspace. */
 lcbg_status lcbg_batch_finite_source_device(lcbg_batch* batch, const int64_t* offsets,
                                             const double* times, int64_t n_epochs, double* mag,
                                             int32_t* method, void* stream);
 
 /* lcbg_batch_finite_source_device, then waits and reports the device time of
- * its three stages (stage_ms[3]: point source + bounding box, caustic distance
- * + hexadecapole, inverse ray), the queue sizes (queue_counts[2]: epochs past
+ * its four stages (stage_ms[4]: caustic-cache build (0 when the curves did not
+ * change), point source + bounding box, caustic distance + hexadecapole,
+ * inverse ray), the queue sizes (queue_counts[2]: epochs past
  * the bounding box, inverse-ray epochs) and the overflow flags (error_flags). */
 lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const int64_t* offsets,
                                              const double* times, int64_t n_epochs, double* mag,
@@ -141,6 +151,18 @@ lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const int64_t* o
  * lcbg_wall_clock_rate_khz() gives the tick rate of the current device. */
 lcbg_status lcbg_batch_set_epoch_cost(lcbg_batch* batch, int64_t* cost);
 int64_t lcbg_wall_clock_rate_khz(void);
+
+/* Work of the inverse-ray stage of the batch's last finite-source run
+ * (synchronous): counts[0] quintic probe solves, [1] quartic solves of the
+ * critical-source refinement, [2] Cartesian lattice cells the row scan of
+ * cartesian_image_area_impl (finite_source_magnifier.cpp:1892-2014) evaluates
+ * (its guard count), [3] the same for Jacobian-guarded fold seeds, [4] polar
+ * lattice cells (run cells, the two cells ending each run, neighbour-row and
+ * start-cell tests), [5] device wall-clock ticks spent in seed searches and
+ * [6] in the area scans (summed over epochs), [7] Cartesian rows closed plus
+ * polar runs recorded.  Cells the device evaluates beyond a run's end are not
+ * counted.  counts has 8 entries. */
+lcbg_status lcbg_batch_work_counts(lcbg_batch* batch, uint64_t* counts);
 
 /* Stage accessor for one epoch of the batch's last finite-source run (host
  * pointers, synchronous, allocates a few KB; a diagnostic, not for hot loops).

@@ -63,6 +63,8 @@ lcbg_batch_create = _sig(
     "lcbg_batch_create", ctypes.c_int, ctypes.POINTER(Curve), ctypes.c_int32,
     ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_void_p))
 lcbg_batch_destroy = _sig("lcbg_batch_destroy", ctypes.c_int, ctypes.c_void_p)
+lcbg_batch_update_curves = _sig("lcbg_batch_update_curves", ctypes.c_int, ctypes.c_void_p,
+                                ctypes.POINTER(Curve), ctypes.c_void_p)
 lcbg_batch_set_source_bins = _sig("lcbg_batch_set_source_bins", ctypes.c_int, ctypes.c_void_p,
                                   c_int32_p)
 lcbg_batch_point_source_device = _sig(
@@ -106,6 +108,12 @@ class Batch:
         self.handle = handle
         self.n_curves = len(curves)
 
+    def update_curves(self, curves, stream: int = 0):
+        arr = (Curve * len(curves))(*curves)
+        if len(curves) != self.n_curves:
+            raise ValueError(f"update_curves needs {self.n_curves} curves, got {len(curves)}")
+        check(lcbg_batch_update_curves(self.handle, arr, stream), "lcbg_batch_update_curves")
+
     def set_source_bins(self, source_bins):
         import numpy as np
         nb = np.ascontiguousarray(source_bins, dtype=np.int32)
@@ -146,13 +154,13 @@ lcbg_light_curve = _sig(
 def finite_source(batch: Batch, offsets_ptr: int, times_ptr: int, n_epochs: int, mag_ptr: int,
                   method_ptr: int, stream: int = 0, profile: bool = False):
     """Runs the finite-source pipeline; with profile=True returns
-    (stage_ms[3], queue_counts[2], error_flags)."""
+    (stage_ms[4], queue_counts[2], error_flags)."""
     if not profile:
         check(lcbg_batch_finite_source_device(batch.handle, offsets_ptr, times_ptr, n_epochs,
                                               mag_ptr, method_ptr, stream),
               "lcbg_batch_finite_source_device")
         return None
-    ms = (ctypes.c_float * 3)()
+    ms = (ctypes.c_float * 4)()
     qc = (ctypes.c_uint32 * 2)()
     fl = ctypes.c_uint32()
     check(lcbg_batch_finite_source_profile(batch.handle, offsets_ptr, times_ptr, n_epochs, mag_ptr,
@@ -200,3 +208,15 @@ def debug_epoch(batch: Batch, epoch: int, method: int, seeds=None, capacity: int
     n = n_out.value
     out = buf[:2 * max(min(n, capacity), 0)].reshape(-1, 2).copy()
     return out, mag.value
+
+lcbg_batch_work_counts = _sig("lcbg_batch_work_counts", ctypes.c_int, ctypes.c_void_p,
+                              ctypes.POINTER(ctypes.c_uint64))
+
+
+def work_counts(batch: Batch):
+    """Inverse-ray work of the last run: dict of quintic, quartic, cart,
+    cart_guarded, polar counts."""
+    out = (ctypes.c_uint64 * 8)()
+    check(lcbg_batch_work_counts(batch.handle, out), "lcbg_batch_work_counts")
+    return dict(zip(("quintic", "quartic", "cart", "cart_guarded", "polar", "seed_ticks",
+                     "area_ticks", "rows"), list(out)))

@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(kWave) k_fs_stage2(const CurveRec* __restrict_
 
 __global__ void k_reset_counters(unsigned* counters, int n)
 {
-    if (static_cast<int>(threadIdx.x) < n) counters[threadIdx.x] = 0u;
+    for (int i = static_cast<int>(threadIdx.x); i < n; i += static_cast<int>(blockDim.x)) counters[i] = 0u;
 }
 
 lcbg_status launch_reset_counters(unsigned* counters, int n, hipStream_t st)

@@ -31,6 +31,7 @@ struct FsState {
     unsigned* counters = nullptr;
     int stage2_waves = 2048;
     int64_t last_n = 0;  // epochs of the last run (their workspace is kept)
+    IrStreams ir{};  // side streams and events of the inverse-ray stage
 };
 
 namespace {
@@ -48,6 +49,20 @@ T* carve(char*& p, std::size_t count)
 
 std::size_t carve_size(std::size_t bytes) { return bytes + 256; }
 
+constexpr int kCounterAlloc = 128;
+static_assert(kCounterWords <= kCounterAlloc, "counter words");
+
+// Seeds held at once by the inverse-ray stage: 8 per epoch of capacity, at
+// least 2^20 (a typical inverse-ray epoch has 3-16 seeds and most epochs do
+// not reach the inverse-ray stage at all).
+int64_t pool_cap_for(int64_t cap)
+{
+    const int64_t v = 8 * cap;
+    const int64_t lo = int64_t{1} << 20;
+    const int64_t hi = (int64_t{1} << 31) - 1;
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
 lcbg_status ensure_epochs(FsState* fs, int64_t n)
 {
     if (n <= fs->epoch_cap && fs->epoch_mem) return LCBG_OK;
@@ -61,7 +76,10 @@ lcbg_status ensure_epochs(FsState* fs, int64_t n)
     const std::size_t bytes = carve_size(cap * sizeof(int32_t)) + carve_size(cap * sizeof(Pt2)) +
                               carve_size(cap * sizeof(double)) * 2 +
                               carve_size(cap * 10 * sizeof(double)) +
-                              carve_size(cap * sizeof(int64_t)) * 2 + carve_size(64 * sizeof(unsigned));
+                              carve_size(cap * sizeof(int64_t)) * 2 +
+                              carve_size(kCounterAlloc * sizeof(unsigned)) +
+                              carve_size(cap * sizeof(int64_t)) + carve_size(cap * sizeof(int)) * 9 +
+                              carve_size(pool_cap_for(cap) * sizeof(Pt2));
     lcbg_status st = map_hip(hipMalloc(&fs->epoch_mem, bytes));
     if (st != LCBG_OK) {
         fs->epoch_mem = nullptr;
@@ -89,8 +107,37 @@ FsEpochs epoch_view(FsState* fs, int64_t n, const int64_t* offsets, const double
     ep.roots = carve<double>(p, cap * 10);
     ep.near_list = carve<int64_t>(p, cap);
     ep.ir_list = carve<int64_t>(p, cap);
-    ep.counters = carve<unsigned>(p, 64);
+    ep.counters = carve<unsigned>(p, kCounterAlloc);
     return ep;
+}
+
+// The inverse-ray work lists, carved after the epoch arrays.
+IrLists lists_view(FsState* fs)
+{
+    char* p = static_cast<char*>(fs->epoch_mem);
+    const int64_t cap = fs->epoch_cap;
+    (void)carve<int32_t>(p, cap);
+    (void)carve<Pt2>(p, cap);
+    (void)carve<double>(p, cap);
+    (void)carve<double>(p, cap);
+    (void)carve<double>(p, cap * 10);
+    (void)carve<int64_t>(p, cap);
+    (void)carve<int64_t>(p, cap);
+    (void)carve<unsigned>(p, kCounterAlloc);
+    IrLists L{};
+    L.ir_sorted = carve<int64_t>(p, cap);
+    L.bucket_of = carve<int>(p, cap);
+    L.seed_off = carve<int>(p, cap);
+    L.seed_n = carve<int>(p, cap);
+    L.cart_q = carve<int>(p, cap);
+    L.polar_q = carve<int>(p, cap);
+    L.cart_retry = carve<int>(p, cap);
+    L.polar_retry = carve<int>(p, cap);
+    L.cart_big = carve<int>(p, cap);
+    L.polar_big = carve<int>(p, cap);
+    L.pool_cap = static_cast<unsigned>(pool_cap_for(cap));
+    L.seed_pool = carve<Pt2>(p, L.pool_cap);
+    return L;
 }
 
 // Inverse-ray scratch, one pool per device shared by every batch: its use is
@@ -109,46 +156,73 @@ ScratchPool g_pool[kMaxDevices];
 lcbg_status ensure_scratch(ScratchPool& pool)
 {
     if (pool.mem) return LCBG_OK;
-    // k_fs_ir holds one wave per SIMD (its register use allows no more), so one
-    // slot per SIMD of the device: 4 SIMDs per CU
     int cus = 0;
     int dev = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
-    const int slots = 4 * cus;
-    const std::size_t per_slot = carve_size(kSeedCap * sizeof(Pt2)) + carve_size(kSeedCap * sizeof(double)) +
-                                 carve_size(kSeedCap * sizeof(int)) +
-                                 carve_size(static_cast<std::size_t>(kRowCap) * 5 * sizeof(double)) +
-                                 carve_size(static_cast<std::size_t>(kRowIntsPerSlot) * sizeof(int)) +
-                                 carve_size(static_cast<std::size_t>(kPhiHash) * sizeof(int)) +
-                                 carve_size(static_cast<std::size_t>(kRunCap) * 4 * sizeof(int));
-    const std::size_t bytes = per_slot * slots;
+    // as many slots as the kernel keeps resident: its occupancy per CU
+    const int seed_slots = cus * ir_resident_per_cu(0);
+    // two Cartesian waves per SIMD: the largest epochs bound the stage, and a
+    // slot of 2^17 rows holds them without a second pass
+    const int cart_occ = ir_resident_per_cu(1);
+    const int cart_slots = cus * (cart_occ < 8 ? cart_occ : 8);
+    const int polar_slots = cus;  // few polar epochs; they share the CUs with the Cartesian pass
+    const int large_slots = 64;
+    IrScratch& s = pool.scr;
+    s.seed.n_slots = seed_slots;
+    s.cart.n_slots = cart_slots;
+    s.cart.row_cap = kRowCapSmall;
+    s.cart_large.n_slots = large_slots;
+    s.cart_large.row_cap = kRowCapLarge;
+    s.polar.n_slots = polar_slots;
+    s.polar.run_cap = kRunCapSmall;
+    s.polar_large.n_slots = large_slots;
+    s.polar_large.run_cap = kRunCapLarge;
+    auto cart_bytes = [](const CartScratch& c) {
+        const std::size_t n = static_cast<std::size_t>(c.n_slots);
+        return carve_size(n * kSeedCap * sizeof(double)) + carve_size(n * kSeedCap * sizeof(int)) +
+               carve_size(n * c.row_cap * 5 * sizeof(double)) +
+               carve_size(n * static_cast<std::size_t>(row_ints_per_slot(c.row_cap)) * sizeof(int));
+    };
+    auto polar_bytes = [](const PolarScratch& c) {
+        const std::size_t n = static_cast<std::size_t>(c.n_slots);
+        return carve_size(n * kPhiHash * sizeof(int)) + carve_size(n * c.run_cap * 4 * sizeof(int));
+    };
+    const std::size_t bytes = carve_size(static_cast<std::size_t>(seed_slots) * kSeedCap * sizeof(Pt2)) +
+                              cart_bytes(s.cart) + cart_bytes(s.cart_large) + polar_bytes(s.polar) +
+                              polar_bytes(s.polar_large);
     lcbg_status st = map_hip(hipMalloc(&pool.mem, bytes));
     if (st != LCBG_OK) {
         pool.mem = nullptr;
         return st;
     }
     char* p = static_cast<char*>(pool.mem);
-    IrScratch& s = pool.scr;
-    s.n_slots = slots;
-    s.seeds = carve<Pt2>(p, static_cast<std::size_t>(slots) * kSeedCap);
-    s.areaimage = carve<double>(p, static_cast<std::size_t>(slots) * kSeedCap);
-    s.overlap = carve<int>(p, static_cast<std::size_t>(slots) * kSeedCap);
-    s.rows = carve<double>(p, static_cast<std::size_t>(slots) * kRowCap * 5);
-    s.row_key = carve<int>(p, static_cast<std::size_t>(slots) * kRowIntsPerSlot);
-    s.phi_head = carve<int>(p, static_cast<std::size_t>(slots) * kPhiHash);
-    s.runs = carve<int>(p, static_cast<std::size_t>(slots) * kRunCap * 4);
+    s.seed.seeds = carve<Pt2>(p, static_cast<std::size_t>(seed_slots) * kSeedCap);
+    for (CartScratch* c : {&s.cart, &s.cart_large}) {
+        const std::size_t n = static_cast<std::size_t>(c->n_slots);
+        c->areaimage = carve<double>(p, n * kSeedCap);
+        c->overlap = carve<int>(p, n * kSeedCap);
+        c->rows = carve<double>(p, n * c->row_cap * 5);
+        c->row_key = carve<int>(p, n * static_cast<std::size_t>(row_ints_per_slot(c->row_cap)));
+        // empty row-key chains: -1 heads, dirty count -1 (nothing to clear);
+        // the kernel restores them per seed
+        st = map_hip(hipMemset(c->row_key, 0xff,
+                               n * static_cast<std::size_t>(row_ints_per_slot(c->row_cap)) * sizeof(int)));
+        if (st != LCBG_OK) return st;
+    }
+    for (PolarScratch* c : {&s.polar, &s.polar_large}) {
+        const std::size_t n = static_cast<std::size_t>(c->n_slots);
+        c->phi_head = carve<int>(p, n * kPhiHash);
+        c->runs = carve<int>(p, n * c->run_cap * 4);
+        // empty visited-run lists; the kernel restores them per epoch
+        st = map_hip(hipMemset(c->phi_head, 0xff, n * kPhiHash * sizeof(int)));
+        if (st != LCBG_OK) return st;
+    }
     st = map_hip(hipEventCreateWithFlags(&pool.last_use, hipEventDisableTiming));
     if (st != LCBG_OK) return st;
     // record once so that the first run's stream wait has a completed event
     st = map_hip(hipEventRecord(pool.last_use, nullptr));
-    if (st != LCBG_OK) return st;
-    // empty visited-run lists and row-key chains (-1 heads, dirty count -1 =
-    // nothing to clear); the kernel restores them per epoch and per seed
-    st = map_hip(hipMemset(s.phi_head, 0xff, static_cast<std::size_t>(slots) * kPhiHash * sizeof(int)));
-    if (st != LCBG_OK) return st;
-    st = map_hip(hipMemset(s.row_key, 0xff, static_cast<std::size_t>(slots) * kRowIntsPerSlot * sizeof(int)));
     if (st != LCBG_OK) return st;
     return map_hip(hipDeviceSynchronize());
 }
@@ -161,10 +235,15 @@ void fs_destroy(FsState* fs)
     if (fs->table_mem) (void)hipFree(fs->table_mem);
     if (fs->cache_mem) (void)hipFree(fs->cache_mem);
     if (fs->epoch_mem) (void)hipFree(fs->epoch_mem);
+    if (fs->ir.polar) (void)hipStreamDestroy(fs->ir.polar);
+    if (fs->ir.big) (void)hipStreamDestroy(fs->ir.big);
+    for (hipEvent_t e : {fs->ir.fork, fs->ir.join_polar, fs->ir.join_big})
+        if (e) (void)hipEventDestroy(e);
     delete fs;
 }
 
-// Builds the trig tables and the per-curve caches of a batch.
+// Allocates the trig tables and the per-curve cache storage of a batch; the
+// caches themselves are built in-stream by fs_refresh.
 lcbg_status fs_prepare(lcbg_batch* b)
 {
     if (b->fs) return LCBG_OK;
@@ -236,9 +315,21 @@ lcbg_status fs_prepare(lcbg_batch* b)
     fs->cache.crit = carve<Pt2>(p, per * kSeedPhases * 4);
     fs->cache.assign = carve<uint8_t>(p, per * kSeedPhases * 4);
     fs->cache.sc = carve<CausticScalars>(p, per);
-    st = launch_curve_prep(b->d_curves, b->n_curves, fs->cache, fs->tab, nullptr);
-    if (st != LCBG_OK) return st;
-    return map_hip(hipDeviceSynchronize());
+    b->caches_stale = true;
+    st = map_hip(hipStreamCreateWithFlags(&fs->ir.polar, hipStreamNonBlocking));
+    if (st == LCBG_OK) st = map_hip(hipStreamCreateWithFlags(&fs->ir.big, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&fs->ir.fork, &fs->ir.join_polar, &fs->ir.join_big})
+        if (st == LCBG_OK) st = map_hip(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    return st;
+}
+
+// Rebuilds the per-curve caches on `stream` if the curves changed since.
+lcbg_status fs_refresh(lcbg_batch* b, hipStream_t stream)
+{
+    if (!b->caches_stale) return LCBG_OK;
+    lcbg_status st = launch_curve_prep(b->d_curves, b->n_curves, b->fs->cache, b->fs->tab, stream);
+    if (st == LCBG_OK) b->caches_stale = false;
+    return st;
 }
 
 lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, int64_t n_epochs,
@@ -252,11 +343,14 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     FsEpochs ep = epoch_view(fs, n_epochs, offsets, times, mag, method);
     ep.cost = b->d_cost;
     fs->last_n = n_epochs;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     if (stage_ms) {
         for (auto& e : ev) (void)hipEventCreate(&e);
         (void)hipEventRecord(ev[0], stream);
     }
+    st = fs_refresh(b, stream);
+    if (st != LCBG_OK) return st;
+    if (stage_ms) (void)hipEventRecord(ev[1], stream);
     ScratchPool& pool = g_pool[b->device % kMaxDevices];
     std::lock_guard<std::mutex> lock(pool.mu);
     st = ensure_scratch(pool);
@@ -274,20 +368,22 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
         std::fflush(stderr);
     };
     st = map_hip(hipStreamWaitEvent(stream, pool.last_use, 0));
-    if (st == LCBG_OK) st = launch_reset_counters(ep.counters, 8, stream);
+    if (st == LCBG_OK) st = launch_reset_counters(ep.counters, kCounterWords, stream);
     if (st == LCBG_OK) st = launch_fs_stage1(b->d_curves, b->n_curves, fs->cache, fs->opt, ep, stream);
     stage_done("stage 1 (point source, bounding box)");
-    if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[1], stream);
+    if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[2], stream);
     if (st == LCBG_OK) st = launch_fs_stage2(b->d_curves, fs->cache, fs->opt, ep, fs->stage2_waves, stream);
     stage_done("stage 2 (caustic distance, hexadecapole)");
-    if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[2], stream);
-    if (st == LCBG_OK) st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, pool.scr, stream);
+    if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[3], stream);
+    if (st == LCBG_OK)
+        st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, lists_view(fs), pool.scr, stream,
+                          fs->ir);
     stage_done("stage 3 (inverse ray)");
     if (st == LCBG_OK) st = map_hip(hipEventRecord(pool.last_use, stream));
     if (stage_ms && st == LCBG_OK) {
-        (void)hipEventRecord(ev[3], stream);
-        (void)hipEventSynchronize(ev[3]);
-        for (int k = 0; k < 3; ++k) (void)hipEventElapsedTime(&stage_ms[k], ev[k], ev[k + 1]);
+        (void)hipEventRecord(ev[4], stream);
+        (void)hipEventSynchronize(ev[4]);
+        for (int k = 0; k < 4; ++k) (void)hipEventElapsedTime(&stage_ms[k], ev[k], ev[k + 1]);
     }
     if (stage_ms) {
         for (auto& e : ev) (void)hipEventDestroy(e);
@@ -388,12 +484,22 @@ extern "C" lcbg_status lcbg_batch_debug_critical_sources(lcbg_batch* batch, int3
 {
     if (!batch || !out || curve < 0 || curve >= batch->n_curves) return LCBG_INVALID_ARGUMENT;
     lcbg_status st = fs_prepare(batch);
-    if (st != LCBG_OK) return st;
-    st = map_hip(hipDeviceSynchronize());
+    if (st == LCBG_OK) st = fs_refresh(batch, nullptr);
+    if (st == LCBG_OK) st = map_hip(hipDeviceSynchronize());
     if (st != LCBG_OK) return st;
     const std::size_t n = static_cast<std::size_t>(kSeedPhases) * 4;
     return map_hip(hipMemcpy(out, batch->fs->cache.crit + static_cast<std::size_t>(curve) * n,
                              n * sizeof(Pt2), hipMemcpyDeviceToHost));
+}
+
+extern "C" lcbg_status lcbg_batch_work_counts(lcbg_batch* batch, uint64_t* counts)
+{
+    if (!batch || !counts || !batch->fs || !batch->fs->epoch_mem) return LCBG_INVALID_ARGUMENT;
+    const FsEpochs ep = epoch_view(batch->fs, 0, nullptr, nullptr, nullptr, nullptr);
+    lcbg_status st = map_hip(hipDeviceSynchronize());
+    if (st != LCBG_OK) return st;
+    return map_hip(hipMemcpy(counts, ep.counters + kWorkOffset, 8 * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost));
 }
 
 extern "C" lcbg_status lcbg_batch_set_epoch_cost(lcbg_batch* batch, int64_t* cost)

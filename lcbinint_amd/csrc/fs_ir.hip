@@ -36,6 +36,14 @@ struct Ctx {
     long long steps;  // wave steps spent on this epoch
     int last_phase;   // seed phases run: 0 images, 1 first crossing, 2 branches,
                       // 3 boundary, 4 interior, 5 caustic segments (diagnostics)
+    // algorithmic work of the epoch (uniform): quintic and quartic solves, the
+    // lattice cells the reference's scans evaluate (plain / Jacobian-guarded
+    // Cartesian, polar)
+    long long n_quintic, n_quartic, n_cart, n_cart_guarded, n_polar;
+    long long n_rows;  // Cartesian rows closed plus polar runs recorded
+    int row_cap;       // row records of this slot (Cartesian)
+    int run_cap;       // visited runs of this slot (polar)
+    bool cap_overflow; // a slot capacity (not the seed list or work budget) ran out
 };
 
 // Counts one wave step; past the budget the epoch is abandoned with an error.
@@ -136,6 +144,7 @@ __device__ void merge_probe(Ctx& C, int n_img, const Pt* img, const double* md)
 // Per-lane probe evaluation: inside-disk test, R3 images, mapped distances.
 struct ProbeResult {
     int n;
+    bool solved;  // the probe was inside and its quintic was solved
     Pt img[5];
     double md[5];
 };
@@ -144,8 +153,10 @@ __device__ __forceinline__ ProbeResult eval_probe(const Ctx& C, Pt probe)
 {
     ProbeResult r;
     r.n = 0;
+    r.solved = false;
     if (!(C.rho > 0.0) || dist2(probe, C.src) >= C.r2 * (1.0 + 1.0e-10)) return r;
     Pt im[5];
+    r.solved = true;
     const int n = probe_r3_images(C, probe, im);
     if (n <= 3) return r;
     r.n = n;
@@ -186,12 +197,14 @@ __device__ void caustic_probes(Ctx& C, Pt cs, const FsTables& tab)
         const double steps[4] = {0.02 * C.rho, 0.05 * C.rho, 0.15 * C.rho, 0.35 * C.rho};
         ProbeResult mine;
         mine.n = 0;
+        mine.solved = false;
         if (C.lane < 8) {
             const double st = steps[C.lane >> 1];
             const Pt p = (C.lane & 1) ? Pt{cs.x - ux * st, cs.y - uy * st}
                                       : Pt{cs.x + ux * st, cs.y + uy * st};
             mine = eval_probe(C, p);
         }
+        C.n_quintic = uni(C.n_quintic + popcount64(ballot(mine.solved)));
         for (int k = 0; k < 8; ++k) merge_lane(C, mine, k);
     }
     if (C.n_seeds <= 3) {
@@ -201,11 +214,13 @@ __device__ void caustic_probes(Ctx& C, Pt cs, const FsTables& tab)
             const int k = base + C.lane;
             ProbeResult mine;
             mine.n = 0;
+            mine.solved = false;
             if (k < 5 * kAngularProbes) {
                 const double st = radial[k / kAngularProbes];
                 const Pt t = tab.angular_trig[k % kAngularProbes];
                 mine = eval_probe(C, Pt{cs.x + st * t.x, cs.y + st * t.y});
             }
+            C.n_quintic = uni(C.n_quintic + popcount64(ballot(mine.solved)));
             const int cnt = (5 * kAngularProbes - base) < kWave ? (5 * kAngularProbes - base) : kWave;
             for (int j = 0; j < cnt; ++j) {
                 merge_lane(C, mine, j);
@@ -236,7 +251,7 @@ __device__ double nearest_critical(const Ctx& C, double phase, Pt* nearest)
 }
 
 // refine_nearest_critical_source (finite_source_magnifier.cpp:1666-1707)
-__device__ Pt refine_critical(const Ctx& C, double center, double step)
+__device__ Pt refine_critical(Ctx& C, double center, double step)
 {
     double left = center - step;
     double right = center + step;
@@ -244,6 +259,7 @@ __device__ Pt refine_critical(const Ctx& C, double center, double step)
     double x1 = right - golden * (right - left);
     double x2 = left + golden * (right - left);
     Pt n1{0.0, 0.0}, n2{0.0, 0.0};
+    C.n_quartic = uni(C.n_quartic + 34);
     double f1 = uni(nearest_critical(C, x1, &n1));
     double f2 = uni(nearest_critical(C, x2, &n2));
     for (int iter = 0; iter < 32; ++iter) {
@@ -419,10 +435,12 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             const int k = base + C.lane;
             ProbeResult mine;
             mine.n = 0;
+            mine.solved = false;
             if (k < kBoundaryProbes) {
                 const Pt t = tab.boundary_trig[k];
                 mine = eval_probe(C, Pt{C.src.x + pr * t.x, C.src.y + pr * t.y});
             }
+            C.n_quintic = uni(C.n_quintic + popcount64(ballot(mine.solved)));
             const int cnt = (kBoundaryProbes - base) < kWave ? (kBoundaryProbes - base) : kWave;
             for (int j = 0; j < cnt && C.n_seeds < 128; ++j) merge_lane(C, mine, j);
         }
@@ -435,6 +453,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             const double pr = C.rho * fr[ri];
             const Pt t = tab.interior_trig[C.lane];
             const ProbeResult mine = eval_probe(C, Pt{C.src.x + pr * t.x, C.src.y + pr * t.y});
+            C.n_quintic = uni(C.n_quintic + popcount64(ballot(mine.solved)));
             for (int j = 0; j < kInteriorProbes; ++j) merge_lane(C, mine, j);
         }
     }
@@ -505,6 +524,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             if (dist2(probe, C.src) >= C.rho * C.rho) continue;
             Pt im[5];
             const int ni = probe_r3_images(C, probe, im);
+            C.n_quintic = uni(C.n_quintic + 1);
             if (ni > C.n_seeds) {
                 wave_sync();
                 for (int i = 0; i < ni; ++i) wave_store(&C.seeds[i], im[i]);
@@ -517,39 +537,35 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
 
 // ------------------------------------------------------------ Cartesian scan
 
-// x after k sequential additions of `step` to x0 (the reference advances
-// lattice cells with `x += incr`, finite_source_magnifier.cpp:2004-2010).
-// Within one binade, with no rounding tie, every addition adds the same
-// rounded increment d, so x_k = x0 + k d exactly; otherwise the lanes replay
-// the additions one by one.
-__device__ __forceinline__ double seq_coord(double x0, double step, int k)
+// Lane k receives x0 after k sequential additions of `step` (the reference
+// advances lattice cells with `x += incr`, finite_source_magnifier.cpp:2004-2010).
+// Every lane from a verified base lane b takes base + (k - b) d with
+// d = (base + step) - base, the increment that repeats while the additions
+// stay in one binade; each lane then checks its value against the previous
+// lane's value plus step.  Values up to the first failing lane are exact by
+// induction from the base, so that lane becomes the next base, holding the
+// previous lane's value plus step.  The loop runs once per binade crossing
+// (at most 64 times) and yields exactly the sequential sums.
+__device__ __forceinline__ double seq_lanes(double x0, double step)
 {
-    bool fast = false;
-    double d = 0.0;
-    if (x0 != 0.0 && isfinite(x0)) {
-        const double a0 = fabs(x0);
-        int e;
-        frexp(a0, &e);  // a0 in [2^(e-1), 2^e)
-        const double lo = ldexp(1.0, e - 1);
-        const double hi = ldexp(1.0, e);
-        const double ulp = ldexp(1.0, e - 53);
-        const double x1 = x0 + step;
-        d = x1 - x0;
-        const double xe = x0 + 64.0 * d;
-        const double as = fabs(step);
-        const double ae = fabs(xe);
-        const bool same_sign = (x0 > 0.0) == (xe > 0.0);
-        const bool in_binade = a0 - as >= lo && a0 + as < hi && ae - as >= lo && ae + as < hi;
-        const bool no_tie = fabs(step - d) != 0.5 * ulp;
-        const bool exact = fabs(d) * 64.0 < 0x1p53 * ulp;
-        fast = same_sign && in_binade && no_tie && exact && lo >= 0x1p-1000;
+    const int lane = lane_id();
+    double base = x0;
+    int b = 0;
+    double cand = x0;
+    for (int it = 0; it < kWave; ++it) {
+        const double d = (base + step) - base;
+        if (lane == b)
+            cand = base;
+        else if (lane > b)
+            cand = base + static_cast<double>(lane - b) * d;
+        const double prev = __shfl_up(cand, 1, kWave);
+        const uint64_t bad = ballot(lane > b && !(cand == prev + step));
+        if (!bad) break;
+        const int f = first_set(bad);
+        base = read_lane(cand, f - 1) + step;
+        b = f;
     }
-    if (fast) return x0 + static_cast<double>(k) * d;
-    double x = x0;
-    for (int j = 0; j < kWave; ++j) {
-        if (j < k) x = x + step;
-    }
-    return x;
+    return cand;
 }
 
 // The reference keeps, per seed, an unordered_map from row key to the rows
@@ -559,9 +575,9 @@ __device__ __forceinline__ double seq_coord(double x0, double step, int k)
 // heads are set; they are cleared before the next seed's scan.
 struct Rows {
     double* r;     // [row*5 + field]
-    int* key;      // [kRowCap] pushed row keys
-    int* idx;      // [kRowCap] pushed row indices
-    int* next;     // [kRowCap] older push with the same hash, -1 at the end
+    int* key;      // [row_cap] pushed row keys
+    int* idx;      // [row_cap] pushed row indices
+    int* next;     // [row_cap] older push with the same hash, -1 at the end
     int* head;     // [kRowHash]
     int* dirty;    // keys pushed since the heads were last cleared (<0: none)
     int size;      // rows allocated (ensure)
@@ -591,8 +607,9 @@ struct Cart {
 __device__ __forceinline__ bool rows_ensure(Ctx& C, Rows& R, int index)
 {
     if (index < R.size) return true;
-    if (index >= kRowCap) {
+    if (index >= C.row_cap) {
         C.overflow = true;
+        C.cap_overflow = true;
         return false;
     }
     const int from = R.size * 5;
@@ -634,11 +651,25 @@ __device__ __forceinline__ bool cart_cell(const Ctx& C, const Cart& K, double x,
     return in;
 }
 
-// cartesian_image_area_impl (finite_source_magnifier.cpp:1892-2014).
+__device__ __forceinline__ void cart_count(Ctx& C, const Cart& K, long long cells)
+{
+    if (K.jac_sign == 0)
+        C.n_cart = uni(C.n_cart + cells);
+    else
+        C.n_cart_guarded = uni(C.n_cart_guarded + cells);
+}
+
+// cartesian_image_area_impl (finite_source_magnifier.cpp:1892-2014).  The
+// scan keeps the current row's xmin/xmax and the previous row's xmin in
+// registers (written through to the row records), and each lane keeps its
+// share of the row's brightness sum, reduced once when the row closes; a
+// row's sum is nonzero exactly when one of its counted weights is, which is
+// what the scan's decisions test.
 __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi)
 {
     const double incr = K.incr;
-    double countx = 0.0;
+    double acc = 0.0;     // this lane's share of the row's brightness sum (countx)
+    bool any_w = false;   // countx != 0
     double countall = 0.0;
     bool last_in = 99999999.9 <= C.r2;
     int dir = 1;
@@ -647,14 +678,17 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
     double x0 = seed.x;
     long long guard = 0;
     const long long budget = K.max_steps - 1;
+    if (!rows_ensure(C, R, yi)) return __builtin_nan("");
+    double cur_xmin = uni(rget(R, yi, XMIN));
+    double cur_xmax = uni(rget(R, yi, XMAX));
+    double prev_xmin = yi != 0 ? uni(rget(R, yi - 1, XMIN)) : 0.0;
     for (;;) {
         if (!step_ok(C)) return __builtin_nan("");
-        if (!rows_ensure(C, R, yi)) return __builtin_nan("");
         const long long remaining = budget - guard;
         if (remaining <= 0) return countall;
         const int n_eval = remaining < kWave ? static_cast<int>(remaining) : kWave;
         const double step = dir > 0 ? incr : -incr;
-        const double xk = seq_coord(x, step, C.lane);
+        const double xk = seq_lanes(x, step);
         double w = 0.0;
         bool in = false;
         if (C.lane < n_eval) in = cart_cell(C, K, xk, y, &w);
@@ -663,71 +697,85 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
         if (dir > 0) {
             const int s = first_set(ballot(!in && valid));
             if (s < 0) {
-                countx = uni(countx + wave_sum(w));
+                acc += w;
+                any_w = any_w || ballot(w != 0.0) != 0;
                 guard += n_eval;
+                cart_count(C, K, n_eval);
                 last_in = true;
-                x = uni(shfl(xk, n_eval - 1) + step);
+                x = read_lane(xk, n_eval - 1) + step;
                 continue;
             }
-            countx = uni(countx + wave_sum(C.lane < s ? w : 0.0));
+            const double wl = C.lane < s ? w : 0.0;
+            acc += wl;
+            any_w = any_w || ballot(wl != 0.0) != 0;
             guard += s + 1;
-            const double xs = uni(shfl(xk, s));
+            cart_count(C, K, s + 1);
+            const double xs = read_lane(xk, s);
             const bool prev_in = s > 0 ? true : last_in;
-            if (prev_in) rset(C, R, yi, XMAX, xs);
+            if (prev_in) {
+                cur_xmax = xs;
+                rset(C, R, yi, XMAX, xs);
+            }
             dir = -1;
             x = x0;
-            rset(C, R, yi, XMIN, x + (-incr));
+            cur_xmin = x + (-incr);
+            rset(C, R, yi, XMIN, cur_xmin);
             last_in = false;
             x = x + (-incr);
             continue;
         }
         // left scan
-        const bool cb = countx != 0.0;
-        double thr = 0.0;
-        if (yi != 0) thr = rget(R, yi - 1, XMIN) - (-incr);
+        const bool cb = any_w;
+        const double thr = yi != 0 ? prev_xmin - (-incr) : 0.0;
         const bool cond = (yi != 0) && (xk >= thr);
         const int f = first_set(in_m);
         const bool stop = valid && !in && (cb || (f >= 0 && C.lane > f) || yi == 0 || !cond);
         const int s = first_set(ballot(stop));
         const int limit = s >= 0 ? s : n_eval;
-        const double add = uni(wave_sum((C.lane < limit && in) ? w : 0.0));
-        if (!cb && f >= 0 && f < limit) rset(C, R, yi, XMAX, uni(shfl(xk, f)) - (-incr));
+        const double wl = (C.lane < limit && in) ? w : 0.0;
+        acc += wl;
+        if (!cb && f >= 0 && f < limit) {
+            cur_xmax = read_lane(xk, f) - (-incr);
+            rset(C, R, yi, XMAX, cur_xmax);
+        }
+        any_w = any_w || ballot(wl != 0.0) != 0;
         if (s < 0) {
-            countx = uni(countx + add);
             guard += n_eval;
+            cart_count(C, K, n_eval);
             last_in = ((in_m >> (n_eval - 1)) & 1ull) != 0;
-            x = uni(shfl(xk, n_eval - 1) + (-incr));
+            x = read_lane(xk, n_eval - 1) + (-incr);
             continue;
         }
-        countx = uni(countx + add);
         guard += s + 1;
-        const double xs = uni(shfl(xk, s));
+        cart_count(C, K, s + 1);
+        const double xs = read_lane(xk, s);
         const bool prev_in = s > 0 ? (((in_m >> (s - 1)) & 1ull) != 0) : last_in;
-        if (prev_in) rset(C, R, yi, XMIN, xs);
+        if (prev_in) {
+            cur_xmin = xs;
+            rset(C, R, yi, XMIN, xs);
+        }
         // close the row
+        C.n_rows = uni(C.n_rows + 1);
+        const double countx = uni(wave_sum(acc));
         countall += countx;
         rset(C, R, yi, AX, countx);
         rset(C, R, yi, YY, y);
-        rset(C, R, yi, DYS, dy);
-        if (countx == 0.0) {
-            rset(C, R, yi, DYS, -dy);
-            return countall;
-        }
+        rset(C, R, yi, DYS, countx == 0.0 ? -dy : dy);
+        if (countx == 0.0) return countall;
         const int key = static_cast<int>(y * K.inv_incr + 2000000);
         {
-            const double xmin_y = rget(R, yi, XMIN);
-            const double xmax_y = rget(R, yi, XMAX);
             bool dup = false;
             for (int p = uni(R.head[key & (kRowHash - 1)]); p >= 0 && !dup; p = uni(R.next[p])) {
                 if (uni(R.key[p]) == key) {
                     const int e = uni(R.idx[p]);
-                    dup = xmin_y + incr < rget(R, e, XMAX) && xmax_y - incr > rget(R, e, XMIN);
+                    dup = cur_xmin + incr < rget(R, e, XMAX) && cur_xmax - incr > rget(R, e, XMIN);
                 }
             }
             if (dup) return countall - countx;
         }
-        if (R.pushed >= kRowCap) {
+        if (R.pushed >= C.row_cap) {
             C.overflow = true;
+            C.cap_overflow = true;
             return __builtin_nan("");
         }
         wave_sync();
@@ -745,10 +793,14 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
         ++yi;
         if (!rows_ensure(C, R, yi)) return __builtin_nan("");
         dir = 1;
-        x0 = rget(R, yi - 1, XMAX);
+        prev_xmin = cur_xmin;
+        x0 = cur_xmax;
+        cur_xmin = uni(rget(R, yi, XMIN));
+        cur_xmax = uni(rget(R, yi, XMAX));
         x = x0 - incr;
         y = y + dy;
-        countx = 0.0;
+        acc = 0.0;
+        any_w = false;
         last_in = false;
         x = x + incr;
     }
@@ -781,10 +833,10 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* 
         Rows R;
         R.r = rows_mem;
         R.key = rows_key;
-        R.idx = rows_key + kRowCap;
-        R.next = rows_key + 2 * kRowCap;
-        R.head = rows_key + 3 * kRowCap;
-        R.dirty = rows_key + 3 * kRowCap + kRowHash;
+        R.idx = rows_key + C.row_cap;
+        R.next = rows_key + 2 * C.row_cap;
+        R.head = rows_key + 3 * C.row_cap;
+        R.dirty = rows_key + 3 * C.row_cap + kRowHash;
         rows_reset(C, R);
         if (!rows_ensure(C, R, 1)) return __builtin_nan("");
         double area0 = 0.0, areai = 0.0;
@@ -993,10 +1045,14 @@ __device__ double polar_add_run(Ctx& C, Polar& P, int ir, int iphi, double cph, 
         right += stop;
         break;
     }
-    if (P.n_runs >= kRunCap) {
+    if (P.n_runs >= C.run_cap) {
         C.overflow = true;
+        C.cap_overflow = true;
         return 0.0;
     }
+    // the run's cells and the two outside cells that end it
+    C.n_polar = uni(C.n_polar + (right - left + 3));
+    C.n_rows = uni(C.n_rows + 1);
     const int old_head = uni(P.head[iphi & (kPhiHash - 1)]);
     wave_sync();
     {
@@ -1090,6 +1146,7 @@ __device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* hea
             const uint64_t m = ballot(in);
             if (m) found_ord = base + first_set(m);
         }
+        C.n_polar = uni(C.n_polar + (found_ord < 0 ? 242 : found_ord + 1));
         if (found_ord < 0) continue;
         int shell = 0, rem = found_ord;
         while (rem >= 2 * (2 * shell + 1)) {
@@ -1133,9 +1190,12 @@ __device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* hea
                 }
                 const uint64_t m = ballot(ok);
                 if (!m) {
+                    const int span = Rr - c + 1;
+                    C.n_polar = uni(C.n_polar + (span < kWave ? span : kWave));
                     c += kWave;
                     continue;
                 }
+                C.n_polar = uni(C.n_polar + first_set(m));
                 const int start = c + first_set(m);
                 total += polar_add_run(C, P, start, nb, cph, sph);
                 c = runs[4 * (P.n_runs - 1) + 2] + 1;
@@ -1151,31 +1211,7 @@ __device__ double polar_mag(Ctx& C, int source_bins, double grid_ratio, int* hea
     return image_flux / flux;
 }
 
-// ------------------------------------------------------------------- kernel
-
-// Slot scratch of one resident wave.
-struct SlotMem {
-    Pt* seeds;
-    double* areaimage;
-    int* flags;
-    double* rows;
-    int* row_key;
-    int* head;
-    int* runs;
-};
-
-__device__ __forceinline__ SlotMem slot_mem(const IrScratch& scr, int slot)
-{
-    SlotMem M;
-    M.seeds = scr.seeds + static_cast<size_t>(slot) * kSeedCap;
-    M.areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
-    M.flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
-    M.rows = scr.rows + static_cast<size_t>(slot) * kRowCap * 5;
-    M.row_key = scr.row_key + static_cast<size_t>(slot) * kRowIntsPerSlot;
-    M.head = scr.phi_head + static_cast<size_t>(slot) * kPhiHash;
-    M.runs = scr.runs + static_cast<size_t>(slot) * kRunCap * 4;
-    return M;
-}
+// ------------------------------------------------------------------ kernels
 
 __device__ __forceinline__ void init_ctx(Ctx& C, const CurveRec& cv, Pt src, Pt* seeds)
 {
@@ -1195,6 +1231,42 @@ __device__ __forceinline__ void init_ctx(Ctx& C, const CurveRec& cv, Pt src, Pt*
     C.overflow = false;
     C.steps = 0;
     C.last_phase = 5;
+    C.n_quintic = 0;
+    C.n_quartic = 0;
+    C.n_cart = 0;
+    C.n_cart_guarded = 0;
+    C.n_polar = 0;
+    C.n_rows = 0;
+    C.row_cap = 0;
+    C.run_cap = 0;
+    C.cap_overflow = false;
+}
+
+__device__ __forceinline__ unsigned long long* work_totals(const FsEpochs& ep)
+{
+    return reinterpret_cast<unsigned long long*>(ep.counters + kWorkOffset);
+}
+
+// Adds a uniform amount to a run total (every lane adds, all but lane 0 zero).
+__device__ __forceinline__ void add_total(unsigned long long* p, long long v)
+{
+    atomicAdd(p, lane_id() == 0 ? static_cast<unsigned long long>(v) : 0ull);
+}
+
+__device__ __forceinline__ void flush_work(const Ctx& C, const FsEpochs& ep)
+{
+    unsigned long long* w = work_totals(ep);
+    if (C.n_quintic) add_total(&w[0], C.n_quintic);
+    if (C.n_quartic) add_total(&w[1], C.n_quartic);
+    if (C.n_cart) add_total(&w[2], C.n_cart);
+    if (C.n_cart_guarded) add_total(&w[3], C.n_cart_guarded);
+    if (C.n_polar) add_total(&w[4], C.n_polar);
+    if (C.n_rows) add_total(&w[7], C.n_rows);
+}
+
+__device__ __forceinline__ int curve_nbin(const CurveRec& cv, const FsOptions& opt)
+{
+    return cv.source_bins > 0 ? cv.source_bins : opt.source_bins;
 }
 
 // Seeds of epoch e into C (fixed_inverse_ray_binary's seed step,
@@ -1211,52 +1283,211 @@ __device__ __forceinline__ void epoch_seeds(Ctx& C, const FsCache& cache, const 
                 ep.roots + 10 * e);
 }
 
-// The inverse-ray magnification on the seeds in C: polar flood fill or the
-// Cartesian row scan, at the curve's resolution.
-__device__ __forceinline__ double epoch_area(Ctx& C, const CurveRec& cv, const FsOptions& opt,
-                                             int method, const SlotMem& M)
+__device__ __forceinline__ double polar_epoch(Ctx& C, const CurveRec& cv, const FsOptions& opt,
+                                              int* head, int* runs)
 {
     if (C.overflow) return __builtin_nan("");
-    const int nbin = cv.source_bins > 0 ? cv.source_bins : opt.source_bins;
-    if (method == LCBG_METHOD_INVERSE_RAY_POLAR) {
-        const int pbins = opt.polar_source_bins > 0 ? opt.polar_source_bins : nbin;
-        const int psb = pbins > 1 ? pbins : 1;
-        double ratio = opt.polar_grid_ratio > 0.0 ? opt.polar_grid_ratio : opt.grid_ratio;
-        ratio = dmax(ratio, 1.0e-12);
-        if (opt.mode == 4) ratio = dmax(ratio, 12.0);
-        return polar_mag(C, psb, ratio, M.head, M.runs);
-    }
-    return cartesian_mag(C, nbin, M.rows, M.row_key, M.areaimage, M.flags);
+    const int pbins = opt.polar_source_bins > 0 ? opt.polar_source_bins : curve_nbin(cv, opt);
+    const int psb = pbins > 1 ? pbins : 1;
+    double ratio = opt.polar_grid_ratio > 0.0 ? opt.polar_grid_ratio : opt.grid_ratio;
+    ratio = dmax(ratio, 1.0e-12);
+    if (opt.mode == 4) ratio = dmax(ratio, 12.0);
+    return polar_mag(C, psb, ratio, head, runs);
 }
 
-__global__ void __launch_bounds__(kWave) k_fs_ir(const CurveRec* __restrict__ curves, FsCache cache,
-                                                  FsTables tab, FsOptions opt, FsEpochs ep,
-                                                  IrScratch scr)
+__device__ __forceinline__ double cart_epoch(Ctx& C, const CurveRec& cv, const FsOptions& opt,
+                                             const CartScratch& scr, int slot)
 {
-    const SlotMem M = slot_mem(scr, blockIdx.x);
-    const int count = uni(static_cast<int>(ep.counters[2]));
+    if (C.overflow) return __builtin_nan("");
+    double* areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
+    int* flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
+    double* rows = scr.rows + static_cast<size_t>(slot) * scr.row_cap * 5;
+    int* row_key = scr.row_key + static_cast<size_t>(slot) * row_ints_per_slot(scr.row_cap);
+    C.row_cap = scr.row_cap;
+    return cartesian_mag(C, curve_nbin(cv, opt), rows, row_key, areaimage, flags);
+}
+
+// Cost bucket of an inverse-ray epoch: the lattice cells of an epoch scale as
+// |A_ps| nbin^2, and the largest epochs are started first so that none of
+// them starts last.
+__device__ __forceinline__ int cost_bucket(double a_ps, int nbin)
+{
+    const double est = fabs(a_ps) * static_cast<double>(nbin) * static_cast<double>(nbin);
+    int lg = est >= 1.0 ? ilogb(est) : 0;
+    lg = lg < kCostBuckets - 1 ? lg : kCostBuckets - 1;
+    return kCostBuckets - 1 - lg;
+}
+
+__global__ void k_ir_bucket_count(const CurveRec* __restrict__ curves, FsOptions opt, FsEpochs ep,
+                                  IrLists L)
+{
+    const int n = static_cast<int>(ep.counters[kQIrCount]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int64_t e = ep.ir_list[i];
+        const CurveRec cv = curves[ep.curve_of[e]];
+        const int b = cost_bucket(ep.a_ps[e], curve_nbin(cv, opt));
+        L.bucket_of[i] = b;
+        atomicAdd(&ep.counters[kBucketOffset + b], 1u);
+    }
+}
+
+__global__ void k_ir_bucket_scan(FsEpochs ep)
+{
+    if (threadIdx.x != 0) return;
+    unsigned run = 0;
+    for (int b = 0; b < kCostBuckets; ++b) {
+        const unsigned c = ep.counters[kBucketOffset + b];
+        ep.counters[kBucketOffset + kCostBuckets + b] = run;
+        run += c;
+    }
+}
+
+__global__ void k_ir_bucket_scatter(FsEpochs ep, IrLists L)
+{
+    const int n = static_cast<int>(ep.counters[kQIrCount]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned pos = atomicAdd(&ep.counters[kBucketOffset + kCostBuckets + L.bucket_of[i]], 1u);
+        L.ir_sorted[pos] = ep.ir_list[i];
+    }
+}
+
+// Seed search of every inverse-ray epoch (largest first); the seeds go to the
+// shared pool and the epoch to the Cartesian or polar queue.
+__global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__ curves,
+                                                     FsCache cache, FsTables tab, FsOptions opt,
+                                                     FsEpochs ep, IrLists L, SeedScratch scr)
+{
+    Pt* slot_seeds = scr.seeds + static_cast<size_t>(blockIdx.x) * kSeedCap;
+    const int count = uni(static_cast<int>(ep.counters[kQIrCount]));
     for (;;) {
-        const int item = next_item(&ep.counters[3]);
+        const int item = next_item(&ep.counters[kQIrCursor]);
         if (item >= count) break;
-        const int64_t e = ep.ir_list[item];
+        const int64_t e = L.ir_sorted[item];
         const long long t_begin = static_cast<long long>(wall_clock64());
         const int c = ep.curve_of[e];
         const CurveRec cv = curves[c];
         Ctx C;
-        init_ctx(C, cv, ep.src[e], M.seeds);
+        init_ctx(C, cv, ep.src[e], slot_seeds);
         epoch_seeds(C, cache, tab, ep, c, e);
-        const double mag = epoch_area(C, cv, opt, ep.method[e], M);
-        wave_store(&ep.mag[e], mag);
-        if (ep.cost) wave_store(&ep.cost[e], static_cast<int64_t>(wall_clock64()) - t_begin);
-        if (C.overflow) atomicOr(&ep.counters[4], kErrSeedCap);
+        const int n = C.n_seeds;
+        bool ok = !C.overflow;
+        int off = 0;
+        if (ok && n > 0) {
+            off = wave_fetch_add(&ep.counters[kQPoolCursor], static_cast<unsigned>(n));
+            if (static_cast<unsigned>(off) + static_cast<unsigned>(n) > L.pool_cap) {
+                ok = false;
+                atomicOr(&ep.counters[kQFlags], kErrSeedPool);
+            }
+        }
+        if (ok) {
+            wave_sync();
+            for (int k = C.lane; k < n; k += kWave) L.seed_pool[off + k] = slot_seeds[k];
+            wave_store(&L.seed_off[item], off);
+            wave_store(&L.seed_n[item], n);
+            const bool polar = uni(ep.method[e]) == LCBG_METHOD_INVERSE_RAY_POLAR;
+            // an image of |A| times the source's area scans about 2 |A| nbin
+            // rows; such epochs start on a large slot right away
+            const bool large = fabs(ep.a_ps[e]) * static_cast<double>(curve_nbin(cv, opt)) > 8192.0;
+            const int qword = polar ? (large ? kQPolarBigCount : kQPolarCount)
+                                    : (large ? kQCartBigCount : kQCartCount);
+            int* qarr = polar ? (large ? L.polar_big : L.polar_q) : (large ? L.cart_big : L.cart_q);
+            const int pos = wave_fetch_inc(&ep.counters[qword]);
+            wave_store(&qarr[pos], item);
+        } else {
+            wave_store(&L.seed_n[item], -1);
+            wave_store(&ep.mag[e], __builtin_nan(""));
+            if (C.overflow) atomicOr(&ep.counters[kQFlags], kErrSeedCap);
+        }
+        flush_work(C, ep);
+        const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
+        add_total(&work_totals(ep)[5], ticks);
+        if (ep.cost) wave_store(&ep.cost[e], static_cast<int64_t>(ticks));
         wave_sync();
     }
 }
 
-// One epoch of the last run, on slot 0: with n_in < 0 the seeds are searched
-// and written to seeds_io (*n_out = their count, up to kSeedCap); otherwise the
-// n_in seeds of seeds_io are used.  *mag_out receives the inverse-ray
-// magnification with method `method`.
+// Cartesian row scans of the queued epochs, in one of three passes: small
+// slots (an epoch that runs out of rows is queued for the retry pass), large
+// slots for the largest epochs, and the retry pass on large slots.
+__global__ void __launch_bounds__(kWave) k_ir_cart(const CurveRec* __restrict__ curves,
+                                                    FsOptions opt, FsEpochs ep, IrLists L,
+                                                    CartScratch scr, int pass)
+{
+    const bool large = pass != kPassSmall;
+    const int* queue = pass == kPassSmall ? L.cart_q : pass == kPassBig ? L.cart_big : L.cart_retry;
+    const int qword = pass == kPassSmall ? kQCartCount : pass == kPassBig ? kQCartBigCount : kQCartRetryCount;
+    unsigned* cursor = &ep.counters[qword + 1];
+    const int count = uni(static_cast<int>(ep.counters[qword]));
+    for (;;) {
+        const int qi = next_item(cursor);
+        if (qi >= count) break;
+        const int item = uni(queue[qi]);
+        const int64_t e = L.ir_sorted[item];
+        const long long t_begin = static_cast<long long>(wall_clock64());
+        const CurveRec cv = curves[ep.curve_of[e]];
+        Ctx C;
+        init_ctx(C, cv, ep.src[e], L.seed_pool + uni(L.seed_off[item]));
+        C.n_seeds = uni(L.seed_n[item]);
+        const double mag = cart_epoch(C, cv, opt, scr, blockIdx.x);
+        if (C.cap_overflow && !large) {
+            const int pos = wave_fetch_inc(&ep.counters[kQCartRetryCount]);
+            wave_store(&L.cart_retry[pos], item);
+        } else {
+            wave_store(&ep.mag[e], mag);
+            if (C.overflow) atomicOr(&ep.counters[kQFlags], C.cap_overflow ? kErrRowCap : kErrBudget);
+        }
+        flush_work(C, ep);
+        const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
+        add_total(&work_totals(ep)[6], ticks);
+        if (ep.cost) wave_store(&ep.cost[e], ep.cost[e] + static_cast<int64_t>(ticks));
+        wave_sync();
+    }
+}
+
+// Polar flood fills of the queued epochs, in the same three passes.
+__global__ void __launch_bounds__(kWave) k_ir_polar(const CurveRec* __restrict__ curves,
+                                                     FsOptions opt, FsEpochs ep, IrLists L,
+                                                     PolarScratch scr, int pass)
+{
+    int* head = scr.phi_head + static_cast<size_t>(blockIdx.x) * kPhiHash;
+    int* runs = scr.runs + static_cast<size_t>(blockIdx.x) * scr.run_cap * 4;
+    const bool large = pass != kPassSmall;
+    const int* queue = pass == kPassSmall ? L.polar_q : pass == kPassBig ? L.polar_big : L.polar_retry;
+    const int qword = pass == kPassSmall ? kQPolarCount : pass == kPassBig ? kQPolarBigCount : kQPolarRetryCount;
+    unsigned* cursor = &ep.counters[qword + 1];
+    const int count = uni(static_cast<int>(ep.counters[qword]));
+    for (;;) {
+        const int qi = next_item(cursor);
+        if (qi >= count) break;
+        const int item = uni(queue[qi]);
+        const int64_t e = L.ir_sorted[item];
+        const long long t_begin = static_cast<long long>(wall_clock64());
+        const CurveRec cv = curves[ep.curve_of[e]];
+        Ctx C;
+        init_ctx(C, cv, ep.src[e], L.seed_pool + uni(L.seed_off[item]));
+        C.n_seeds = uni(L.seed_n[item]);
+        C.run_cap = scr.run_cap;
+        const double mag = polar_epoch(C, cv, opt, head, runs);
+        if (C.cap_overflow && !large) {
+            const int pos = wave_fetch_inc(&ep.counters[kQPolarRetryCount]);
+            wave_store(&L.polar_retry[pos], item);
+        } else {
+            wave_store(&ep.mag[e], mag);
+            if (C.overflow) atomicOr(&ep.counters[kQFlags], C.cap_overflow ? kErrPolarCap : kErrBudget);
+        }
+        flush_work(C, ep);
+        const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
+        add_total(&work_totals(ep)[6], ticks);
+        if (ep.cost) wave_store(&ep.cost[e], ep.cost[e] + static_cast<int64_t>(ticks));
+        wave_sync();
+    }
+}
+
+// One epoch of the last run, on the first seed slot and the first large
+// Cartesian / polar slot: with n_in < 0 the seeds are searched (phases up to
+// last_phase) and written to seeds_io (*n_out = their count, up to kSeedCap);
+// otherwise the n_in seeds of seeds_io are used.  *mag_out receives the
+// inverse-ray magnification with method `method`.
 __global__ void __launch_bounds__(kWave) k_ir_epoch_debug(const CurveRec* __restrict__ curves,
                                                           FsCache cache, FsTables tab,
                                                           FsOptions opt, FsEpochs ep,
@@ -1264,24 +1495,80 @@ __global__ void __launch_bounds__(kWave) k_ir_epoch_debug(const CurveRec* __rest
                                                           int last_phase, Pt* seeds_io, int n_in,
                                                           int* n_out, double* mag_out)
 {
-    const SlotMem M = slot_mem(scr, 0);
+    Pt* slot_seeds = scr.seed.seeds;
     const int c = ep.curve_of[e];
     const CurveRec cv = curves[c];
     Ctx C;
-    init_ctx(C, cv, ep.src[e], M.seeds);
+    init_ctx(C, cv, ep.src[e], slot_seeds);
     if (last_phase >= 0) C.last_phase = last_phase;
     if (n_in < 0) {
         epoch_seeds(C, cache, tab, ep, c, e);
         wave_sync();
-        for (int k = C.lane; k < C.n_seeds; k += kWave) seeds_io[k] = M.seeds[k];
+        for (int k = C.lane; k < C.n_seeds; k += kWave) seeds_io[k] = slot_seeds[k];
     } else {
-        for (int k = C.lane; k < n_in; k += kWave) M.seeds[k] = seeds_io[k];
+        for (int k = C.lane; k < n_in; k += kWave) slot_seeds[k] = seeds_io[k];
         C.n_seeds = n_in;
     }
     wave_sync();
     wave_store(n_out, C.overflow ? -1 : C.n_seeds);
-    const double mag = epoch_area(C, cv, opt, method, M);
+    double mag;
+    if (method == LCBG_METHOD_INVERSE_RAY_POLAR) {
+        C.run_cap = scr.polar_large.run_cap;
+        mag = polar_epoch(C, cv, opt, scr.polar_large.phi_head, scr.polar_large.runs);
+    } else {
+        mag = cart_epoch(C, cv, opt, scr.cart_large, 0);
+    }
     wave_store(mag_out, mag);
+}
+
+lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
+                         const FsOptions& opt, const FsEpochs& ep, const IrLists& lists,
+                         const IrScratch& scr, hipStream_t st, const IrStreams& ss)
+{
+    hipLaunchKernelGGL(k_ir_bucket_count, dim3(256), dim3(256), 0, st, d_curves, opt, ep, lists);
+    hipLaunchKernelGGL(k_ir_bucket_scan, dim3(1), dim3(64), 0, st, ep);
+    hipLaunchKernelGGL(k_ir_bucket_scatter, dim3(256), dim3(256), 0, st, ep, lists);
+    hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
+                       d_curves, cache, tab, opt, ep, lists, scr.seed);
+    lcbg_status status = check_launch();
+    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.fork, st));
+    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.polar, ss.fork, 0));
+    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.big, ss.fork, 0));
+    if (status != LCBG_OK) return status;
+    // the largest epochs on large slots and the polar epochs beside the
+    // Cartesian small-slot pass; launched first so their waves are resident
+    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots)), dim3(kWave), 0,
+                       ss.big, d_curves, opt, ep, lists, scr.cart_large, int{kPassBig});
+    hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar_large.n_slots)), dim3(kWave), 0,
+                       ss.big, d_curves, opt, ep, lists, scr.polar_large, int{kPassBig});
+    hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar.n_slots)), dim3(kWave), 0,
+                       ss.polar, d_curves, opt, ep, lists, scr.polar, int{kPassSmall});
+    status = check_launch();
+    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_big, ss.big));
+    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_polar, ss.polar));
+    if (status != LCBG_OK) return status;
+    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart.n_slots)), dim3(kWave), 0, st,
+                       d_curves, opt, ep, lists, scr.cart, int{kPassSmall});
+    status = check_launch();
+    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(st, ss.join_big, 0));
+    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(st, ss.join_polar, 0));
+    if (status != LCBG_OK) return status;
+    // the epochs that outgrew a small slot, on the (now free) large slots
+    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots)), dim3(kWave), 0,
+                       st, d_curves, opt, ep, lists, scr.cart_large, int{kPassRetry});
+    hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar_large.n_slots)), dim3(kWave), 0,
+                       st, d_curves, opt, ep, lists, scr.polar_large, int{kPassRetry});
+    return check_launch();
+}
+
+int ir_resident_per_cu(int kind)
+{
+    const void* fn = kind == 0 ? reinterpret_cast<const void*>(k_ir_seeds)
+                   : kind == 1 ? reinterpret_cast<const void*>(k_ir_cart)
+                               : reinterpret_cast<const void*>(k_ir_polar);
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kWave, 0) != hipSuccess || n < 1) n = 1;
+    return n;
 }
 
 lcbg_status launch_ir_epoch_debug(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
@@ -1291,15 +1578,6 @@ lcbg_status launch_ir_epoch_debug(const CurveRec* d_curves, const FsCache& cache
 {
     hipLaunchKernelGGL(k_ir_epoch_debug, dim3(1), dim3(kWave), 0, st, d_curves, cache, tab, opt,
                        ep, scr, e, method, last_phase, seeds_io, n_in, n_out, mag_out);
-    return check_launch();
-}
-
-lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
-                         const FsOptions& opt, const FsEpochs& ep, const IrScratch& scr,
-                         hipStream_t st)
-{
-    hipLaunchKernelGGL(k_fs_ir, dim3(static_cast<unsigned>(scr.n_slots)), dim3(kWave), 0, st,
-                       d_curves, cache, tab, opt, ep, scr);
     return check_launch();
 }
 

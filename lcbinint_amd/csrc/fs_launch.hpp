@@ -22,11 +22,20 @@ lcbg_status launch_fs_stage1(const CurveRec* d_curves, int n_curves, const FsCac
 lcbg_status launch_fs_stage2(const CurveRec* d_curves, const FsCache& cache, const FsOptions& opt,
                              const FsEpochs& ep, int n_waves, hipStream_t st);
 
-// Stage 3 (persistent waves over the queue): image seeds, then the Cartesian
-// row scan or the polar flood fill.
+// Stage 3: the inverse-ray epochs ordered by estimated cost, their image seeds
+// (persistent waves), then the Cartesian row scans on `st` beside the polar
+// flood fills and the large-slot pass of the largest epochs on the two side
+// streams (forked and joined through the events), then a large-slot retry
+// pass for the epochs that outgrew a small slot.
+struct IrStreams {
+    hipStream_t polar;   // polar small-slot pass
+    hipStream_t big;     // large-slot pass of the largest epochs
+    hipEvent_t fork, join_polar, join_big;
+};
+
 lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
-                         const FsOptions& opt, const FsEpochs& ep, const IrScratch& scr,
-                         hipStream_t st);
+                         const FsOptions& opt, const FsEpochs& ep, const IrLists& lists,
+                         const IrScratch& scr, hipStream_t st, const IrStreams& ss);
 
 // One inverse-ray epoch of the last run on slot 0 (diagnostics): seed search
 // (n_in < 0) or given seeds, then the magnification with `method`.
@@ -34,6 +43,10 @@ lcbg_status launch_ir_epoch_debug(const CurveRec* d_curves, const FsCache& cache
                                   const FsOptions& opt, const FsEpochs& ep, const IrScratch& scr,
                                   int64_t e, int method, int last_phase, Pt2* seeds_io, int n_in,
                                   int* n_out, double* mag_out, hipStream_t st);
+
+// Workgroups of one inverse-ray kernel resident per CU (kind 0 seeds,
+// 1 Cartesian, 2 polar), from its register and LDS use.
+int ir_resident_per_cu(int kind);
 
 // Zeroes the queue counters in-stream with a kernel (graph-capturable).
 lcbg_status launch_reset_counters(unsigned* counters, int n, hipStream_t st);

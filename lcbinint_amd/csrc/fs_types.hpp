@@ -31,16 +31,22 @@ constexpr int kInteriorProbes = 64;     // finite_source_magnifier.cpp:1604
 // seed.  Phi rows: the polar lattice has ceil(2 pi R / (dr g)) rows, about
 // 2.1e6 at rho 1e-4, nbin 100, g 12 and R 4.
 // Row keys and phi rows are found through hashed chains (kRowHash, kPhiHash
-// heads), so no dense per-row array is needed.
+// heads), so no dense per-row array is needed.  Slots come in two tiers: many
+// small ones for the first pass and a few large ones that redo the epochs
+// which ran out of a small slot's rows or runs.
 constexpr int kSeedCap = 1024;
-constexpr int kRowCap = 1 << 17;
+constexpr int kRowCapSmall = 1 << 17;
+constexpr int kRowCapLarge = 1 << 18;
+constexpr int kRunCapSmall = 1 << 16;
+constexpr int kRunCapLarge = 1 << 20;
 constexpr int kRowHash = 1 << 16;
 constexpr int kPhiCap = 1 << 30;
 constexpr int kPhiHash = 1 << 16;
-constexpr int kRunCap = 1 << 18;
-// ints per slot of IrScratch::row_key: key, idx, next per row, the hash heads,
-// and the count of keys whose heads are still set (padded to 64)
-constexpr int kRowIntsPerSlot = 3 * kRowCap + kRowHash + 64;
+// ints per slot of CartScratch::row_key: key, idx, next per row, the hash
+// heads, and the count of keys whose heads are still set (padded to 64)
+constexpr long long row_ints_per_slot(int row_cap) { return 3ll * row_cap + kRowHash + 64; }
+// cost buckets of the inverse-ray work order (largest a_ps * nbin^2 first)
+constexpr int kCostBuckets = 32;
 
 struct CausticScalars {
     double min_x, max_x, min_y, max_y, step_x, step_y, max_seg_len;
@@ -96,16 +102,71 @@ struct FsEpochs {
     int64_t* cost;      // [epoch] wall-clock ticks of each inverse-ray epoch, or null
 };
 
-struct IrScratch {
+// Scratch of the resident waves of each inverse-ray kernel.
+struct SeedScratch {
     int n_slots;
     Pt2* seeds;        // [slot][kSeedCap]
+};
+
+struct CartScratch {
+    int n_slots;
+    int row_cap;
     double* areaimage; // [slot][kSeedCap]
     int* overlap;      // [slot][kSeedCap]
-    double* rows;      // [slot][kRowCap*5]: xmin, xmax, ax, y, dys
-    int* row_key;      // [slot][kRowIntsPerSlot]: key, idx, next, heads, dirty count
-    int* phi_head;     // [slot][kPhiHash]: hashed run-chain heads
-    int* runs;         // [slot][kRunCap*4]: iphi, left, right, next
+    double* rows;      // [slot][row_cap*5]: xmin, xmax, ax, y, dys
+    int* row_key;      // [slot][row_ints_per_slot(row_cap)]: key, idx, next, heads, dirty
 };
+
+struct PolarScratch {
+    int n_slots;
+    int run_cap;
+    int* phi_head;     // [slot][kPhiHash]: hashed run-chain heads
+    int* runs;         // [slot][run_cap*4]: iphi, left, right, next
+};
+
+struct IrScratch {
+    SeedScratch seed;
+    CartScratch cart, cart_large;
+    PolarScratch polar, polar_large;
+};
+
+// Work lists of the inverse-ray stage, in the epoch workspace.  Items index
+// ir_sorted; each holds its seeds in the shared seed pool.
+struct IrLists {
+    int64_t* ir_sorted;  // [cap] inverse-ray epochs, largest estimated cost first
+    int* bucket_of;      // [cap] cost bucket of ir_list[i]
+    int* seed_off;       // [cap] first seed of item i in seed_pool
+    int* seed_n;         // [cap] seed count of item i (-1: none, the epoch failed)
+    int* cart_q;         // [cap] items for the Cartesian kernel
+    int* polar_q;        // [cap] items for the polar kernel
+    int* cart_retry;     // [cap] items that ran out of a small Cartesian slot
+    int* polar_retry;    // [cap] items that ran out of a small polar slot
+    int* cart_big;       // [cap] largest Cartesian items, large slots from the start
+    int* polar_big;      // [cap] largest polar items
+    Pt2* seed_pool;      // [pool_cap]
+    unsigned pool_cap;
+};
+
+// counters[kWorkOffset..] hold 8 uint64 work totals of the inverse-ray stage:
+// quintic solves, quartic solves, Cartesian cells, guarded Cartesian cells,
+// polar cells, seed-search and area ticks, rows; counters[kBucketOffset..]
+// the cost-bucket counts and then their cursors.  All are reset each run.
+// Queue words: near and inverse-ray epoch lists, error flags, the Cartesian
+// and polar queues of the small-slot pass, of the large-slot pass for the
+// largest epochs, and of the large-slot retry pass, the seed-pool cursor.
+constexpr int kWorkOffset = 24;
+constexpr int kBucketOffset = 40;
+constexpr int kCounterWords = kBucketOffset + 2 * kCostBuckets;
+enum : int {
+    kQNearCount = 0, kQNearCursor = 1, kQIrCount = 2, kQIrCursor = 3, kQFlags = 4,
+    kQCartCount = 5, kQCartCursor = 6, kQPolarCount = 7, kQPolarCursor = 8,
+    kQCartRetryCount = 9, kQCartRetryCursor = 10, kQPolarRetryCount = 11,
+    kQPolarRetryCursor = 12, kQPoolCursor = 13, kQCartBigCount = 14, kQCartBigCursor = 15,
+    kQPolarBigCount = 16, kQPolarBigCursor = 17
+};
+// inverse-ray area passes: small slots, large slots for the largest epochs
+// (beside the small pass), large slots for the epochs a small slot could not hold
+enum : int { kPassSmall = 0, kPassBig = 1, kPassRetry = 2 };
 
 // error flags in counters[4]
 constexpr unsigned kErrSeedCap = 1u;
@@ -113,6 +174,7 @@ constexpr unsigned kErrRowCap = 2u;
 constexpr unsigned kErrPolarCap = 4u;
 constexpr unsigned kErrUnsupported = 8u;
 constexpr unsigned kErrBudget = 16u;
+constexpr unsigned kErrSeedPool = 32u;
 
 // Work budget of one inverse-ray epoch in wave steps (64-cell chunks, probe
 // merges, polar runs); an epoch that exceeds it ends with an error flag, so

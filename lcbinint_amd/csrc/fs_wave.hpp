@@ -53,6 +53,15 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// v of lane l (l uniform), as a scalar.
+__device__ __forceinline__ double read_lane(double v, int l)
+{
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u & 0xffffffffull), l));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), l));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 // Minimum value and the smallest index attaining it (ties -> lowest index;
 // indices with a NaN value never win).
 __device__ __forceinline__ void wave_argmin(double& v, int& idx)
@@ -69,7 +78,18 @@ __device__ __forceinline__ void wave_argmin(double& v, int& idx)
     }
 }
 
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+// The wave kernels run one wavefront per workgroup, and their control flow is
+// scalar (below), so every memory operation is issued by the whole wave in
+// program order; the hardware performs one wavefront's accesses in that order
+// (the AMDGPU memory model needs no wait for wavefront-scope ordering).
+// wave_sync() therefore only has to keep the compiler from moving memory
+// operations across it: a wavefront-scope fence, which emits no wait for
+// outstanding stores, plus a wave barrier for convergent operations.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
 
 // Values of the wave's control program are the same on every lane.  uni()
 // hands such a value to the compiler as a scalar (lane 0's copy, which equals
@@ -116,6 +136,14 @@ __device__ __forceinline__ void wave_store(T* p, typename same_type<T>::type v)
 __device__ __forceinline__ int wave_fetch_inc(unsigned* counter)
 {
     const unsigned inc = lane_id() == 0 ? 1u : 0u;
+    return __builtin_amdgcn_readfirstlane(static_cast<int>(atomicAdd(counter, inc)));
+}
+
+// atomicAdd(counter, n) on behalf of the wave (n uniform); every lane receives
+// the old value.
+__device__ __forceinline__ int wave_fetch_add(unsigned* counter, unsigned n)
+{
+    const unsigned inc = lane_id() == 0 ? n : 0u;
     return __builtin_amdgcn_readfirstlane(static_cast<int>(atomicAdd(counter, inc)));
 }
 

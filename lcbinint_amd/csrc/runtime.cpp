@@ -142,6 +142,30 @@ extern "C" lcbg_status lcbg_batch_destroy(lcbg_batch* batch)
     return LCBG_OK;
 }
 
+extern "C" lcbg_status lcbg_batch_update_curves(lcbg_batch* batch, const lcbg_curve* curves,
+                                                void* stream)
+{
+    if (!batch || (batch->n_curves > 0 && !curves)) return LCBG_INVALID_ARGUMENT;
+    for (int32_t c = 0; c < batch->n_curves; ++c) {
+        if (!std::isfinite(curves[c].tE) || curves[c].tE == 0.0 || !std::isfinite(curves[c].q) ||
+            !std::isfinite(curves[c].s))
+            return LCBG_INVALID_ARGUMENT;  // LensParameters::is_valid (lens_parameters.cpp:7-11)
+    }
+    for (int32_t c = 0; c < batch->n_curves; ++c) {
+        CurveRec& r = batch->h_curves[static_cast<std::size_t>(c)];
+        const int nb = r.source_bins;
+        r = make_curve_rec(curves[c]);
+        r.source_bins = nb;
+    }
+    batch->caches_stale = true;
+    if (batch->n_curves == 0) return LCBG_OK;
+    // pageable source: the copy is staged before the call returns, and the
+    // stream orders it after the batch's earlier work on that stream
+    return map_hip(hipMemcpyAsync(batch->d_curves, batch->h_curves.data(),
+                                  batch->h_curves.size() * sizeof(CurveRec), hipMemcpyHostToDevice,
+                                  static_cast<hipStream_t>(stream)));
+}
+
 extern "C" lcbg_status lcbg_batch_set_source_bins(lcbg_batch* batch, const int32_t* source_bins)
 {
     if (!batch || (batch->n_curves > 0 && !source_bins)) return LCBG_INVALID_ARGUMENT;

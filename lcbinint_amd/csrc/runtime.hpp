@@ -53,4 +53,5 @@ struct lcbg_batch {
     std::vector<lcbg::CurveRec> h_curves; // host copy of the same records
     lcbg::FsState* fs = nullptr;         // finite-source state, built on first use
     int64_t* d_cost = nullptr;           // optional per-epoch inverse-ray cost output (device)
+    bool caches_stale = true;            // per-curve caustic caches need a rebuild
 };
