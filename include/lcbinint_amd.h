@@ -133,6 +133,37 @@ lcbg_status lcbg_batch_finite_source_device(lcbg_batch* batch, const int64_t* of
                                             const double* times, int64_t n_epochs, double* mag,
                                             int32_t* method, void* stream);
 
+/* Value and gradient of the finite-source light curves (no counterpart in the
+ * reference, which has no derivatives).  Runs lcbg_batch_finite_source_device
+ * (mag, method as there, the reference's values) and then, for every epoch,
+ * the derivatives of the magnification with respect to the curve parameters
+ * (t0, tE, u0, alpha, s, q, rho, u_LD), dmag[8*i + k] (device, n_epochs x 8),
+ * q and alpha as in lcbg_curve.  Point-source and hexadecapole epochs are
+ * differentiated exactly (implicit differentiation of the lens equation per
+ * image); inverse-ray epochs through an antialiased quadrature on a lattice
+ * anchored at the lens-frame origin (cell side rho/nbin, the source-disk
+ * indicator replaced by the covered fraction of each cell).  mag_aa (may be
+ * NULL) receives the value of the differentiated function: equal to mag up to
+ * rounding for point-source and hexadecapole epochs, the antialiased
+ * quadrature for inverse-ray epochs. */
+lcbg_status lcbg_batch_finite_source_grad_device(lcbg_batch* batch, const int64_t* offsets,
+                                                 const double* times, int64_t n_epochs,
+                                                 double* mag, int32_t* method, double* mag_aa,
+                                                 double* dmag, void* stream);
+
+/* chi^2 of each curve against observed fluxes, chi2[c] = sum over its epochs of
+ * ((flux - (fs * mag + fb)) / sigma)^2 with mag the finite-source
+ * magnification, and its gradient grad[8*c + k] with respect to (t0, tE, u0,
+ * alpha, s, q, rho, u_LD) through the derivatives of
+ * lcbg_batch_finite_source_grad_device (fs and fb held fixed).  flux and sigma
+ * have n_epochs entries; mag, method (and mag_aa, dmag if not NULL) are filled
+ * as there.  Device pointers. */
+lcbg_status lcbg_batch_chi2_grad_device(lcbg_batch* batch, const int64_t* offsets,
+                                        const double* times, int64_t n_epochs, const double* flux,
+                                        const double* sigma, double fs, double fb, double* mag,
+                                        int32_t* method, double* mag_aa, double* dmag,
+                                        double* chi2, double* grad, void* stream);
+
 /* lcbg_batch_finite_source_device, then waits and reports the device time of
  * its four stages (stage_ms[4]: caustic-cache build (0 when the curves did not
  * change), point source + bounding box, caustic distance + hexadecapole,

@@ -14,6 +14,13 @@ build module stays importable (``python -m lcbinint_amd.build``).
 """
 _API = ("LightCurve", "Options", "LimbDarkening", "OrbitalMotionMode", "FiniteSourceMethod")
 
+# torch first, so that the extension binds to torch's HIP runtime (one HIP
+# runtime per process; see capi.py)
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 try:
     from ._lcbinint import (  # noqa: F401
         FiniteSourceMethod,

@@ -14,6 +14,14 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "liblcbinint_amd.so"
 HEADER = PKG.parent / "include" / "lcbinint_amd.h"
 
+# PyTorch-ROCm carries its own HIP runtime.  Loading torch before this
+# library makes the library bind to that same runtime; loading the library
+# first would start the system runtime, and torch would then find no GPU.
+try:
+    import torch  # noqa: F401
+except ImportError:  # the library itself does not need torch
+    pass
+
 if not LIB_PATH.exists():
     raise ImportError(
         f"{LIB_PATH} is not built; run `python -m lcbinint_amd.build` (needs hipcc for gfx950)"
@@ -220,3 +228,33 @@ def work_counts(batch: Batch):
     check(lcbg_batch_work_counts(batch.handle, out), "lcbg_batch_work_counts")
     return dict(zip(("quintic", "quartic", "cart", "cart_guarded", "polar", "seed_ticks",
                      "area_ticks", "rows"), list(out)))
+
+lcbg_batch_finite_source_grad_device = _sig(
+    "lcbg_batch_finite_source_grad_device", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_void_p)
+lcbg_batch_chi2_grad_device = _sig(
+    "lcbg_batch_chi2_grad_device", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+    ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+
+GRAD_PARAMS = ("t0", "tE", "u0", "alpha", "s", "q", "rho", "ld_c")
+
+
+def finite_source_grad(batch: Batch, offsets_ptr: int, times_ptr: int, n_epochs: int, mag_ptr: int,
+                       method_ptr: int, mag_aa_ptr: int, dmag_ptr: int, stream: int = 0):
+    """Values, methods, differentiated values and d mag / d GRAD_PARAMS (n x 8)."""
+    check(lcbg_batch_finite_source_grad_device(batch.handle, offsets_ptr, times_ptr, n_epochs,
+                                               mag_ptr, method_ptr, mag_aa_ptr, dmag_ptr, stream),
+          "lcbg_batch_finite_source_grad_device")
+
+
+def chi2_grad(batch: Batch, offsets_ptr: int, times_ptr: int, n_epochs: int, flux_ptr: int,
+              sigma_ptr: int, fs: float, fb: float, mag_ptr: int, method_ptr: int,
+              chi2_ptr: int, grad_ptr: int, mag_aa_ptr=None, dmag_ptr=None, stream: int = 0):
+    """chi^2 per curve and its gradient with respect to GRAD_PARAMS (n_curves x 8)."""
+    check(lcbg_batch_chi2_grad_device(batch.handle, offsets_ptr, times_ptr, n_epochs, flux_ptr,
+                                      sigma_ptr, fs, fb, mag_ptr, method_ptr, mag_aa_ptr, dmag_ptr,
+                                      chi2_ptr, grad_ptr, stream),
+          "lcbg_batch_chi2_grad_device")

@@ -1,0 +1,628 @@
+// Value-and-gradient kernels of the finite-source light curve.
+//
+// Per epoch the derivative of the magnification is taken for the method the
+// dispatcher chose in the value run, with forward-mode tangents (dual.hpp) in
+// six slots: source x, y, s, q_eff, rho and the linear limb-darkening c.  The
+// reference has no gradients; these follow the mathematics of each method:
+//
+//  * point source: each physical image z of the source y solves the lens
+//    equation w(z; s, q) = y; its tangent is J^-1 (dy - dw/dp dp) (implicit
+//    differentiation), and A = sum 1 / |det J(z)| is differentiated through z,
+//    s and q;
+//  * hexadecapole: the 13 point-source samples of hexadecapole_binary
+//    (finite_source_magnifier.cpp:792-847) at y + rho e_k, each with its own
+//    implicit-differentiation tangent, combined by the same formula;
+//  * inverse ray: an antialiased quadrature on a lattice anchored at the
+//    origin of the lens frame, cell (i, j) at z = h (i + 1/2, j + 1/2) with
+//    h the power of two at or below rho / nbin, fixed while the parameters
+//    move (a lattice that scaled with rho would sweep every edge cell across
+//    the edge |z| / rho times faster than the source moves).  With
+//    g = rho^2 - |w(z) - y|^2 the source brightness is
+//    (1 - c) [g > 0] + c sqrt(max(g, 0)) / rho; each cell weighs it averaged
+//    over its footprint, g linearised over the cell and smeared by a Gaussian
+//    of one cell (the antialiased edge), and A = sum W h^2 / (pi rho^2
+//    (1 - c/3)).  The cells with nonzero weight are enumerated by a scan-line
+//    flood fill from the value run's image seeds.  This is a differentiable
+//    stand-in for the plain quadrature, which it matches to its
+//    discretisation error; the value run's magnifications are unchanged.
+//
+// k_grad_chain turns the six tangents into derivatives with respect to the
+// curve parameters (t0, tE, u0, alpha, s, q, rho, u_LD) and reduces chi^2 and
+// its gradient per curve.
+#include "dual.hpp"
+#include "fs_launch.hpp"
+#include "fs_math.hpp"
+#include "fs_wave.hpp"
+
+namespace lcbg {
+
+namespace {
+
+constexpr double kSqrtHalfG = 0.70710678118654752440;
+
+struct DLens {
+    Dual a, m1, m2;
+};
+
+// make_lens with tangents
+__device__ __forceinline__ DLens make_dlens(const Dual& s, const Dual& q)
+{
+    const bool small = q.v < 1.0;
+    const Dual qq = small ? q : 1.0 / q;
+    DLens L;
+    L.a = small ? -s : s;
+    L.m1 = 1.0 / (1.0 + qq);
+    L.m2 = qq * L.m1;
+    return L;
+}
+
+template <class T>
+struct MapOut {
+    T wx, wy, kr, ki;  // w(z), and kappa = dw/d(conj z) = m1/(zbar-a)^2 + m2/zbar^2
+};
+
+// Real lens map w = z - m1 (z - a)/|z - a|^2 - m2 z/|z|^2 - a m1 and kappa;
+// the Jacobian of (x, y) -> w is [[1 + kr, ki], [ki, 1 - kr]].  Divisions are
+// written as products with reciprocals so that the value-only and the tangent
+// evaluations round identically.
+template <class T, class LT>
+__device__ __forceinline__ MapOut<T> lens_map_k(const LT& L, const T& x, const T& y)
+{
+    const T xa = x - L.a;
+    const T y2 = y * y;
+    const T i1 = 1.0 / (xa * xa + y2);
+    const T i2 = 1.0 / (x * x + y2);
+    MapOut<T> m;
+    m.wx = x - L.m1 * xa * i1 - L.m2 * x * i2 - L.a * L.m1;
+    m.wy = y - L.m1 * y * i1 - L.m2 * y * i2;
+    const T i1s = i1 * i1;
+    const T i2s = i2 * i2;
+    m.kr = L.m1 * (xa * xa - y2) * i1s + L.m2 * (x * x - y2) * i2s;
+    m.ki = 2.0 * (L.m1 * xa * y * i1s + L.m2 * x * y * i2s);
+    return m;
+}
+
+// Magnification 1/|det J| of the image at z of source y, with tangents: the
+// image's tangent is J^-1 (dy - dw/dp dp) (implicit differentiation of
+// w(z; s, q) = y).
+__device__ Dual image_dual(const DLens& L, double zxv, double zyv, const Dual& yx, const Dual& yy)
+{
+    const Dual zx0 = dconst(zxv);
+    const Dual zy0 = dconst(zyv);
+    const MapOut<Dual> m0 = lens_map_k(L, zx0, zy0);  // tangents: dw/dp at fixed z
+    const double j11 = 1.0 + m0.kr.v, j12 = m0.ki.v, j22 = 1.0 - m0.kr.v;
+    const double det = j11 * j22 - j12 * j12;
+    const double idet = 1.0 / det;
+    Dual zx = zx0, zy = zy0;
+#pragma unroll
+    for (int k = 0; k < kTangents; ++k) {
+        const double rx = yx.d[k] - m0.wx.d[k];
+        const double ry = yy.d[k] - m0.wy.d[k];
+        zx.d[k] = (j22 * rx - j12 * ry) * idet;
+        zy.d[k] = (-j12 * rx + j11 * ry) * idet;
+    }
+    const MapOut<Dual> m = lens_map_k(L, zx, zy);
+    const Dual detd = 1.0 - (m.kr * m.kr + m.ki * m.ki);
+    return 1.0 / dabs(detd);
+}
+
+// Point-source magnification with tangents.
+__device__ Dual ps_dual(const Dual& yx, const Dual& yy, const Dual& s, const Dual& q)
+{
+    if (s.v == 0.0 || !(q.v > 0.0)) return dconst(0.0);
+    const Geom g = make_geom(s.v, q.v, yx.v, yy.v);
+    cx roots[5];
+    solve_quintic(g, roots);
+    const unsigned mask = select_physical(g, roots);
+    const DLens L = make_dlens(s, q);
+    Dual A = dconst(0.0);
+    for (int i = 0; i < 5; ++i) {
+        if (!(mask & (1u << i))) continue;
+        A += image_dual(L, roots[i].re, roots[i].im, yx, yy);
+    }
+    return A;
+}
+
+// hexadecapole_binary (finite_source_magnifier.cpp:792-847) with tangents; the
+// linear limb-darkening Gamma = 10 c / (15 - 5 c), Lambda = 0 (:349-367).
+__device__ Dual hex_dual(const Dual& yx, const Dual& yy, const Dual& s, const Dual& q,
+                         const Dual& rho, const Dual& c)
+{
+    const double sineta[8] = {0.0, 1.0, 0.0, -1.0, kSqrtHalfG, kSqrtHalfG, -kSqrtHalfG, -kSqrtHalfG};
+    const double coseta[8] = {1.0, 0.0, -1.0, 0.0, kSqrtHalfG, -kSqrtHalfG, -kSqrtHalfG, kSqrtHalfG};
+    const Dual a0 = ps_dual(yx, yy, s, q);
+    Dual a1p = dconst(0.0), a2p = dconst(0.0), a1c = dconst(0.0);
+    for (int i = 0; i < 4; ++i) {
+        a1p += ps_dual(yx + rho * coseta[i], yy + rho * sineta[i], s, q);
+        a2p += ps_dual(yx + (0.5 * rho) * coseta[i], yy + (0.5 * rho) * sineta[i], s, q);
+        a1c += ps_dual(yx + rho * coseta[i + 4], yy + rho * sineta[i + 4], s, q);
+    }
+    a1p = a1p * 0.25 - a0;
+    a2p = a2p * 0.25 - a0;
+    a1c = a1c * 0.25 - a0;
+    const Dual a2rho2 = (16.0 * a2p - a1p) * (1.0 / 3.0);
+    const Dual a4rho4 = (a1p + a1c) * 0.5 - a2rho2;
+    const Dual den = 15.0 - 5.0 * c;
+    const Dual gamma = den.v == 0.0 ? dconst(0.0) : (10.0 * c) / den;
+    const Dual quad = 0.5 * a2rho2 * (1.0 - 0.2 * gamma);
+    const Dual hex = a4rho4 * (1.0 / 3.0) * (1.0 - (11.0 / 35.0) * gamma);
+    return a0 + quad + hex;
+}
+
+// ------------------------------------------------ antialiased inverse ray
+
+struct GradCtx {
+    Lens L;        // values
+    DLens Ld;      // with tangents
+    Dual yx, yy, rho, c;
+    double q_eff;  // lens-frame mass ratio (value)
+    double h;      // cell side: the power of two at or below rho / nbin
+    int lane;
+};
+
+// 64-point Gauss-Legendre nodes and weights on [-1, 1] (numpy.polynomial.legendre.leggauss(64))
+__constant__ double kGLx[64] = {
+    -0.9993050417357722, -0.9963401167719552, -0.9910133714767443, -0.983336253884626,
+    -0.973326827789911, -0.9610087996520538, -0.9464113748584028, -0.9295691721319396,
+    -0.9105221370785028, -0.8893154459951141, -0.8659993981540928, -0.8406292962525803,
+    -0.8132653151227975, -0.7839723589433414, -0.7528199072605319, -0.7198818501716108,
+    -0.6852363130542333, -0.6489654712546573, -0.6111553551723933, -0.571895646202634,
+    -0.5312794640198946, -0.48940314570705296, -0.4463660172534641, -0.4022701579639916,
+    -0.3572201583376681, -0.31132287199021097, -0.2646871622087674, -0.21742364374000708,
+    -0.1696444204239928, -0.12146281929612056, -0.07299312178779904, -0.02435029266342443,
+    0.02435029266342443, 0.07299312178779904, 0.12146281929612056, 0.1696444204239928,
+    0.21742364374000708, 0.2646871622087674, 0.31132287199021097, 0.3572201583376681,
+    0.4022701579639916, 0.4463660172534641, 0.48940314570705296, 0.5312794640198946,
+    0.571895646202634, 0.6111553551723933, 0.6489654712546573, 0.6852363130542333,
+    0.7198818501716108, 0.7528199072605319, 0.7839723589433414, 0.8132653151227975,
+    0.8406292962525803, 0.8659993981540928, 0.8893154459951141, 0.9105221370785028,
+    0.9295691721319396, 0.9464113748584028, 0.9610087996520538, 0.973326827789911,
+    0.983336253884626, 0.9910133714767443, 0.9963401167719552, 0.9993050417357722,
+};
+__constant__ double kGLw[64] = {
+    0.0017832807216942152, 0.004147033260562923, 0.006504457968979654, 0.008846759826364391,
+    0.011168139460131466, 0.013463047896718231, 0.015726030476025082, 0.0179517157756973,
+    0.020134823153530094, 0.022270173808383007, 0.024352702568710853, 0.026377469715054627,
+    0.028339672614259702, 0.030234657072402495, 0.03205792835485145, 0.03380516183714179,
+    0.03547221325688232, 0.03705512854024015, 0.03855015317861559, 0.03995374113272035,
+    0.041262563242623486, 0.0424735151236536, 0.043583724529323464, 0.044590558163756545,
+    0.045491627927418114, 0.046284796581314375, 0.04696818281621, 0.0475401657148303,
+    0.04799938859645832, 0.048344762234802954, 0.048575467441503456, 0.04869095700913975,
+    0.04869095700913975, 0.048575467441503456, 0.048344762234802954, 0.04799938859645832,
+    0.0475401657148303, 0.04696818281621, 0.046284796581314375, 0.045491627927418114,
+    0.044590558163756545, 0.043583724529323464, 0.0424735151236536, 0.041262563242623486,
+    0.03995374113272035, 0.03855015317861559, 0.03705512854024015, 0.03547221325688232,
+    0.03380516183714179, 0.03205792835485145, 0.030234657072402495, 0.028339672614259702,
+    0.026377469715054627, 0.024352702568710853, 0.022270173808383007, 0.020134823153530094,
+    0.0179517157756973, 0.015726030476025082, 0.013463047896718231, 0.011168139460131466,
+    0.008846759826364391, 0.006504457968979654, 0.004147033260562923, 0.0017832807216942152,
+};
+// E[sqrt(1 + Z/t)] = sum_k binom(1/2, 2k) E[Z^2k] t^(-2k), k = 0..7
+constexpr double kSqrtSeries[8] = {1.0, -0.125, -0.11718750000000001, -0.30761718750000006, -1.3746643066406252, -8.763484954833984, -72.8464686870575, -747.9771338403226};
+
+// F(t) = E[sqrt(max(t + Z, 0))] and F'(t) for a standard normal Z:
+// F = int_0^inf 2 v^2 phi(v^2 - t) dv, F' = int_0^inf phi(v^2 - t) dv
+// (u = t + Z = v^2), by 64-point Gauss-Legendre on [0, sqrt(max(t, 0) + 9)]
+// for t < 12 (relative error below 1e-15), and by the asymptotic series
+// sqrt(t) E[sqrt(1 + Z/t)] beyond (below 6e-14).
+__device__ __forceinline__ void sqrt_gauss(double t, double* F, double* dF)
+{
+    const double kInvSqrt2Pi = 0.39894228040143267794;
+    if (t >= 12.0) {
+        const double it2 = 1.0 / (t * t);
+        double p = 1.0, f = 0.0, df = 0.0;
+        for (int k = 0; k < 8; ++k) {
+            f += kSqrtSeries[k] * p;
+            df += kSqrtSeries[k] * (0.5 - 2.0 * k) * p;
+            p *= it2;
+        }
+        const double st = __builtin_sqrt(t);
+        *F = st * f;
+        *dF = df / st;
+        return;
+    }
+    const double V = __builtin_sqrt((t > 0.0 ? t : 0.0) + 9.0);
+    double f = 0.0, df = 0.0;
+    for (int k = 0; k < 64; ++k) {
+        const double v = 0.5 * V * (kGLx[k] + 1.0);
+        const double a = v * v - t;
+        const double e = kGLw[k] * exp(-0.5 * a * a);
+        f += e * v * v;
+        df += e;
+    }
+    *F = 0.5 * V * kInvSqrt2Pi * 2.0 * f;
+    *dF = 0.5 * V * kInvSqrt2Pi * df;
+}
+
+// Cells the fill enumerates: those within eight Gaussian widths outside the
+// edge (g + 8 h |grad g|_1 > 0; the 1-norm bounds the 2-norm from above).
+__device__ __forceinline__ bool aa_in(const GradCtx& G, int i, int j)
+{
+    const double rho = G.rho.v;
+    const double x = (static_cast<double>(i) + 0.5) * G.h;
+    const double y = (static_cast<double>(j) + 0.5) * G.h;
+    const MapOut<double> m = lens_map_k(G.L, x, y);
+    const double dx = m.wx - G.yx.v;
+    const double dy = m.wy - G.yy.v;
+    const double g = rho * rho - (dx * dx + dy * dy);
+    const double gx = -2.0 * (dx * (1.0 + m.kr) + dy * m.ki);
+    const double gy = -2.0 * (dx * m.ki + dy * (1.0 - m.kr));
+    return g + 8.0 * G.h * (dabs(gx) + dabs(gy)) > 0.0;
+}
+
+// Weight of cell (i, j) with tangents: the source brightness
+// I = (1 - c) [g > 0] + c sqrt(max(g, 0)) / rho  (g = rho^2 - |w - y|^2,
+// linear limb darkening) averaged over the cell's linearised footprint with a
+// Gaussian of one cell: g -> g + s Z, s = h |grad g|.  With t = g / s,
+// W = (1 - c) Phi(t) + (c / rho) sqrt(s) F(t).  Unlike a box footprint, the
+// Gaussian makes the lattice sum smooth at the cell scale (its ripple scales
+// with the kernel's transform at the lattice frequency, ~exp(-2 pi^2)).
+__device__ Dual aa_weight(const GradCtx& G, int i, int j)
+{
+    const Dual x = dconst((static_cast<double>(i) + 0.5) * G.h);
+    const Dual y = dconst((static_cast<double>(j) + 0.5) * G.h);
+    const MapOut<Dual> m = lens_map_k(G.Ld, x, y);
+    const Dual dx = m.wx - G.yx;
+    const Dual dy = m.wy - G.yy;
+    const Dual g = G.rho * G.rho - (dx * dx + dy * dy);
+    const Dual gx = -2.0 * (dx * (1.0 + m.kr) + dy * m.ki);
+    const Dual gy = -2.0 * (dx * m.ki + dy * (1.0 - m.kr));
+    const Dual sc = G.h * dsqrt(gx * gx + gy * gy);
+    if (!(sc.v > 0.0)) return dconst(g.v > 0.0 ? 1.0 - G.c.v + G.c.v * __builtin_sqrt(g.v) / G.rho.v : 0.0);
+    const Dual t = g / sc;
+    if (t.v < -9.0) return dconst(0.0);
+    Dual phi = t;
+    phi.v = 0.5 * erfc(-t.v * 0.70710678118654752440);
+    const double dens = 0.39894228040143267794 * exp(-0.5 * t.v * t.v);
+#pragma unroll
+    for (int k = 0; k < kTangents; ++k) phi.d[k] = dens * t.d[k];
+    double F, dF;
+    sqrt_gauss(t.v, &F, &dF);
+    Dual Fd = t;
+    Fd.v = F;
+#pragma unroll
+    for (int k = 0; k < kTangents; ++k) Fd.d[k] = dF * t.d[k];
+    return (1.0 - G.c) * phi + G.c * (1.0 / G.rho) * dsqrt(sc) * Fd;
+}
+
+struct AaFill {
+    int* head;   // [kPhiHash] hashed run-chain heads, keyed on the row
+    int* runs;   // [run*4]: row, left, right, next
+    int n_runs;
+    int run_cap;
+    bool overflow;
+    long long steps;
+};
+
+__device__ __forceinline__ bool aa_visited(const AaFill& F, int i, int j)
+{
+    for (int r = F.head[j & (kPhiHash - 1)]; r >= 0; r = F.runs[4 * r + 3]) {
+        if (F.runs[4 * r] == j && i >= F.runs[4 * r + 1] && i <= F.runs[4 * r + 2]) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool aa_step(AaFill& F)
+{
+    F.steps = uni(F.steps + 1);
+    if (F.steps > kStepBudget) F.overflow = true;
+    return !F.overflow;
+}
+
+// Extends cell (i, j) to its run along the row, records it and adds its
+// weights to this lane's accumulator.
+__device__ void aa_add_run(const GradCtx& G, AaFill& F, int i, int j, Dual& acc)
+{
+    int left = i;
+    for (;;) {
+        if (!aa_step(F)) return;
+        const int c = left - 1 - G.lane;
+        const bool ok = !aa_visited(F, c, j) && aa_in(G, c, j);
+        const int stop = first_set(~ballot(ok));
+        if (stop < 0) {
+            left -= kWave;
+            continue;
+        }
+        left -= stop;
+        break;
+    }
+    int right = i;
+    for (;;) {
+        if (!aa_step(F)) return;
+        const int c = right + 1 + G.lane;
+        const bool ok = !aa_visited(F, c, j) && aa_in(G, c, j);
+        const int stop = first_set(~ballot(ok));
+        if (stop < 0) {
+            right += kWave;
+            continue;
+        }
+        right += stop;
+        break;
+    }
+    if (F.n_runs >= F.run_cap) {
+        F.overflow = true;
+        return;
+    }
+    const int older = uni(F.head[j & (kPhiHash - 1)]);
+    wave_sync();
+    {
+        const int r = F.n_runs;
+        wave_store(&F.runs[4 * r + 0], j);
+        wave_store(&F.runs[4 * r + 1], left);
+        wave_store(&F.runs[4 * r + 2], right);
+        wave_store(&F.runs[4 * r + 3], older);
+        wave_store(&F.head[j & (kPhiHash - 1)], r);
+    }
+    F.n_runs = uni(F.n_runs + 1);
+    wave_sync();
+    for (int c = left + G.lane; c <= right; c += kWave) acc += aa_weight(G, c, j);
+}
+
+// The antialiased magnification of one epoch with tangents, flood-filled from
+// the seeds.  Images smaller than the lattice can resolve (linear size
+// rho sqrt(A_i) under four cells) enter as point images with their exact
+// tangents, and the seeds near them start no fill.  Returns NaN (and
+// F.overflow) past a capacity or the budget.
+__device__ Dual aa_magnification(const GradCtx& G, AaFill& F, const Pt* seeds, int n_seeds)
+{
+    Dual acc = dconst(0.0);
+    F.n_runs = 0;
+    F.overflow = false;
+    F.steps = 0;
+    const double h = G.h;
+    Pt unresolved[5];
+    int n_unres = 0;
+    Dual a_points = dconst(0.0);
+    {
+        const Geom g = make_geom(G.Ld.a.v < 0.0 ? -G.Ld.a.v : G.Ld.a.v, G.q_eff, G.yx.v, G.yy.v);
+        cx roots[5];
+        solve_quintic(g, roots);
+        const unsigned mask = select_physical(g, roots);
+        for (int i = 0; i < 5; ++i) {
+            if (!(mask & (1u << i))) continue;
+            const double ai = 1.0 / fabs(jacobian_det(g, roots[i]));
+            if (G.rho.v * __builtin_sqrt(ai) < 4.0 * h) {
+                unresolved[n_unres++] = Pt{roots[i].re, roots[i].im};
+                a_points += image_dual(G.Ld, roots[i].re, roots[i].im, G.yx, G.yy);
+            }
+        }
+    }
+    for (int sidx = 0; sidx < n_seeds && !F.overflow; ++sidx) {
+        const Pt sd = seeds[sidx];
+        bool near_point = false;
+        for (int u = 0; u < n_unres; ++u)
+            near_point = near_point || dist2(sd, unresolved[u]) < 64.0 * h * h;
+        if (near_point) continue;
+        const int i0 = static_cast<int>(floor(sd.x / h));
+        const int j0 = static_cast<int>(floor(sd.y / h));
+        // start: the first cell with nonzero weight in a 9 x 9 block around
+        // the seed's cell, nearest first (|di| + |dj|, then row, then column)
+        int found = -1;
+        for (int base = 0; base < 81 && found < 0; base += kWave) {
+            const int o = base + G.lane;
+            bool in = false;
+            if (o < 81) {
+                const int di = (o % 9) - 4;
+                const int dj = (o / 9) - 4;
+                in = aa_in(G, i0 + di, j0 + dj);
+            }
+            // order candidates by Manhattan distance, then index
+            const int key = in ? ((abs((o % 9) - 4) + abs((o / 9) - 4)) * 128 + o) : 0x7fffffff;
+            int best = key;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const int ob = __shfl_xor(best, off, kWave);
+                best = ob < best ? ob : best;
+            }
+            best = uni(best);
+            if (best != 0x7fffffff) found = best & 127;
+        }
+        if (found < 0) continue;
+        const int si = i0 + (found % 9) - 4;
+        const int sj = j0 + (found / 9) - 4;
+        if (aa_visited(F, si, sj)) continue;
+        aa_add_run(G, F, si, sj, acc);
+    }
+    // grow: each recorded run visits its two neighbour rows, diagonal
+    // neighbours included (8-connected), so that a thin diagonal image stays
+    // one component while its fringe cells' weights go to zero
+    for (int q = 0; q < F.n_runs && !F.overflow; ++q) {
+        const int row = F.runs[4 * q + 0];
+        const int L = F.runs[4 * q + 1] - 1;
+        const int R = F.runs[4 * q + 2] + 1;
+        for (int side = 0; side < 2; ++side) {
+            const int nb = side == 0 ? row - 1 : row + 1;
+            int c = L;
+            while (c <= R && aa_step(F)) {
+                const int cc = c + G.lane;
+                bool ok = false;
+                if (cc <= R) ok = !aa_visited(F, cc, nb) && aa_in(G, cc, nb);
+                const uint64_t m = ballot(ok);
+                if (!m) {
+                    c += kWave;
+                    continue;
+                }
+                aa_add_run(G, F, c + first_set(m), nb, acc);
+                c = F.runs[4 * (F.n_runs - 1) + 2] + 1;
+            }
+        }
+    }
+    wave_sync();
+    for (int q = G.lane; q < F.n_runs; q += kWave) F.head[F.runs[4 * q] & (kPhiHash - 1)] = -1;
+    wave_sync();
+    Dual total;
+    total.v = uni(wave_sum(acc.v));
+#pragma unroll
+    for (int k = 0; k < kTangents; ++k) total.d[k] = uni(wave_sum(acc.d[k]));
+    if (F.overflow) return dconst(__builtin_nan(""));
+    // sum W h^2 / (pi rho^2 (1 - c/3)), plus the unresolved point images
+    const Dual flux = (1.0 - G.c * (1.0 / 3.0)) * (kPi * G.rho * G.rho);
+    return total * (G.h * G.h) / flux + a_points;
+}
+
+__device__ __forceinline__ void epoch_inputs(const CurveRec& cv, Pt src, Dual& yx, Dual& yy, Dual& s,
+                                             Dual& q, Dual& rho, Dual& c)
+{
+    yx = dvar(src.x, kTx);
+    yy = dvar(src.y, kTy);
+    s = dvar(cv.s, kTs);
+    q = dvar(cv.q_eff, kTq);
+    rho = dvar(cv.rho, kTrho);
+    c = dvar(cv.ld_c, kTc);
+}
+
+__device__ __forceinline__ void store_grad(const FsEpochs& ep, double* mag_aa, int64_t e, const Dual& A)
+{
+    if (mag_aa) mag_aa[e] = A.v;
+#pragma unroll
+    for (int k = 0; k < kTangents; ++k) ep.dint[e * kTangents + k] = A.d[k];
+}
+
+}  // namespace
+
+// Point-source and hexadecapole epochs: one thread each.
+__global__ void __launch_bounds__(256) k_grad_point(const CurveRec* __restrict__ curves, FsEpochs ep,
+                                                    double* mag_aa)
+{
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= ep.n) return;
+    const int method = ep.method[e];
+    if (method != LCBG_METHOD_POINT_SOURCE && method != LCBG_METHOD_HEXADECAPOLE) return;
+    const CurveRec cv = curves[ep.curve_of[e]];
+    Dual yx, yy, s, q, rho, c;
+    epoch_inputs(cv, ep.src[e], yx, yy, s, q, rho, c);
+    const Dual A = method == LCBG_METHOD_HEXADECAPOLE ? hex_dual(yx, yy, s, q, rho, c)
+                                                      : ps_dual(yx, yy, s, q);
+    store_grad(ep, mag_aa, e, A);
+}
+
+// Inverse-ray epochs of the value run (its item list and seeds), one wave
+// each; pass kPassSmall queues the epochs that outgrow a small slot's runs for
+// the kPassRetry pass on large slots.
+__global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ curves,
+                                                    FsOptions opt, FsEpochs ep, IrLists Lst,
+                                                    PolarScratch scr, int pass, double* mag_aa)
+{
+    const bool retry = pass == kPassRetry;
+    AaFill F;
+    F.head = scr.phi_head + static_cast<size_t>(blockIdx.x) * kPhiHash;
+    F.runs = scr.runs + static_cast<size_t>(blockIdx.x) * scr.run_cap * 4;
+    F.run_cap = scr.run_cap;
+    const int count = uni(static_cast<int>(ep.counters[retry ? kQGradRetryCount : kQIrCount]));
+    unsigned* cursor = &ep.counters[retry ? kQGradRetryCursor : kQGradCursor];
+    for (;;) {
+        const int qi = next_item(cursor);
+        if (qi >= count) break;
+        const int item = retry ? uni(Lst.cart_retry[qi]) : qi;
+        const int64_t e = Lst.ir_sorted[item];
+        const int n_seeds = uni(Lst.seed_n[item]);
+        const CurveRec cv = curves[ep.curve_of[e]];
+        GradCtx G;
+        G.L = make_lens(cv.s, cv.q_eff);
+        Dual s, q;
+        epoch_inputs(cv, ep.src[e], G.yx, G.yy, s, q, G.rho, G.c);
+        G.Ld = make_dlens(s, q);
+        G.q_eff = cv.q_eff;
+        const int nb = cv.source_bins > 0 ? cv.source_bins : opt.source_bins;
+        // a cell side that does not move with rho: the power of two at or
+        // below rho / nbin (between one and two cells per rho / nbin)
+        G.h = exp2(floor(log2(cv.rho / static_cast<double>(nb > 1 ? nb : 1))));
+        G.lane = lane_id();
+        Dual A;
+        if (n_seeds < 0) {
+            A = dconst(__builtin_nan(""));
+        } else {
+            A = aa_magnification(G, F, Lst.seed_pool + uni(Lst.seed_off[item]), n_seeds);
+        }
+        const bool cap = F.overflow && F.n_runs >= F.run_cap;
+        if (cap && !retry) {
+            const int pos = wave_fetch_inc(&ep.counters[kQGradRetryCount]);
+            wave_store(&Lst.cart_retry[pos], item);
+        } else {
+            wave_sync();
+            store_grad(ep, mag_aa, e, A);  // uniform values, stored from every lane
+            if (F.overflow) atomicOr(&ep.counters[kQFlags], cap ? kErrPolarCap : kErrBudget);
+        }
+        wave_sync();
+    }
+}
+
+// d magnification / d (t0, tE, u0, alpha, s, q, rho, u_LD) per epoch from the six
+// tangents, and per curve chi^2 = sum ((F - fs A - fb) / sigma)^2 with its
+// gradient.  One workgroup per curve.
+__global__ void __launch_bounds__(256) k_grad_chain(const CurveRec* __restrict__ curves, FsEpochs ep,
+                                                    const double* __restrict__ flux,
+                                                    const double* __restrict__ sigma, double fs,
+                                                    double fb, double* dmag, double* chi2,
+                                                    double* grad)
+{
+    const int c = blockIdx.x;
+    const CurveRec cv = curves[c];
+    const int64_t lo = ep.offsets[c];
+    const int64_t hi = ep.offsets[c + 1];
+    double acc[kGradParams + 1];
+#pragma unroll
+    for (int k = 0; k <= kGradParams; ++k) acc[k] = 0.0;
+    for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+        const double* t = ep.dint + e * kTangents;
+        const double tau = (ep.times[e] - cv.t0) * cv.inverse_tE;
+        const double yx = tau * cv.cos_alpha - cv.u0 * cv.sin_alpha;
+        const double yy = tau * cv.sin_alpha + cv.u0 * cv.cos_alpha;
+        double g[kGradParams];
+        const double along = t[kTx] * cv.cos_alpha + t[kTy] * cv.sin_alpha;
+        g[0] = -along * cv.inverse_tE;                              // t0
+        g[1] = -along * tau * cv.inverse_tE;                        // tE
+        g[2] = -t[kTx] * cv.sin_alpha + t[kTy] * cv.cos_alpha;      // u0
+        g[3] = -t[kTx] * yy + t[kTy] * yx;                          // alpha
+        g[4] = t[kTs];                                              // s
+        g[5] = -t[kTq] * cv.q_eff * cv.q_eff;                       // q (q_eff = 1/q)
+        g[6] = t[kTrho];                                            // rho
+        g[7] = t[kTc];                                              // u_LD
+        if (dmag) {
+#pragma unroll
+            for (int k = 0; k < kGradParams; ++k) dmag[e * kGradParams + k] = g[k];
+        }
+        if (chi2) {
+            const double r = (flux[e] - (fs * ep.mag[e] + fb)) / sigma[e];
+            acc[kGradParams] += r * r;
+            const double w = -2.0 * r / sigma[e] * fs;
+#pragma unroll
+            for (int k = 0; k < kGradParams; ++k) acc[k] += w * g[k];
+        }
+    }
+    if (!chi2) return;
+    __shared__ double red[kGradParams + 1][256];
+#pragma unroll
+    for (int k = 0; k <= kGradParams; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int off = 128; off >= 1; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) {
+#pragma unroll
+            for (int k = 0; k <= kGradParams; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        chi2[c] = red[kGradParams][0];
+#pragma unroll
+        for (int k = 0; k < kGradParams; ++k) grad[static_cast<int64_t>(c) * kGradParams + k] = red[k][0];
+    }
+}
+
+lcbg_status launch_fs_grad(const CurveRec* d_curves, int n_curves, const FsOptions& opt,
+                           const FsEpochs& ep, const IrLists& lists, const IrScratch& scr,
+                           double* mag_aa, double* dmag, const double* flux, const double* sigma,
+                           double fs, double fb, double* chi2, double* grad, hipStream_t st)
+{
+    const unsigned blocks = static_cast<unsigned>((ep.n + 255) / 256);
+    hipLaunchKernelGGL(k_grad_point, dim3(blocks), dim3(256), 0, st, d_curves, ep, mag_aa);
+    hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(scr.polar.n_slots)), dim3(kWave), 0, st,
+                       d_curves, opt, ep, lists, scr.polar, int{kPassSmall}, mag_aa);
+    hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(scr.polar_large.n_slots)), dim3(kWave), 0,
+                       st, d_curves, opt, ep, lists, scr.polar_large, int{kPassRetry}, mag_aa);
+    hipLaunchKernelGGL(k_grad_chain, dim3(static_cast<unsigned>(n_curves)), dim3(256), 0, st, d_curves,
+                       ep, flux, sigma, fs, fb, dmag, chi2, grad);
+    return check_launch();
+}
+
+}  // namespace lcbg

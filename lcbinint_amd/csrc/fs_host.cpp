@@ -50,6 +50,7 @@ T* carve(char*& p, std::size_t count)
 std::size_t carve_size(std::size_t bytes) { return bytes + 256; }
 
 constexpr int kCounterAlloc = 128;
+constexpr int kTangentsHost = 6;  // tangent slots of the gradient kernels (dual.hpp)
 static_assert(kCounterWords <= kCounterAlloc, "counter words");
 
 // Seeds held at once by the inverse-ray stage: 8 per epoch of capacity, at
@@ -79,7 +80,8 @@ lcbg_status ensure_epochs(FsState* fs, int64_t n)
                               carve_size(cap * sizeof(int64_t)) * 2 +
                               carve_size(kCounterAlloc * sizeof(unsigned)) +
                               carve_size(cap * sizeof(int64_t)) + carve_size(cap * sizeof(int)) * 9 +
-                              carve_size(pool_cap_for(cap) * sizeof(Pt2));
+                              carve_size(pool_cap_for(cap) * sizeof(Pt2)) +
+                              carve_size(cap * kTangentsHost * sizeof(double));
     lcbg_status st = map_hip(hipMalloc(&fs->epoch_mem, bytes));
     if (st != LCBG_OK) {
         fs->epoch_mem = nullptr;
@@ -138,6 +140,24 @@ IrLists lists_view(FsState* fs)
     L.pool_cap = static_cast<unsigned>(pool_cap_for(cap));
     L.seed_pool = carve<Pt2>(p, L.pool_cap);
     return L;
+}
+
+double* dint_view(FsState* fs)
+{
+    char* p = static_cast<char*>(fs->epoch_mem);
+    const int64_t cap = fs->epoch_cap;
+    (void)carve<int32_t>(p, cap);
+    (void)carve<Pt2>(p, cap);
+    (void)carve<double>(p, cap);
+    (void)carve<double>(p, cap);
+    (void)carve<double>(p, cap * 10);
+    (void)carve<int64_t>(p, cap);
+    (void)carve<int64_t>(p, cap);
+    (void)carve<unsigned>(p, kCounterAlloc);
+    (void)carve<int64_t>(p, cap);
+    for (int k = 0; k < 9; ++k) (void)carve<int>(p, cap);
+    (void)carve<Pt2>(p, pool_cap_for(cap));
+    return carve<double>(p, cap * kTangentsHost);
 }
 
 // Inverse-ray scratch, one pool per device shared by every batch: its use is
@@ -332,8 +352,20 @@ lcbg_status fs_refresh(lcbg_batch* b, hipStream_t stream)
     return st;
 }
 
+// Outputs of a gradient run (any pointer may be null).
+struct GradArgs {
+    double* mag_aa;
+    double* dmag;
+    const double* flux;
+    const double* sigma;
+    double fs, fb;
+    double* chi2;
+    double* grad;
+};
+
 lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, int64_t n_epochs,
-                   double* mag, int32_t* method, hipStream_t stream, float* stage_ms)
+                   double* mag, int32_t* method, hipStream_t stream, float* stage_ms,
+                   const GradArgs* ga = nullptr)
 {
     lcbg_status st = fs_prepare(b);
     if (st != LCBG_OK) return st;
@@ -342,6 +374,7 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     if (st != LCBG_OK) return st;
     FsEpochs ep = epoch_view(fs, n_epochs, offsets, times, mag, method);
     ep.cost = b->d_cost;
+    ep.dint = dint_view(fs);
     fs->last_n = n_epochs;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     if (stage_ms) {
@@ -379,6 +412,9 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
         st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, lists_view(fs), pool.scr, stream,
                           fs->ir);
     stage_done("stage 3 (inverse ray)");
+    if (ga && st == LCBG_OK)
+        st = launch_fs_grad(b->d_curves, b->n_curves, fs->opt, ep, lists_view(fs), pool.scr, ga->mag_aa,
+                            ga->dmag, ga->flux, ga->sigma, ga->fs, ga->fb, ga->chi2, ga->grad, stream);
     if (st == LCBG_OK) st = map_hip(hipEventRecord(pool.last_use, stream));
     if (stage_ms && st == LCBG_OK) {
         (void)hipEventRecord(ev[4], stream);
@@ -419,6 +455,36 @@ extern "C" lcbg_status lcbg_batch_finite_source_device(lcbg_batch* batch, const 
     if (!offsets || !times || !mag || !method) return LCBG_INVALID_ARGUMENT;
     return fs_run(batch, offsets, times, n_epochs, mag, method, static_cast<hipStream_t>(stream),
                   nullptr);
+}
+
+extern "C" lcbg_status lcbg_batch_finite_source_grad_device(lcbg_batch* batch, const int64_t* offsets,
+                                                            const double* times, int64_t n_epochs,
+                                                            double* mag, int32_t* method,
+                                                            double* mag_aa, double* dmag,
+                                                            void* stream)
+{
+    if (!batch || n_epochs < 0) return LCBG_INVALID_ARGUMENT;
+    if (batch->n_curves == 0 || n_epochs == 0) return LCBG_OK;
+    if (!offsets || !times || !mag || !method || !dmag) return LCBG_INVALID_ARGUMENT;
+    const GradArgs ga{mag_aa, dmag, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr};
+    return fs_run(batch, offsets, times, n_epochs, mag, method, static_cast<hipStream_t>(stream),
+                  nullptr, &ga);
+}
+
+extern "C" lcbg_status lcbg_batch_chi2_grad_device(lcbg_batch* batch, const int64_t* offsets,
+                                                   const double* times, int64_t n_epochs,
+                                                   const double* flux, const double* sigma,
+                                                   double fs, double fb, double* mag,
+                                                   int32_t* method, double* mag_aa, double* dmag,
+                                                   double* chi2, double* grad, void* stream)
+{
+    if (!batch || n_epochs < 0) return LCBG_INVALID_ARGUMENT;
+    if (batch->n_curves == 0 || n_epochs == 0) return LCBG_OK;
+    if (!offsets || !times || !mag || !method || !flux || !sigma || !chi2 || !grad)
+        return LCBG_INVALID_ARGUMENT;
+    const GradArgs ga{mag_aa, dmag, flux, sigma, fs, fb, chi2, grad};
+    return fs_run(batch, offsets, times, n_epochs, mag, method, static_cast<hipStream_t>(stream),
+                  nullptr, &ga);
 }
 
 extern "C" lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const int64_t* offsets,

@@ -44,6 +44,15 @@ lcbg_status launch_ir_epoch_debug(const CurveRec* d_curves, const FsCache& cache
                                   int64_t e, int method, int last_phase, Pt2* seeds_io, int n_in,
                                   int* n_out, double* mag_out, hipStream_t st);
 
+// Gradient kernels after a value run: tangents of every epoch (point source,
+// hexadecapole, antialiased inverse ray), then per-epoch derivatives dmag
+// [n][8] and, if chi2 is not null, chi^2 and its gradient per curve.  mag_aa,
+// dmag, flux/sigma/chi2/grad may be null.
+lcbg_status launch_fs_grad(const CurveRec* d_curves, int n_curves, const FsOptions& opt,
+                           const FsEpochs& ep, const IrLists& lists, const IrScratch& scr,
+                           double* mag_aa, double* dmag, const double* flux, const double* sigma,
+                           double fs, double fb, double* chi2, double* grad, hipStream_t st);
+
 // Workgroups of one inverse-ray kernel resident per CU (kind 0 seeds,
 // 1 Cartesian, 2 polar), from its register and LDS use.
 int ir_resident_per_cu(int kind);

@@ -100,6 +100,7 @@ struct FsEpochs {
     int64_t* ir_list;
     unsigned* counters; // [0] near count, [1] near cursor, [2] ir count, [3] ir cursor, [4] error flags
     int64_t* cost;      // [epoch] wall-clock ticks of each inverse-ray epoch, or null
+    double* dint;       // [epoch][6] tangents of the magnification (gradient runs)
 };
 
 // Scratch of the resident waves of each inverse-ray kernel.
@@ -162,8 +163,11 @@ enum : int {
     kQCartCount = 5, kQCartCursor = 6, kQPolarCount = 7, kQPolarCursor = 8,
     kQCartRetryCount = 9, kQCartRetryCursor = 10, kQPolarRetryCount = 11,
     kQPolarRetryCursor = 12, kQPoolCursor = 13, kQCartBigCount = 14, kQCartBigCursor = 15,
-    kQPolarBigCount = 16, kQPolarBigCursor = 17
+    kQPolarBigCount = 16, kQPolarBigCursor = 17, kQGradCursor = 18, kQGradRetryCount = 19,
+    kQGradRetryCursor = 20
 };
+// derivatives per epoch: t0, tE, u0, alpha, s, q, rho, u_LD
+constexpr int kGradParams = 8;
 // inverse-ray area passes: small slots, large slots for the largest epochs
 // (beside the small pass), large slots for the epochs a small slot could not hold
 enum : int { kPassSmall = 0, kPassBig = 1, kPassRetry = 2 };

@@ -10,6 +10,11 @@ finite_source_batch: 2048 sets with the geometry above plus rho log-uniform
 count floor(log-uniform [50, 2001)), epochs uniform in [-1, 1]; nbin uniform
 integer in [25, 100]; automatic Cartesian/polar choice.
 
+differentiation_batch: 256 sets with the point-source geometry plus rho
+log-uniform [1e-4, 3e-2] and linear limb darkening u uniform [0.1, 0.9]; 1000
+sorted epochs per set uniform in [-2, 2]; observed fluxes are made from the
+model itself (F = fs A + fb, fs = 1, fb = 0.1, Gaussian noise of 1% of F).
+
 plain_curve: the single light curve t0=0, tE=1, u0=-0.01, alpha=0.5, s=0.95,
 q=1e-2, rho=5e-3, linear LD u=0.5, 200 epochs linspace(-0.5, 0.5), nbin 50.
 """
@@ -53,3 +58,27 @@ def plain_curve():
     curve = dict(t0=0.0, tE=1.0, u0=-0.01, alpha=0.5, s=0.95, q=1e-2, rho=5e-3, ld_c=0.5,
                  ld_d=0.0)
     return curve, np.linspace(-0.5, 0.5, 200)
+
+
+def differentiation_batch(n_curves: int = 256, n_epochs: int = 1000, seed: int = 4):
+    rng = np.random.default_rng(seed)
+    s = np.exp(rng.uniform(np.log(0.3), np.log(3.0), n_curves))
+    q = np.exp(rng.uniform(np.log(1e-5), 0.0, n_curves))
+    u0 = rng.uniform(-0.5, 0.5, n_curves)
+    alpha = rng.uniform(0.0, 2.0 * np.pi, n_curves)
+    rho = np.exp(rng.uniform(np.log(1e-4), np.log(3e-2), n_curves))
+    ld = rng.uniform(0.1, 0.9, n_curves)
+    times = np.sort(rng.uniform(-2.0, 2.0, (n_curves, n_epochs)), axis=1)
+    curves = [dict(t0=0.0, tE=1.0, u0=float(u0[i]), alpha=float(alpha[i]), s=float(s[i]),
+                   q=float(q[i]), rho=float(rho[i]), ld_c=float(ld[i]), ld_d=0.0)
+              for i in range(n_curves)]
+    offsets = np.arange(n_curves + 1, dtype=np.int64) * n_epochs
+    return curves, offsets, times.ravel()
+
+
+def synthetic_fluxes(mag, fs: float = 1.0, fb: float = 0.1, noise: float = 0.01, seed: int = 5):
+    """F = fs A + fb with Gaussian noise of `noise` times F; returns (flux, sigma)."""
+    rng = np.random.default_rng(seed)
+    f = fs * np.asarray(mag) + fb
+    sigma = noise * f
+    return f + sigma * rng.standard_normal(len(f)), sigma
