@@ -72,7 +72,7 @@ def _cpu_worker(args):
     return time.perf_counter() - t0, n
 
 
-def cpu_baseline(curves, offsets, times, nbin, stride=8):
+def cpu_baseline(curves, offsets, times, nbin, stride=2):
     """The reference core (oracle/_ref, compiled from its own sources) on every
     `stride`-th curve of the same workload, one process per core (the
     reference solver keeps static state, so it is not run threaded)."""
@@ -143,6 +143,81 @@ def bench_batched(capi, torch, steps, warmup):
     stage_ms = np.mean(np.array(stages), axis=0)
     return dict(n_curves=len(curves), n_epochs=n_epochs, wall=wall, steps=steps,
                 stage_ms=stage_ms.tolist(), queue=qc, work=work, methods=meth.tolist())
+
+
+def bench_value_grad(capi, torch, steps, warmup, per_epoch_calls=64):
+    """chi^2 and its gradient w.r.t. (t0, tE, u0, alpha, s, q, rho, u_LD) of
+    256 curves x 1000 epochs (lcbinint_amd.workloads.differentiation_batch),
+    against observed fluxes made from the model (fs = 1, fb = 0.1, 1% noise).
+    A step uploads the curve parameters and evaluates magnifications, their
+    derivatives and the per-curve chi^2 gradient.  The plain per-epoch path
+    evaluates one epoch per call (its own batch, host round trip)."""
+    from lcbinint_amd.workloads import differentiation_batch, synthetic_fluxes
+    curves, offsets, times = differentiation_batch()
+    n, nc = len(times), len(curves)
+    recs = (capi.Curve * nc)(*[capi.Curve(**c) for c in curves])
+    batch = capi.Batch(list(recs))
+    d_off = torch.from_numpy(offsets).cuda()
+    d_t = torch.from_numpy(times).cuda()
+    mag = torch.empty(n, dtype=torch.float64, device="cuda")
+    meth = torch.empty(n, dtype=torch.int32, device="cuda")
+    sptr = torch.cuda.current_stream().cuda_stream
+    capi.finite_source(batch, d_off.data_ptr(), d_t.data_ptr(), n, mag.data_ptr(), meth.data_ptr(),
+                       sptr)
+    flux, sigma = synthetic_fluxes(mag.cpu().numpy())
+    d_flux = torch.from_numpy(flux).cuda()
+    d_sig = torch.from_numpy(sigma).cuda()
+    chi2 = torch.empty(nc, dtype=torch.float64, device="cuda")
+    grad = torch.empty(nc * 8, dtype=torch.float64, device="cuda")
+
+    def step():
+        capi.check(capi.lcbg_batch_update_curves(batch.handle, recs, sptr), "update_curves")
+        capi.chi2_grad(batch, d_off.data_ptr(), d_t.data_ptr(), n, d_flux.data_ptr(),
+                       d_sig.data_ptr(), 1.0, 0.1, mag.data_ptr(), meth.data_ptr(),
+                       chi2.data_ptr(), grad.data_ptr(), None, None, sptr)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    methods = np.bincount(meth.cpu().numpy(), minlength=4).tolist()
+    finite = bool(torch.isfinite(grad).all().item()) and bool(torch.isfinite(chi2).all().item())
+    batch.close()
+    if not finite:
+        raise RuntimeError("non-finite chi^2 gradient")
+    # plain per-epoch path: one epoch per call
+    P = capi.c_double_p
+    rng = np.random.default_rng(0)
+    picks = rng.choice(n, per_epoch_calls, replace=False)
+    curve_of = np.repeat(np.arange(nc), np.diff(offsets))
+    one_t = torch.empty(1, dtype=torch.float64, device="cuda")
+    one_off = torch.tensor([0, 1], dtype=torch.int64, device="cuda")
+    one_m = torch.empty(1, dtype=torch.float64, device="cuda")
+    one_k = torch.empty(1, dtype=torch.int32, device="cuda")
+    one_d = torch.empty(8, dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    for e in picks:
+        b1 = capi.Batch([capi.Curve(**curves[curve_of[e]])])
+        one_t.fill_(float(times[e]))
+        capi.finite_source_grad(b1, one_off.data_ptr(), one_t.data_ptr(), 1, one_m.data_ptr(),
+                                one_k.data_ptr(), 0, one_d.data_ptr(), sptr)
+        _ = one_d.cpu()
+        b1.close()
+    per_epoch = len(picks) / (time.perf_counter() - t0)
+    value_grad = n * steps / wall
+    return {
+        "workload": "chi^2 and gradient w.r.t. (t0,tE,u0,alpha,s,q,rho,u_LD): 256 curves x 1000 "
+                    "epochs in [-2, 2], s log-uniform [0.3, 3], q log-uniform [1e-5, 1], rho "
+                    "log-uniform [1e-4, 3e-2], linear LD u in [0.1, 0.9], nbin 50, seed 4; "
+                    "fluxes F = A + 0.1 with 1% noise",
+        "value_grad_per_s": value_grad, "unit": "epochs/s", "ms_per_step": 1e3 * wall / steps,
+        "methods": methods, "plain_per_epoch_value_grad_per_s": per_epoch,
+        "batched_over_plain_per_epoch": value_grad / per_epoch,
+    }
 
 
 def bench_plain(capi, calls_per_epoch=50):
@@ -244,6 +319,8 @@ def main() -> int:
             "batched_over_per_curve": value / per_curve,
         }
         extras["point_source_batch"] = bench_point_source(capi, torch)
+        print("[bench] value and gradient workload", file=sys.stderr, flush=True)
+        extras["value_grad"] = bench_value_grad(capi, torch, max(2, args.steps // 2), 1)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] reference CPU baseline", file=sys.stderr, flush=True)

@@ -1,0 +1,87 @@
+"""PyTorch autograd bridge of the finite-source light curves.
+
+    from lcbinint_amd.torch_ops import FiniteSourceLightCurves
+    lc = FiniteSourceLightCurves(offsets, times, nbin=None, options=None)
+    mag = lc(params)          # params: (n_curves, 8) float64 on cuda:
+                              # t0, tE, u0, alpha, s, q, rho, u_LD per curve
+    loss(mag).backward()      # params.grad via the device derivatives
+
+The forward pass is the plain finite-source method (the reference's values);
+the backward pass contracts the incoming gradient with the per-epoch
+derivatives d mag / d params of lcbg_batch_finite_source_grad_device and sums
+them per curve on the device.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import capi
+
+PARAM_NAMES = capi.GRAD_PARAMS  # t0, tE, u0, alpha, s, q, rho, ld_c
+
+
+def _curves_from(params: torch.Tensor):
+    host = params.detach().to("cpu", torch.float64).numpy()
+    return [capi.Curve(t0=r[0], tE=r[1], u0=r[2], alpha=r[3], s=r[4], q=r[5], rho=r[6],
+                       ld_c=r[7], ld_d=0.0) for r in host]
+
+
+class _LightCurveFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, params, owner):
+        n = owner.n_epochs
+        owner.batch.update_curves(_curves_from(params))
+        dev = params.device
+        mag = torch.empty(n, dtype=torch.float64, device=dev)
+        method = torch.empty(n, dtype=torch.int32, device=dev)
+        dmag = torch.empty((n, 8), dtype=torch.float64, device=dev)
+        capi.finite_source_grad(owner.batch, owner.d_off.data_ptr(), owner.d_t.data_ptr(), n,
+                                mag.data_ptr(), method.data_ptr(), 0, dmag.data_ptr(),
+                                torch.cuda.current_stream(dev).cuda_stream)
+        ctx.save_for_backward(dmag)
+        ctx.owner = owner
+        owner.last_method = method
+        return mag
+
+    @staticmethod
+    def backward(ctx, grad_mag):
+        (dmag,) = ctx.saved_tensors
+        owner = ctx.owner
+        contrib = grad_mag.to(torch.float64)[:, None] * dmag
+        grad = torch.zeros((owner.n_curves, 8), dtype=torch.float64, device=dmag.device)
+        grad.index_add_(0, owner.curve_of, contrib)
+        return grad, None
+
+
+class FiniteSourceLightCurves:
+    """Light curves of n_curves curves with per-curve epochs
+    times[offsets[c]:offsets[c+1]] as a differentiable function of the
+    (n_curves, 8) parameter tensor (VBM convention, as LightCurve)."""
+
+    def __init__(self, offsets, times, nbin=None, options: capi.Options | None = None,
+                 device: str = "cuda"):
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        times = np.ascontiguousarray(times, dtype=np.float64)
+        self.n_curves = len(offsets) - 1
+        self.n_epochs = int(offsets[-1])
+        if len(times) != self.n_epochs:
+            raise ValueError(f"times has {len(times)} entries, offsets end at {self.n_epochs}")
+        placeholder = [capi.Curve(0.0, 1.0, 0.1, 0.0, 1.0, 1.0, 1e-3, 0.0, 0.0)] * self.n_curves
+        self.batch = capi.Batch(placeholder, options)
+        if nbin is not None:
+            self.batch.set_source_bins(np.broadcast_to(np.asarray(nbin, dtype=np.int32),
+                                                       (self.n_curves,)))
+        self.d_off = torch.from_numpy(offsets).to(device)
+        self.d_t = torch.from_numpy(times).to(device)
+        self.curve_of = torch.repeat_interleave(
+            torch.arange(self.n_curves, device=device), torch.from_numpy(np.diff(offsets)).to(device))
+        self.last_method = None
+
+    def __call__(self, params: torch.Tensor) -> torch.Tensor:
+        if params.shape != (self.n_curves, 8):
+            raise ValueError(f"params must be ({self.n_curves}, 8), got {tuple(params.shape)}")
+        return _LightCurveFn.apply(params, self)
+
+    def close(self):
+        self.batch.close()

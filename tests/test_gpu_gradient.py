@@ -230,3 +230,26 @@ def test_chi2_gradient(gpu):
         assert float(np.median(rel)) < 1e-3, (name, float(np.median(rel)))
         checked += 1
     assert checked == 8
+
+
+def test_torch_autograd_bridge(gpu):
+    import torch
+    from lcbinint_amd.torch_ops import FiniteSourceLightCurves
+    from lcbinint_amd.workloads import differentiation_batch
+    curves, offsets, times = differentiation_batch(n_curves=16, n_epochs=100)
+    params = torch.tensor([[c[k] for k in PARAMS] for c in curves], dtype=torch.float64,
+                          device="cuda", requires_grad=True)
+    lc = FiniteSourceLightCurves(offsets, times, nbin=50)
+    mag = lc(params)
+    target = mag.detach() * 1.01
+    loss = ((mag - target) ** 2).sum()
+    loss.backward()
+    # the same contraction from the C-ABI derivatives
+    run = Runner(curves, offsets, times)
+    m, k, aa, g = run.grad()
+    assert np.array_equal(mag.detach().cpu().numpy(), m)
+    w = 2.0 * (m - target.cpu().numpy())
+    exp = np.stack([np.add.reduceat(w * g[:, p], offsets[:-1]) for p in range(8)], axis=1)
+    got = params.grad.cpu().numpy()
+    assert np.allclose(got, exp, rtol=1e-10, atol=1e-12 * np.abs(exp).max())
+    lc.close()
