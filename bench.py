@@ -49,11 +49,13 @@ FLOPS_CART_CELL = 30.0
 FLOPS_CART_GUARDED_CELL = 53.0
 FLOPS_POLAR_CELL = 37.0
 
-# HBM bytes per k_fs_ir launch on the headline workload, from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE (profiles/k_fs_ir_batched_fs_pmc.txt), each scaled
-# by the same counter's reading for a device copy of known size; None until
-# measured.
-K_FS_IR_TRAFFIC_BYTES = None
+# HBM bytes per k_ir_cart launch on the headline workload (mean of its three
+# launches of a step), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (profiles/k_ir_batched_fs.md): KiB per dispatch x 1024, FETCH_SIZE scaled by
+# 2.0 and WRITE_SIZE by 1.0, the factors that bring each counter's reading for
+# a 256 MiB device copy to 256 MiB (bench.py --copy-calibration).
+K_IR_CART_TRAFFIC_BYTES = (2.0 * (903555.25 + 58.0) + 1.0 * (1084607.25 + 4.0)) * 1024 / 3
+COPY_BYTES = 1 << 28
 
 
 def _cpu_worker(args):
@@ -125,13 +127,14 @@ def bench_batched(capi, torch, steps, warmup):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    stages = []
+    stages, launches = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         ms, qc, flags = step()
         if flags:
             raise RuntimeError(f"finite-source run reported capacity flags {flags}")
         stages.append(ms)
+        launches.append(capi.ir_kernel_ms(batch))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     work = capi.work_counts(batch)
@@ -141,8 +144,10 @@ def bench_batched(capi, torch, steps, warmup):
     if not finite:
         raise RuntimeError("non-finite magnification in the batched run")
     stage_ms = np.mean(np.array(stages), axis=0)
+    launch_ms = {k: float(np.mean([l[k] for l in launches])) for k in launches[0]}
     return dict(n_curves=len(curves), n_epochs=n_epochs, wall=wall, steps=steps,
-                stage_ms=stage_ms.tolist(), queue=qc, work=work, methods=meth.tolist())
+                stage_ms=stage_ms.tolist(), launch_ms=launch_ms, queue=qc, work=work,
+                methods=meth.tolist())
 
 
 def bench_value_grad(capi, torch, steps, warmup, per_epoch_calls=64):
@@ -285,6 +290,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the plain-path and point-source measurements")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three device copies of 256 MiB (the reference reading for "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for {args.gpus}): light curves are independent, "
@@ -297,6 +305,15 @@ def main() -> int:
     from lcbinint_amd import capi
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": COPY_BYTES, "copies": 3}))
+        return 0
     print("[bench] batched finite-source workload", file=sys.stderr, flush=True)
     b = bench_batched(capi, torch, args.steps, args.warmup)
     value = b["n_epochs"] * b["steps"] / b["wall"]
@@ -305,7 +322,12 @@ def main() -> int:
                 w["cart"] * FLOPS_CART_CELL + w["cart_guarded"] * FLOPS_CART_GUARDED_CELL +
                 w["polar"] * FLOPS_POLAR_CELL)
     ir_ms = b["stage_ms"][3]
-    achieved = ir_flops / (ir_ms * 1e-3) / 1e12
+    # dominant kernel: k_ir_cart (three launches per step: large slots, small
+    # slots, retries); per-launch work and duration are the step's totals / 3
+    lm = b["launch_ms"]
+    cart_ms = (lm["cart_big"] + lm["cart_small"] + lm["cart_retry"]) / 3
+    cart_flops = (w["cart"] * FLOPS_CART_CELL + w["cart_guarded"] * FLOPS_CART_GUARDED_CELL) / 3
+    achieved = cart_flops / (cart_ms * 1e-3) / 1e12
     extras = {}
     if not args.no_extras:
         print("[bench] plain path and point-source batch", file=sys.stderr, flush=True)
@@ -354,15 +376,22 @@ def main() -> int:
             "parallelism": "one process, one GPU",
         },
         "roofline": {
-            "bound": "valu",
-            "kernel": "k_fs_ir",
+            # FP64 compute roof: MI355X's FP64 vector and FP64 matrix peaks are
+            # the same 78.6 TFLOP/s; the lattice scan issues FP64 VALU work
+            "bound": "mfma",
+            "kernel": "k_ir_cart",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": K_FS_IR_TRAFFIC_BYTES,
-            "kernel_ms": ir_ms,
-            "algorithmic_flops": ir_flops,
+            "traffic": K_IR_CART_TRAFFIC_BYTES,
+            "kernel_ms": cart_ms,
+            "launches_per_step": 3,
+            "algorithmic_flops_per_launch": cart_flops,
+            "achieved_hbm_gbps": K_IR_CART_TRAFFIC_BYTES / (cart_ms * 1e-3) / 1e9,
+            "inverse_ray_stage_ms": ir_ms,
+            "inverse_ray_stage_flops": ir_flops,
+            "launch_ms": lm,
             "work": w,
         },
         "cpu_baseline": cpu,

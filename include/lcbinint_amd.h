@@ -195,6 +195,15 @@ int64_t lcbg_wall_clock_rate_khz(void);
  * counted.  counts has 8 entries. */
 lcbg_status lcbg_batch_work_counts(lcbg_batch* batch, uint64_t* counts);
 
+/* Durations (ms, HIP events on the stream each kernel runs on) of the seven
+ * kernel launches of the inverse-ray stage of the batch's last profiled run
+ * (lcbg_batch_finite_source_profile): [0] seed search (k_ir_seeds), [1]
+ * Cartesian and [2] polar area scans of the largest epochs on large slots, [3]
+ * polar and [4] Cartesian area scans on small slots, [5] Cartesian and [6]
+ * polar retries of epochs that outgrew a small slot.  A diagnostic with no
+ * counterpart in the reference.  ms has 7 entries. */
+lcbg_status lcbg_batch_ir_kernel_ms(lcbg_batch* batch, float* ms);
+
 /* Stage accessor for one epoch of the batch's last finite-source run (host
  * pointers, synchronous, allocates a few KB; a diagnostic, not for hot loops).
  * With n_in < 0 it runs the device seed search of that epoch (the stage of

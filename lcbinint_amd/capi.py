@@ -229,6 +229,19 @@ def work_counts(batch: Batch):
     return dict(zip(("quintic", "quartic", "cart", "cart_guarded", "polar", "seed_ticks",
                      "area_ticks", "rows"), list(out)))
 
+lcbg_batch_ir_kernel_ms = _sig("lcbg_batch_ir_kernel_ms", ctypes.c_int, ctypes.c_void_p,
+                               ctypes.POINTER(ctypes.c_float))
+IR_LAUNCHES = ("seeds", "cart_big", "polar_big", "polar_small", "cart_small", "cart_retry",
+               "polar_retry")
+
+
+def ir_kernel_ms(batch: Batch):
+    """Per-launch durations (ms) of the inverse-ray kernels of the last
+    profiled run, keyed by IR_LAUNCHES."""
+    out = (ctypes.c_float * 7)()
+    check(lcbg_batch_ir_kernel_ms(batch.handle, out), "lcbg_batch_ir_kernel_ms")
+    return dict(zip(IR_LAUNCHES, list(out)))
+
 lcbg_batch_finite_source_grad_device = _sig(
     "lcbg_batch_finite_source_grad_device", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
     ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

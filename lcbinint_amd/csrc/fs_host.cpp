@@ -32,6 +32,7 @@ struct FsState {
     int stage2_waves = 2048;
     int64_t last_n = 0;  // epochs of the last run (their workspace is kept)
     IrStreams ir{};  // side streams and events of the inverse-ray stage
+    float ir_kernel_ms[kIrLaunches] = {};  // per-launch times of the last profiled run
 };
 
 namespace {
@@ -376,9 +377,26 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     ep.cost = b->d_cost;
     ep.dint = dint_view(fs);
     fs->last_n = n_epochs;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // timing events of a profiled run, released on every return path
+    struct RunEvents {
+        hipEvent_t ev[5] = {};
+        hipEvent_t kev[2 * kIrLaunches] = {};
+        IrStreams* ir = nullptr;
+        ~RunEvents()
+        {
+            if (!ir) return;
+            ir->kev = nullptr;
+            for (auto& e : ev) (void)hipEventDestroy(e);
+            for (auto& e : kev) (void)hipEventDestroy(e);
+        }
+    } events;
+    hipEvent_t* ev = events.ev;
+    hipEvent_t* kev = events.kev;
     if (stage_ms) {
-        for (auto& e : ev) (void)hipEventCreate(&e);
+        for (auto& e : events.ev) (void)hipEventCreate(&e);
+        for (auto& e : events.kev) (void)hipEventCreate(&e);
+        events.ir = &fs->ir;
+        fs->ir.kev = kev;
         (void)hipEventRecord(ev[0], stream);
     }
     st = fs_refresh(b, stream);
@@ -420,9 +438,8 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
         (void)hipEventRecord(ev[4], stream);
         (void)hipEventSynchronize(ev[4]);
         for (int k = 0; k < 4; ++k) (void)hipEventElapsedTime(&stage_ms[k], ev[k], ev[k + 1]);
-    }
-    if (stage_ms) {
-        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (int k = 0; k < kIrLaunches; ++k)
+            (void)hipEventElapsedTime(&fs->ir_kernel_ms[k], kev[2 * k], kev[2 * k + 1]);
     }
     return st;
 }
@@ -566,6 +583,13 @@ extern "C" lcbg_status lcbg_batch_work_counts(lcbg_batch* batch, uint64_t* count
     if (st != LCBG_OK) return st;
     return map_hip(hipMemcpy(counts, ep.counters + kWorkOffset, 8 * sizeof(uint64_t),
                              hipMemcpyDeviceToHost));
+}
+
+extern "C" lcbg_status lcbg_batch_ir_kernel_ms(lcbg_batch* batch, float* ms)
+{
+    if (!batch || !ms || !batch->fs) return LCBG_INVALID_ARGUMENT;
+    for (int k = 0; k < kIrLaunches; ++k) ms[k] = batch->fs->ir_kernel_ms[k];
+    return LCBG_OK;
 }
 
 extern "C" lcbg_status lcbg_batch_set_epoch_cost(lcbg_batch* batch, int64_t* cost)

@@ -1525,11 +1525,16 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
                          const FsOptions& opt, const FsEpochs& ep, const IrLists& lists,
                          const IrScratch& scr, hipStream_t st, const IrStreams& ss)
 {
+    auto mark = [&](int k, hipStream_t s) {
+        if (ss.kev) (void)hipEventRecord(ss.kev[k], s);
+    };
     hipLaunchKernelGGL(k_ir_bucket_count, dim3(256), dim3(256), 0, st, d_curves, opt, ep, lists);
     hipLaunchKernelGGL(k_ir_bucket_scan, dim3(1), dim3(64), 0, st, ep);
     hipLaunchKernelGGL(k_ir_bucket_scatter, dim3(256), dim3(256), 0, st, ep, lists);
+    mark(2 * kIrSeeds, st);
     hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
                        d_curves, cache, tab, opt, ep, lists, scr.seed);
+    mark(2 * kIrSeeds + 1, st);
     lcbg_status status = check_launch();
     if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.fork, st));
     if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.polar, ss.fork, 0));
@@ -1537,27 +1542,39 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     if (status != LCBG_OK) return status;
     // the largest epochs on large slots and the polar epochs beside the
     // Cartesian small-slot pass; launched first so their waves are resident
+    mark(2 * kIrCartBig, ss.big);
     hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots)), dim3(kWave), 0,
                        ss.big, d_curves, opt, ep, lists, scr.cart_large, int{kPassBig});
+    mark(2 * kIrCartBig + 1, ss.big);
+    mark(2 * kIrPolarBig, ss.big);
     hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar_large.n_slots)), dim3(kWave), 0,
                        ss.big, d_curves, opt, ep, lists, scr.polar_large, int{kPassBig});
+    mark(2 * kIrPolarBig + 1, ss.big);
+    mark(2 * kIrPolarSmall, ss.polar);
     hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar.n_slots)), dim3(kWave), 0,
                        ss.polar, d_curves, opt, ep, lists, scr.polar, int{kPassSmall});
+    mark(2 * kIrPolarSmall + 1, ss.polar);
     status = check_launch();
     if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_big, ss.big));
     if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_polar, ss.polar));
     if (status != LCBG_OK) return status;
+    mark(2 * kIrCartSmall, st);
     hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart.n_slots)), dim3(kWave), 0, st,
                        d_curves, opt, ep, lists, scr.cart, int{kPassSmall});
+    mark(2 * kIrCartSmall + 1, st);
     status = check_launch();
     if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(st, ss.join_big, 0));
     if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(st, ss.join_polar, 0));
     if (status != LCBG_OK) return status;
     // the epochs that outgrew a small slot, on the (now free) large slots
+    mark(2 * kIrCartRetry, st);
     hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots)), dim3(kWave), 0,
                        st, d_curves, opt, ep, lists, scr.cart_large, int{kPassRetry});
+    mark(2 * kIrCartRetry + 1, st);
+    mark(2 * kIrPolarRetry, st);
     hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar_large.n_slots)), dim3(kWave), 0,
                        st, d_curves, opt, ep, lists, scr.polar_large, int{kPassRetry});
+    mark(2 * kIrPolarRetry + 1, st);
     return check_launch();
 }
 

@@ -31,6 +31,21 @@ struct IrStreams {
     hipStream_t polar;   // polar small-slot pass
     hipStream_t big;     // large-slot pass of the largest epochs
     hipEvent_t fork, join_polar, join_big;
+    // per-launch timing (profiled runs only, else null): events 2k and 2k+1
+    // bracket launch k on the stream it runs on, in the order of IrLaunch
+    hipEvent_t* kev;
+};
+
+// The kernel launches of the inverse-ray stage, in launch order.
+enum IrLaunch {
+    kIrSeeds = 0,      // k_ir_seeds, main stream
+    kIrCartBig = 1,    // k_ir_cart, large slots, big-epoch stream
+    kIrPolarBig = 2,   // k_ir_polar, large slots, big-epoch stream
+    kIrPolarSmall = 3, // k_ir_polar, small slots, polar stream
+    kIrCartSmall = 4,  // k_ir_cart, small slots, main stream
+    kIrCartRetry = 5,  // k_ir_cart, large slots, main stream
+    kIrPolarRetry = 6, // k_ir_polar, large slots, main stream
+    kIrLaunches = 7
 };
 
 lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
