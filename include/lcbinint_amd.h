@@ -177,8 +177,12 @@ lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const int64_t* o
 /* Diagnostics with no counterpart in the reference (its closest relative is the
  * LCBININT_AREA_DIAGNOSTICS switch of finite_source_magnifier.cpp): after this
  * call every finite-source run of the batch writes, for each inverse-ray epoch
- * e, the device wall-clock ticks that epoch took to cost[e] (device pointer,
- * n_epochs entries; other epochs are left untouched).  NULL turns it off.
+ * e of an n-epoch run, cost[e] = device wall-clock ticks of its seed search,
+ * cost[n + e] = ticks of its area scans (added to the value found there, so
+ * the caller zeroes it), cost[2n + e] = quintic probe solves of the seed
+ * search, cost[3n + e] = Cartesian rows or polar runs closed (added likewise).
+ * cost is a device pointer with 4 n_epochs entries; other epochs are left
+ * untouched.  NULL turns it off.
  * lcbg_wall_clock_rate_khz() gives the tick rate of the current device. */
 lcbg_status lcbg_batch_set_epoch_cost(lcbg_batch* batch, int64_t* cost);
 int64_t lcbg_wall_clock_rate_khz(void);

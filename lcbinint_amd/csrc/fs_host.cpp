@@ -192,6 +192,7 @@ lcbg_status ensure_scratch(ScratchPool& pool)
     const int large_slots = 64;
     IrScratch& s = pool.scr;
     s.seed.n_slots = seed_slots;
+    s.farm_slots = cus * ir_resident_per_cu(3);
     s.cart.n_slots = cart_slots;
     s.cart.row_cap = kRowCapSmall;
     s.cart_large.n_slots = large_slots;
@@ -210,9 +211,13 @@ lcbg_status ensure_scratch(ScratchPool& pool)
         const std::size_t n = static_cast<std::size_t>(c.n_slots);
         return carve_size(n * kPhiHash * sizeof(int)) + carve_size(n * c.run_cap * 4 * sizeof(int));
     };
+    const std::size_t heavy_bytes =
+        carve_size(kHeavyCap * sizeof(int)) * 3 +
+        carve_size(static_cast<std::size_t>(kHeavyCap) * kScanPositions * sizeof(uint16_t)) +
+        carve_size(static_cast<std::size_t>(kHeavyCap) * kScanPositions * kProbesPerPosition);
     const std::size_t bytes = carve_size(static_cast<std::size_t>(seed_slots) * kSeedCap * sizeof(Pt2)) +
-                              cart_bytes(s.cart) + cart_bytes(s.cart_large) + polar_bytes(s.polar) +
-                              polar_bytes(s.polar_large);
+                              heavy_bytes + cart_bytes(s.cart) + cart_bytes(s.cart_large) +
+                              polar_bytes(s.polar) + polar_bytes(s.polar_large);
     lcbg_status st = map_hip(hipMalloc(&pool.mem, bytes));
     if (st != LCBG_OK) {
         pool.mem = nullptr;
@@ -220,6 +225,12 @@ lcbg_status ensure_scratch(ScratchPool& pool)
     }
     char* p = static_cast<char*>(pool.mem);
     s.seed.seeds = carve<Pt2>(p, static_cast<std::size_t>(seed_slots) * kSeedCap);
+    s.seed.heavy.item = carve<int>(p, kHeavyCap);
+    s.seed.heavy.start = carve<int>(p, kHeavyCap);
+    s.seed.heavy.n_list = carve<int>(p, kHeavyCap);
+    s.seed.heavy.list = carve<uint16_t>(p, static_cast<std::size_t>(kHeavyCap) * kScanPositions);
+    s.seed.heavy.flags = carve<uint8_t>(p, static_cast<std::size_t>(kHeavyCap) * kScanPositions *
+                                               kProbesPerPosition);
     for (CartScratch* c : {&s.cart, &s.cart_large}) {
         const std::size_t n = static_cast<std::size_t>(c->n_slots);
         c->areaimage = carve<double>(p, n * kSeedCap);
@@ -322,7 +333,7 @@ lcbg_status fs_prepare(lcbg_batch* b)
     const int nc = b->n_curves > 0 ? b->n_curves : 1;
     const std::size_t per = static_cast<std::size_t>(nc);
     const std::size_t bytes = carve_size(per * bins * 4 * sizeof(Pt2)) * 2 +
-                              carve_size(per * bins * 4 * sizeof(int)) * 2 +
+                              carve_size(per * bins * 4 * sizeof(int)) * 3 +
                               carve_size(per * kSeedPhases * 4 * sizeof(Pt2)) +
                               carve_size(per * kSeedPhases * 4) + carve_size(per * sizeof(CausticScalars));
     st = map_hip(hipMalloc(&fs->cache_mem, bytes));
@@ -333,6 +344,7 @@ lcbg_status fs_prepare(lcbg_batch* b)
     fs->cache.br = carve<Pt2>(p, per * bins * 4);
     fs->cache.pt_cell = carve<int>(p, per * bins * 4);
     fs->cache.br_cell = carve<int>(p, per * bins * 4);
+    fs->cache.br_order = carve<int>(p, per * bins * 4);
     fs->cache.crit = carve<Pt2>(p, per * kSeedPhases * 4);
     fs->cache.assign = carve<uint8_t>(p, per * kSeedPhases * 4);
     fs->cache.sc = carve<CausticScalars>(p, per);

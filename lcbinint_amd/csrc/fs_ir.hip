@@ -44,6 +44,14 @@ struct Ctx {
     int row_cap;       // row records of this slot (Cartesian)
     int run_cap;       // visited runs of this slot (polar)
     bool cap_overflow; // a slot capacity (not the seed list or work budget) ran out
+    // first-crossing scan hand-off to the probe farm (seed search only)
+    const uint8_t* farm;  // probe flags of this search (resumed scan), or null
+    int p1_start;         // first scan position whose probes are applied
+    int defer_budget;     // inside positions to probe before deferring (<0: never)
+    int defer_at;         // scan position the search was deferred at, or -1
+    int defer_slot;       // its heavy-list slot
+    unsigned* heavy_count;
+    bool seeds_changed;   // a probe added or replaced a seed
 };
 
 // Counts one wave step; past the budget the epoch is abandoned with an error.
@@ -134,9 +142,11 @@ __device__ void merge_probe(Ctx& C, int n_img, const Pt* img, const double* md)
                 wave_sync();  // every lane has read seeds[si] before it changes
                 wave_store(&C.seeds[si], img[i]);
                 wave_sync();
+                C.seeds_changed = true;
             }
         } else {
             push_seed(C, img[i]);
+            C.seeds_changed = true;
         }
     }
 }
@@ -183,43 +193,64 @@ __device__ __forceinline__ void merge_lane(Ctx& C, const ProbeResult& mine, int 
     merge_probe(C, n, img, md);
 }
 
-// append_caustic_probe_image_seeds (finite_source_magnifier.cpp:1500-1563)
-__device__ void caustic_probes(Ctx& C, Pt cs, const FsTables& tab)
+// Probe k of a caustic point cs (append_caustic_probe_image_seeds,
+// finite_source_magnifier.cpp:1500-1563): k < 8 along the unit vector (ux, uy)
+// from the source centre, steps 0.02, 0.05, 0.15, 0.35 rho, outward then
+// inward; k >= 8 the angular probes at radii 0.02, 0.05, 0.10, 0.20, 0.35 rho.
+__device__ __forceinline__ Pt caustic_probe_point(const Ctx& C, Pt cs, double ux, double uy, int k,
+                                                  const FsTables& tab)
+{
+    if (k < 8) {
+        const int si = k >> 1;
+        const double st = si == 0 ? 0.02 * C.rho : si == 1 ? 0.05 * C.rho : si == 2 ? 0.15 * C.rho
+                                                                                   : 0.35 * C.rho;
+        return (k & 1) ? Pt{cs.x - ux * st, cs.y - uy * st} : Pt{cs.x + ux * st, cs.y + uy * st};
+    }
+    const int a = k - 8;
+    const int ri = a / kAngularProbes;
+    const double st = ri == 0 ? 0.02 * C.rho : ri == 1 ? 0.05 * C.rho : ri == 2 ? 0.10 * C.rho
+                    : ri == 3 ? 0.20 * C.rho : 0.35 * C.rho;
+    const Pt t = tab.angular_trig[a % kAngularProbes];
+    return Pt{cs.x + st * t.x, cs.y + st * t.y};
+}
+
+// Direction of the caustic probes of cs; false when cs is at the centre or
+// not inside the disk (no probes).
+__device__ __forceinline__ bool caustic_probe_axis(const Ctx& C, Pt cs, double* ux, double* uy)
 {
     const double dx = cs.x - C.src.x;
     const double dy = cs.y - C.src.y;
-    const double distance = uni(hypot_glibc(dx, dy));
-    if (distance <= 0.0 || distance >= C.rho) return;
-    const double ux = dx / distance;
-    const double uy = dy / distance;
+    const double distance = hypot_glibc(dx, dy);
+    if (distance <= 0.0 || distance >= C.rho) return false;
+    *ux = dx / distance;
+    *uy = dy / distance;
+    return true;
+}
+
+// append_caustic_probe_image_seeds (finite_source_magnifier.cpp:1500-1563).
+// With probe flags `fl` (a resumed scan) only the probes flagged as finding
+// more than three images are solved again; the others cannot change the seeds.
+__device__ void caustic_probes(Ctx& C, Pt cs, const FsTables& tab, const uint8_t* fl)
+{
+    double ux = 0.0, uy = 0.0;
+    if (!uni(caustic_probe_axis(C, cs, &ux, &uy) ? 1 : 0)) return;
     {
         // 4 steps x 2 directions in the reference's order, one probe per lane
-        const double steps[4] = {0.02 * C.rho, 0.05 * C.rho, 0.15 * C.rho, 0.35 * C.rho};
         ProbeResult mine;
         mine.n = 0;
         mine.solved = false;
-        if (C.lane < 8) {
-            const double st = steps[C.lane >> 1];
-            const Pt p = (C.lane & 1) ? Pt{cs.x - ux * st, cs.y - uy * st}
-                                      : Pt{cs.x + ux * st, cs.y + uy * st};
-            mine = eval_probe(C, p);
-        }
+        if (C.lane < 8 && (!fl || fl[C.lane])) mine = eval_probe(C, caustic_probe_point(C, cs, ux, uy, C.lane, tab));
         C.n_quintic = uni(C.n_quintic + popcount64(ballot(mine.solved)));
         for (int k = 0; k < 8; ++k) merge_lane(C, mine, k);
     }
     if (C.n_seeds <= 3) {
-        const double radial[5] = {0.02 * C.rho, 0.05 * C.rho, 0.10 * C.rho, 0.20 * C.rho,
-                                  0.35 * C.rho};
         for (int base = 0; base < 5 * kAngularProbes; base += kWave) {
             const int k = base + C.lane;
             ProbeResult mine;
             mine.n = 0;
             mine.solved = false;
-            if (k < 5 * kAngularProbes) {
-                const double st = radial[k / kAngularProbes];
-                const Pt t = tab.angular_trig[k % kAngularProbes];
-                mine = eval_probe(C, Pt{cs.x + st * t.x, cs.y + st * t.y});
-            }
+            if (k < 5 * kAngularProbes && (!fl || fl[8 + k]))
+                mine = eval_probe(C, caustic_probe_point(C, cs, ux, uy, 8 + k, tab));
             C.n_quintic = uni(C.n_quintic + popcount64(ballot(mine.solved)));
             const int cnt = (5 * kAngularProbes - base) < kWave ? (5 * kAngularProbes - base) : kWave;
             for (int j = 0; j < cnt; ++j) {
@@ -285,7 +316,7 @@ __device__ Pt refine_critical(Ctx& C, double center, double step)
 // augmented_image_seeds (finite_source_magnifier.cpp:1709-1871) without seed
 // hints, then augment_seeds_from_caustic_branches (:4845-4926).
 __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const Pt* br,
-                            const int* br_cell, const CausticScalars& sc, int bins,
+                            const int* br_cell, const int* br_order, const CausticScalars& sc, int bins,
                             const FsTables& tab, const double* roots_ps)
 {
     // Phase 0: binary_images (point_source_magnifier.cpp:723-778) of the source
@@ -321,7 +352,8 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
     bool found = false;
     double best_d2 = __builtin_inf();
     int best_ord = 0x7fffffff;
-    constexpr int kOrd = kSeedPhases * 4;  // scan positions: (k*35 + j)*4 + root
+    constexpr int kOrd = kScanPositions;  // scan positions: (k*35 + j)*4 + root
+    int probed = 0;                        // inside positions probed so far
     for (int base = 0; base < kOrd && !found; base += kWave) {
         const int o = base + C.lane;
         double d2 = __builtin_inf();
@@ -333,7 +365,8 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             const int jphi = ph - kphi * 35;
             const int sample = jphi * 40 + kphi;
             d2 = dist2(crit[4 * sample + r], C.src);
-            inside = d2 < C.r2 && d2 > 0.0;
+            // positions before a resumed scan's start were probed without effect
+            inside = d2 < C.r2 && d2 > 0.0 && o >= C.p1_start;
         }
         // running best in scan order (ties keep the earliest)
         {
@@ -348,6 +381,21 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             }
         }
         uint64_t m = ballot(inside);
+        // positions whose probes can change the seeds (all of them, unless the
+        // probe farm flagged this search's probes)
+        uint64_t live = m;
+        if (C.farm) {
+            bool any = false;
+            if (inside) {
+                const uint64_t* f8 = reinterpret_cast<const uint64_t*>(
+                    C.farm + static_cast<size_t>(o) * kProbesPerPosition);
+                uint64_t acc = 0;
+#pragma unroll
+                for (int w = 0; w < kProbesPerPosition / 8; ++w) acc |= f8[w];
+                any = acc != 0;
+            }
+            live = ballot(any);
+        }
         int cur_phase = -1;
         bool phase_found = false;
         while (m) {
@@ -360,11 +408,26 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
                 cur_phase = ph;
                 phase_found = false;
             }
-            const int kphi = ph / 35;
-            const int jphi = ph - kphi * 35;
-            const int sample = jphi * 40 + kphi;
+            if (C.defer_budget >= 0 && probed >= C.defer_budget && !C.seeds_changed) {
+                // a long scan with no effect so far: hand the positions from
+                // o2 on to the probe farm and resume later
+                const int h = wave_fetch_inc(C.heavy_count);
+                if (h < kHeavyCap) {
+                    C.defer_at = o2;
+                    C.defer_slot = h;
+                    return;
+                }
+                C.defer_budget = -1;
+            }
+            ++probed;
             const int before = C.n_seeds;
-            caustic_probes(C, crit[4 * sample + (o2 & 3)], tab);
+            if ((live >> b) & 1ull) {
+                const int kphi = ph / 35;
+                const int jphi = ph - kphi * 35;
+                const int sample = jphi * 40 + kphi;
+                caustic_probes(C, crit[4 * sample + (o2 & 3)], tab,
+                               C.farm ? C.farm + static_cast<size_t>(o2) * kProbesPerPosition : nullptr);
+            }
             phase_found = uni(C.n_seeds > before);
         }
         if (cur_phase >= 0 && phase_found) {
@@ -379,7 +442,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
         const int sample = jphi * 40 + kphi;
         const double best_phase = phase_step * static_cast<double>(sample);
         const Pt cs = refine_critical(C, best_phase, phase_step);
-        caustic_probes(C, cs, tab);
+        caustic_probes(C, cs, tab, nullptr);
     }
 
     if (C.last_phase < 2) return;
@@ -422,7 +485,7 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
                 const int o2 = base + b;
                 const int sample = o2 >> 2;
                 const int j = assign[4 * sample + (o2 & 3)];
-                caustic_probes(C, crit[4 * sample + j], tab);
+                caustic_probes(C, crit[4 * sample + j], tab, nullptr);
             }
         }
     }
@@ -468,68 +531,82 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
         const int iy0 = iclamp(static_cast<int>((C.src.y - seg_search - sc.min_y) / sc.step_y), 0, kGrid - 1);
         const int iy1 = iclamp(static_cast<int>((C.src.y + seg_search - sc.min_y) / sc.step_y), 0, kGrid - 1);
         const int n = sc.n;
-        // Candidates are visited in (cell, branch, position) order; the seed
-        // count is tested only when the cell changes, as in the reference.
+        const int nk = 4 * bins;
+        // Candidates are visited in (cell, branch, position) order, the order
+        // of br_order; the seed count is tested only when the cell changes, as
+        // in the reference.  Each lane takes one candidate of a 64-candidate
+        // chunk and solves its probe; the results are then applied in order,
+        // and the solves past a stop are discarded.
         int last_key = -1;
-        long long cursor = -1;  // last processed composite key
-        for (;;) {
-            // find the smallest composite key greater than cursor among candidates
-            long long best = 0x7fffffffffffffffLL;
-            for (int k = C.lane; k < 4 * bins; k += kWave) {
-                const int cell = br_cell[k];
+        bool stop = false;
+        for (int base = 0; base < nk && !stop; base += kWave) {
+            const int i = base + C.lane;
+            bool cand = false;
+            bool probe_ok = false;
+            int cell = -1;
+            Pt probe{0.0, 0.0};
+            if (i < nk) {
+                const int k = br_order[i];
+                cell = br_cell[k];
                 const int iy = cell / kGrid;
                 const int ix = cell - iy * kGrid;
-                if (ix < ix0 || ix > ix1 || iy < iy0 || iy > iy1) continue;
-                const long long key = static_cast<long long>(cell) * (4LL * bins) + k;
-                if (key <= cursor || key >= best) continue;
                 const int b = k / bins;
                 const int pos = k - b * bins;
                 const int next = (pos + 1) % n;
-                if (next == 0) continue;
-                const Pt* bb = br + static_cast<size_t>(b) * bins;
-                if (point_segment_distance(C.src, bb[pos], bb[next]) >= C.rho) continue;
-                best = key;
+                if (ix >= ix0 && ix <= ix1 && iy >= iy0 && iy <= iy1 && next != 0) {
+                    const Pt* bb = br + static_cast<size_t>(b) * bins;
+                    const Pt p0 = bb[pos], p1 = bb[next];
+                    if (point_segment_distance(C.src, p0, p1) < C.rho) {
+                        cand = true;
+                        const double seg_dx = p1.x - p0.x;
+                        const double seg_dy = p1.y - p0.y;
+                        const double seg_len2 = seg_dx * seg_dx + seg_dy * seg_dy;
+                        const double t = seg_len2 > 0.0
+                                             ? dclamp(((C.src.x - p0.x) * seg_dx + (C.src.y - p0.y) * seg_dy) / seg_len2,
+                                                      0.0, 1.0)
+                                             : 0.0;
+                        const Pt nearest{p0.x + t * seg_dx, p0.y + t * seg_dy};
+                        const double distance = __builtin_sqrt(dist2(nearest, C.src));
+                        if (distance > 0.0 && distance < C.rho) {
+                            const double step = C.rho * 0.05 / distance;
+                            probe = Pt{nearest.x + (nearest.x - C.src.x) * step,
+                                       nearest.y + (nearest.y - C.src.y) * step};
+                            probe_ok = dist2(probe, C.src) < C.rho * C.rho;
+                        }
+                    }
+                }
             }
+            uint64_t cm = ballot(cand);
+            if (!cm) continue;
+            Pt im[5] = {};
+            int ni = 0;
+            if (probe_ok) ni = probe_r3_images(C, probe, im);
+            while (cm) {
+                const int j = first_set(cm);
+                cm &= cm - 1;
+                const int cj = uni(shfl(cell, j));
+                if (cj != last_key) {
+                    if (C.n_seeds >= 5) {
+                        stop = true;
+                        break;
+                    }
+                    last_key = cj;
+                }
+                if (uni(shfl(probe_ok ? 1 : 0, j)) == 0) continue;
+                C.n_quintic = uni(C.n_quintic + 1);
+                const int nij = uni(shfl(ni, j));
+                if (nij > C.n_seeds) {
+                    Pt imj[5];
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                const long long o = __shfl_xor(best, off, kWave);
-                best = o < best ? o : best;
-            }
-            best = uni(best);
-            if (best == 0x7fffffffffffffffLL) break;
-            cursor = best;
-            const int cell = static_cast<int>(best / (4LL * bins));
-            if (cell != last_key) {
-                if (C.n_seeds >= 5) break;
-                last_key = cell;
-            }
-            const int k = static_cast<int>(best - static_cast<long long>(cell) * (4LL * bins));
-            const int b = k / bins;
-            const int pos = k - b * bins;
-            const int next = (pos + 1) % n;
-            const Pt* bb = br + static_cast<size_t>(b) * bins;
-            const Pt p0 = bb[pos], p1 = bb[next];
-            const double seg_dx = p1.x - p0.x;
-            const double seg_dy = p1.y - p0.y;
-            const double seg_len2 = seg_dx * seg_dx + seg_dy * seg_dy;
-            const double t = seg_len2 > 0.0
-                                 ? dclamp(((C.src.x - p0.x) * seg_dx + (C.src.y - p0.y) * seg_dy) / seg_len2,
-                                          0.0, 1.0)
-                                 : 0.0;
-            const Pt nearest{p0.x + t * seg_dx, p0.y + t * seg_dy};
-            const double distance = __builtin_sqrt(dist2(nearest, C.src));
-            if (distance <= 0.0 || distance >= C.rho) continue;
-            const double step = C.rho * 0.05 / distance;
-            const Pt probe{nearest.x + (nearest.x - C.src.x) * step, nearest.y + (nearest.y - C.src.y) * step};
-            if (dist2(probe, C.src) >= C.rho * C.rho) continue;
-            Pt im[5];
-            const int ni = probe_r3_images(C, probe, im);
-            C.n_quintic = uni(C.n_quintic + 1);
-            if (ni > C.n_seeds) {
-                wave_sync();
-                for (int i = 0; i < ni; ++i) wave_store(&C.seeds[i], im[i]);
-                C.n_seeds = ni;
-                wave_sync();
+                    for (int q = 0; q < 5; ++q) {
+                        imj[q].x = uni(shfl(im[q].x, j));
+                        imj[q].y = uni(shfl(im[q].y, j));
+                    }
+                    wave_sync();
+                    for (int q = 0; q < nij; ++q) wave_store(&C.seeds[q], imj[q]);
+                    C.n_seeds = nij;
+                    wave_sync();
+                }
             }
         }
     }
@@ -1240,6 +1317,13 @@ __device__ __forceinline__ void init_ctx(Ctx& C, const CurveRec& cv, Pt src, Pt*
     C.row_cap = 0;
     C.run_cap = 0;
     C.cap_overflow = false;
+    C.farm = nullptr;
+    C.p1_start = 0;
+    C.defer_budget = -1;
+    C.defer_at = -1;
+    C.defer_slot = -1;
+    C.heavy_count = nullptr;
+    C.seeds_changed = false;
 }
 
 __device__ __forceinline__ unsigned long long* work_totals(const FsEpochs& ep)
@@ -1279,7 +1363,8 @@ __device__ __forceinline__ void epoch_seeds(Ctx& C, const FsCache& cache, const 
     image_seeds(C, cache.crit + static_cast<size_t>(c) * kSeedPhases * 4,
                 cache.assign + static_cast<size_t>(c) * kSeedPhases * 4,
                 cache.br + static_cast<size_t>(c) * bins * 4,
-                cache.br_cell + static_cast<size_t>(c) * bins * 4, sc, bins, tab,
+                cache.br_cell + static_cast<size_t>(c) * bins * 4,
+                cache.br_order + static_cast<size_t>(c) * bins * 4, sc, bins, tab,
                 ep.roots + 10 * e);
 }
 
@@ -1352,23 +1437,51 @@ __global__ void k_ir_bucket_scatter(FsEpochs ep, IrLists L)
 }
 
 // Seed search of every inverse-ray epoch (largest first); the seeds go to the
-// shared pool and the epoch to the Cartesian or polar queue.
+// shared pool and the epoch to the Cartesian or polar queue.  Pass 0 runs
+// every search and defers the long first-crossing scans to the probe farm;
+// pass 1 resumes the deferred ones with the farm's probe flags.
 __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__ curves,
                                                      FsCache cache, FsTables tab, FsOptions opt,
-                                                     FsEpochs ep, IrLists L, SeedScratch scr)
+                                                     FsEpochs ep, IrLists L, SeedScratch scr, int pass)
 {
     Pt* slot_seeds = scr.seeds + static_cast<size_t>(blockIdx.x) * kSeedCap;
-    const int count = uni(static_cast<int>(ep.counters[kQIrCount]));
+    const HeavyScratch& hv = scr.heavy;
+    const int n_heavy = uni(static_cast<int>(ep.counters[kQHeavyCount]));
+    const int count = pass == 0 ? uni(static_cast<int>(ep.counters[kQIrCount]))
+                                : (n_heavy < kHeavyCap ? n_heavy : kHeavyCap);
+    unsigned* cursor = &ep.counters[pass == 0 ? kQIrCursor : kQHeavyCursor];
     for (;;) {
-        const int item = next_item(&ep.counters[kQIrCursor]);
-        if (item >= count) break;
+        const int task = next_item(cursor);
+        if (task >= count) break;
+        const int item = pass == 0 ? task : uni(hv.item[task]);
         const int64_t e = L.ir_sorted[item];
         const long long t_begin = static_cast<long long>(wall_clock64());
         const int c = ep.curve_of[e];
         const CurveRec cv = curves[c];
         Ctx C;
         init_ctx(C, cv, ep.src[e], slot_seeds);
+        if (pass == 0) {
+            C.defer_budget = kHeavyBudget;
+            C.heavy_count = &ep.counters[kQHeavyCount];
+        } else {
+            C.farm = hv.flags + static_cast<size_t>(task) * kScanPositions * kProbesPerPosition;
+            C.p1_start = uni(hv.start[task]);
+        }
         epoch_seeds(C, cache, tab, ep, c, e);
+        if (C.defer_at >= 0) {
+            // deferred: the farm and pass 1 finish this search
+            wave_store(&hv.item[C.defer_slot], item);
+            wave_store(&hv.start[C.defer_slot], C.defer_at);
+            flush_work(C, ep);
+            const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
+            add_total(&work_totals(ep)[5], ticks);
+            if (ep.cost) {
+                wave_store(&ep.cost[e], static_cast<int64_t>(ticks));
+                wave_store(&ep.cost[2 * ep.n + e], static_cast<int64_t>(C.n_quintic));
+            }
+            wave_sync();
+            continue;
+        }
         const int n = C.n_seeds;
         bool ok = !C.overflow;
         int off = 0;
@@ -1401,9 +1514,103 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
         flush_work(C, ep);
         const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
         add_total(&work_totals(ep)[5], ticks);
-        if (ep.cost) wave_store(&ep.cost[e], static_cast<int64_t>(ticks));
+        if (ep.cost) {
+            const int64_t prior = pass == 0 ? 0 : ep.cost[e];
+            const int64_t prior_q = pass == 0 ? 0 : ep.cost[2 * ep.n + e];
+            wave_store(&ep.cost[e], prior + static_cast<int64_t>(ticks));
+            wave_store(&ep.cost[2 * ep.n + e], prior_q + static_cast<int64_t>(C.n_quintic));
+        }
         wave_sync();
     }
+}
+
+// Inside scan positions of each deferred search from its start on, in scan
+// order (one wave per search).
+__global__ void __launch_bounds__(kWave) k_ir_heavy_list(const CurveRec* __restrict__ curves,
+                                                          FsCache cache, FsEpochs ep, IrLists L,
+                                                          HeavyScratch hv)
+{
+    const int h = blockIdx.x;
+    const int n_heavy = uni(static_cast<int>(ep.counters[kQHeavyCount]));
+    if (h >= n_heavy || h >= kHeavyCap) return;
+    const int lane = lane_id();
+    const int item = uni(hv.item[h]);
+    const int start = uni(hv.start[h]);
+    const int64_t e = L.ir_sorted[item];
+    const int c = ep.curve_of[e];
+    const double rho = curves[c].rho;
+    const double r2 = rho * rho;
+    const Pt src = ep.src[e];
+    const Pt* crit = cache.crit + static_cast<size_t>(c) * kSeedPhases * 4;
+    uint16_t* list = hv.list + static_cast<size_t>(h) * kScanPositions;
+    int n = 0;
+    for (int base = start & ~(kWave - 1); base < kScanPositions; base += kWave) {
+        const int o = base + lane;
+        bool inside = false;
+        if (o < kScanPositions && o >= start) {
+            const int ph = o >> 2;
+            const int kphi = ph / 35;
+            const int jphi = ph - kphi * 35;
+            const int sample = jphi * 40 + kphi;
+            const double d2 = dist2(crit[4 * sample + (o & 3)], src);
+            inside = d2 < r2 && d2 > 0.0;
+        }
+        const uint64_t m = ballot(inside);
+        if (inside) list[n + popcount64(m & ((1ull << lane) - 1ull))] = static_cast<uint16_t>(o);
+        n += popcount64(m);
+    }
+    wave_store(&hv.n_list[h], n);
+}
+
+// The probe farm: every caustic and angular probe of the listed positions of
+// every deferred search, one probe per lane, kFarmChunk positions per task;
+// flags[o * kProbesPerPosition + k] = probe k of position o is inside the disk
+// and finds more than three images.
+__global__ void __launch_bounds__(kWave) k_ir_farm(const CurveRec* __restrict__ curves,
+                                                    FsCache cache, FsTables tab, FsEpochs ep,
+                                                    IrLists L, HeavyScratch hv)
+{
+    const int n_heavy = uni(static_cast<int>(ep.counters[kQHeavyCount]));
+    const int tasks = (n_heavy < kHeavyCap ? n_heavy : kHeavyCap) * kFarmChunks;
+    long long solved = 0;
+    for (;;) {
+        const int t = next_item(&ep.counters[kQFarmCursor]);
+        if (t >= tasks) break;
+        const int h = t / kFarmChunks;
+        const int i0 = (t - h * kFarmChunks) * kFarmChunk;
+        const int nl = uni(hv.n_list[h]);
+        if (i0 >= nl) continue;
+        const int np = nl - i0 < kFarmChunk ? nl - i0 : kFarmChunk;
+        const int item = uni(hv.item[h]);
+        const int64_t e = L.ir_sorted[item];
+        const int c = ep.curve_of[e];
+        Ctx C;
+        init_ctx(C, curves[c], ep.src[e], nullptr);
+        const Pt* crit = cache.crit + static_cast<size_t>(c) * kSeedPhases * 4;
+        const uint16_t* list = hv.list + static_cast<size_t>(h) * kScanPositions + i0;
+        uint8_t* flags = hv.flags + static_cast<size_t>(h) * kScanPositions * kProbesPerPosition;
+        for (int q0 = 0; q0 < np * kProbesPerPosition; q0 += kWave) {
+            const int q = q0 + C.lane;
+            if (q >= np * kProbesPerPosition) continue;
+            const int pi = q / kProbesPerPosition;
+            const int k = q - pi * kProbesPerPosition;
+            const int o = list[pi];
+            const int ph = o >> 2;
+            const int kphi = ph / 35;
+            const int jphi = ph - kphi * 35;
+            const int sample = jphi * 40 + kphi;
+            const Pt cs = crit[4 * sample + (o & 3)];
+            bool valid = false;
+            double ux, uy;
+            if (caustic_probe_axis(C, cs, &ux, &uy)) {
+                const ProbeResult r = eval_probe(C, caustic_probe_point(C, cs, ux, uy, k, tab));
+                valid = r.n > 3;
+                solved += r.solved ? 1 : 0;
+            }
+            flags[static_cast<size_t>(o) * kProbesPerPosition + k] = valid ? 1 : 0;
+        }
+    }
+    if (solved) atomicAdd(&work_totals(ep)[0], static_cast<unsigned long long>(solved));
 }
 
 // Cartesian row scans of the queued epochs, in one of three passes: small
@@ -1439,7 +1646,10 @@ __global__ void __launch_bounds__(kWave) k_ir_cart(const CurveRec* __restrict__ 
         flush_work(C, ep);
         const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
         add_total(&work_totals(ep)[6], ticks);
-        if (ep.cost) wave_store(&ep.cost[e], ep.cost[e] + static_cast<int64_t>(ticks));
+        if (ep.cost) {
+            wave_store(&ep.cost[ep.n + e], ep.cost[ep.n + e] + static_cast<int64_t>(ticks));
+            wave_store(&ep.cost[3 * ep.n + e], ep.cost[3 * ep.n + e] + static_cast<int64_t>(C.n_rows));
+        }
         wave_sync();
     }
 }
@@ -1478,7 +1688,10 @@ __global__ void __launch_bounds__(kWave) k_ir_polar(const CurveRec* __restrict__
         flush_work(C, ep);
         const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
         add_total(&work_totals(ep)[6], ticks);
-        if (ep.cost) wave_store(&ep.cost[e], ep.cost[e] + static_cast<int64_t>(ticks));
+        if (ep.cost) {
+            wave_store(&ep.cost[ep.n + e], ep.cost[ep.n + e] + static_cast<int64_t>(ticks));
+            wave_store(&ep.cost[3 * ep.n + e], ep.cost[3 * ep.n + e] + static_cast<int64_t>(C.n_rows));
+        }
         wave_sync();
     }
 }
@@ -1533,7 +1746,14 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     hipLaunchKernelGGL(k_ir_bucket_scatter, dim3(256), dim3(256), 0, st, ep, lists);
     mark(2 * kIrSeeds, st);
     hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
-                       d_curves, cache, tab, opt, ep, lists, scr.seed);
+                       d_curves, cache, tab, opt, ep, lists, scr.seed, 0);
+    // the deferred first-crossing scans: list, farm, resume
+    hipLaunchKernelGGL(k_ir_heavy_list, dim3(kHeavyCap), dim3(kWave), 0, st, d_curves, cache, ep, lists,
+                       scr.seed.heavy);
+    hipLaunchKernelGGL(k_ir_farm, dim3(static_cast<unsigned>(scr.farm_slots)), dim3(kWave), 0, st,
+                       d_curves, cache, tab, ep, lists, scr.seed.heavy);
+    hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
+                       d_curves, cache, tab, opt, ep, lists, scr.seed, 1);
     mark(2 * kIrSeeds + 1, st);
     lcbg_status status = check_launch();
     if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.fork, st));
@@ -1582,7 +1802,8 @@ int ir_resident_per_cu(int kind)
 {
     const void* fn = kind == 0 ? reinterpret_cast<const void*>(k_ir_seeds)
                    : kind == 1 ? reinterpret_cast<const void*>(k_ir_cart)
-                               : reinterpret_cast<const void*>(k_ir_polar);
+                   : kind == 2 ? reinterpret_cast<const void*>(k_ir_polar)
+                               : reinterpret_cast<const void*>(k_ir_farm);
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kWave, 0) != hipSuccess || n < 1) n = 1;
     return n;

@@ -216,12 +216,75 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
     }
 }
 
+// The branch positions of each curve in (grid cell, position) order, the
+// order in which augment_seeds_from_caustic_branches (:4845-4926) visits its
+// candidate segments: a counting sort over the 128x128 cells in LDS, scattered
+// 256 positions at a time with each position's rank among the earlier ones of
+// its cell, which keeps positions of one cell in index order.
+constexpr int kOrderBlock = 256;
+constexpr int kCells = kGrid * kGrid;
+
+__global__ void __launch_bounds__(kOrderBlock) k_branch_order(const CurveRec* __restrict__ curves,
+                                                              FsCache cache)
+{
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const CurveRec cv = curves[c];
+    if (!(cv.rho > 0.0) || cv.s == 0.0 || !(cv.q_eff > 0.0)) return;  // no caustic cache
+    const int n = 4 * cache.bins;
+    const int* cell = cache.br_cell + static_cast<size_t>(c) * n;
+    int* order = cache.br_order + static_cast<size_t>(c) * n;
+    __shared__ int hist[kCells];
+    __shared__ int part[kOrderBlock];
+    __shared__ int chunk[kOrderBlock];
+    for (int i = tid; i < kCells; i += kOrderBlock) hist[i] = 0;
+    __syncthreads();
+    for (int k = tid; k < n; k += kOrderBlock) atomicAdd(&hist[iclamp(cell[k], 0, kCells - 1)], 1);
+    __syncthreads();
+    // exclusive prefix sum: thread t owns cells [64 t, 64 t + 64)
+    constexpr int kPer = kCells / kOrderBlock;
+    int sum = 0;
+    for (int i = 0; i < kPer; ++i) sum += hist[tid * kPer + i];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < kOrderBlock; off <<= 1) {
+        const int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - sum;
+    for (int i = 0; i < kPer; ++i) {
+        const int h = hist[tid * kPer + i];
+        hist[tid * kPer + i] = run;
+        run += h;
+    }
+    __syncthreads();
+    for (int base = 0; base < n; base += kOrderBlock) {
+        const int k = base + tid;
+        const int my = k < n ? iclamp(cell[k], 0, kCells - 1) : -1;
+        chunk[tid] = my;
+        __syncthreads();
+        int rank = 0;
+        for (int j = 0; j < tid; ++j) rank += chunk[j] == my ? 1 : 0;
+        const int pos = k < n ? hist[my] + rank : 0;
+        __syncthreads();
+        if (k < n) {
+            order[pos] = k;
+            atomicAdd(&hist[my], 1);
+        }
+        __syncthreads();
+    }
+}
+
 lcbg_status launch_curve_prep(const CurveRec* d_curves, int n_curves, const FsCache& cache,
                               const FsTables& tab, hipStream_t st)
 {
     if (n_curves <= 0) return LCBG_OK;
     hipLaunchKernelGGL(k_curve_prep, dim3(static_cast<unsigned>(n_curves)), dim3(kPrepBlock), 0,
                        st, d_curves, cache, tab);
+    hipLaunchKernelGGL(k_branch_order, dim3(static_cast<unsigned>(n_curves)), dim3(kOrderBlock), 0,
+                       st, d_curves, cache);
     return check_launch();
 }
 

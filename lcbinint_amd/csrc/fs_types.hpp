@@ -68,6 +68,7 @@ struct FsCache {
     Pt2* br;           // [curve][4*bins]   tracked branches
     int* pt_cell;      // [curve][bins*4]   grid cell (iy*128+ix) of each point
     int* br_cell;      // [curve][4*bins]
+    int* br_order;     // [curve][4*bins]   branch positions sorted by (br_cell, index)
     Pt2* crit;         // [curve][1400*4]   critical sources of the seed scan
     uint8_t* assign;   // [curve][1400*4]   Phase-2 branch -> root index
     CausticScalars* sc;// [curve]
@@ -104,9 +105,30 @@ struct FsEpochs {
 };
 
 // Scratch of the resident waves of each inverse-ray kernel.
+// Seed searches whose first-crossing scan probes more than kHeavyBudget
+// caustic positions without a change hand the rest of the scan to the probe
+// farm: every probe of the remaining inside positions is solved across the
+// whole device, then the scan resumes and solves again only the probes that
+// found more than three images (the only ones that can change the seeds).
+constexpr int kHeavyCap = 256;
+constexpr int kHeavyBudget = 16;
+constexpr int kScanPositions = kSeedPhases * 4;             // (phase, root) positions
+constexpr int kProbesPerPosition = 8 + 5 * kAngularProbes;  // caustic + angular probes
+constexpr int kFarmChunk = 8;                               // positions per farm task
+constexpr int kFarmChunks = (kScanPositions + kFarmChunk - 1) / kFarmChunk;
+
+struct HeavyScratch {
+    int* item;        // [kHeavyCap] inverse-ray item of the deferred search
+    int* start;       // [kHeavyCap] first scan position left to probe
+    int* n_list;      // [kHeavyCap] inside positions from `start` on
+    uint16_t* list;   // [kHeavyCap][kScanPositions] those positions in scan order
+    uint8_t* flags;   // [kHeavyCap][kScanPositions * kProbesPerPosition] probe found > 3 images
+};
+
 struct SeedScratch {
     int n_slots;
     Pt2* seeds;        // [slot][kSeedCap]
+    HeavyScratch heavy;
 };
 
 struct CartScratch {
@@ -129,6 +151,7 @@ struct IrScratch {
     SeedScratch seed;
     CartScratch cart, cart_large;
     PolarScratch polar, polar_large;
+    int farm_slots;  // resident waves of the probe farm
 };
 
 // Work lists of the inverse-ray stage, in the epoch workspace.  Items index
@@ -164,7 +187,7 @@ enum : int {
     kQCartRetryCount = 9, kQCartRetryCursor = 10, kQPolarRetryCount = 11,
     kQPolarRetryCursor = 12, kQPoolCursor = 13, kQCartBigCount = 14, kQCartBigCursor = 15,
     kQPolarBigCount = 16, kQPolarBigCursor = 17, kQGradCursor = 18, kQGradRetryCount = 19,
-    kQGradRetryCursor = 20
+    kQGradRetryCursor = 20, kQHeavyCount = 21, kQHeavyCursor = 22, kQFarmCursor = 23
 };
 // derivatives per epoch: t0, tE, u0, alpha, s, q, rho, u_LD
 constexpr int kGradParams = 8;
