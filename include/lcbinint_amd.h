@@ -180,9 +180,10 @@ lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const int64_t* o
  * e of an n-epoch run, cost[e] = device wall-clock ticks of its seed search,
  * cost[n + e] = ticks of its area scans (added to the value found there, so
  * the caller zeroes it), cost[2n + e] = quintic probe solves of the seed
- * search, cost[3n + e] = Cartesian rows or polar runs closed (added likewise).
- * cost is a device pointer with 4 n_epochs entries; other epochs are left
- * untouched.  NULL turns it off.
+ * search, cost[3n + e] = Cartesian rows or polar runs closed and cost[4n + e]
+ * = wave steps of the area scans (both added likewise).  cost is a device
+ * pointer with 5 n_epochs entries; other epochs are left untouched.  NULL
+ * turns it off.
  * lcbg_wall_clock_rate_khz() gives the tick rate of the current device. */
 lcbg_status lcbg_batch_set_epoch_cost(lcbg_batch* batch, int64_t* cost);
 int64_t lcbg_wall_clock_rate_khz(void);

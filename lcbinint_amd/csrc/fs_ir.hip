@@ -645,6 +645,45 @@ __device__ __forceinline__ double seq_lanes(double x0, double step)
     return cand;
 }
 
+// Two chains in one wave: lane k of [0, a) receives xa after k sequential
+// additions of sa, lane k of [a, 64) receives xb after k - a sequential
+// additions of sb; each chain is verified as in seq_lanes.  0 < a < 64.
+__device__ __forceinline__ double seq_lanes2(double xa, double sa, double xb, double sb, int a)
+{
+    const int lane = lane_id();
+    const bool hi = lane >= a;
+    const double step = hi ? sb : sa;
+    const uint64_t lo_mask = (1ull << a) - 1ull;
+    int b0 = 0, b1 = a;
+    double v0 = xa, v1 = xb;
+    double cand = hi ? xb : xa;
+    for (int it = 0; it < kWave; ++it) {
+        const int b = hi ? b1 : b0;
+        const double base = hi ? v1 : v0;
+        const double d = (base + step) - base;
+        if (lane == b)
+            cand = base;
+        else if (lane > b)
+            cand = base + static_cast<double>(lane - b) * d;
+        const double prev = __shfl_up(cand, 1, kWave);
+        const uint64_t bad = ballot(lane > b && !(cand == prev + step));
+        if (!bad) break;
+        const uint64_t bad0 = bad & lo_mask;
+        const uint64_t bad1 = bad & ~lo_mask;
+        if (bad0) {
+            const int f = first_set(bad0);
+            v0 = read_lane(cand, f - 1) + sa;
+            b0 = f;
+        }
+        if (bad1) {
+            const int f = first_set(bad1);
+            v1 = read_lane(cand, f - 1) + sb;
+            b1 = f;
+        }
+    }
+    return cand;
+}
+
 // The reference keeps, per seed, an unordered_map from row key to the rows
 // pushed with that key (finite_source_magnifier.cpp:917, 1989-1999).  Here the
 // map is a chained hash: head[key & (kRowHash-1)] -> newest row, next[] links.
@@ -736,102 +775,210 @@ __device__ __forceinline__ void cart_count(Ctx& C, const Cart& K, long long cell
         C.n_cart_guarded = uni(C.n_cart_guarded + cells);
 }
 
+// 128 consecutive lattice cells of a row, two per lane: cell c is held by
+// lane c & 63 in half c >> 6.
+struct Cells {
+    double x0, x1;  // coordinates
+    double w0, w1;  // brightness weights (0 outside)
+    bool in0, in1;
+};
+
+__device__ __forceinline__ int first_set128(uint64_t lo, uint64_t hi)
+{
+    return lo ? first_set(lo) : (hi ? 64 + first_set(hi) : -1);
+}
+
+// Highest set cell below `limit` (0 <= limit <= 128), or -1.
+__device__ __forceinline__ int last_set_below128(uint64_t lo, uint64_t hi, int limit)
+{
+    if (limit > 64) {
+        const uint64_t h = limit >= 128 ? hi : (hi & ((1ull << (limit - 64)) - 1ull));
+        if (h) return 127 - __builtin_clzll(h);
+        return lo ? 63 - __builtin_clzll(lo) : -1;
+    }
+    const uint64_t l = limit >= 64 ? lo : (lo & ((1ull << limit) - 1ull));
+    return l ? 63 - __builtin_clzll(l) : -1;
+}
+
+__device__ __forceinline__ double cell_x(const Cells& K2, int c)
+{
+    return c < kWave ? read_lane(K2.x0, c) : read_lane(K2.x1, c - kWave);
+}
+
+__device__ __forceinline__ bool cell_in(uint64_t lo, uint64_t hi, int c)
+{
+    return c < kWave ? ((lo >> c) & 1ull) != 0 : ((hi >> (c - kWave)) & 1ull) != 0;
+}
+
 // cartesian_image_area_impl (finite_source_magnifier.cpp:1892-2014).  The
 // scan keeps the current row's xmin/xmax and the previous row's xmin in
 // registers (written through to the row records), and each lane keeps its
 // share of the row's brightness sum, reduced once when the row closes; a
 // row's sum is nonzero exactly when one of its counted weights is, which is
 // what the scan's decisions test.
-__device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi)
+//
+// A step evaluates 128 cells, two per lane, whose lens maps are independent
+// and overlap in the FP64 pipeline.  A row is a right scan from x0 followed by
+// a left scan from x0 - incr; the first step of a row evaluates both at once:
+// `split` cells continue the right scan and the others are the first cells of
+// the left scan, used once the right scan has ended (a cell's value does not
+// depend on the scan state; only the decisions taken on it do, and they are
+// taken in the reference's order).  `split` follows the right-scan length of
+// the previous row.
+__device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi)
 {
+    constexpr int kCells = 2 * kWave;
     const double incr = K.incr;
+    const int lane = C.lane;
     double acc = 0.0;     // this lane's share of the row's brightness sum (countx)
     bool any_w = false;   // countx != 0
     double countall = 0.0;
     bool last_in = 99999999.9 <= C.r2;
-    int dir = 1;
-    double x = seed.x;
+    double x = seed.x;    // first cell of the right scan
     double y = seed.y;
     double x0 = seed.x;
     long long guard = 0;
     const long long budget = K.max_steps - 1;
+    int split = kWave / 2;
     if (!rows_ensure(C, R, yi)) return __builtin_nan("");
     double cur_xmin = uni(rget(R, yi, XMIN));
     double cur_xmax = uni(rget(R, yi, XMAX));
     double prev_xmin = yi != 0 ? uni(rget(R, yi - 1, XMIN)) : 0.0;
+    Cells S;              // the left-scan cells of the combined step
     for (;;) {
-        if (!step_ok(C)) return __builtin_nan("");
-        const long long remaining = budget - guard;
-        if (remaining <= 0) return countall;
-        const int n_eval = remaining < kWave ? static_cast<int>(remaining) : kWave;
-        const double step = dir > 0 ? incr : -incr;
-        const double xk = seq_lanes(x, step);
-        double w = 0.0;
-        bool in = false;
-        if (C.lane < n_eval) in = cart_cell(C, K, xk, y, &w);
-        const bool valid = C.lane < n_eval;
-        const uint64_t in_m = ballot(in && valid);
-        if (dir > 0) {
-            const int s = first_set(ballot(!in && valid));
+        // ---------------------------------------------------------- right scan
+        int stash_n = 0;  // left-scan cells held in cells [kCells - stash_n, kCells)
+        int right_cells = 0;
+        for (bool first = true;; first = false) {
+            if (!step_ok(C)) return __builtin_nan("");
+            const long long remaining = budget - guard;
+            if (remaining <= 0) return countall;
+            const bool combined = first && remaining >= kCells;
+            const int n_eval = combined ? split : (remaining < kCells ? static_cast<int>(remaining) : kCells);
+            Cells E;
+            if (combined) {
+                E.x0 = seq_lanes2(x, incr, x0 + (-incr), -incr, split);
+                E.x1 = seq_lanes(read_lane(E.x0, kWave - 1) + (-incr), -incr);
+            } else {
+                E.x0 = seq_lanes(x, incr);
+                E.x1 = seq_lanes(read_lane(E.x0, kWave - 1) + incr, incr);
+            }
+            const int n_cells = combined ? kCells : n_eval;
+            E.w0 = 0.0;
+            E.w1 = 0.0;
+            E.in0 = lane < n_cells && cart_cell(C, K, E.x0, y, &E.w0);
+            E.in1 = kWave + lane < n_cells && cart_cell(C, K, E.x1, y, &E.w1);
+            if (combined) {
+                stash_n = kCells - split;
+                S = E;
+            }
+            const bool v0 = lane < n_eval;
+            const bool v1 = kWave + lane < n_eval;
+            const int s = first_set128(ballot(!E.in0 && v0), ballot(!E.in1 && v1));
             if (s < 0) {
-                acc += w;
-                any_w = any_w || ballot(w != 0.0) != 0;
+                acc += v0 ? E.w0 : 0.0;
+                acc += v1 ? E.w1 : 0.0;
+                any_w = any_w || ballot((v0 && E.w0 != 0.0) || (v1 && E.w1 != 0.0)) != 0;
                 guard += n_eval;
+                right_cells += n_eval;
                 cart_count(C, K, n_eval);
                 last_in = true;
-                x = read_lane(xk, n_eval - 1) + step;
+                x = cell_x(E, n_eval - 1) + incr;
                 continue;
             }
-            const double wl = C.lane < s ? w : 0.0;
-            acc += wl;
-            any_w = any_w || ballot(wl != 0.0) != 0;
+            const double wl0 = lane < s ? E.w0 : 0.0;
+            const double wl1 = kWave + lane < s ? E.w1 : 0.0;
+            acc += wl0;
+            acc += wl1;
+            any_w = any_w || ballot(wl0 != 0.0 || wl1 != 0.0) != 0;
             guard += s + 1;
+            right_cells += s + 1;
             cart_count(C, K, s + 1);
-            const double xs = read_lane(xk, s);
+            const double xs = cell_x(E, s);
             const bool prev_in = s > 0 ? true : last_in;
             if (prev_in) {
                 cur_xmax = xs;
                 rset(C, R, yi, XMAX, xs);
             }
-            dir = -1;
-            x = x0;
-            cur_xmin = x + (-incr);
-            rset(C, R, yi, XMIN, cur_xmin);
-            last_in = false;
-            x = x + (-incr);
-            continue;
+            break;
         }
-        // left scan
-        const bool cb = any_w;
+        x = x0;
+        cur_xmin = x + (-incr);
+        rset(C, R, yi, XMIN, cur_xmin);
+        last_in = false;
+        x = x + (-incr);
+        // ----------------------------------------------------------- left scan
         const double thr = yi != 0 ? prev_xmin - (-incr) : 0.0;
-        const bool cond = (yi != 0) && (xk >= thr);
-        const int f = first_set(in_m);
-        const bool stop = valid && !in && (cb || (f >= 0 && C.lane > f) || yi == 0 || !cond);
-        const int s = first_set(ballot(stop));
-        const int limit = s >= 0 ? s : n_eval;
-        const double wl = (C.lane < limit && in) ? w : 0.0;
-        acc += wl;
-        if (!cb && f >= 0 && f < limit) {
-            cur_xmax = read_lane(xk, f) - (-incr);
-            rset(C, R, yi, XMAX, cur_xmax);
+        bool use_stash = stash_n > 0 && budget - guard >= stash_n;
+        for (;;) {
+            if (!step_ok(C)) return __builtin_nan("");
+            const long long remaining = budget - guard;
+            if (remaining <= 0) return countall;
+            int lo, n_eval;
+            Cells E;
+            if (use_stash) {
+                lo = kCells - stash_n;
+                n_eval = stash_n;
+                E = S;
+                use_stash = false;
+            } else {
+                lo = 0;
+                n_eval = remaining < kCells ? static_cast<int>(remaining) : kCells;
+                E.x0 = seq_lanes(x, -incr);
+                E.x1 = seq_lanes(read_lane(E.x0, kWave - 1) + (-incr), -incr);
+                E.w0 = 0.0;
+                E.w1 = 0.0;
+                E.in0 = lane < n_eval && cart_cell(C, K, E.x0, y, &E.w0);
+                E.in1 = kWave + lane < n_eval && cart_cell(C, K, E.x1, y, &E.w1);
+            }
+            const int end = lo + n_eval;
+            const int c0 = lane, c1 = kWave + lane;
+            const bool v0 = c0 >= lo && c0 < end;
+            const bool v1 = c1 >= lo && c1 < end;
+            const bool i0 = E.in0 && v0;
+            const bool i1 = E.in1 && v1;
+            const bool cb = any_w;
+            const uint64_t in_lo = ballot(i0), in_hi = ballot(i1);
+            // the row's sum becomes nonzero after cell g (-1: not in this step)
+            const int g = first_set128(ballot(i0 && E.w0 != 0.0), ballot(i1 && E.w1 != 0.0));
+            const bool st0 = v0 && !E.in0 &&
+                             (cb || (g >= 0 && c0 > g) || yi == 0 || !((yi != 0) && (E.x0 >= thr)));
+            const bool st1 = v1 && !E.in1 &&
+                             (cb || (g >= 0 && c1 > g) || yi == 0 || !((yi != 0) && (E.x1 >= thr)));
+            const int s = first_set128(ballot(st0), ballot(st1));
+            const int limit = s >= 0 ? s : end;
+            const double wl0 = (c0 < limit && i0) ? E.w0 : 0.0;
+            const double wl1 = (c1 < limit && i1) ? E.w1 : 0.0;
+            acc += wl0;
+            acc += wl1;
+            if (!cb) {
+                // every in-cell met while the sum is zero sets xmax = x + incr:
+                // the last is the first nonzero-weight cell, else the last in-cell
+                const int j = (g >= 0 && g < limit) ? g : last_set_below128(in_lo, in_hi, limit);
+                if (j >= 0) {
+                    cur_xmax = cell_x(E, j) - (-incr);
+                    rset(C, R, yi, XMAX, cur_xmax);
+                }
+            }
+            any_w = any_w || ballot(wl0 != 0.0 || wl1 != 0.0) != 0;
+            if (s < 0) {
+                guard += n_eval;
+                cart_count(C, K, n_eval);
+                last_in = cell_in(in_lo, in_hi, end - 1);
+                x = cell_x(E, end - 1) + (-incr);
+                continue;
+            }
+            guard += s - lo + 1;
+            cart_count(C, K, s - lo + 1);
+            const double xs = cell_x(E, s);
+            const bool prev_in = s > lo ? cell_in(in_lo, in_hi, s - 1) : last_in;
+            if (prev_in) {
+                cur_xmin = xs;
+                rset(C, R, yi, XMIN, xs);
+            }
+            break;
         }
-        any_w = any_w || ballot(wl != 0.0) != 0;
-        if (s < 0) {
-            guard += n_eval;
-            cart_count(C, K, n_eval);
-            last_in = ((in_m >> (n_eval - 1)) & 1ull) != 0;
-            x = read_lane(xk, n_eval - 1) + (-incr);
-            continue;
-        }
-        guard += s + 1;
-        cart_count(C, K, s + 1);
-        const double xs = read_lane(xk, s);
-        const bool prev_in = s > 0 ? (((in_m >> (s - 1)) & 1ull) != 0) : last_in;
-        if (prev_in) {
-            cur_xmin = xs;
-            rset(C, R, yi, XMIN, xs);
-        }
-        // close the row
+        // ------------------------------------------------------- close the row
         C.n_rows = uni(C.n_rows + 1);
         const double countx = uni(wave_sum(acc));
         countall += countx;
@@ -868,18 +1015,24 @@ __device__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, 
         R.pushed = uni(R.pushed + 1);
         wave_sync();
         ++yi;
+        const bool fresh = yi >= R.size;  // a new record is all zero
         if (!rows_ensure(C, R, yi)) return __builtin_nan("");
-        dir = 1;
         prev_xmin = cur_xmin;
         x0 = cur_xmax;
-        cur_xmin = uni(rget(R, yi, XMIN));
-        cur_xmax = uni(rget(R, yi, XMAX));
+        if (fresh) {
+            cur_xmin = 0.0;
+            cur_xmax = 0.0;
+        } else {
+            cur_xmin = uni(rget(R, yi, XMIN));
+            cur_xmax = uni(rget(R, yi, XMAX));
+        }
         x = x0 - incr;
         y = y + dy;
         acc = 0.0;
         any_w = false;
         last_in = false;
         x = x + incr;
+        split = iclamp(right_cells + 4, 8, 48);
     }
 }
 
@@ -916,89 +1069,124 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* 
         R.dirty = rows_key + 3 * C.row_cap + kRowHash;
         rows_reset(C, R);
         if (!rows_ensure(C, R, 1)) return __builtin_nan("");
-        double area0 = 0.0, areai = 0.0;
-        double dy = incr;
         int yi = 0;
         const Pt seed = C.seeds[i];
         const double J_seed = jacobian_real(C.L, seed.x, seed.y);
         K.jac_sign = fabs(J_seed) < 0.02 ? sign_of(J_seed) : 0;
-        rset(C, R, 0, XMIN, seed.x);
-        rset(C, R, 0, XMAX, seed.x);
-        areai = cart_scan(C, K, R, seed, dy, yi);
-        if (C.overflow) return __builtin_nan("");
-        dy = -incr;
-        if (!rows_ensure(C, R, yi)) return __builtin_nan("");
-        const Pt lower{rget(R, 0, XMAX), seed.y + dy};
-        rset(C, R, yi, XMIN, rget(R, 0, XMIN));
-        rset(C, R, yi, XMAX, rget(R, 0, XMAX));
-        rset(C, R, yi, YY, rget(R, 0, YY));
-        rset(C, R, yi, DYS, dy);
-        ++yi;
-        areai += cart_scan(C, K, R, lower, dy, yi);
-        if (C.overflow) return __builtin_nan("");
-        int nyi = yi;
-        double areabound = 0.0;
+        // The image's scans in the reference's order, through one call site:
+        // stage 0 up from the seed, stage 1 down from the row below it, stage
+        // 2 the gap-repair scans of rows 0..nyi-1, four cases per row.
+        int stage = 0;
+        int row = -1;      // gap-repair row being examined
+        int cs = 4;        // next case of `row` (4: the row is done)
+        double dxmax = 0.0, dxmin = 0.0;
+        int nyi = 0;
+        double areai = 0.0, areabound = 0.0;
         const double g = 1.1 * incr;
-        for (int row = 0; row < nyi; ++row) {
-            if (!rows_ensure(C, R, row + 1)) return __builtin_nan("");
-            const double dxmax = rget(R, row + 1, XMAX) - rget(R, row, XMAX);
-            const double dxmin = rget(R, row + 1, XMIN) - rget(R, row, XMIN);
-            if (rget(R, row + 1, AX) > 0.0) {
-                if (dxmax > g) {
-                    const Pt extra{rget(R, row + 1, XMAX), rget(R, row, YY)};
-                    if (!rows_ensure(C, R, yi)) return __builtin_nan("");
-                    rset(C, R, yi, XMIN, rget(R, row, XMAX));
-                    rset(C, R, yi, XMAX, rget(R, row + 1, XMAX));
-                    dy = -rget(R, row, DYS);
-                    rset(C, R, yi, DYS, dy);
-                    ++yi;
-                    area0 = cart_scan(C, K, R, extra, dy, yi);
-                    areai += area0;
-                    areabound += area0;
-                    if (area0 <= 0.0) --yi;
+        for (;;) {
+            Pt start{0.0, 0.0};
+            double sdy = 0.0;
+            if (stage == 0) {
+                rset(C, R, 0, XMIN, seed.x);
+                rset(C, R, 0, XMAX, seed.x);
+                start = seed;
+                sdy = incr;
+            } else if (stage == 1) {
+                sdy = -incr;
+                if (!rows_ensure(C, R, yi)) return __builtin_nan("");
+                start = Pt{rget(R, 0, XMAX), seed.y + sdy};
+                rset(C, R, yi, XMIN, rget(R, 0, XMIN));
+                rset(C, R, yi, XMAX, rget(R, 0, XMAX));
+                rset(C, R, yi, YY, rget(R, 0, YY));
+                rset(C, R, yi, DYS, sdy);
+                ++yi;
+            } else {
+                bool have = false;
+                while (!have) {
+                    if (cs >= 4) {
+                        if (row >= 0 && row == nyi - 1 && areabound > 0.0 && yi > nyi) nyi = yi;
+                        ++row;
+                        // rows before the last whose four cases all fail change
+                        // nothing: skip them 64 at a time (their records and the
+                        // next row's are final while the loop runs)
+                        while (row < nyi - 1) {
+                            const int rr = row + C.lane;
+                            bool act = rr == nyi - 1;
+                            if (rr < nyi - 1 && rget(R, rr + 1, AX) > 0.0) {
+                                const double ex = rget(R, rr + 1, XMAX) - rget(R, rr, XMAX);
+                                const double en = rget(R, rr + 1, XMIN) - rget(R, rr, XMIN);
+                                act = ex > g || en > g || en < -g || ex < -g;
+                            }
+                            const uint64_t mm = ballot(act);
+                            if (mm) {
+                                row += first_set(mm);
+                                break;
+                            }
+                            row += kWave;
+                        }
+                        if (row >= nyi) break;
+                        if (!rows_ensure(C, R, row + 1)) return __builtin_nan("");
+                        dxmax = rget(R, row + 1, XMAX) - rget(R, row, XMAX);
+                        dxmin = rget(R, row + 1, XMIN) - rget(R, row, XMIN);
+                        cs = rget(R, row + 1, AX) > 0.0 ? 0 : 4;
+                        continue;
+                    }
+                    const int k = cs++;
+                    double nxmin = 0.0, nxmax = 0.0;
+                    if (k == 0 && dxmax > g) {
+                        start = Pt{rget(R, row + 1, XMAX), rget(R, row, YY)};
+                        nxmin = rget(R, row, XMAX);
+                        nxmax = rget(R, row + 1, XMAX);
+                        sdy = -rget(R, row, DYS);
+                        have = true;
+                    } else if (k == 1 && dxmin > g) {
+                        start = Pt{rget(R, row + 1, XMIN) - incr, rget(R, row + 1, YY)};
+                        nxmin = rget(R, row, XMIN);
+                        nxmax = rget(R, row + 1, XMIN);
+                        sdy = rget(R, row, DYS);
+                        have = true;
+                    } else if (k == 2 && dxmin < -g) {
+                        start = Pt{rget(R, row, XMIN) - incr, rget(R, row, YY)};
+                        nxmin = rget(R, row + 1, XMIN);
+                        nxmax = rget(R, row, XMIN);
+                        sdy = -rget(R, row, DYS);
+                        have = true;
+                    } else if (k == 3 && dxmax < -g) {
+                        start = Pt{rget(R, row + 1, XMAX) + incr, rget(R, row + 1, YY)};
+                        nxmin = rget(R, row + 1, XMAX);
+                        nxmax = rget(R, row, XMAX);
+                        sdy = rget(R, row, DYS);
+                        have = true;
+                    }
+                    if (have) {
+                        start.x = uni(start.x);
+                        start.y = uni(start.y);
+                        sdy = uni(sdy);
+                        nxmin = uni(nxmin);
+                        nxmax = uni(nxmax);
+                        if (!rows_ensure(C, R, yi)) return __builtin_nan("");
+                        rset(C, R, yi, XMIN, nxmin);
+                        rset(C, R, yi, XMAX, nxmax);
+                        rset(C, R, yi, DYS, sdy);
+                        ++yi;
+                    }
                 }
-                if (dxmin > g) {
-                    const Pt extra{rget(R, row + 1, XMIN) - incr, rget(R, row + 1, YY)};
-                    if (!rows_ensure(C, R, yi)) return __builtin_nan("");
-                    rset(C, R, yi, XMIN, rget(R, row, XMIN));
-                    rset(C, R, yi, XMAX, rget(R, row + 1, XMIN));
-                    dy = rget(R, row, DYS);
-                    rset(C, R, yi, DYS, dy);
-                    ++yi;
-                    area0 = cart_scan(C, K, R, extra, dy, yi);
-                    areai += area0;
-                    areabound += area0;
-                    if (area0 <= 0.0) --yi;
-                }
-                if (dxmin < -g) {
-                    const Pt extra{rget(R, row, XMIN) - incr, rget(R, row, YY)};
-                    if (!rows_ensure(C, R, yi)) return __builtin_nan("");
-                    rset(C, R, yi, XMIN, rget(R, row + 1, XMIN));
-                    rset(C, R, yi, XMAX, rget(R, row, XMIN));
-                    dy = -rget(R, row, DYS);
-                    rset(C, R, yi, DYS, dy);
-                    ++yi;
-                    area0 = cart_scan(C, K, R, extra, dy, yi);
-                    areai += area0;
-                    areabound += area0;
-                    if (area0 <= 0.0) --yi;
-                }
-                if (dxmax < -g) {
-                    const Pt extra{rget(R, row + 1, XMAX) + incr, rget(R, row + 1, YY)};
-                    if (!rows_ensure(C, R, yi)) return __builtin_nan("");
-                    rset(C, R, yi, XMIN, rget(R, row + 1, XMAX));
-                    rset(C, R, yi, XMAX, rget(R, row, XMAX));
-                    dy = rget(R, row, DYS);
-                    rset(C, R, yi, DYS, dy);
-                    ++yi;
-                    area0 = cart_scan(C, K, R, extra, dy, yi);
-                    areai += area0;
-                    areabound += area0;
-                    if (area0 <= 0.0) --yi;
-                }
-                if (C.overflow) return __builtin_nan("");
+                if (!have) break;
             }
-            if (row == nyi - 1 && areabound > 0.0 && yi > nyi) nyi = yi;
+            const double r = cart_scan(C, K, R, start, sdy, yi);
+            if (C.overflow) return __builtin_nan("");
+            if (stage == 0) {
+                areai = r;
+                stage = 1;
+            } else if (stage == 1) {
+                areai += r;
+                nyi = yi;
+                stage = 2;
+            } else {
+                areai += r;
+                areabound += r;
+                if (r <= 0.0) --yi;
+            }
         }
         area += areai;
         wave_sync();
@@ -1649,6 +1837,7 @@ __global__ void __launch_bounds__(kWave) k_ir_cart(const CurveRec* __restrict__ 
         if (ep.cost) {
             wave_store(&ep.cost[ep.n + e], ep.cost[ep.n + e] + static_cast<int64_t>(ticks));
             wave_store(&ep.cost[3 * ep.n + e], ep.cost[3 * ep.n + e] + static_cast<int64_t>(C.n_rows));
+            wave_store(&ep.cost[4 * ep.n + e], ep.cost[4 * ep.n + e] + static_cast<int64_t>(C.steps));
         }
         wave_sync();
     }
@@ -1691,6 +1880,7 @@ __global__ void __launch_bounds__(kWave) k_ir_polar(const CurveRec* __restrict__
         if (ep.cost) {
             wave_store(&ep.cost[ep.n + e], ep.cost[ep.n + e] + static_cast<int64_t>(ticks));
             wave_store(&ep.cost[3 * ep.n + e], ep.cost[3 * ep.n + e] + static_cast<int64_t>(C.n_rows));
+            wave_store(&ep.cost[4 * ep.n + e], ep.cost[4 * ep.n + e] + static_cast<int64_t>(C.steps));
         }
         wave_sync();
     }
