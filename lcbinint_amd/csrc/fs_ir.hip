@@ -775,41 +775,6 @@ __device__ __forceinline__ void cart_count(Ctx& C, const Cart& K, long long cell
         C.n_cart_guarded = uni(C.n_cart_guarded + cells);
 }
 
-// 128 consecutive lattice cells of a row, two per lane: cell c is held by
-// lane c & 63 in half c >> 6.
-struct Cells {
-    double x0, x1;  // coordinates
-    double w0, w1;  // brightness weights (0 outside)
-    bool in0, in1;
-};
-
-__device__ __forceinline__ int first_set128(uint64_t lo, uint64_t hi)
-{
-    return lo ? first_set(lo) : (hi ? 64 + first_set(hi) : -1);
-}
-
-// Highest set cell below `limit` (0 <= limit <= 128), or -1.
-__device__ __forceinline__ int last_set_below128(uint64_t lo, uint64_t hi, int limit)
-{
-    if (limit > 64) {
-        const uint64_t h = limit >= 128 ? hi : (hi & ((1ull << (limit - 64)) - 1ull));
-        if (h) return 127 - __builtin_clzll(h);
-        return lo ? 63 - __builtin_clzll(lo) : -1;
-    }
-    const uint64_t l = limit >= 64 ? lo : (lo & ((1ull << limit) - 1ull));
-    return l ? 63 - __builtin_clzll(l) : -1;
-}
-
-__device__ __forceinline__ double cell_x(const Cells& K2, int c)
-{
-    return c < kWave ? read_lane(K2.x0, c) : read_lane(K2.x1, c - kWave);
-}
-
-__device__ __forceinline__ bool cell_in(uint64_t lo, uint64_t hi, int c)
-{
-    return c < kWave ? ((lo >> c) & 1ull) != 0 : ((hi >> (c - kWave)) & 1ull) != 0;
-}
-
 // cartesian_image_area_impl (finite_source_magnifier.cpp:1892-2014).  The
 // scan keeps the current row's xmin/xmax and the previous row's xmin in
 // registers (written through to the row records), and each lane keeps its
@@ -817,19 +782,16 @@ __device__ __forceinline__ bool cell_in(uint64_t lo, uint64_t hi, int c)
 // row's sum is nonzero exactly when one of its counted weights is, which is
 // what the scan's decisions test.
 //
-// A step evaluates 128 cells, two per lane, whose lens maps are independent
-// and overlap in the FP64 pipeline.  A row is a right scan from x0 followed by
-// a left scan from x0 - incr; the first step of a row evaluates both at once:
-// `split` cells continue the right scan and the others are the first cells of
-// the left scan, used once the right scan has ended (a cell's value does not
-// depend on the scan state; only the decisions taken on it do, and they are
-// taken in the reference's order).  `split` follows the right-scan length of
-// the previous row.
+// A row is a right scan from x0 followed by a left scan from x0 - incr.  The
+// first step of a row evaluates both at once: `split` lanes continue the right
+// scan and the other lanes hold the first cells of the left scan, which are
+// used once the right scan has ended (a cell's value does not depend on the
+// scan state; only the decisions taken on it do, and they are taken in the
+// reference's order).  `split` follows the right-scan length of the previous
+// row.
 __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi)
 {
-    constexpr int kCells = 2 * kWave;
     const double incr = K.incr;
-    const int lane = C.lane;
     double acc = 0.0;     // this lane's share of the row's brightness sum (countx)
     bool any_w = false;   // countx != 0
     double countall = 0.0;
@@ -844,57 +806,48 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
     double cur_xmin = uni(rget(R, yi, XMIN));
     double cur_xmax = uni(rget(R, yi, XMAX));
     double prev_xmin = yi != 0 ? uni(rget(R, yi - 1, XMIN)) : 0.0;
-    Cells S;              // the left-scan cells of the combined step
     for (;;) {
         // ---------------------------------------------------------- right scan
-        int stash_n = 0;  // left-scan cells held in cells [kCells - stash_n, kCells)
+        int stash_n = 0;      // left-scan cells held in lanes [64 - stash_n, 64)
+        double sx = 0.0, sw = 0.0;
+        bool s_in = false;
         int right_cells = 0;
         for (bool first = true;; first = false) {
             if (!step_ok(C)) return __builtin_nan("");
             const long long remaining = budget - guard;
             if (remaining <= 0) return countall;
-            const bool combined = first && remaining >= kCells;
-            const int n_eval = combined ? split : (remaining < kCells ? static_cast<int>(remaining) : kCells);
-            Cells E;
+            const bool combined = first && remaining >= kWave;
+            const int n_eval = combined ? split : (remaining < kWave ? static_cast<int>(remaining) : kWave);
+            const double xk = combined ? seq_lanes2(x, incr, x0 + (-incr), -incr, split) : seq_lanes(x, incr);
+            double w = 0.0;
+            bool in = false;
+            if (combined || C.lane < n_eval) in = cart_cell(C, K, xk, y, &w);
             if (combined) {
-                E.x0 = seq_lanes2(x, incr, x0 + (-incr), -incr, split);
-                E.x1 = seq_lanes(read_lane(E.x0, kWave - 1) + (-incr), -incr);
-            } else {
-                E.x0 = seq_lanes(x, incr);
-                E.x1 = seq_lanes(read_lane(E.x0, kWave - 1) + incr, incr);
+                stash_n = kWave - split;
+                sx = xk;
+                sw = w;
+                s_in = in;
             }
-            const int n_cells = combined ? kCells : n_eval;
-            E.w0 = 0.0;
-            E.w1 = 0.0;
-            E.in0 = lane < n_cells && cart_cell(C, K, E.x0, y, &E.w0);
-            E.in1 = kWave + lane < n_cells && cart_cell(C, K, E.x1, y, &E.w1);
-            if (combined) {
-                stash_n = kCells - split;
-                S = E;
-            }
-            const bool v0 = lane < n_eval;
-            const bool v1 = kWave + lane < n_eval;
-            const int s = first_set128(ballot(!E.in0 && v0), ballot(!E.in1 && v1));
+            const bool valid = C.lane < n_eval;
+            const int s = first_set(ballot(!in && valid));
             if (s < 0) {
-                acc += v0 ? E.w0 : 0.0;
-                acc += v1 ? E.w1 : 0.0;
-                any_w = any_w || ballot((v0 && E.w0 != 0.0) || (v1 && E.w1 != 0.0)) != 0;
+                const double wv = valid ? w : 0.0;
+                acc += wv;
+                any_w = any_w || ballot(wv != 0.0) != 0;
                 guard += n_eval;
                 right_cells += n_eval;
                 cart_count(C, K, n_eval);
                 last_in = true;
-                x = cell_x(E, n_eval - 1) + incr;
+                x = read_lane(xk, n_eval - 1) + incr;
                 continue;
             }
-            const double wl0 = lane < s ? E.w0 : 0.0;
-            const double wl1 = kWave + lane < s ? E.w1 : 0.0;
-            acc += wl0;
-            acc += wl1;
-            any_w = any_w || ballot(wl0 != 0.0 || wl1 != 0.0) != 0;
+            const double wl = C.lane < s ? w : 0.0;
+            acc += wl;
+            any_w = any_w || ballot(wl != 0.0) != 0;
             guard += s + 1;
             right_cells += s + 1;
             cart_count(C, K, s + 1);
-            const double xs = cell_x(E, s);
+            const double xs = read_lane(xk, s);
             const bool prev_in = s > 0 ? true : last_in;
             if (prev_in) {
                 cur_xmax = xs;
@@ -915,63 +868,63 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
             const long long remaining = budget - guard;
             if (remaining <= 0) return countall;
             int lo, n_eval;
-            Cells E;
+            double xk, w;
+            bool in;
             if (use_stash) {
-                lo = kCells - stash_n;
+                lo = kWave - stash_n;
                 n_eval = stash_n;
-                E = S;
+                xk = sx;
+                w = sw;
+                in = s_in;
                 use_stash = false;
             } else {
                 lo = 0;
-                n_eval = remaining < kCells ? static_cast<int>(remaining) : kCells;
-                E.x0 = seq_lanes(x, -incr);
-                E.x1 = seq_lanes(read_lane(E.x0, kWave - 1) + (-incr), -incr);
-                E.w0 = 0.0;
-                E.w1 = 0.0;
-                E.in0 = lane < n_eval && cart_cell(C, K, E.x0, y, &E.w0);
-                E.in1 = kWave + lane < n_eval && cart_cell(C, K, E.x1, y, &E.w1);
+                n_eval = remaining < kWave ? static_cast<int>(remaining) : kWave;
+                xk = seq_lanes(x, -incr);
+                w = 0.0;
+                in = false;
+                if (C.lane < n_eval) in = cart_cell(C, K, xk, y, &w);
             }
             const int end = lo + n_eval;
-            const int c0 = lane, c1 = kWave + lane;
-            const bool v0 = c0 >= lo && c0 < end;
-            const bool v1 = c1 >= lo && c1 < end;
-            const bool i0 = E.in0 && v0;
-            const bool i1 = E.in1 && v1;
+            const bool valid = C.lane >= lo && C.lane < end;
+            const bool inv = in && valid;
             const bool cb = any_w;
-            const uint64_t in_lo = ballot(i0), in_hi = ballot(i1);
-            // the row's sum becomes nonzero after cell g (-1: not in this step)
-            const int g = first_set128(ballot(i0 && E.w0 != 0.0), ballot(i1 && E.w1 != 0.0));
-            const bool st0 = v0 && !E.in0 &&
-                             (cb || (g >= 0 && c0 > g) || yi == 0 || !((yi != 0) && (E.x0 >= thr)));
-            const bool st1 = v1 && !E.in1 &&
-                             (cb || (g >= 0 && c1 > g) || yi == 0 || !((yi != 0) && (E.x1 >= thr)));
-            const int s = first_set128(ballot(st0), ballot(st1));
+            const uint64_t in_m = ballot(inv);
+            // the row's sum becomes nonzero after lane g (-1: not in this step)
+            const int g = first_set(ballot(inv && w != 0.0));
+            const bool cond = (yi != 0) && (xk >= thr);
+            const bool stop = valid && !in && (cb || (g >= 0 && C.lane > g) || yi == 0 || !cond);
+            const int s = first_set(ballot(stop));
             const int limit = s >= 0 ? s : end;
-            const double wl0 = (c0 < limit && i0) ? E.w0 : 0.0;
-            const double wl1 = (c1 < limit && i1) ? E.w1 : 0.0;
-            acc += wl0;
-            acc += wl1;
+            const double wl = (C.lane < limit && inv) ? w : 0.0;
+            acc += wl;
             if (!cb) {
                 // every in-cell met while the sum is zero sets xmax = x + incr:
                 // the last is the first nonzero-weight cell, else the last in-cell
-                const int j = (g >= 0 && g < limit) ? g : last_set_below128(in_lo, in_hi, limit);
+                int j = -1;
+                if (g >= 0 && g < limit) {
+                    j = g;
+                } else {
+                    const uint64_t mm = in_m & (limit >= kWave ? ~0ull : ((1ull << limit) - 1ull));
+                    if (mm) j = 63 - __builtin_clzll(mm);
+                }
                 if (j >= 0) {
-                    cur_xmax = cell_x(E, j) - (-incr);
+                    cur_xmax = read_lane(xk, j) - (-incr);
                     rset(C, R, yi, XMAX, cur_xmax);
                 }
             }
-            any_w = any_w || ballot(wl0 != 0.0 || wl1 != 0.0) != 0;
+            any_w = any_w || ballot(wl != 0.0) != 0;
             if (s < 0) {
                 guard += n_eval;
                 cart_count(C, K, n_eval);
-                last_in = cell_in(in_lo, in_hi, end - 1);
-                x = cell_x(E, end - 1) + (-incr);
+                last_in = ((in_m >> (end - 1)) & 1ull) != 0;
+                x = read_lane(xk, end - 1) + (-incr);
                 continue;
             }
             guard += s - lo + 1;
             cart_count(C, K, s - lo + 1);
-            const double xs = cell_x(E, s);
-            const bool prev_in = s > lo ? cell_in(in_lo, in_hi, s - 1) : last_in;
+            const double xs = read_lane(xk, s);
+            const bool prev_in = s > lo ? (((in_m >> (s - 1)) & 1ull) != 0) : last_in;
             if (prev_in) {
                 cur_xmin = xs;
                 rset(C, R, yi, XMIN, xs);
@@ -1032,7 +985,7 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
         any_w = false;
         last_in = false;
         x = x + incr;
-        split = iclamp(right_cells + 4, 8, 48);
+        split = iclamp(right_cells + 4, 8, 32);
     }
 }
 

@@ -186,6 +186,8 @@ __device__ __forceinline__ double ld_node_value(int i, double c, double d)
     const double r2 = static_cast<double>(i) / 20000.0;
     const double b = dclamp(r2, 0.0, 1.0);
     const double mu = __builtin_sqrt(dmax(0.0, 1.0 - b));
+    // with d = 0 the last term is +0, and subtracting +0 changes no value
+    if (d == 0.0) return 1.0 - c * (1.0 - mu);
     return 1.0 - c * (1.0 - mu) - d * (1.0 - __builtin_sqrt(mu));
 }
 
