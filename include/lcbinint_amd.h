@@ -120,6 +120,64 @@ lcbg_status lcbg_batch_point_source_device(lcbg_batch* batch, const int64_t* off
 lcbg_status lcbg_light_curve_point_source(const lcbg_curve* curve, const double* times,
                                           int64_t n, double* mag);
 
+/* Second-order effects of a curve: annual parallax (piEN, piEE, applied when
+ * either is nonzero, as LightCurve(parallax=True) passes them,
+ * python/lcbinint_pybind.cpp:2734-2735), the event coordinates (ra, dec in
+ * degrees, reference time tfix, 0 = t0) of EventCoordinates
+ * (python/lcbinint_pybind.cpp:95-101), and the orbital motion parameters of
+ * binary_params_from_dict (:145-179): VBM (w1, w2, w3) = (g1, g2, g3) in
+ * 1/day, and for the Keplerian orbit lom_szs, lom_ar (default 1). */
+typedef struct lcbg_dynamics {
+    double piEN;
+    double piEE;
+    double ra;
+    double dec;
+    double tfix;
+    double g1;
+    double g2;
+    double g3;
+    double lom_szs;
+    double lom_ar;
+} lcbg_dynamics;
+
+/* Orbital motion models, the values of lcbi_orbital_motion_mode
+ * (include/lcbinint/lcbinint.h:15-19). */
+enum { LCBG_ORBIT_STATIC = 0, LCBG_ORBIT_CIRCULAR = 1, LCBG_ORBIT_KEPLER = 2 };
+
+/* Earth ephemeris for annual parallax: the Horizons vector-table CSV that
+ * load_earth_ephemeris reads (src/lcbinint/model/trajectory.cpp:147-183).
+ * path NULL reads LCBININT_EARTH_EPHEMERIS, as the reference does
+ * (trajectory.cpp:134-145); without a call the table is read from that
+ * variable when first needed.  Process-wide.  LCBG_INVALID_ARGUMENT if fewer
+ * than two rows were read (parallax epochs then evaluate to NaN). */
+lcbg_status lcbg_load_earth_ephemeris(const char* path);
+
+/* Host computation, no device: for each epoch the source position the
+ * magnifier receives and the projected separation, xys[3i..3i+2] = (x, y, s),
+ * as LensModel::magnification (src/lcbinint/model/lens_model.cpp:84-104,
+ * 123-178) derives them: Trajectory::source_position with annual parallax
+ * (trajectory.cpp:436-484), orbital_state (orbital_motion.cpp:183-199) and
+ * the rotation into the instantaneous binary frame (VBM convention). */
+lcbg_status lcbg_dynamic_sources(const lcbg_curve* curve, const lcbg_dynamics* dyn,
+                                 int32_t orbital_motion_mode, const double* times, int64_t n,
+                                 double* xys);
+
+/* Light curves of the batch's curves with annual parallax and/or orbital
+ * motion (dyn: n_curves records), the LensModel path of
+ * evaluate_binary_light_curve_numpy (python/lcbinint_pybind.cpp:1159-1173):
+ * per epoch the source position and separation of lcbg_dynamic_sources, the
+ * point-source magnification summed over binary_images
+ * (point_source_magnifier.cpp:723-778), those images as seed hints, and
+ * FiniteSourceMagnifier::binary_mag on the caustic of that epoch's
+ * separation.  Host pointers (offsets: n_curves + 1, times/mag/method:
+ * n_epochs); per-curve source_bins as set on the batch.  The trajectories run
+ * on the host, the magnifications on the device in chunks of up to 16384
+ * epochs, each epoch with its own caustic cache. */
+lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                            int32_t orbital_motion_mode, const int64_t* offsets,
+                                            const double* times, int64_t n_epochs, double* mag,
+                                            int32_t* method);
+
 /* Finite-source light curves of every curve of the batch: the plain method of
  * FiniteSourceMagnifier::binary_mag (finite_source_magnifier.cpp:4633-4843) as
  * the static VBM epoch loop of evaluate_binary_light_curve_numpy

@@ -7,6 +7,8 @@ a CPU implementation.
 from __future__ import annotations
 
 import ctypes
+
+import numpy as np
 import re
 from pathlib import Path
 
@@ -228,6 +230,56 @@ def work_counts(batch: Batch):
     check(lcbg_batch_work_counts(batch.handle, out), "lcbg_batch_work_counts")
     return dict(zip(("quintic", "quartic", "cart", "cart_guarded", "polar", "seed_ticks",
                      "area_ticks", "rows"), list(out)))
+
+class Dynamics(ctypes.Structure):
+    """lcbg_dynamics: annual parallax, event coordinates and orbital motion."""
+    _fields_ = [(k, ctypes.c_double) for k in ("piEN", "piEE", "ra", "dec", "tfix", "g1", "g2", "g3",
+                                               "lom_szs", "lom_ar")]
+
+    def __init__(self, **kw):
+        kw.setdefault("lom_ar", 1.0)
+        super().__init__(**kw)
+
+
+ORBIT_STATIC, ORBIT_CIRCULAR, ORBIT_KEPLER = 0, 1, 2
+
+lcbg_load_earth_ephemeris = _sig("lcbg_load_earth_ephemeris", ctypes.c_int, ctypes.c_char_p)
+lcbg_dynamic_sources = _sig("lcbg_dynamic_sources", ctypes.c_int, ctypes.POINTER(Curve),
+                            ctypes.POINTER(Dynamics), ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                            ctypes.c_void_p)
+lcbg_batch_light_curves_dynamic = _sig("lcbg_batch_light_curves_dynamic", ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.POINTER(Dynamics), ctypes.c_int32, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p)
+
+
+def load_earth_ephemeris(path=None):
+    """Reads the Horizons-format Earth table (None: LCBININT_EARTH_EPHEMERIS)."""
+    check(lcbg_load_earth_ephemeris(None if path is None else str(path).encode()),
+          "lcbg_load_earth_ephemeris")
+
+
+def dynamic_sources(curve: Curve, dyn: Dynamics, orbit_mode: int, times):
+    """Host trajectory: (n, 3) array of source x, y and separation s(t)."""
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    out = np.empty((len(t), 3))
+    check(lcbg_dynamic_sources(ctypes.byref(curve), ctypes.byref(dyn), int(orbit_mode),
+                               t.ctypes.data, len(t), out.ctypes.data), "lcbg_dynamic_sources")
+    return out
+
+
+def light_curves_dynamic(batch: "Batch", dyn, orbit_mode: int, offsets, times):
+    """Magnifications and methods of the batch's curves with parallax / orbital
+    motion (host arrays; dyn: one Dynamics per curve)."""
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    d = (Dynamics * len(dyn))(*dyn)
+    mag = np.empty(len(t))
+    meth = np.empty(len(t), dtype=np.int32)
+    check(lcbg_batch_light_curves_dynamic(batch.handle, d, int(orbit_mode), off.ctypes.data, t.ctypes.data,
+                                          len(t), mag.ctypes.data, meth.ctypes.data),
+          "lcbg_batch_light_curves_dynamic")
+    return mag, meth
+
 
 lcbg_batch_ir_kernel_ms = _sig("lcbg_batch_ir_kernel_ms", ctypes.c_int, ctypes.c_void_p,
                                ctypes.POINTER(ctypes.c_float))

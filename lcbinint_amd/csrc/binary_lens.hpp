@@ -159,6 +159,36 @@ __device__ __forceinline__ PointMag ps_from_roots(const Geom& g, const cx* roots
     return PointMag{mag, n};
 }
 
+// Point-source magnification as LensModel::magnification sums it
+// (src/lcbinint/model/lens_model.cpp:199-205) over binary_images
+// (point_source_magnifier.cpp:723-778): roots sorted by residual (a stable
+// sort, as std::sort's insertion sort orders five elements), three images if
+// r[3] * 1e-4 > r[2] + 1e-12, else five; sum of 1 / |J| in that order.
+__device__ __forceinline__ double ps_from_images(const Geom& g, const cx* roots)
+{
+    double res[5];
+    int ord[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        res[i] = __builtin_sqrt(residual_squared(g, roots[i]));
+        ord[i] = i;
+    }
+    for (int i = 1; i < 5; ++i) {
+        const int v = ord[i];
+        const double rv = res[v];
+        int j = i;
+        while (j > 0 && rv < res[ord[j - 1]]) {
+            ord[j] = ord[j - 1];
+            --j;
+        }
+        ord[j] = v;
+    }
+    const int physical = (res[ord[3]] * 1.0e-4 > res[ord[2]] + 1.0e-12) ? 3 : 5;
+    double mag = 0.0;
+    for (int i = 0; i < physical; ++i) mag += 1.0 / fabs(jacobian_det(g, roots[ord[i]]));
+    return mag;
+}
+
 // binary_mag0 (point_source_magnifier.cpp:567 -> 652-693), no root cache.
 __device__ __forceinline__ PointMag binary_mag0(double s, double q, double x, double y)
 {

@@ -44,7 +44,10 @@ __global__ void __launch_bounds__(kStage1Block) k_fs_stage1(const CurveRec* __re
         cx roots[5];
         solve_quintic(g, roots);
         const unsigned mask = select_physical(g, roots);
-        const double a_ps = ps_from_roots(g, roots, mask).mag;
+        // LensModel path with rho > 0 (lens_model.cpp:199-212): the sum over
+        // binary_images in their residual order; rho == 0 keeps binary_mag0
+        const double a_ps = (opt.ps_images && cv.rho != 0.0) ? ps_from_images(g, roots)
+                                                              : ps_from_roots(g, roots, mask).mag;
         ep.a_ps[e] = a_ps;
         double* rr = ep.roots + 10 * e;
 #pragma unroll

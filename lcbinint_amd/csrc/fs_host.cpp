@@ -292,6 +292,8 @@ lcbg_status fs_prepare(lcbg_batch* b)
     fs->opt.kinji = o.point_source_threshold;
     fs->opt.hex_threshold = o.hexadecapole_threshold;
     fs->opt.adaptive_hex = o.adaptive_hex_threshold;
+    fs->opt.ps_images = b->dynamic_images ? 1 : 0;
+    fs->opt.pad = 0;
     const int bins = o.caustic_bins > 32 ? o.caustic_bins : 32;
 
     // angle tables, each expression as the reference writes it
@@ -484,6 +486,111 @@ extern "C" lcbg_status lcbg_batch_finite_source_device(lcbg_batch* batch, const 
     if (!offsets || !times || !mag || !method) return LCBG_INVALID_ARGUMENT;
     return fs_run(batch, offsets, times, n_epochs, mag, method, static_cast<hipStream_t>(stream),
                   nullptr);
+}
+
+namespace lcbg {
+bool dynamic_sources_batch(const lcbg_curve* curves, const lcbg_dynamics* dyn, int32_t n_curves,
+                           int orbit_mode, const int64_t* offsets, const double* times, double* xys);
+}
+
+namespace {
+// Epochs per device pass of the dynamic path: each epoch is a curve of its own
+// (its own separation, its own caustic cache); 16384 caches take ~5.6 GB.
+constexpr int32_t kDynChunk = 16384;
+}  // namespace
+
+extern "C" lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                                       int32_t orbital_motion_mode,
+                                                       const int64_t* offsets, const double* times,
+                                                       int64_t n_epochs, double* mag, int32_t* method)
+{
+    if (!batch || n_epochs < 0) return LCBG_INVALID_ARGUMENT;
+    if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER)
+        return LCBG_UNSUPPORTED;
+    const int32_t nc = batch->n_curves;
+    if (nc == 0 || n_epochs == 0) return LCBG_OK;
+    if (!dyn || !offsets || !times || !mag || !method) return LCBG_INVALID_ARGUMENT;
+    if (offsets[0] != 0 || offsets[nc] != n_epochs) return LCBG_INVALID_ARGUMENT;
+    for (int32_t c = 0; c < nc; ++c)
+        if (offsets[c + 1] < offsets[c]) return LCBG_INVALID_ARGUMENT;
+    std::vector<double> xys(static_cast<std::size_t>(3 * n_epochs));
+    if (!dynamic_sources_batch(batch->h_params.data(), dyn, nc, orbital_motion_mode, offsets, times,
+                               xys.data()))
+        return LCBG_NUMERICAL_ERROR;  // a non-finite orbit (lens_model.cpp:117-120)
+    lcbg_status st = LCBG_OK;
+    // pseudo-curve capacity: the run's epoch count rounded up to a power of two,
+    // at least 64 and at most kDynChunk; a larger later run replaces the child
+    int32_t want = 64;
+    while (want < kDynChunk && want < n_epochs) want *= 2;
+    if (batch->dyn_child && batch->dyn_child->n_curves < want) {
+        lcbg_batch_destroy(batch->dyn_child);
+        batch->dyn_child = nullptr;
+    }
+    if (!batch->dyn_child) {
+        std::vector<lcbg_curve> idle(static_cast<std::size_t>(want),
+                                     lcbg_curve{0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0});
+        st = lcbg_batch_create(idle.data(), want, &batch->options, &batch->dyn_child);
+        if (st != LCBG_OK) return st;
+        batch->dyn_child->dynamic_images = true;
+    }
+    lcbg_batch* ch = batch->dyn_child;
+    const int32_t chunk = ch->n_curves;
+    int64_t* d_off = nullptr;
+    double* d_buf = nullptr;
+    int32_t* d_meth = nullptr;
+    st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_off), (chunk + 1) * sizeof(int64_t)));
+    if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_buf), 2 * chunk * sizeof(double)));
+    if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_meth), chunk * sizeof(int32_t)));
+    std::vector<int64_t> h_off(static_cast<std::size_t>(chunk) + 1);
+    std::vector<double> h_t(static_cast<std::size_t>(chunk));
+    int32_t c = 0;  // curve of the chunk's first epoch
+    for (int64_t base = 0; st == LCBG_OK && base < n_epochs; base += chunk) {
+        const int32_t m = static_cast<int32_t>(std::min<int64_t>(chunk, n_epochs - base));
+        // pseudo-curve k = epoch base + k: source (x, y) = (time, u0) with
+        // t0 = 0, tE = 1, alpha = 0, the epoch's separation, the curve's q, rho, LD, nbin
+        for (int32_t k = 0; k < chunk; ++k) {
+            CurveRec& r = ch->h_curves[static_cast<std::size_t>(k)];
+            h_off[static_cast<std::size_t>(k)] = k < m ? k : m;
+            if (k >= m) {
+                r = make_curve_rec(lcbg_curve{0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0});
+                continue;
+            }
+            const int64_t e = base + k;
+            while (offsets[c + 1] <= e) ++c;
+            const CurveRec& pc = batch->h_curves[static_cast<std::size_t>(c)];
+            r.t0 = 0.0;
+            r.inverse_tE = 1.0;
+            r.u0 = xys[static_cast<std::size_t>(3 * e + 1)];
+            r.cos_alpha = 1.0;
+            r.sin_alpha = 0.0;
+            r.s = xys[static_cast<std::size_t>(3 * e + 2)];
+            r.q_eff = pc.q_eff;
+            r.rho = pc.rho;
+            r.ld_c = pc.ld_c;
+            r.ld_d = pc.ld_d;
+            r.source_bins = pc.source_bins;
+            r.pad = 0;
+            h_t[static_cast<std::size_t>(k)] = xys[static_cast<std::size_t>(3 * e)];
+        }
+        h_off[chunk] = m;
+        ch->caches_stale = true;
+        st = map_hip(hipMemcpy(ch->d_curves, ch->h_curves.data(), chunk * sizeof(CurveRec),
+                               hipMemcpyHostToDevice));
+        if (st == LCBG_OK)
+            st = map_hip(hipMemcpy(d_off, h_off.data(), (chunk + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        if (st == LCBG_OK)
+            st = map_hip(hipMemcpy(d_buf, h_t.data(), m * sizeof(double), hipMemcpyHostToDevice));
+        if (st == LCBG_OK) st = fs_run(ch, d_off, d_buf, m, d_buf + chunk, d_meth, nullptr, nullptr);
+        if (st == LCBG_OK) st = map_hip(hipStreamSynchronize(nullptr));
+        if (st == LCBG_OK)
+            st = map_hip(hipMemcpy(mag + base, d_buf + chunk, m * sizeof(double), hipMemcpyDeviceToHost));
+        if (st == LCBG_OK)
+            st = map_hip(hipMemcpy(method + base, d_meth, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    if (d_off) (void)hipFree(d_off);
+    if (d_buf) (void)hipFree(d_buf);
+    if (d_meth) (void)hipFree(d_meth);
+    return st;
 }
 
 extern "C" lcbg_status lcbg_batch_finite_source_grad_device(lcbg_batch* batch, const int64_t* offsets,

@@ -52,6 +52,7 @@ struct Ctx {
     int defer_slot;       // its heavy-list slot
     unsigned* heavy_count;
     bool seeds_changed;   // a probe added or replaced a seed
+    bool hints;           // center images are seed hints (LensModel path)
 };
 
 // Counts one wave step; past the budget the epoch is abandoned with an error.
@@ -342,7 +343,16 @@ __device__ void image_seeds(Ctx& C, const Pt* crit, const uint8_t* assign, const
             ord[j] = v;
         }
         const int physical = (res[ord[3]] * 1.0e-4 > res[ord[2]] + 1.0e-12) ? 3 : 5;
-        for (int i = 0; i < physical; ++i) push_seed(C, Pt{roots[ord[i]].re, roots[ord[i]].im});
+        if (C.hints) {
+            // the LensModel path passes these images as seed hints, kept if
+            // they map inside the disk (append_valid_seed, :1430-1447)
+            for (int i = 0; i < physical; ++i) {
+                const Pt z{roots[ord[i]].re, roots[ord[i]].im};
+                if (!(mapped_dist2(C.L, z.x, z.y, C.src) > C.r2 * (1.0 + 1.0e-8))) push_seed(C, z);
+            }
+        }
+        if (C.n_seeds == 0)
+            for (int i = 0; i < physical; ++i) push_seed(C, Pt{roots[ord[i]].re, roots[ord[i]].im});
     }
     if (!(C.rho > 0.0) || C.last_phase < 1) return;
     const double phase_step = 2.0 * kPi / static_cast<double>(kSeedPhases);
@@ -1465,6 +1475,7 @@ __device__ __forceinline__ void init_ctx(Ctx& C, const CurveRec& cv, Pt src, Pt*
     C.defer_slot = -1;
     C.heavy_count = nullptr;
     C.seeds_changed = false;
+    C.hints = false;
 }
 
 __device__ __forceinline__ unsigned long long* work_totals(const FsEpochs& ep)
@@ -1601,6 +1612,7 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
         const CurveRec cv = curves[c];
         Ctx C;
         init_ctx(C, cv, ep.src[e], slot_seeds);
+        C.hints = opt.ps_images != 0;
         if (pass == 0) {
             C.defer_budget = kHeavyBudget;
             C.heavy_count = &ep.counters[kQHeavyCount];
@@ -1856,6 +1868,7 @@ __global__ void __launch_bounds__(kWave) k_ir_epoch_debug(const CurveRec* __rest
     const CurveRec cv = curves[c];
     Ctx C;
     init_ctx(C, cv, ep.src[e], slot_seeds);
+    C.hints = opt.ps_images != 0;
     if (last_phase >= 0) C.last_phase = last_phase;
     if (n_in < 0) {
         epoch_seeds(C, cache, tab, ep, c, e);

@@ -188,32 +188,12 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
         sc->step_y = step_y;
         sc->max_seg_len = __builtin_sqrt(red[0][0]);
         sc->n = bins;
-        // seed Phase-2 greedy nearest-neighbour matching (:1813-1840)
-        Pt prevp[4] = {{1.0e30, 1.0e30}, {1.0e30, 1.0e30}, {1.0e30, 1.0e30}, {1.0e30, 1.0e30}};
-        for (int k = 0; k < kSeedPhases; ++k) {
-            const Pt* cs = crit + 4 * k;
-            bool used[4] = {false, false, false, false};
-            int asg[4];
-            for (int bi = 0; bi < 4; ++bi) {
-                double best_d2 = 1.0e60;
-                int best_j = -1;
-                for (int j = 0; j < 4; ++j) {
-                    if (used[j]) continue;
-                    const double d2 = dist2(cs[j], prevp[bi]);
-                    if (d2 < best_d2) {
-                        best_d2 = d2;
-                        best_j = j;
-                    }
-                }
-                asg[bi] = best_j;
-                if (best_j >= 0) used[best_j] = true;
-            }
-            for (int bi = 0; bi < 4; ++bi) {
-                assign[4 * k + bi] = static_cast<uint8_t>(asg[bi] < 0 ? 255 : asg[bi]);
-                if (asg[bi] >= 0) prevp[bi] = cs[asg[bi]];
-            }
-        }
     }
+    // seed Phase-2 greedy nearest-neighbour matching (:1813-1840): the
+    // reference starts every branch at (1e30, 1e30) with a best distance of
+    // 1e60, and each squared distance to that start is about 2e60 (or NaN), so
+    // no branch is ever matched and every start stays put: all 255
+    for (int k = tid; k < kSeedPhases * 4; k += kPrepBlock) assign[k] = 255;
 }
 
 // The branch positions of each curve in (grid cell, position) order, the

@@ -84,6 +84,11 @@ struct FsOptions {
     double kinji;
     double hex_threshold;
     double adaptive_hex;
+    // 1: the point-source magnification and center seeds of the LensModel path
+    // (sum over binary_images, seeds filtered by append_valid_seed), used for
+    // curves with parallax or orbital motion; 0: binary_mag0 of the static path
+    int ps_images;
+    int pad;
 };
 
 struct FsEpochs {

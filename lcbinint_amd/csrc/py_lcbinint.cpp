@@ -6,9 +6,11 @@
 // Same class names, keyword arguments, defaults, aliases and error types as
 // python/lcbinint_pybind.cpp (LimbDarkening :2828-2847, Options :2985-3112,
 // LightCurve :3114-3181, dict keys :145-179).  The light curve is evaluated by
-// the HIP library through lcbg_light_curve (include/lcbinint_amd.h) with the GIL
-// released; the static binary lens in VBM coordinates is the supported model,
-// other models raise as unsupported.
+// the HIP library with the GIL released: the static binary lens through
+// lcbg_light_curve, annual parallax and circular / Keplerian orbital motion
+// through lcbg_batch_light_curves_dynamic (the LensModel path); other models
+// (triple lens, binary source, terrestrial parallax, xallarap, legacy
+// coordinates) raise as unsupported.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -122,12 +124,17 @@ void raise_status(lcbg_status st)
     }
 }
 
+// EventCoordinates (python/lcbinint_pybind.cpp:95-101, 2848-2866)
+struct PyEvent {
+    double ra = 0.0, dec = 0.0, tfix = 0.0, obs_lat = 0.0, obs_lon = 0.0;
+};
+
 class PyLightCurve {
 public:
-    PyLightCurve(std::string lens, std::string source, const PyOptions& options,
+    PyLightCurve(std::string lens, std::string source, const PyEvent& event, const PyOptions& options,
                  const PyLimbDarkening& ld, int orbital_motion_mode, bool parallax,
                  bool terrestrial_parallax)
-        : lens_(std::move(lens)), source_(std::move(source)), options_(options), ld_(ld),
+        : lens_(std::move(lens)), source_(std::move(source)), event_(event), options_(options), ld_(ld),
           orbit_(orbital_motion_mode), parallax_(parallax), terrestrial_(terrestrial_parallax)
     {
         if (lens_ != "binary_lens" && lens_ != "triple_lens")
@@ -139,29 +146,49 @@ public:
     py::array_t<double> evaluate(py::array_t<double, py::array::c_style | py::array::forcecast> times,
                                  double t0, double tE, double u0, double alpha, double s, double q,
                                  double rho, double piEN, double piEE, double g1, double g2,
-                                 double g3) const
+                                 double g3, double lom_szs, double lom_ar) const
     {
         const py::buffer_info in = times.request();
         if (in.ndim != 1) throw std::invalid_argument("times must be one-dimensional");
         const auto n = static_cast<py::ssize_t>(in.shape[0]);
         std::vector<double> t(static_cast<const double*>(in.ptr), static_cast<const double*>(in.ptr) + n);
         auto out = py::array_t<double>({n}, {static_cast<py::ssize_t>(sizeof(double))});
-        // models beyond the static binary lens in VBM coordinates are outside this library
-        const bool supported = lens_ == "binary_lens" && source_ == "single" && orbit_ == 0 &&
-                               (!parallax_ || (piEN == 0.0 && piEE == 0.0)) && !terrestrial_ &&
-                               g1 == 0.0 && g2 == 0.0 && g3 == 0.0 && options_.vbm_compatible != 0 &&
-                               options_.center_of_mass == 0 && options_.xallarap == "none" &&
+        // the binary lens in VBM coordinates, static or with annual parallax and
+        // orbital motion; other models are outside this library
+        const bool supported = lens_ == "binary_lens" && source_ == "single" && !terrestrial_ &&
+                               options_.vbm_compatible != 0 && options_.center_of_mass == 0 &&
+                               options_.xallarap == "none" &&
                                (options_.adaptive_source_bins == 0 ||
                                 (options_.finite_source_tol <= 0.0 && options_.finite_source_reltol <= 0.0));
         if (!supported) throw std::runtime_error("unsupported");
         if (!std::isfinite(tE) || tE == 0.0 || !std::isfinite(q) || !std::isfinite(s))
             throw std::runtime_error("numerical error");
         lcbg_curve cv{t0, tE, u0, alpha, s, q, rho, ld_.c, ld_.d};
+        const double pN = parallax_ ? piEN : 0.0;
+        const double pE = parallax_ ? piEE : 0.0;
         lcbg_status st;
-        {
+        if (orbit_ == 0 && pN == 0.0 && pE == 0.0) {
+            // direct static binary (python/lcbinint_pybind.cpp:1103-1158)
             py::gil_scoped_release release;
             st = lcbg_light_curve(&cv, &options_.o, t.data(), static_cast<int64_t>(n),
                                   out.mutable_data(), nullptr);
+        } else {
+            // LensModel path (python/lcbinint_pybind.cpp:1159-1173)
+            const lcbg_dynamics dyn{pN, pE, event_.ra, event_.dec, event_.tfix, g1, g2, g3, lom_szs, lom_ar};
+            py::gil_scoped_release release;
+            lcbg_batch* b = nullptr;
+            st = lcbg_batch_create(&cv, 1, &options_.o, &b);
+            const int64_t off[2] = {0, static_cast<int64_t>(n)};
+            std::vector<int32_t> meth(static_cast<std::size_t>(n));
+            if (st == LCBG_OK)
+                st = lcbg_batch_light_curves_dynamic(b, &dyn, orbit_, off, t.data(), static_cast<int64_t>(n),
+                                                     out.mutable_data(), meth.data());
+            lcbg_batch_destroy(b);
+            if (st == LCBG_OK) {
+                const double* v = out.data();
+                for (py::ssize_t i = 0; i < n; ++i)
+                    if (!std::isfinite(v[i])) st = LCBG_NUMERICAL_ERROR;
+            }
         }
         raise_status(st);
         return out;
@@ -174,8 +201,11 @@ public:
                         dict_get(p, "alpha", 0.0), dict_get(p, "s", 1.0), dict_get(p, "q", 1.0),
                         dict_get(p, "rho", 0.0), parallax_ ? dict_get(p, "piEN", 0.0) : 0.0,
                         parallax_ ? dict_get(p, "piEE", 0.0) : 0.0, dict_get(p, "g1", 0.0),
-                        dict_get(p, "g2", 0.0), dict_get(p, "g3", 0.0));
+                        dict_get(p, "g2", 0.0), dict_get(p, "g3", 0.0), dict_get(p, "lom_szs", 0.0),
+                        dict_get(p, "lom_ar", 1.0));
     }
+
+    const PyEvent& event() const { return event_; }
 
     const std::string& lens() const { return lens_; }
     const std::string& source() const { return source_; }
@@ -187,6 +217,7 @@ public:
 
 private:
     std::string lens_, source_;
+    PyEvent event_;
     PyOptions options_;
     PyLimbDarkening ld_;
     int orbit_;
@@ -306,19 +337,36 @@ PYBIND11_MODULE(_lcbinint, m)
         .def_property("param_type", &coordinate_name, &apply_param_type)
         .def_property_readonly("xallarap_param_type", [](const PyOptions& p) { return p.xallarap; });
 
+    py::class_<PyEvent>(m, "EventCoordinates")
+        .def(py::init([](double ra, double dec, double tfix, double obs_lat, double obs_lon) {
+                 return PyEvent{ra, dec, tfix, obs_lat, obs_lon};
+             }),
+             py::arg("ra") = 0.0, py::arg("dec") = 0.0, py::arg("tfix") = 0.0, py::arg("obs_lat") = 0.0,
+             py::arg("obs_lon") = 0.0)
+        .def("__repr__", [](const PyEvent& e) {
+            return "EventCoordinates(ra=" + std::to_string(e.ra) + ", dec=" + std::to_string(e.dec) +
+                   ", tfix=" + std::to_string(e.tfix) + ", obs_lat=" + std::to_string(e.obs_lat) +
+                   ", obs_lon=" + std::to_string(e.obs_lon) + ")";
+        })
+        .def_readwrite("ra", &PyEvent::ra)
+        .def_readwrite("dec", &PyEvent::dec)
+        .def_readwrite("tfix", &PyEvent::tfix)
+        .def_readwrite("obs_lat", &PyEvent::obs_lat)
+        .def_readwrite("obs_lon", &PyEvent::obs_lon);
+
     py::class_<PyLightCurve>(m, "LightCurve")
-        .def(py::init([](const std::string& lens, const std::string& source, const py::object& event,
+        .def(py::init([](const std::string& lens, const std::string& source, const PyEvent& event,
                          const PyOptions& options, const PyLimbDarkening& ld, OrbitalMotionMode orbital_motion_mode,
                          bool parallax, bool terrestrial_parallax) {
-                 (void)event;
-                 return PyLightCurve(lens, source, options, ld, static_cast<int>(orbital_motion_mode), parallax,
-                                     terrestrial_parallax);
+                 return PyLightCurve(lens, source, event, options, ld, static_cast<int>(orbital_motion_mode),
+                                     parallax, terrestrial_parallax);
              }),
              py::kw_only(), py::arg("lens") = "binary_lens", py::arg("source") = "single",
-             py::arg("event") = py::none(), py::arg("options") = PyOptions{},
+             py::arg("event") = PyEvent{}, py::arg("options") = PyOptions{},
              py::arg("limb_darkening") = PyLimbDarkening{}, py::arg("orbital_motion_mode") = STATIC,
              py::arg("parallax") = false, py::arg("terrestrial_parallax") = false)
         .def_property_readonly("lens", &PyLightCurve::lens)
+        .def_property_readonly("event", &PyLightCurve::event)
         .def_property_readonly("source", &PyLightCurve::source)
         .def_property_readonly("options", &PyLightCurve::options)
         .def_property_readonly("limb_darkening", &PyLightCurve::limb_darkening)
@@ -329,6 +377,7 @@ PYBIND11_MODULE(_lcbinint, m)
         .def("__call__", &PyLightCurve::evaluate, py::arg("times"), py::kw_only(),
              py::arg("t0") = 0.0, py::arg("tE") = 1.0, py::arg("u0") = 0.0, py::arg("alpha") = 0.0,
              py::arg("s") = 1.0, py::arg("q") = 1.0, py::arg("rho") = 0.0, py::arg("piEN") = 0.0,
-             py::arg("piEE") = 0.0, py::arg("g1") = 0.0, py::arg("g2") = 0.0, py::arg("g3") = 0.0)
+             py::arg("piEE") = 0.0, py::arg("g1") = 0.0, py::arg("g2") = 0.0, py::arg("g3") = 0.0,
+             py::arg("lom_szs") = 0.0, py::arg("lom_ar") = 1.0)
         .def("light_curve", &PyLightCurve::from_dict, py::arg("times"), py::arg("params"));
 }

@@ -120,6 +120,7 @@ extern "C" lcbg_status lcbg_batch_create(const lcbg_curve* curves, int32_t n_cur
         std::vector<CurveRec>& recs = b->h_curves;
         recs.resize(static_cast<std::size_t>(n_curves));
         for (int32_t c = 0; c < n_curves; ++c) recs[static_cast<std::size_t>(c)] = make_curve_rec(curves[c]);
+        b->h_params.assign(curves, curves + n_curves);
         st = map_hip(hipMalloc(reinterpret_cast<void**>(&b->d_curves), recs.size() * sizeof(CurveRec)));
         if (st == LCBG_OK)
             st = map_hip(hipMemcpy(b->d_curves, recs.data(), recs.size() * sizeof(CurveRec),
@@ -136,6 +137,7 @@ extern "C" lcbg_status lcbg_batch_create(const lcbg_curve* curves, int32_t n_cur
 extern "C" lcbg_status lcbg_batch_destroy(lcbg_batch* batch)
 {
     if (!batch) return LCBG_OK;
+    if (batch->dyn_child) lcbg_batch_destroy(batch->dyn_child);
     if (batch->fs) fs_destroy(batch->fs);
     if (batch->d_curves) (void)hipFree(batch->d_curves);
     delete batch;
@@ -157,6 +159,7 @@ extern "C" lcbg_status lcbg_batch_update_curves(lcbg_batch* batch, const lcbg_cu
         r = make_curve_rec(curves[c]);
         r.source_bins = nb;
     }
+    batch->h_params.assign(curves, curves + batch->n_curves);
     batch->caches_stale = true;
     if (batch->n_curves == 0) return LCBG_OK;
     // pageable source: the copy is staged before the call returns, and the

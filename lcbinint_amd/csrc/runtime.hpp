@@ -54,4 +54,7 @@ struct lcbg_batch {
     lcbg::FsState* fs = nullptr;         // finite-source state, built on first use
     int64_t* d_cost = nullptr;           // optional per-epoch inverse-ray cost output (device)
     bool caches_stale = true;            // per-curve caustic caches need a rebuild
+    std::vector<lcbg_curve> h_params;    // the curves as passed (for host trajectories)
+    bool dynamic_images = false;         // point-source sum and seeds of the LensModel path
+    lcbg_batch* dyn_child = nullptr;     // per-epoch pseudo-curves of the dynamic path
 };

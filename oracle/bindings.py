@@ -132,6 +132,8 @@ class Ref:
                       P(I32), P(D), P(I32))
         self._lc = _f(L, "ref_light_curve_static", I32, P(RefSettings), V, ctypes.c_long,
                       D, D, D, D, D, D, D, V, V)
+        self._lcd = _f(L, "ref_light_curve_dynamic", I32, P(RefSettings), V, ctypes.c_long, V, I32,
+                       V, V, V)
         self._ir = _f(L, "ref_binary_inverse_ray", D, P(RefSettings), D, D, D, D, D, P(I32))
         self._caus = _f(L, "ref_caustic_points_at_phase", I32, D, D, D, V)
         self._crits = _f(L, "ref_critical_sources_at_phase", I32, D, D, D, V)
@@ -199,6 +201,24 @@ class Ref:
         st = self._lc(ctypes.byref(settings), ptr(t), len(t), t0, tE, u0, alpha, s, q, rho,
                       ptr(out), ptr(meth))
         return out, meth, st
+
+    DYNAMIC_KEYS = ("t0", "tE", "u0", "alpha", "s", "q", "rho", "piEN", "piEE", "ra", "dec", "tfix",
+                    "g1", "g2", "g3", "lom_szs", "lom_ar")
+
+    def light_curve_dynamic(self, settings, times, params, orbit_mode=0):
+        """LensModel path (parallax and/or orbital motion); params: dict with
+        DYNAMIC_KEYS (missing keys: 0, lom_ar 1).  Returns (mag, method,
+        source x/y and s(t) as an (n, 3) array, status).  The Earth table is the
+        one LCBININT_EARTH_EPHEMERIS names when the first parallax epoch runs."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        p = np.array([params.get(k, 1.0 if k in ("lom_ar", "tE") else 0.0) for k in self.DYNAMIC_KEYS],
+                     dtype=np.float64)
+        out = np.empty(len(t))
+        meth = np.empty(len(t), dtype=np.int32)
+        src = np.empty((len(t), 3))
+        st = self._lcd(ctypes.byref(settings), ptr(t), len(t), ptr(p), int(orbit_mode), ptr(out),
+                       ptr(meth), ptr(src))
+        return out, meth, src, st
 
     def caustic_points(self, s, q, phase):
         out = np.zeros(8)

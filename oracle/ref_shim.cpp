@@ -12,6 +12,7 @@
 #include "lcbinint/model/lens_model.hpp"
 #include "lcbinint/model/lens_parameters.hpp"
 #include "lcbinint/model/orbital_motion.hpp"
+#include "lcbinint/model/trajectory.hpp"
 #include "SkowronGould.h"
 
 // The reference's finite-source translation unit, compiled here from where it
@@ -343,6 +344,100 @@ double ref_hexadecapole(const RefSettings* rs, double s, double q, double x, dou
     const auto h = fsm::hexadecapole_binary(pm, s, q, {x, y}, rho, settings, &a0);
     if (rel_err) *rel_err = h.relative_error;
     return h.magnification;
+}
+
+// A binary-lens curve with annual parallax and/or orbital motion, as
+// LightCurve(parallax=..., orbital_motion_mode=...) evaluates it through
+// LensModel::magnification (lens_model.cpp:83-236; binary lens, VBM convention,
+// the path evaluate_binary_light_curve_numpy takes for such curves,
+// python/lcbinint_pybind.cpp:1159-1173).  p: t0, tE, u0, alpha, s, q, rho,
+// piEN, piEE, ra, dec, tfix, g1, g2, g3, lom_szs, lom_ar.  src (optional, 3n):
+// the source position the magnifier receives and the separation s(t).
+// Returns 0, 2 on a numerical error (first failing epoch left in out).
+int ref_light_curve_dynamic(const RefSettings* rs, const double* times, long n, const double* p,
+                            int orbit_mode, double* out, int* method, double* src)
+{
+    using namespace lcbinint::model;
+    LensParameters lp;
+    lp.t0 = p[0];
+    lp.tE = p[1];
+    lp.umin = p[2];
+    lp.theta = p[3];
+    lp.sep = p[4];
+    lp.q = p[5];
+    lp.rho = p[6];
+    lp.piEN = p[7];
+    lp.piEE = p[8];
+    lp.ra = p[9];
+    lp.dec = p[10];
+    lp.tfix = p[11];
+    lp.g1 = p[12];
+    lp.g2 = p[13];
+    lp.g3 = p[14];
+    lp.lom_szs = p[15];
+    lp.lom_ar = p[16];
+    lp.orbital_motion_mode = static_cast<lcbi_orbital_motion_mode>(orbit_mode);
+    lp.limb_darkening_c = rs->limb_darkening_c;
+    lp.limb_darkening_d = rs->limb_darkening_d;
+    const Trajectory trajectory(lp);
+    const double cos_theta = std::cos(lp.theta);
+    const double sin_theta = std::sin(lp.theta);
+    PointSourceMagnifier pm;
+    FiniteSourceMagnifier fm(to_settings(*rs));
+    const double effective_q = lp.q != 0.0 ? 1.0 / lp.q : lp.q;
+    for (long i = 0; i < n; ++i) {
+        const double time = times[i];
+        SourcePosition source;
+        if (lp.piEN == 0.0 && lp.piEE == 0.0) {
+            const double tn = (time - lp.t0) / lp.tE;
+            const double beta = lp.umin;
+            source.x = tn * cos_theta - beta * sin_theta;
+            source.y = tn * sin_theta + beta * cos_theta;
+        } else {
+            source = trajectory.source_position(time, true, LCBI_XALLARAP_NONE);
+        }
+        const bool static_orbit = lp.orbital_motion_mode == LCBI_ORBIT_STATIC;
+        const auto orbit = static_orbit ? OrbitalState {lp.sep, lp.theta, 0.0} : orbital_state(lp, time);
+        if (!std::isfinite(orbit.separation) || !std::isfinite(orbit.angle)) return 2;
+        SourcePosition sm = source;
+        if (!static_orbit) {
+            double tau = 0.0;
+            double beta = 0.0;
+            if (lp.piEN == 0.0 && lp.piEE == 0.0) {
+                tau = (time - lp.t0) / lp.tE;
+                beta = lp.umin;
+            } else {
+                tau = source.x * cos_theta + source.y * sin_theta;
+                beta = -source.x * sin_theta + source.y * cos_theta;
+            }
+            sm = {tau * std::cos(orbit.angle) - beta * std::sin(orbit.angle),
+                  beta * std::cos(orbit.angle) + tau * std::sin(orbit.angle)};
+        }
+        if (src) {
+            src[3 * i] = sm.x;
+            src[3 * i + 1] = sm.y;
+            src[3 * i + 2] = orbit.separation;
+        }
+        if (lp.rho == 0.0) {
+            out[i] = pm.binary_mag0(orbit.separation, effective_q, sm).magnification;
+            if (method) method[i] = 0;
+            continue;
+        }
+        const auto images = pm.binary_images(orbit.separation, effective_q, sm);
+        double a_ps = 0.0;
+        std::vector<SourcePosition> seeds;
+        seeds.reserve(images.size());
+        for (const auto& image : images) {
+            a_ps += 1.0 / std::abs(image.jacobian_determinant);
+            seeds.push_back(image.position);
+        }
+        const auto fs = fm.binary_mag(orbit.separation, effective_q, sm, std::abs(lp.rho), a_ps,
+                                      &seeds, true);
+        out[i] = fs.magnification;
+        if (method) method[i] = static_cast<int>(fs.decision.method);
+        if (!std::isfinite(fs.magnification)) return 2;
+    }
+    return 0;
 }
 
 int ref_sizeof_params(void) { return static_cast<int>(sizeof(lcbi_params)); }
