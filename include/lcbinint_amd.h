@@ -162,6 +162,14 @@ lcbg_status lcbg_dynamic_sources(const lcbg_curve* curve, const lcbg_dynamics* d
                                  int32_t orbital_motion_mode, const double* times, int64_t n,
                                  double* xys);
 
+/* lcbg_dynamic_sources with the forward-mode derivatives of (x, y, s) with
+ * respect to (t0, tE, u0, alpha, s, piEN, piEE, g1, g2):
+ * jac[27 i + 9 j + k] = d xys[3 i + j] / d param_k (no counterpart in the
+ * reference).  Host computation. */
+lcbg_status lcbg_dynamic_sources_jacobian(const lcbg_curve* curve, const lcbg_dynamics* dyn,
+                                          int32_t orbital_motion_mode, const double* times, int64_t n,
+                                          double* xys, double* jac);
+
 /* Light curves of the batch's curves with annual parallax and/or orbital
  * motion (dyn: n_curves records), the LensModel path of
  * evaluate_binary_light_curve_numpy (python/lcbinint_pybind.cpp:1159-1173):
@@ -177,6 +185,22 @@ lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const lcbg_dynami
                                             int32_t orbital_motion_mode, const int64_t* offsets,
                                             const double* times, int64_t n_epochs, double* mag,
                                             int32_t* method);
+
+/* lcbg_batch_light_curves_dynamic with, for every epoch, the derivatives of
+ * the magnification with respect to (t0, tE, u0, alpha, s, q, rho, u_LD, piEN,
+ * piEE, g1, g2): dmag[12 i + k] (host, n_epochs x 12; no counterpart in the
+ * reference).  The host trajectory is differentiated in forward mode (x, y
+ * and s(t) with respect to t0, tE, u0, alpha, s, piEN, piEE, g1, g2; g3,
+ * lom_szs, lom_ar, ra, dec and tfix held fixed), the magnification as in
+ * lcbg_batch_finite_source_grad_device with respect to the source position,
+ * s, q, rho and u_LD, and the two are chained per epoch.  mag_aa (may be
+ * NULL) as there.  The rates in the config-5 parameterization follow by the
+ * chain rule: d/d(ds/dt) = d/dg1 / (s 365.25), d/d(dalpha/dt) = d/dg2 / 365.25
+ * (with g1 = (ds/dt) / s / 365.25 also moving with s). */
+lcbg_status lcbg_batch_light_curves_dynamic_grad(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                                 int32_t orbital_motion_mode, const int64_t* offsets,
+                                                 const double* times, int64_t n_epochs, double* mag,
+                                                 int32_t* method, double* mag_aa, double* dmag);
 
 /* Finite-source light curves of every curve of the batch: the plain method of
  * FiniteSourceMagnifier::binary_mag (finite_source_magnifier.cpp:4633-4843) as

@@ -252,6 +252,45 @@ lcbg_batch_light_curves_dynamic = _sig("lcbg_batch_light_curves_dynamic", ctypes
                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p)
 
 
+lcbg_dynamic_sources_jacobian = _sig("lcbg_dynamic_sources_jacobian", ctypes.c_int, ctypes.POINTER(Curve),
+                                     ctypes.POINTER(Dynamics), ctypes.c_int32, ctypes.c_void_p,
+                                     ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p)
+lcbg_batch_light_curves_dynamic_grad = _sig("lcbg_batch_light_curves_dynamic_grad", ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.POINTER(Dynamics), ctypes.c_int32,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p)
+TRAJ_PARAMS = ("t0", "tE", "u0", "alpha", "s", "piEN", "piEE", "g1", "g2")
+DYN_GRAD_PARAMS = ("t0", "tE", "u0", "alpha", "s", "q", "rho", "ld_c", "piEN", "piEE", "g1", "g2")
+
+
+def dynamic_sources_jacobian(curve: Curve, dyn: Dynamics, orbit_mode: int, times):
+    """Host trajectory and its Jacobian: ((n, 3), (n, 3, 9)) over TRAJ_PARAMS."""
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    out = np.empty((len(t), 3))
+    jac = np.empty((len(t), 3, 9))
+    check(lcbg_dynamic_sources_jacobian(ctypes.byref(curve), ctypes.byref(dyn), int(orbit_mode),
+                                        t.ctypes.data, len(t), out.ctypes.data, jac.ctypes.data),
+          "lcbg_dynamic_sources_jacobian")
+    return out, jac
+
+
+def light_curves_dynamic_grad(batch: "Batch", dyn, orbit_mode: int, offsets, times):
+    """(mag, method, mag_aa, dmag (n, 12) over DYN_GRAD_PARAMS), host arrays."""
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    d = (Dynamics * len(dyn))(*dyn)
+    mag = np.empty(len(t))
+    meth = np.empty(len(t), dtype=np.int32)
+    aa = np.empty(len(t))
+    dm = np.empty((len(t), 12))
+    check(lcbg_batch_light_curves_dynamic_grad(batch.handle, d, int(orbit_mode), off.ctypes.data,
+                                               t.ctypes.data, len(t), mag.ctypes.data, meth.ctypes.data,
+                                               aa.ctypes.data, dm.ctypes.data),
+          "lcbg_batch_light_curves_dynamic_grad")
+    return mag, meth, aa, dm
+
+
 def load_earth_ephemeris(path=None):
     """Reads the Horizons-format Earth table (None: LCBININT_EARTH_EPHEMERIS)."""
     check(lcbg_load_earth_ephemeris(None if path is None else str(path).encode()),

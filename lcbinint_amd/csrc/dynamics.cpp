@@ -124,22 +124,146 @@ const std::vector<EarthRow>& earth_table()
     return g_earth;
 }
 
-struct EarthAt {
-    V3 p, v;
+// Forward-mode derivatives: the value part of every operation is the plain
+// double operation, so a Dual run yields the double run's values bitwise.
+template <int N>
+struct Dual {
+    double v = 0.0;
+    double d[N] = {};
+    Dual() = default;
+    Dual(double x) : v(x) {}  // NOLINT: constants promote
 };
 
-EarthAt earth_at(const std::vector<EarthRow>& rows, double t)
+template <int N> Dual<N> operator+(const Dual<N>& a, const Dual<N>& b)
 {
-    if (rows.size() < 2 || t < rows.front().t || t > rows.back().t)
-        return {{kNaN, kNaN, kNaN}, {kNaN, kNaN, kNaN}};
-    auto hi = std::upper_bound(rows.begin(), rows.end(), t,
+    Dual<N> r(a.v + b.v);
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+template <int N> Dual<N> operator-(const Dual<N>& a, const Dual<N>& b)
+{
+    Dual<N> r(a.v - b.v);
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+template <int N> Dual<N> operator-(const Dual<N>& a)
+{
+    Dual<N> r(-a.v);
+    for (int k = 0; k < N; ++k) r.d[k] = -a.d[k];
+    return r;
+}
+template <int N> Dual<N> operator*(const Dual<N>& a, const Dual<N>& b)
+{
+    Dual<N> r(a.v * b.v);
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+template <int N> Dual<N> operator/(const Dual<N>& a, const Dual<N>& b)
+{
+    Dual<N> r(a.v / b.v);
+    for (int k = 0; k < N; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+    return r;
+}
+template <int N> Dual<N> operator+(const Dual<N>& a, double b) { return a + Dual<N>(b); }
+template <int N> Dual<N> operator+(double a, const Dual<N>& b) { return Dual<N>(a) + b; }
+template <int N> Dual<N> operator-(const Dual<N>& a, double b) { return a - Dual<N>(b); }
+template <int N> Dual<N> operator-(double a, const Dual<N>& b) { return Dual<N>(a) - b; }
+template <int N> Dual<N> operator*(const Dual<N>& a, double b)
+{
+    Dual<N> r(a.v * b);
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * b;
+    return r;
+}
+template <int N> Dual<N> operator*(double a, const Dual<N>& b) { return b * a; }
+template <int N> Dual<N> operator/(const Dual<N>& a, double b)
+{
+    Dual<N> r(a.v / b);
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] / b;
+    return r;
+}
+template <int N> Dual<N> operator/(double a, const Dual<N>& b) { return Dual<N>(a) / b; }
+
+inline double val(double x) { return x; }
+template <int N> double val(const Dual<N>& x) { return x.v; }
+
+inline double f_sqrt(double x) { return std::sqrt(x); }
+inline double f_cos(double x) { return std::cos(x); }
+inline double f_sin(double x) { return std::sin(x); }
+inline double f_acos(double x) { return std::acos(x); }
+inline double f_atan2(double y, double x) { return std::atan2(y, x); }
+template <int N> Dual<N> f_sqrt(const Dual<N>& a)
+{
+    Dual<N> r(std::sqrt(a.v));
+    for (int k = 0; k < N; ++k) r.d[k] = r.v > 0.0 ? a.d[k] / (2.0 * r.v) : 0.0;
+    return r;
+}
+template <int N> Dual<N> f_cos(const Dual<N>& a)
+{
+    Dual<N> r(std::cos(a.v));
+    const double s = -std::sin(a.v);
+    for (int k = 0; k < N; ++k) r.d[k] = s * a.d[k];
+    return r;
+}
+template <int N> Dual<N> f_sin(const Dual<N>& a)
+{
+    Dual<N> r(std::sin(a.v));
+    const double c = std::cos(a.v);
+    for (int k = 0; k < N; ++k) r.d[k] = c * a.d[k];
+    return r;
+}
+template <int N> Dual<N> f_acos(const Dual<N>& a)
+{
+    Dual<N> r(std::acos(a.v));
+    const double den = std::sqrt(std::max(0.0, 1.0 - a.v * a.v));
+    for (int k = 0; k < N; ++k) r.d[k] = den > 0.0 ? -a.d[k] / den : 0.0;
+    return r;
+}
+template <int N> Dual<N> f_atan2(const Dual<N>& y, const Dual<N>& x)
+{
+    Dual<N> r(std::atan2(y.v, x.v));
+    const double den = x.v * x.v + y.v * y.v;
+    for (int k = 0; k < N; ++k) r.d[k] = den > 0.0 ? (x.v * y.d[k] - y.v * x.d[k]) / den : 0.0;
+    return r;
+}
+// max(0, a) and clamp(a, lo, hi) as std::max / std::clamp choose
+template <class T> T f_max0(const T& a) { return val(a) < 0.0 ? T(0.0) : a; }
+template <class T> T f_clamp(const T& a, double lo, double hi)
+{
+    return val(a) < lo ? T(lo) : (hi < val(a) ? T(hi) : a);
+}
+
+template <class T>
+struct Earth3 {
+    T p[3], v[3];
+};
+
+template <class T>
+Earth3<T> earth_at(const std::vector<EarthRow>& rows, const T& t)
+{
+    Earth3<T> out;
+    const double tv = val(t);
+    if (rows.size() < 2 || tv < rows.front().t || tv > rows.back().t) {
+        for (int i = 0; i < 3; ++i) out.p[i] = out.v[i] = T(kNaN);
+        return out;
+    }
+    auto hi = std::upper_bound(rows.begin(), rows.end(), tv,
                                [](double value, const EarthRow& r) { return value < r.t; });
     if (hi == rows.begin()) hi = rows.begin() + 1;
     if (hi == rows.end()) hi = rows.end() - 1;  // t equals the last row's time
     const auto lo = hi - 1;
     const double dt = hi->t - lo->t;
-    const double w = dt != 0.0 ? (t - lo->t) / dt : 0.0;
-    return {v_add(lo->p, v_scale(v_sub(hi->p, lo->p), w)), v_add(lo->v, v_scale(v_sub(hi->v, lo->v), w))};
+    const T w = dt != 0.0 ? (t - lo->t) / dt : T(0.0);
+    for (int i = 0; i < 3; ++i) {
+        out.p[i] = lo->p[i] + (hi->p[i] - lo->p[i]) * w;
+        out.v[i] = lo->v[i] + (hi->v[i] - lo->v[i]) * w;
+    }
+    return out;
+}
+
+template <class T>
+T dot3(const T* a, const V3& b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
 V3 normalized(V3 a)
@@ -150,12 +274,13 @@ V3 normalized(V3 a)
 }
 
 // Annual-parallax offsets (tau, beta) of one event direction and reference time.
+template <class T>
 class Parallax {
 public:
-    Parallax(const std::vector<EarthRow>& rows, double ra_deg, double dec_deg, double ref_time)
+    Parallax(const std::vector<EarthRow>& rows, double ra_deg, double dec_deg, const T& ref_time)
         : rows_(rows)
     {
-        offset_ = ref_time < kReducedJdOrigin ? kReducedJdOrigin : 0.0;
+        offset_ = val(ref_time) < kReducedJdOrigin ? kReducedJdOrigin : 0.0;
         eph_ref_ = ref_time + offset_;
         const double ra = ra_deg * kDeg;
         const double dec = dec_deg * kDeg;
@@ -163,144 +288,169 @@ public:
         event_ = {std::cos(ra) * cos_dec, std::sin(ra) * cos_dec, std::sin(dec)};
         east_ = normalized(v_cross({0.0, 0.0, 1.0}, event_));
         north_ = v_cross(event_, east_);
-        const EarthAt r = earth_at(rows_, emit_time(eph_ref_));
-        ref_pn_ = -v_dot(r.p, north_);
-        ref_pe_ = -v_dot(r.p, east_);
-        ref_vn_ = -v_dot(r.v, north_);
-        ref_ve_ = -v_dot(r.v, east_);
+        const Earth3<T> r = earth_at(rows_, emit_time(eph_ref_));
+        ref_pn_ = -dot3(r.p, north_);
+        ref_pe_ = -dot3(r.p, east_);
+        ref_vn_ = -dot3(r.v, north_);
+        ref_ve_ = -dot3(r.v, east_);
     }
 
-    void offset(double t, double pi_n, double pi_e, double* dtau, double* dbeta) const
+    void offset(double t, const T& pi_n, const T& pi_e, T* dtau, T* dbeta) const
     {
         const double eph = t + offset_;
-        const EarthAt s = earth_at(rows_, emit_time(eph));
-        const double pn = -v_dot(s.p, north_);
-        const double pe = -v_dot(s.p, east_);
-        const double dt = eph - eph_ref_;
-        const double dn = pn - ref_pn_ - ref_vn_ * dt;
-        const double de = pe - ref_pe_ - ref_ve_ * dt;
+        const Earth3<T> s = earth_at(rows_, emit_time(T(eph)));
+        const T pn = -dot3(s.p, north_);
+        const T pe = -dot3(s.p, east_);
+        const T dt = eph - eph_ref_;
+        const T dn = pn - ref_pn_ - ref_vn_ * dt;
+        const T de = pe - ref_pe_ - ref_ve_ * dt;
         *dtau = pi_n * dn + pi_e * de;
         *dbeta = -pi_e * dn + pi_n * de;
     }
 
 private:
-    double emit_time(double obs) const
+    T emit_time(const T& obs) const
     {
-        double emit = obs;
+        T emit = obs;
         for (int i = 0; i < 5; ++i) {
-            const EarthAt s = earth_at(rows_, emit);
-            emit = obs - v_dot(s.p, event_) * kAuLightDays;
+            const Earth3<T> s = earth_at(rows_, emit);
+            emit = obs - dot3(s.p, event_) * kAuLightDays;
         }
         return emit;
     }
 
     const std::vector<EarthRow>& rows_;
-    double offset_ = 0.0, eph_ref_ = 0.0;
+    double offset_ = 0.0;
+    T eph_ref_{};
     V3 event_{}, east_{}, north_{};
-    double ref_pn_ = 0.0, ref_pe_ = 0.0, ref_vn_ = 0.0, ref_ve_ = 0.0;
+    T ref_pn_{}, ref_pe_{}, ref_vn_{}, ref_ve_{};
 };
 
 // ----------------------------------------------------------------- orbits
 
-double kepler_e(double m, double e)
+template <class T>
+T kepler_e(const T& m, const T& e)
 {
-    double E = m + e * std::sin(m);
+    T E = m + e * f_sin(m);
     for (int i = 0; i < 10; ++i) {
-        const double f = E - e * std::sin(E) - m;
-        const double fp = 1.0 - e * std::cos(E);
-        const double d = f / fp;
-        E -= d;
-        if (std::abs(d) <= 1.0e-13) break;
+        const T f = E - e * f_sin(E) - m;
+        const T fp = 1.0 - e * f_cos(E);
+        const T d = f / fp;
+        E = E - d;
+        if (std::abs(val(d)) <= 1.0e-13) break;
     }
     return E;
 }
 
+// The orbit parameters a gradient may follow: g1, g2 (g3, lom_szs, lom_ar are
+// held fixed).
+template <class T>
+struct OrbitIn {
+    T g1, g2;
+    double g3, szs, ar_in;
+};
+
 // Per-curve constants of orbital_state; at() gives (s, angle) at a time.
+template <class T>
 struct Orbit {
     int mode = LCBG_ORBIT_STATIC;
-    double sep = 1.0, angle = 0.0, t_ref = 0.0;
+    T sep{}, angle{}, t_ref{};
     // circular
-    double w_orb = 0.0, phi0 = 0.0, c_inc = 1.0, s_true = 0.0, c_om = 0.0, s_om = 0.0;
+    T w_orb{}, phi0{}, c_inc{}, s_true{}, c_om{}, s_om{};
     // Keplerian
-    double n = 0.0, ecc = 0.0, t_peri = 0.0, a = 0.0, one_minus_e2 = 0.0;
-    V3 xa{}, ya{};
+    T n{}, ecc{}, t_peri{}, a{}, one_minus_e2{};
+    T xa[3], ya[3];
 
-    Orbit(int m, double s, double alpha, const lcbg_dynamics& d, double t0)
-        : mode(m), sep(s), angle(alpha)
+    Orbit(int m, const T& s, const T& alpha, const OrbitIn<T>& d, const T& tref)
+        : mode(m), sep(s), angle(alpha), t_ref(tref)
     {
-        t_ref = d.tfix != 0.0 ? d.tfix : t0;
         if (mode == LCBG_ORBIT_CIRCULAR) {
-            const double w13_sq = d.g1 * d.g1 + d.g3 * d.g3;
-            const double w13 = std::sqrt(w13_sq);
-            const double w123 = std::sqrt(w13_sq + d.g2 * d.g2);
+            const T w13_sq = d.g1 * d.g1 + d.g3 * d.g3;
+            const T w13 = f_sqrt(w13_sq);
+            const T w123 = f_sqrt(w13_sq + d.g2 * d.g2);
             w_orb = d.g2;
-            double inc = 0.0;
-            if (w13 > kOrbitEps) {
+            phi0 = T(0.0);
+            T inc(0.0);
+            if (val(w13) > kOrbitEps) {
                 const double w3 = d.g3 > kOrbitEps ? d.g3 : kOrbitEps;
                 w_orb = w3 * w123 / w13;
-                inc = std::acos(std::clamp(d.g2 * w3 / (w13 * w123), -1.0, 1.0));
-                phi0 = std::atan2(-d.g1 * w123, w3 * w13);
+                inc = f_acos(f_clamp(d.g2 * w3 / (w13 * w123), -1.0, 1.0));
+                phi0 = f_atan2(-d.g1 * w123, w3 * w13);
             }
-            const double c_phi0 = std::cos(phi0);
-            const double s_phi0 = std::sin(phi0);
-            c_inc = std::cos(inc);
-            const double den0 = std::sqrt(c_phi0 * c_phi0 + c_inc * c_inc * s_phi0 * s_phi0);
+            const T c_phi0 = f_cos(phi0);
+            const T s_phi0 = f_sin(phi0);
+            c_inc = f_cos(inc);
+            const T den0 = f_sqrt(c_phi0 * c_phi0 + c_inc * c_inc * s_phi0 * s_phi0);
             s_true = sep / den0;
-            const double c_a = std::cos(angle);
-            const double s_a = std::sin(angle);
+            const T c_a = f_cos(angle);
+            const T s_a = f_sin(angle);
             c_om = (c_phi0 * c_a + c_inc * s_a * s_phi0) / den0;
             s_om = (c_phi0 * s_a - c_inc * c_a * s_phi0) / den0;
         } else if (mode == LCBG_ORBIT_KEPLER) {
-            const double ar = d.lom_ar + kOrbitEps;
-            const double szs = d.lom_szs;
+            const double ar = d.ar_in + kOrbitEps;
+            const double szs = d.szs;
             const double smix = 1.0 + szs * szs;
             const double sqsmix = std::sqrt(smix);
-            const double w11 = d.g1 * d.g1, w22 = d.g2 * d.g2, w33 = d.g3 * d.g3;
-            const double w12 = w11 + w22;
-            const double wt2 = w12 + w33;
+            const T w11 = d.g1 * d.g1, w22 = d.g2 * d.g2;
+            const double w33 = d.g3 * d.g3;
+            const T w12 = w11 + w22;
+            const T wt2 = w12 + w33;
             const double arm1 = ar - 1.0;
             const double arm2 = 2.0 * ar - 1.0;
-            n = std::sqrt(wt2 / arm2 / smix) / ar;
-            const V3 za = normalized({-szs * d.g2, szs * d.g1 - d.g3, d.g2});
-            xa = {-ar * w11 + arm1 * w22 - arm2 * szs * d.g1 * d.g3 + arm1 * w33,
-                  -arm2 * d.g2 * (d.g1 + szs * d.g3),
-                  arm1 * szs * w12 - arm2 * d.g1 * d.g3 - ar * szs * w33};
-            const double xn = std::sqrt(std::max(0.0, v_dot(xa, xa)));
-            xa = v_scale(xa, 1.0 / xn);
+            n = f_sqrt(wt2 / arm2 / smix) / ar;
+            // z axis (normalized)
+            T za[3] = {-szs * d.g2, szs * d.g1 - d.g3, d.g2};
+            {
+                const T zn = f_sqrt(za[0] * za[0] + za[1] * za[1] + za[2] * za[2]);
+                if (val(zn) == 0.0 || !std::isfinite(val(zn))) {
+                    for (auto& z : za) z = T(kNaN);
+                } else {
+                    const T inv = 1.0 / zn;
+                    for (auto& z : za) z = z * inv;
+                }
+            }
+            xa[0] = -ar * w11 + arm1 * w22 - arm2 * szs * d.g1 * d.g3 + arm1 * w33;
+            xa[1] = -arm2 * d.g2 * (d.g1 + szs * d.g3);
+            xa[2] = arm1 * szs * w12 - arm2 * d.g1 * d.g3 - ar * szs * w33;
+            const T xn = f_sqrt(f_max0(xa[0] * xa[0] + xa[1] * xa[1] + xa[2] * xa[2]));
+            const T inv_xn = 1.0 / xn;
+            for (auto& x : xa) x = x * inv_xn;
             ecc = xn / (ar * sqsmix * wt2);
-            ya = v_cross(za, xa);
-            const double conu = (xa[0] + xa[2] * szs) / sqsmix;
-            const double cos_e0 = std::clamp((conu + ecc) / (1.0 + ecc * conu), -1.0, 1.0);
-            const double sign = (ya[0] + ya[2] * szs) > 0.0 ? 1.0 : -1.0;
-            const double e0 = std::acos(cos_e0) * sign;
-            const double sin_e0 = std::sqrt(std::max(0.0, 1.0 - cos_e0 * cos_e0)) * sign;
+            ya[0] = za[1] * xa[2] - za[2] * xa[1];
+            ya[1] = za[2] * xa[0] - za[0] * xa[2];
+            ya[2] = za[0] * xa[1] - za[1] * xa[0];
+            const T conu = (xa[0] + xa[2] * szs) / sqsmix;
+            const T cos_e0 = f_clamp((conu + ecc) / (1.0 + ecc * conu), -1.0, 1.0);
+            const double sign = (val(ya[0]) + val(ya[2]) * szs) > 0.0 ? 1.0 : -1.0;
+            const T e0 = f_acos(cos_e0) * sign;
+            const T sin_e0 = f_sqrt(f_max0(1.0 - cos_e0 * cos_e0)) * sign;
             t_peri = t_ref - (e0 - ecc * sin_e0) / n;
             a = ar * sep * sqsmix;
-            one_minus_e2 = std::sqrt(std::max(0.0, 1.0 - ecc * ecc));
+            one_minus_e2 = f_sqrt(f_max0(1.0 - ecc * ecc));
         }
     }
 
-    void at(double t, double* s_out, double* angle_out) const
+    void at(double t, T* s_out, T* angle_out) const
     {
         if (mode == LCBG_ORBIT_CIRCULAR) {
-            const double phi = w_orb * (t - t_ref) + phi0;
-            const double c_phi = std::cos(phi);
-            const double s_phi = std::sin(phi);
-            const double den = std::sqrt(c_phi * c_phi + c_inc * c_inc * s_phi * s_phi);
-            const double sin_a = (c_phi * s_om + c_inc * s_phi * c_om) / den;
-            const double cos_a = (c_phi * c_om - c_inc * s_phi * s_om) / den;
+            const T phi = w_orb * (t - t_ref) + phi0;
+            const T c_phi = f_cos(phi);
+            const T s_phi = f_sin(phi);
+            const T den = f_sqrt(c_phi * c_phi + c_inc * c_inc * s_phi * s_phi);
+            const T sin_a = (c_phi * s_om + c_inc * s_phi * c_om) / den;
+            const T cos_a = (c_phi * c_om - c_inc * s_phi * s_om) / den;
             *s_out = s_true * den;
-            *angle_out = std::atan2(sin_a, cos_a);
+            *angle_out = f_atan2(sin_a, cos_a);
         } else if (mode == LCBG_ORBIT_KEPLER) {
-            const double E = kepler_e(n * (t - t_peri), ecc);
-            const double cos_e = std::cos(E);
-            const double sin_e = std::sin(E);
-            const double r0 = a * (cos_e - ecc);
-            const double r1 = a * one_minus_e2 * sin_e;
-            const double x0 = r0 * xa[0] + r1 * ya[0];
-            const double x1 = r0 * xa[1] + r1 * ya[1];
-            *s_out = std::sqrt(x0 * x0 + x1 * x1);
-            *angle_out = angle + std::atan2(x1, x0);
+            const T E = kepler_e(n * (t - t_peri), ecc);
+            const T cos_e = f_cos(E);
+            const T sin_e = f_sin(E);
+            const T r0 = a * (cos_e - ecc);
+            const T r1 = a * one_minus_e2 * sin_e;
+            const T x0 = r0 * xa[0] + r1 * ya[0];
+            const T x1 = r0 * xa[1] + r1 * ya[1];
+            *s_out = f_sqrt(x0 * x0 + x1 * x1);
+            *angle_out = angle + f_atan2(x1, x0);
         } else {
             *s_out = sep;
             *angle_out = angle;
@@ -308,60 +458,103 @@ struct Orbit {
     }
 };
 
-}  // namespace
+// The parameters a trajectory depends on, as T (double, or Dual seeded on
+// t0, tE, u0, alpha, s, piEN, piEE, g1, g2).
+template <class T>
+struct TrajIn {
+    T t0, tE, u0, alpha, s, piEN, piEE, g1, g2;
+};
 
 // Source position in the instantaneous binary frame and s(t) of every epoch of
 // one curve (VBM convention); false if an epoch's orbit is not finite.
-bool dynamic_sources(const lcbg_curve& cv, const lcbg_dynamics& d, int orbit_mode, const double* t,
-                     int64_t n, double* xys)
+template <class T>
+bool trajectory(const TrajIn<T>& P, const lcbg_dynamics& d, int orbit_mode, const double* t, int64_t n,
+                T* out)
 {
     const bool parallax = d.piEN != 0.0 || d.piEE != 0.0;
-    const std::vector<EarthRow>* rows = parallax ? &earth_table() : nullptr;
-    std::unique_ptr<Parallax> px;
-    if (parallax) px.reset(new Parallax(*rows, d.ra, d.dec, d.tfix != 0.0 ? d.tfix : cv.t0));
-    const Orbit orbit(orbit_mode, cv.s, cv.alpha, d, cv.t0);
-    const double cos_t = std::cos(cv.alpha);
-    const double sin_t = std::sin(cv.alpha);
+    std::unique_ptr<Parallax<T>> px;
+    if (parallax) px.reset(new Parallax<T>(earth_table(), d.ra, d.dec, d.tfix != 0.0 ? T(d.tfix) : P.t0));
+    const OrbitIn<T> oin{P.g1, P.g2, d.g3, d.lom_szs, d.lom_ar};
+    const Orbit<T> orbit(orbit_mode, P.s, P.alpha, oin, d.tfix != 0.0 ? T(d.tfix) : P.t0);
+    const T cos_t = f_cos(P.alpha);
+    const T sin_t = f_sin(P.alpha);
     bool ok = true;
     for (int64_t i = 0; i < n; ++i) {
         const double time = t[i];
-        double tn = (time - cv.t0) / cv.tE;
-        double beta = cv.u0;
+        T tn = (time - P.t0) / P.tE;
+        T beta = P.u0;
         if (parallax) {
-            double dtau, dbeta;
-            px->offset(time, d.piEN, d.piEE, &dtau, &dbeta);
-            tn += dtau;
-            beta += dbeta;
+            T dtau, dbeta;
+            px->offset(time, P.piEN, P.piEE, &dtau, &dbeta);
+            tn = tn + dtau;
+            beta = beta + dbeta;
         }
-        double x = tn * cos_t - beta * sin_t;
-        double y = tn * sin_t + beta * cos_t;
-        double s_t, ang;
+        T x = tn * cos_t - beta * sin_t;
+        T y = tn * sin_t + beta * cos_t;
+        T s_t, ang;
         orbit.at(time, &s_t, &ang);
-        if (!std::isfinite(s_t) || !std::isfinite(ang)) ok = false;
+        if (!std::isfinite(val(s_t)) || !std::isfinite(val(ang))) ok = false;
         if (orbit_mode != LCBG_ORBIT_STATIC) {
-            double tau, b;
+            T tau, b;
             if (!parallax) {
-                tau = (time - cv.t0) / cv.tE;
-                b = cv.u0;
+                tau = (time - P.t0) / P.tE;
+                b = P.u0;
             } else {
                 tau = x * cos_t + y * sin_t;
                 b = -x * sin_t + y * cos_t;
             }
-            const double ca = std::cos(ang);
-            const double sa = std::sin(ang);
+            const T ca = f_cos(ang);
+            const T sa = f_sin(ang);
             x = tau * ca - b * sa;
             y = b * ca + tau * sa;
         }
-        xys[3 * i] = x;
-        xys[3 * i + 1] = y;
-        xys[3 * i + 2] = s_t;
+        out[3 * i] = x;
+        out[3 * i + 1] = y;
+        out[3 * i + 2] = s_t;
     }
     return ok;
 }
 
-// The same for every curve of a batch, curves spread over host threads.
+}  // namespace
+
+namespace {
+constexpr int kTrajParams = 9;  // t0, tE, u0, alpha, s, piEN, piEE, g1, g2
+}
+
+bool dynamic_sources(const lcbg_curve& cv, const lcbg_dynamics& d, int orbit_mode, const double* t,
+                     int64_t n, double* xys)
+{
+    const TrajIn<double> P{cv.t0, cv.tE, cv.u0, cv.alpha, cv.s, d.piEN, d.piEE, d.g1, d.g2};
+    return trajectory(P, d, orbit_mode, t, n, xys);
+}
+
+// The same with the derivatives of (x, y, s) with respect to (t0, tE, u0,
+// alpha, s, piEN, piEE, g1, g2): jac[27 i + 9 j + k] = d out_j / d param_k.
+bool dynamic_sources_jacobian(const lcbg_curve& cv, const lcbg_dynamics& d, int orbit_mode,
+                              const double* t, int64_t n, double* xys, double* jac)
+{
+    using D = Dual<kTrajParams>;
+    TrajIn<D> P;
+    const double seed[kTrajParams] = {cv.t0, cv.tE, cv.u0, cv.alpha, cv.s, d.piEN, d.piEE, d.g1, d.g2};
+    D* fields[kTrajParams] = {&P.t0, &P.tE, &P.u0, &P.alpha, &P.s, &P.piEN, &P.piEE, &P.g1, &P.g2};
+    for (int k = 0; k < kTrajParams; ++k) {
+        *fields[k] = D(seed[k]);
+        fields[k]->d[k] = 1.0;
+    }
+    std::vector<D> out(static_cast<std::size_t>(3 * n));
+    const bool ok = trajectory(P, d, orbit_mode, t, n, out.data());
+    for (int64_t i = 0; i < 3 * n; ++i) {
+        xys[i] = out[static_cast<std::size_t>(i)].v;
+        for (int k = 0; k < kTrajParams; ++k) jac[kTrajParams * i + k] = out[static_cast<std::size_t>(i)].d[k];
+    }
+    return ok;
+}
+
+// The same for every curve of a batch, curves spread over host threads; jac
+// may be null.
 bool dynamic_sources_batch(const lcbg_curve* curves, const lcbg_dynamics* dyn, int32_t n_curves,
-                           int orbit_mode, const int64_t* offsets, const double* times, double* xys)
+                           int orbit_mode, const int64_t* offsets, const double* times, double* xys,
+                           double* jac)
 {
     if (n_curves <= 0) return true;
     bool any_parallax = false;
@@ -371,9 +564,13 @@ bool dynamic_sources_batch(const lcbg_curve* curves, const lcbg_dynamics* dyn, i
     const int n_threads = static_cast<int>(std::min<unsigned>(hw ? hw : 1, 16u));
     std::vector<char> ok(static_cast<std::size_t>(n_curves), 1);
     auto work = [&](int k) {
-        for (int32_t c = k; c < n_curves; c += n_threads)
-            ok[static_cast<std::size_t>(c)] = dynamic_sources(curves[c], dyn[c], orbit_mode, times + offsets[c],
-                                                              offsets[c + 1] - offsets[c], xys + 3 * offsets[c]);
+        for (int32_t c = k; c < n_curves; c += n_threads) {
+            const int64_t b = offsets[c], m = offsets[c + 1] - offsets[c];
+            ok[static_cast<std::size_t>(c)] =
+                jac ? dynamic_sources_jacobian(curves[c], dyn[c], orbit_mode, times + b, m, xys + 3 * b,
+                                               jac + 3 * kTrajParams * b)
+                    : dynamic_sources(curves[c], dyn[c], orbit_mode, times + b, m, xys + 3 * b);
+        }
     };
     std::vector<std::thread> pool;
     for (int k = 1; k < n_threads; ++k) pool.emplace_back(work, k);
@@ -404,4 +601,16 @@ extern "C" lcbg_status lcbg_dynamic_sources(const lcbg_curve* curve, const lcbg_
         return LCBG_UNSUPPORTED;
     return lcbg::dynamic_sources(*curve, *dyn, orbital_motion_mode, times, n, xys) ? LCBG_OK
                                                                                    : LCBG_NUMERICAL_ERROR;
+}
+
+extern "C" lcbg_status lcbg_dynamic_sources_jacobian(const lcbg_curve* curve, const lcbg_dynamics* dyn,
+                                                     int32_t orbital_motion_mode, const double* times,
+                                                     int64_t n, double* xys, double* jac)
+{
+    if (!curve || !dyn || (n > 0 && (!times || !xys || !jac)) || n < 0) return LCBG_INVALID_ARGUMENT;
+    if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER)
+        return LCBG_UNSUPPORTED;
+    return lcbg::dynamic_sources_jacobian(*curve, *dyn, orbital_motion_mode, times, n, xys, jac)
+               ? LCBG_OK
+               : LCBG_NUMERICAL_ERROR;
 }

@@ -490,19 +490,27 @@ extern "C" lcbg_status lcbg_batch_finite_source_device(lcbg_batch* batch, const 
 
 namespace lcbg {
 bool dynamic_sources_batch(const lcbg_curve* curves, const lcbg_dynamics* dyn, int32_t n_curves,
-                           int orbit_mode, const int64_t* offsets, const double* times, double* xys);
+                           int orbit_mode, const int64_t* offsets, const double* times, double* xys,
+                           double* jac);
 }
 
 namespace {
 // Epochs per device pass of the dynamic path: each epoch is a curve of its own
 // (its own separation, its own caustic cache); 16384 caches take ~5.6 GB.
 constexpr int32_t kDynChunk = 16384;
+constexpr int kDynGradParams = 12;
 }  // namespace
 
-extern "C" lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const lcbg_dynamics* dyn,
-                                                       int32_t orbital_motion_mode,
-                                                       const int64_t* offsets, const double* times,
-                                                       int64_t n_epochs, double* mag, int32_t* method)
+namespace {
+
+constexpr int kTrajParams = 9;  // t0, tE, u0, alpha, s, piEN, piEE, g1, g2
+
+// The dynamic path, optionally with the derivatives of each epoch's
+// magnification with respect to the 12 parameters of
+// lcbg_batch_light_curves_dynamic_grad (dmag != nullptr).
+lcbg_status dynamic_run(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t orbital_motion_mode,
+                        const int64_t* offsets, const double* times, int64_t n_epochs, double* mag,
+                        int32_t* method, double* mag_aa, double* dmag)
 {
     if (!batch || n_epochs < 0) return LCBG_INVALID_ARGUMENT;
     if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER)
@@ -513,9 +521,11 @@ extern "C" lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const 
     if (offsets[0] != 0 || offsets[nc] != n_epochs) return LCBG_INVALID_ARGUMENT;
     for (int32_t c = 0; c < nc; ++c)
         if (offsets[c + 1] < offsets[c]) return LCBG_INVALID_ARGUMENT;
+    const bool grad = dmag != nullptr;
     std::vector<double> xys(static_cast<std::size_t>(3 * n_epochs));
+    std::vector<double> jac(grad ? static_cast<std::size_t>(3 * kTrajParams * n_epochs) : 0);
     if (!dynamic_sources_batch(batch->h_params.data(), dyn, nc, orbital_motion_mode, offsets, times,
-                               xys.data()))
+                               xys.data(), grad ? jac.data() : nullptr))
         return LCBG_NUMERICAL_ERROR;  // a non-finite orbit (lens_model.cpp:117-120)
     lcbg_status st = LCBG_OK;
     // pseudo-curve capacity: the run's epoch count rounded up to a power of two,
@@ -536,18 +546,25 @@ extern "C" lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const 
     lcbg_batch* ch = batch->dyn_child;
     const int32_t chunk = ch->n_curves;
     int64_t* d_off = nullptr;
-    double* d_buf = nullptr;
+    double* d_buf = nullptr;  // times, mag, mag_aa, dmag (8 per epoch)
     int32_t* d_meth = nullptr;
+    const std::size_t buf_doubles = static_cast<std::size_t>(chunk) * (grad ? 3 + kGradParams : 2);
     st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_off), (chunk + 1) * sizeof(int64_t)));
-    if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_buf), 2 * chunk * sizeof(double)));
+    if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_buf), buf_doubles * sizeof(double)));
     if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_meth), chunk * sizeof(int32_t)));
+    double* d_t = d_buf;
+    double* d_mag = d_buf + chunk;
+    double* d_aa = grad ? d_buf + 2 * chunk : nullptr;
+    double* d_dm = grad ? d_buf + 3 * chunk : nullptr;
     std::vector<int64_t> h_off(static_cast<std::size_t>(chunk) + 1);
     std::vector<double> h_t(static_cast<std::size_t>(chunk));
+    std::vector<double> h_dm(grad ? static_cast<std::size_t>(chunk) * kGradParams : 0);
     int32_t c = 0;  // curve of the chunk's first epoch
     for (int64_t base = 0; st == LCBG_OK && base < n_epochs; base += chunk) {
         const int32_t m = static_cast<int32_t>(std::min<int64_t>(chunk, n_epochs - base));
         // pseudo-curve k = epoch base + k: source (x, y) = (time, u0) with
         // t0 = 0, tE = 1, alpha = 0, the epoch's separation, the curve's q, rho, LD, nbin
+        int32_t cc = c;
         for (int32_t k = 0; k < chunk; ++k) {
             CurveRec& r = ch->h_curves[static_cast<std::size_t>(k)];
             h_off[static_cast<std::size_t>(k)] = k < m ? k : m;
@@ -556,8 +573,8 @@ extern "C" lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const 
                 continue;
             }
             const int64_t e = base + k;
-            while (offsets[c + 1] <= e) ++c;
-            const CurveRec& pc = batch->h_curves[static_cast<std::size_t>(c)];
+            while (offsets[cc + 1] <= e) ++cc;
+            const CurveRec& pc = batch->h_curves[static_cast<std::size_t>(cc)];
             r.t0 = 0.0;
             r.inverse_tE = 1.0;
             r.u0 = xys[static_cast<std::size_t>(3 * e + 1)];
@@ -572,25 +589,85 @@ extern "C" lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const 
             r.pad = 0;
             h_t[static_cast<std::size_t>(k)] = xys[static_cast<std::size_t>(3 * e)];
         }
-        h_off[chunk] = m;
+        h_off[static_cast<std::size_t>(chunk)] = m;
         ch->caches_stale = true;
         st = map_hip(hipMemcpy(ch->d_curves, ch->h_curves.data(), chunk * sizeof(CurveRec),
                                hipMemcpyHostToDevice));
         if (st == LCBG_OK)
             st = map_hip(hipMemcpy(d_off, h_off.data(), (chunk + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
-        if (st == LCBG_OK)
-            st = map_hip(hipMemcpy(d_buf, h_t.data(), m * sizeof(double), hipMemcpyHostToDevice));
-        if (st == LCBG_OK) st = fs_run(ch, d_off, d_buf, m, d_buf + chunk, d_meth, nullptr, nullptr);
+        if (st == LCBG_OK) st = map_hip(hipMemcpy(d_t, h_t.data(), m * sizeof(double), hipMemcpyHostToDevice));
+        if (st == LCBG_OK) {
+            if (grad) {
+                const GradArgs ga{d_aa, d_dm, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr};
+                st = fs_run(ch, d_off, d_t, m, d_mag, d_meth, nullptr, nullptr, &ga);
+            } else {
+                st = fs_run(ch, d_off, d_t, m, d_mag, d_meth, nullptr, nullptr);
+            }
+        }
         if (st == LCBG_OK) st = map_hip(hipStreamSynchronize(nullptr));
         if (st == LCBG_OK)
-            st = map_hip(hipMemcpy(mag + base, d_buf + chunk, m * sizeof(double), hipMemcpyDeviceToHost));
+            st = map_hip(hipMemcpy(mag + base, d_mag, m * sizeof(double), hipMemcpyDeviceToHost));
         if (st == LCBG_OK)
             st = map_hip(hipMemcpy(method + base, d_meth, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (st == LCBG_OK && grad && mag_aa)
+            st = map_hip(hipMemcpy(mag_aa + base, d_aa, m * sizeof(double), hipMemcpyDeviceToHost));
+        if (st == LCBG_OK && grad)
+            st = map_hip(hipMemcpy(h_dm.data(), d_dm, static_cast<std::size_t>(m) * kGradParams * sizeof(double),
+                                   hipMemcpyDeviceToHost));
+        if (st == LCBG_OK && grad) {
+            // chain: the pseudo-curve's derivatives give dA/dx = -d/dt0,
+            // dA/dy = d/du0, dA/ds, and q, rho, u_LD directly (alpha = 0, tE = 1)
+            for (int32_t k = 0; k < m; ++k) {
+                const int64_t e = base + k;
+                const double* gp = h_dm.data() + static_cast<std::size_t>(k) * kGradParams;
+                const double da[3] = {-gp[0], gp[2], gp[4]};
+                const double* J = jac.data() + static_cast<std::size_t>(3 * kTrajParams * e);
+                double tr[kTrajParams];
+                for (int j = 0; j < kTrajParams; ++j)
+                    tr[j] = da[0] * J[j] + da[1] * J[kTrajParams + j] + da[2] * J[2 * kTrajParams + j];
+                double* out = dmag + e * kDynGradParams;
+                out[0] = tr[0];   // t0
+                out[1] = tr[1];   // tE
+                out[2] = tr[2];   // u0
+                out[3] = tr[3];   // alpha
+                out[4] = tr[4];   // s
+                out[5] = gp[5];   // q
+                out[6] = gp[6];   // rho
+                out[7] = gp[7];   // u_LD
+                out[8] = tr[5];   // piEN
+                out[9] = tr[6];   // piEE
+                out[10] = tr[7];  // g1
+                out[11] = tr[8];  // g2
+            }
+        }
+        c = cc;
     }
     if (d_off) (void)hipFree(d_off);
     if (d_buf) (void)hipFree(d_buf);
     if (d_meth) (void)hipFree(d_meth);
     return st;
+}
+
+}  // namespace
+
+extern "C" lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                                       int32_t orbital_motion_mode,
+                                                       const int64_t* offsets, const double* times,
+                                                       int64_t n_epochs, double* mag, int32_t* method)
+{
+    return dynamic_run(batch, dyn, orbital_motion_mode, offsets, times, n_epochs, mag, method, nullptr,
+                       nullptr);
+}
+
+extern "C" lcbg_status lcbg_batch_light_curves_dynamic_grad(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                                            int32_t orbital_motion_mode,
+                                                            const int64_t* offsets, const double* times,
+                                                            int64_t n_epochs, double* mag, int32_t* method,
+                                                            double* mag_aa, double* dmag)
+{
+    if (n_epochs > 0 && !dmag) return LCBG_INVALID_ARGUMENT;
+    return dynamic_run(batch, dyn, orbital_motion_mode, offsets, times, n_epochs, mag, method, mag_aa,
+                       dmag ? dmag : nullptr);
 }
 
 extern "C" lcbg_status lcbg_batch_finite_source_grad_device(lcbg_batch* batch, const int64_t* offsets,

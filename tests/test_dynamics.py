@@ -110,3 +110,40 @@ def test_circular_orbit_period(capi):
 def test_ephemeris_table_errors(capi):
     assert capi.lcbg_load_earth_ephemeris(b"/nonexistent/earth.txt") != 0
     capi.load_earth_ephemeris(TABLE)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_trajectory_jacobian_vs_finite_differences(capi, mode):
+    # forward-mode derivatives of (x, y, s) against central differences of the
+    # same host trajectory (step 1e-4 relative: the nearly edge-on circular
+    # orbits of g3 = 0 lose digits to cancellation, which smaller steps expose);
+    # errors are relative to each component's largest derivative over the epochs
+    rng = np.random.default_rng(20 + mode)
+    for k in range(4):
+        p = _case(rng, mode, True, 3.0 if k % 2 else None)
+        times = np.sort(rng.uniform(10000, 10200, 40))
+        cv = capi.Curve(t0=p["t0"], tE=p["tE"], u0=p["u0"], alpha=p["alpha"], s=p["s"], q=p["q"],
+                        rho=0.0, ld_c=0.0, ld_d=0.0)
+        dy = capi.Dynamics(**{kk: p[kk] for kk in DYN_KEYS})
+        base, jac = capi.dynamic_sources_jacobian(cv, dy, mode, times)
+        assert np.array_equal(base.view(np.int64), capi.dynamic_sources(cv, dy, mode, times).view(np.int64))
+        for j, name in enumerate(capi.TRAJ_PARAMS):
+            v = p[name]
+            h = 1e-5 * p["tE"] if name == "t0" else (1e-4 * abs(v) if v != 0.0 else 1e-8)
+            if mode == 0 and name in ("g1", "g2"):
+                assert np.all(jac[:, :, j] == 0.0)
+                continue
+            outs = []
+            for sgn in (1, -1):
+                q = dict(p)
+                q[name] = v + sgn * h
+                cq = capi.Curve(t0=q["t0"], tE=q["tE"], u0=q["u0"], alpha=q["alpha"], s=q["s"], q=q["q"],
+                                rho=0.0, ld_c=0.0, ld_d=0.0)
+                dq = capi.Dynamics(**{kk: q[kk] for kk in DYN_KEYS})
+                outs.append(capi.dynamic_sources(cq, dq, mode, times))
+            fd = (outs[0] - outs[1]) / (2 * h)
+            # (a floor of 1e-9 for derivatives that vanish identically, such as
+            # those of the Keplerian frame angle with respect to s)
+            scale = np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max(axis=0, keepdims=True) + 1e-9)
+            err = np.abs(jac[:, :, j] - fd) / scale
+            assert np.median(err) < 1e-5 and err.max() < 1e-2, (mode, name, np.median(err), err.max())

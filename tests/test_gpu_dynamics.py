@@ -16,7 +16,7 @@ os.environ.setdefault("LCBININT_EARTH_EPHEMERIS", str(TABLE))
 pytestmark = pytest.mark.gpu
 
 
-def _curves(n_curves, n_epochs, seed, mode):
+def _curves(n_curves, n_epochs, seed, mode, width=0.6):
     from lcbinint_amd.workloads import dynamic_batch
     curves, dyn, _, _ = dynamic_batch(n_curves=n_curves, seed=seed, min_epochs=2, max_epochs=2)
     rng = np.random.default_rng(seed + 100)
@@ -27,7 +27,7 @@ def _curves(n_curves, n_epochs, seed, mode):
             d["lom_szs"] = 0.3
             d["lom_ar"] = 1.5
         # epochs close to the peak, where the finite source matters
-        times.append(np.sort(c["t0"] + c["tE"] * rng.uniform(-0.6, 0.6, n_epochs)))
+        times.append(np.sort(c["t0"] + c["tE"] * rng.uniform(-width, width, n_epochs)))
     offsets = np.arange(n_curves + 1, dtype=np.int64) * n_epochs
     return curves, dyn, offsets, np.concatenate(times)
 
@@ -39,9 +39,11 @@ def test_dynamic_light_curves_match_reference(gpu, mode):
     from oracle.bindings import default_settings, load_ref
     capi.load_earth_ephemeris(TABLE)
     ref = load_ref(str(REF))
-    curves, dyn, offsets, times = _curves(10, 60, 7 + mode, mode)
+    curves, dyn, offsets, times = _curves(12, 80, 7 + mode, mode, width=0.15)
     for c in curves:
-        c["rho"] = max(c["rho"], 2e-3)
+        # large sources close to the lens axis: many finite-source epochs
+        c["rho"] = max(c["rho"], 5e-3)
+        c["u0"] *= 0.1
     b = capi.Batch([capi.Curve(**c) for c in curves])
     dobj = [capi.Dynamics(**d) for d in dyn]
     print(f"dynamic light curves, orbit mode {mode}: {len(times)} epochs", flush=True)
@@ -95,3 +97,51 @@ def test_lightcurve_api_parallax_orbit(gpu):
                                           dict(params, piEN=0.0, piEE=0.0, ra=270.0, dec=-30.0, tfix=t0), 1)
     got0 = lc0(times, params)
     assert np.max(np.abs(got0 - a0) / np.abs(a0)) <= 1e-9
+
+
+@pytest.mark.skipif(not REF.exists(), reason="oracle/_ref not built")
+def test_dynamic_gradients_match_finite_differences(gpu):
+    # derivatives of the magnification with respect to all twelve parameters,
+    # trajectory included, against central differences (epochs whose method
+    # changes across the step are left out; inverse-ray epochs through the
+    # antialiased quadrature, as in the static gradient tests)
+    from lcbinint_amd import capi
+    capi.load_earth_ephemeris(TABLE)
+    curves, dyn, offsets, times = _curves(8, 40, 11, 1, width=0.15)
+    for c in curves:
+        c["rho"] = max(c["rho"], 5e-3)
+        c["u0"] *= 0.1
+    b = capi.Batch([capi.Curve(**c) for c in curves])
+
+    def run(cs, ds):
+        b.update_curves([capi.Curve(**c) for c in cs])
+        return capi.light_curves_dynamic_grad(b, [capi.Dynamics(**d) for d in ds], 1, offsets, times)
+
+    mag, meth, aa, dm = run(curves, dyn)
+    assert np.all(np.isfinite(dm))
+    ir = meth >= 2
+    print(f"dynamic gradient: {len(times)} epochs, {int(ir.sum())} inverse-ray", flush=True)
+    for k, name in enumerate(capi.DYN_GRAD_PARAMS):
+        v = np.array([c[name] if name in c else d[name] for c, d in zip(curves, dyn)])
+        # the nearly edge-on circular orbits of g3 = 0 lose digits to
+        # cancellation in g1 and g2, so those take a larger relative step
+        rel = 1e-4 if name in ("g1", "g2") else 1e-6
+        step = {"t0": 1e-5 * np.array([c["tE"] for c in curves])}.get(name, rel * np.abs(v))
+        step = np.where(step == 0.0, 1e-8, step)
+        out = []
+        for sgn in (1, -1):
+            cs = [dict(c) for c in curves]
+            ds = [dict(d) for d in dyn]
+            for i in range(len(cs)):
+                (cs[i] if name in cs[i] else ds[i])[name] = v[i] + sgn * step[i]
+            out.append(run(cs, ds))
+        h = np.repeat(step, np.diff(offsets))
+        fd = (out[0][2] - out[1][2]) / (2 * h)
+        same = (out[0][1] == meth) & (out[1][1] == meth)
+        scale = np.maximum(np.abs(fd), 1e-3 * np.abs(fd[same]).max() + 1e-12)
+        err = np.abs(dm[:, k] - fd) / scale
+        for sel, med_bar, label in ((same & ~ir, 1e-4, "point source / hexadecapole"),
+                                    (same & ir, 1e-2, "inverse ray")):
+            if sel.sum() >= 5:
+                assert np.median(err[sel]) <= med_bar, (name, label, np.median(err[sel]), err[sel].max())
+    b.close()
