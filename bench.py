@@ -54,7 +54,7 @@ FLOPS_POLAR_CELL = 37.0
 # (profiles/k_ir_batched_fs.md): KiB per dispatch x 1024, FETCH_SIZE scaled by
 # 2.0 and WRITE_SIZE by 1.0, the factors that bring each counter's reading for
 # a 256 MiB device copy to 256 MiB (bench.py --copy-calibration).
-K_IR_CART_TRAFFIC_BYTES = (2.0 * (903555.25 + 58.0) + 1.0 * (1084607.25 + 4.0)) * 1024 / 3
+K_IR_CART_TRAFFIC_BYTES = (2.0 * (567958.125 + 52.0) + 1.0 * (1088506.21875 + 4.0)) * 1024 / 3
 COPY_BYTES = 1 << 28
 
 
@@ -225,6 +225,42 @@ def bench_value_grad(capi, torch, steps, warmup, per_epoch_calls=64):
     }
 
 
+def bench_dynamic(capi, n_curves=128, steps=2):
+    """Light curves with annual parallax and circular orbital motion: the first
+    n_curves curves of lcbinint_amd.workloads.dynamic_batch (the 1024-curve
+    workload), substitute Earth table of tests/golden.  Value, and value plus
+    the derivatives with respect to 12 parameters; host trajectories included
+    in the timing (host arrays in and out)."""
+    from lcbinint_amd.workloads import dynamic_batch
+    table = ROOT / "tests" / "golden" / "earth_ephemeris_substitute.txt"
+    capi.load_earth_ephemeris(table)
+    curves, dyn, off, t = dynamic_batch()
+    curves, dyn = curves[:n_curves], dyn[:n_curves]
+    off = off[:n_curves + 1]
+    t = t[:off[-1]]
+    b = capi.Batch([capi.Curve(**c) for c in curves])
+    d = [capi.Dynamics(**x) for x in dyn]
+    capi.light_curves_dynamic(b, d, capi.ORBIT_CIRCULAR, off, t)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mag, meth = capi.light_curves_dynamic(b, d, capi.ORBIT_CIRCULAR, off, t)
+    value_rate = len(t) * steps / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, _, _, dm = capi.light_curves_dynamic_grad(b, d, capi.ORBIT_CIRCULAR, off, t)
+    grad_rate = len(t) * steps / (time.perf_counter() - t0)
+    b.close()
+    if not (np.isfinite(mag).all() and np.isfinite(dm).all()):
+        raise RuntimeError("non-finite value or derivative in the dynamic workload")
+    return {
+        "workload": f"annual parallax + circular orbital motion: first {n_curves} of 1024 curves "
+                    f"({len(t)} epochs over 200 days), rho log-uniform [1e-4, 1e-2], linear LD, nbin 50, "
+                    "piEN/piEE U(-0.5, 0.5), ds/dt, dalpha/dt U(-1, 1)/yr, seed 5",
+        "value_epochs_per_s": value_rate, "value_grad_epochs_per_s": grad_rate, "grad_params": 12,
+        "methods": np.bincount(meth, minlength=4).tolist(),
+    }
+
+
 def bench_plain(capi, calls_per_epoch=50):
     """The unbatched drop-in path on the README curve: lcbg_light_curve with
     host arrays, once for the whole curve and once per epoch."""
@@ -343,6 +379,8 @@ def main() -> int:
         extras["point_source_batch"] = bench_point_source(capi, torch)
         print("[bench] value and gradient workload", file=sys.stderr, flush=True)
         extras["value_grad"] = bench_value_grad(capi, torch, max(2, args.steps // 2), 1)
+        print("[bench] parallax and orbital motion workload", file=sys.stderr, flush=True)
+        extras["dynamic"] = bench_dynamic(capi)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] reference CPU baseline", file=sys.stderr, flush=True)
