@@ -140,14 +140,29 @@ def bench_batched(capi, torch, steps, warmup):
     work = capi.work_counts(batch)
     meth = np.bincount(d_k.cpu().numpy(), minlength=4)
     finite = bool(torch.isfinite(d_m).all().item())
-    batch.close()
     if not finite:
         raise RuntimeError("non-finite magnification in the batched run")
     stage_ms = np.mean(np.array(stages), axis=0)
     launch_ms = {k: float(np.mean([l[k] for l in launches])) for k in launches[0]}
+    # the same steps replayed from a captured HIP graph (same buffers)
+    gm = torch.empty_like(d_t)
+    gk = torch.empty(n_epochs, dtype=torch.int32, device="cuda")
+    capi.capture_finite_source(batch, d_off.data_ptr(), d_t.data_ptr(), n_epochs, gm.data_ptr(),
+                               gk.data_ptr(), sptr)
+    capi.check(capi.lcbg_batch_update_curves(batch.handle, recs, sptr), "update_curves")
+    capi.launch_graph(batch, sptr)
+    torch.cuda.synchronize()
+    tg = time.perf_counter()
+    for _ in range(steps):
+        capi.check(capi.lcbg_batch_update_curves(batch.handle, recs, sptr), "update_curves")
+        capi.launch_graph(batch, sptr)
+    torch.cuda.synchronize()
+    graph_ms = 1e3 * (time.perf_counter() - tg) / steps
+    graph_same = bool(torch.equal(gm, d_m)) and bool(torch.equal(gk, d_k))
+    batch.close()
     return dict(n_curves=len(curves), n_epochs=n_epochs, wall=wall, steps=steps,
                 stage_ms=stage_ms.tolist(), launch_ms=launch_ms, queue=qc, work=work,
-                methods=meth.tolist())
+                methods=meth.tolist(), graph_ms=graph_ms, graph_same=graph_same)
 
 
 def bench_value_grad(capi, torch, steps, warmup, per_epoch_calls=64):
@@ -412,6 +427,8 @@ def main() -> int:
                          "dispatch_hexadecapole": b["stage_ms"][2],
                          "inverse_ray": b["stage_ms"][3]},
             "parallelism": "one process, one GPU",
+            "graph_replay": {"ms_per_step": b["graph_ms"], "mags_per_s": b["n_epochs"] / (b["graph_ms"] * 1e-3),
+                             "bitwise_equal_to_eager": b["graph_same"]},
         },
         "roofline": {
             # FP64 compute roof: MI355X's FP64 vector and FP64 matrix peaks are

@@ -246,6 +246,18 @@ lcbg_status lcbg_batch_chi2_grad_device(lcbg_batch* batch, const int64_t* offset
                                         int32_t* method, double* mag_aa, double* dmag,
                                         double* chi2, double* grad, void* stream);
 
+/* Records one run of lcbg_batch_finite_source_device with these arguments
+ * (device pointers, the caustic caches always rebuilt) into a HIP graph kept by
+ * the batch, replacing an earlier one; nothing runs.  The reference evaluates
+ * epoch by epoch and has no counterpart.  All workspace is allocated here, so
+ * a replay allocates nothing.  lcbg_batch_launch_graph replays it on `stream`
+ * with the same buffers: a caller updates the curves (lcbg_batch_update_curves
+ * on the same stream) and the times in place between replays. */
+lcbg_status lcbg_batch_capture_finite_source(lcbg_batch* batch, const int64_t* offsets,
+                                             const double* times, int64_t n_epochs, double* mag,
+                                             int32_t* method, void* stream);
+lcbg_status lcbg_batch_launch_graph(lcbg_batch* batch, void* stream);
+
 /* lcbg_batch_finite_source_device, then waits and reports the device time of
  * its four stages (stage_ms[4]: caustic-cache build (0 when the curves did not
  * change), point source + bounding box, caustic distance + hexadecapole,

@@ -320,6 +320,24 @@ def light_curves_dynamic(batch: "Batch", dyn, orbit_mode: int, offsets, times):
     return mag, meth
 
 
+lcbg_batch_capture_finite_source = _sig("lcbg_batch_capture_finite_source", ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p)
+lcbg_batch_launch_graph = _sig("lcbg_batch_launch_graph", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
+
+
+def capture_finite_source(batch: "Batch", offsets_ptr: int, times_ptr: int, n: int, mag_ptr: int,
+                          method_ptr: int, stream: int = 0):
+    """Records one finite-source run (device pointers) into the batch's HIP graph."""
+    check(lcbg_batch_capture_finite_source(batch.handle, offsets_ptr, times_ptr, n, mag_ptr, method_ptr,
+                                           stream), "lcbg_batch_capture_finite_source")
+
+
+def launch_graph(batch: "Batch", stream: int = 0):
+    """Replays the batch's captured finite-source run on `stream`."""
+    check(lcbg_batch_launch_graph(batch.handle, stream), "lcbg_batch_launch_graph")
+
+
 lcbg_batch_ir_kernel_ms = _sig("lcbg_batch_ir_kernel_ms", ctypes.c_int, ctypes.c_void_p,
                                ctypes.POINTER(ctypes.c_float))
 IR_LAUNCHES = ("seeds", "cart_big", "polar_big", "polar_small", "cart_small", "cart_retry",

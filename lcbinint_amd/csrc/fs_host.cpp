@@ -378,9 +378,13 @@ struct GradArgs {
     double* grad;
 };
 
+// One finite-source run enqueued on `stream`.  capture: the run is being
+// recorded into a HIP graph, so no host waits, no timing events, the caustic
+// caches are always rebuilt, and the workspace hand-over events are left to
+// the graph launch.
 lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, int64_t n_epochs,
                    double* mag, int32_t* method, hipStream_t stream, float* stage_ms,
-                   const GradArgs* ga = nullptr)
+                   const GradArgs* ga = nullptr, bool capture = false)
 {
     lcbg_status st = fs_prepare(b);
     if (st != LCBG_OK) return st;
@@ -422,7 +426,7 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     if (st != LCBG_OK) return st;
     // LCBG_STAGE_SYNC=1: wait after every stage and report it on stderr
     // (a diagnostic in the spirit of the reference's LCBININT_AREA_DIAGNOSTICS)
-    const bool stage_sync = std::getenv("LCBG_STAGE_SYNC") != nullptr;
+    const bool stage_sync = !capture && std::getenv("LCBG_STAGE_SYNC") != nullptr;
     auto stage_done = [&](const char* name) {
         if (!stage_sync || st != LCBG_OK) return;
         st = map_hip(hipStreamSynchronize(stream));
@@ -432,7 +436,7 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
                      static_cast<int>(st), h[0], h[2], h[4]);
         std::fflush(stderr);
     };
-    st = map_hip(hipStreamWaitEvent(stream, pool.last_use, 0));
+    if (!capture) st = map_hip(hipStreamWaitEvent(stream, pool.last_use, 0));
     if (st == LCBG_OK) st = launch_reset_counters(ep.counters, kCounterWords, stream);
     if (st == LCBG_OK) st = launch_fs_stage1(b->d_curves, b->n_curves, fs->cache, fs->opt, ep, stream);
     stage_done("stage 1 (point source, bounding box)");
@@ -447,7 +451,7 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     if (ga && st == LCBG_OK)
         st = launch_fs_grad(b->d_curves, b->n_curves, fs->opt, ep, lists_view(fs), pool.scr, ga->mag_aa,
                             ga->dmag, ga->flux, ga->sigma, ga->fs, ga->fb, ga->chi2, ga->grad, stream);
-    if (st == LCBG_OK) st = map_hip(hipEventRecord(pool.last_use, stream));
+    if (st == LCBG_OK && !capture) st = map_hip(hipEventRecord(pool.last_use, stream));
     if (stage_ms && st == LCBG_OK) {
         (void)hipEventRecord(ev[4], stream);
         (void)hipEventSynchronize(ev[4]);
@@ -486,6 +490,57 @@ extern "C" lcbg_status lcbg_batch_finite_source_device(lcbg_batch* batch, const 
     if (!offsets || !times || !mag || !method) return LCBG_INVALID_ARGUMENT;
     return fs_run(batch, offsets, times, n_epochs, mag, method, static_cast<hipStream_t>(stream),
                   nullptr);
+}
+
+extern "C" lcbg_status lcbg_batch_capture_finite_source(lcbg_batch* batch, const int64_t* offsets,
+                                                         const double* times, int64_t n_epochs,
+                                                         double* mag, int32_t* method, void* stream)
+{
+    (void)stream;
+    if (!batch || n_epochs <= 0 || batch->n_curves == 0) return LCBG_INVALID_ARGUMENT;
+    if (!offsets || !times || !mag || !method) return LCBG_INVALID_ARGUMENT;
+    // everything the run allocates, allocated before the capture
+    lcbg_status st = fs_prepare(batch);
+    if (st == LCBG_OK) st = ensure_epochs(batch->fs, n_epochs);
+    if (st == LCBG_OK) {
+        ScratchPool& pool = g_pool[batch->device % kMaxDevices];
+        std::lock_guard<std::mutex> lock(pool.mu);
+        st = ensure_scratch(pool);
+    }
+    if (st != LCBG_OK) return st;
+    if (batch->graph) {
+        (void)hipGraphExecDestroy(batch->graph);
+        batch->graph = nullptr;
+    }
+    if (!batch->capture_stream) {
+        st = map_hip(hipStreamCreateWithFlags(&batch->capture_stream, hipStreamNonBlocking));
+        if (st != LCBG_OK) return st;
+    }
+    hipStream_t cs = batch->capture_stream;
+    st = map_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    if (st != LCBG_OK) return st;
+    batch->caches_stale = true;
+    const lcbg_status run = fs_run(batch, offsets, times, n_epochs, mag, method, cs, nullptr, nullptr, true);
+    hipGraph_t g = nullptr;
+    st = map_hip(hipStreamEndCapture(cs, &g));  // ends the capture on every path
+    if (st == LCBG_OK) st = run;
+    if (st == LCBG_OK) st = map_hip(hipGraphInstantiate(&batch->graph, g, nullptr, nullptr, 0));
+    if (g) (void)hipGraphDestroy(g);
+    batch->caches_stale = true;  // nothing has run yet
+    return st;
+}
+
+extern "C" lcbg_status lcbg_batch_launch_graph(lcbg_batch* batch, void* stream)
+{
+    if (!batch || !batch->graph) return LCBG_INVALID_ARGUMENT;
+    const hipStream_t st_ = static_cast<hipStream_t>(stream);
+    ScratchPool& pool = g_pool[batch->device % kMaxDevices];
+    std::lock_guard<std::mutex> lock(pool.mu);
+    lcbg_status st = map_hip(hipStreamWaitEvent(st_, pool.last_use, 0));
+    if (st == LCBG_OK) st = map_hip(hipGraphLaunch(batch->graph, st_));
+    if (st == LCBG_OK) st = map_hip(hipEventRecord(pool.last_use, st_));
+    if (st == LCBG_OK) batch->caches_stale = false;  // the graph rebuilt them
+    return st;
 }
 
 namespace lcbg {

@@ -138,6 +138,8 @@ extern "C" lcbg_status lcbg_batch_destroy(lcbg_batch* batch)
 {
     if (!batch) return LCBG_OK;
     if (batch->dyn_child) lcbg_batch_destroy(batch->dyn_child);
+    if (batch->graph) (void)hipGraphExecDestroy(batch->graph);
+    if (batch->capture_stream) (void)hipStreamDestroy(batch->capture_stream);
     if (batch->fs) fs_destroy(batch->fs);
     if (batch->d_curves) (void)hipFree(batch->d_curves);
     delete batch;

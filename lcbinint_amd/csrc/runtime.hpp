@@ -57,4 +57,6 @@ struct lcbg_batch {
     std::vector<lcbg_curve> h_params;    // the curves as passed (for host trajectories)
     bool dynamic_images = false;         // point-source sum and seeds of the LensModel path
     lcbg_batch* dyn_child = nullptr;     // per-epoch pseudo-curves of the dynamic path
+    hipGraphExec_t graph = nullptr;      // captured finite-source run
+    hipStream_t capture_stream = nullptr;
 };
