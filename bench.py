@@ -106,7 +106,7 @@ def cpu_baseline(curves, offsets, times, nbin, stride=2):
     }
 
 
-def bench_batched(capi, torch, steps, warmup):
+def bench_batched(capi, torch, steps, warmup, autotune=True):
     from lcbinint_amd.workloads import finite_source_batch
     curves, offsets, times, nbin = finite_source_batch()
     n_epochs = len(times)
@@ -124,6 +124,9 @@ def bench_batched(capi, torch, steps, warmup):
         return capi.finite_source(batch, d_off.data_ptr(), d_t.data_ptr(), n_epochs,
                                   d_m.data_ptr(), d_k.data_ptr(), sptr, profile=True)
 
+    # launch schedule chosen by the autotuner on this workload (untimed)
+    schedule, tuned_ms = capi.autotune(batch, d_off.data_ptr(), d_t.data_ptr(), n_epochs, d_m.data_ptr(),
+                                       d_k.data_ptr(), sptr, max_runs=18) if autotune else (None, None)
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
@@ -162,7 +165,8 @@ def bench_batched(capi, torch, steps, warmup):
     batch.close()
     return dict(n_curves=len(curves), n_epochs=n_epochs, wall=wall, steps=steps,
                 stage_ms=stage_ms.tolist(), launch_ms=launch_ms, queue=qc, work=work,
-                methods=meth.tolist(), graph_ms=graph_ms, graph_same=graph_same)
+                methods=meth.tolist(), graph_ms=graph_ms, graph_same=graph_same,
+                schedule=schedule.as_dict() if schedule is not None else None, tuned_ms=tuned_ms)
 
 
 def bench_value_grad(capi, torch, steps, warmup, per_epoch_calls=64):
@@ -341,6 +345,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the plain-path and point-source measurements")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch schedule")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device copies of 256 MiB (the reference reading for "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -366,7 +371,7 @@ def main() -> int:
         print(json.dumps({"copy_bytes": COPY_BYTES, "copies": 3}))
         return 0
     print("[bench] batched finite-source workload", file=sys.stderr, flush=True)
-    b = bench_batched(capi, torch, args.steps, args.warmup)
+    b = bench_batched(capi, torch, args.steps, args.warmup, autotune=not args.no_autotune)
     value = b["n_epochs"] * b["steps"] / b["wall"]
     w = b["work"]
     ir_flops = (w["quintic"] * FLOPS_PS_SOLVE + w["quartic"] * FLOPS_CRIT_SOLVE +
@@ -427,6 +432,7 @@ def main() -> int:
                          "dispatch_hexadecapole": b["stage_ms"][2],
                          "inverse_ray": b["stage_ms"][3]},
             "parallelism": "one process, one GPU",
+            "schedule": {"autotuned": b["schedule"], "tuning_run_ms": b["tuned_ms"]},
             "graph_replay": {"ms_per_step": b["graph_ms"], "mags_per_s": b["n_epochs"] / (b["graph_ms"] * 1e-3),
                              "bitwise_equal_to_eager": b["graph_same"]},
         },

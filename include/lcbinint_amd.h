@@ -246,6 +246,33 @@ lcbg_status lcbg_batch_chi2_grad_device(lcbg_batch* batch, const int64_t* offset
                                         int32_t* method, double* mag_aa, double* dmag,
                                         double* chi2, double* grad, void* stream);
 
+/* Launch schedule of the batch's inverse-ray stage (no counterpart in the
+ * reference, which runs one epoch at a time).  Zero fields take the defaults:
+ * resident waves per CU for the seed search and the small-slot Cartesian and
+ * polar scans (default: as many as the slots the device's occupancy gives),
+ * the inside positions a first-crossing scan probes before the probe farm
+ * takes it over (default 16), and the |A_ps| * nbin above which an epoch starts
+ * on a large slot (default 8192).  Results do not depend on the schedule. */
+typedef struct lcbg_schedule {
+    int32_t seed_waves_per_cu;
+    int32_t cart_waves_per_cu;
+    int32_t polar_waves_per_cu;
+    int32_t heavy_budget;
+    double big_threshold;
+} lcbg_schedule;
+
+lcbg_status lcbg_batch_set_schedule(lcbg_batch* batch, const lcbg_schedule* schedule);
+lcbg_status lcbg_batch_get_schedule(const lcbg_batch* batch, lcbg_schedule* schedule);
+
+/* Autotuner: runs lcbg_batch_finite_source_device on the given workload with
+ * candidate schedules (one pass of coordinate descent over the fields of
+ * lcbg_schedule, each candidate timed over one run with HIP events, at most
+ * max_runs runs), keeps the fastest on the batch and writes it to *best and
+ * its run time to *best_ms.  mag and method receive the last run's results. */
+lcbg_status lcbg_batch_autotune(lcbg_batch* batch, const int64_t* offsets, const double* times,
+                                int64_t n_epochs, double* mag, int32_t* method, void* stream,
+                                int32_t max_runs, lcbg_schedule* best, float* best_ms);
+
 /* Records one run of lcbg_batch_finite_source_device with these arguments
  * (device pointers, the caustic caches always rebuilt) into a HIP graph kept by
  * the batch, replacing an earlier one; nothing runs.  The reference evaluates

@@ -320,6 +320,40 @@ def light_curves_dynamic(batch: "Batch", dyn, orbit_mode: int, offsets, times):
     return mag, meth
 
 
+class Schedule(ctypes.Structure):
+    """lcbg_schedule: launch schedule of the inverse-ray stage (0 = default)."""
+    _fields_ = [("seed_waves_per_cu", ctypes.c_int32), ("cart_waves_per_cu", ctypes.c_int32),
+                ("polar_waves_per_cu", ctypes.c_int32), ("heavy_budget", ctypes.c_int32),
+                ("big_threshold", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+lcbg_batch_set_schedule = _sig("lcbg_batch_set_schedule", ctypes.c_int, ctypes.c_void_p,
+                               ctypes.POINTER(Schedule))
+lcbg_batch_get_schedule = _sig("lcbg_batch_get_schedule", ctypes.c_int, ctypes.c_void_p,
+                               ctypes.POINTER(Schedule))
+lcbg_batch_autotune = _sig("lcbg_batch_autotune", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Schedule),
+                           ctypes.POINTER(ctypes.c_float))
+
+
+def autotune(batch: "Batch", offsets_ptr: int, times_ptr: int, n: int, mag_ptr: int, method_ptr: int,
+             stream: int = 0, max_runs: int = 24):
+    """Runs the autotuner on this workload; returns (Schedule, best run ms)."""
+    best = Schedule()
+    ms = ctypes.c_float()
+    check(lcbg_batch_autotune(batch.handle, offsets_ptr, times_ptr, n, mag_ptr, method_ptr, stream,
+                              max_runs, ctypes.byref(best), ctypes.byref(ms)), "lcbg_batch_autotune")
+    return best, ms.value
+
+
+def set_schedule(batch: "Batch", schedule: Schedule):
+    check(lcbg_batch_set_schedule(batch.handle, ctypes.byref(schedule)), "lcbg_batch_set_schedule")
+
+
 lcbg_batch_capture_finite_source = _sig("lcbg_batch_capture_finite_source", ctypes.c_int, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p)

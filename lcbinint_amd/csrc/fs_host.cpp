@@ -167,6 +167,7 @@ double* dint_view(FsState* fs)
 struct ScratchPool {
     std::mutex mu;
     IrScratch scr{};
+    int cus = 0;
     void* mem = nullptr;
     hipEvent_t last_use = nullptr;
 };
@@ -191,6 +192,7 @@ lcbg_status ensure_scratch(ScratchPool& pool)
     const int polar_slots = cus;  // few polar epochs; they share the CUs with the Cartesian pass
     const int large_slots = 64;
     IrScratch& s = pool.scr;
+    pool.cus = cus;
     s.seed.n_slots = seed_slots;
     s.farm_slots = cus * ir_resident_per_cu(3);
     s.cart.n_slots = cart_slots;
@@ -293,7 +295,8 @@ lcbg_status fs_prepare(lcbg_batch* b)
     fs->opt.hex_threshold = o.hexadecapole_threshold;
     fs->opt.adaptive_hex = o.adaptive_hex_threshold;
     fs->opt.ps_images = b->dynamic_images ? 1 : 0;
-    fs->opt.pad = 0;
+    fs->opt.heavy_budget = kHeavyBudget;
+    fs->opt.big_threshold = 8192.0;
     const int bins = o.caustic_bins > 32 ? o.caustic_bins : 32;
 
     // angle tables, each expression as the reference writes it
@@ -444,9 +447,21 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     if (st == LCBG_OK) st = launch_fs_stage2(b->d_curves, fs->cache, fs->opt, ep, fs->stage2_waves, stream);
     stage_done("stage 2 (caustic distance, hexadecapole)");
     if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[3], stream);
+    // the batch's schedule: fewer resident waves than slots, the deferral
+    // budget and the large-slot threshold
+    IrScratch scr = pool.scr;
+    const lcbg_schedule& sc = b->schedule;
+    auto waves = [&](int per_cu, int alloc) {
+        const int n = per_cu > 0 ? per_cu * pool.cus : alloc;
+        return n < alloc ? n : alloc;
+    };
+    scr.seed.n_slots = waves(sc.seed_waves_per_cu, scr.seed.n_slots);
+    scr.cart.n_slots = waves(sc.cart_waves_per_cu, scr.cart.n_slots);
+    scr.polar.n_slots = waves(sc.polar_waves_per_cu, scr.polar.n_slots);
+    fs->opt.heavy_budget = sc.heavy_budget > 0 ? sc.heavy_budget : kHeavyBudget;
+    fs->opt.big_threshold = sc.big_threshold > 0.0 ? sc.big_threshold : 8192.0;
     if (st == LCBG_OK)
-        st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, lists_view(fs), pool.scr, stream,
-                          fs->ir);
+        st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, lists_view(fs), scr, stream, fs->ir);
     stage_done("stage 3 (inverse ray)");
     if (ga && st == LCBG_OK)
         st = launch_fs_grad(b->d_curves, b->n_curves, fs->opt, ep, lists_view(fs), pool.scr, ga->mag_aa,
@@ -490,6 +505,77 @@ extern "C" lcbg_status lcbg_batch_finite_source_device(lcbg_batch* batch, const 
     if (!offsets || !times || !mag || !method) return LCBG_INVALID_ARGUMENT;
     return fs_run(batch, offsets, times, n_epochs, mag, method, static_cast<hipStream_t>(stream),
                   nullptr);
+}
+
+extern "C" lcbg_status lcbg_batch_set_schedule(lcbg_batch* batch, const lcbg_schedule* schedule)
+{
+    if (!batch || !schedule) return LCBG_INVALID_ARGUMENT;
+    if (schedule->seed_waves_per_cu < 0 || schedule->cart_waves_per_cu < 0 ||
+        schedule->polar_waves_per_cu < 0 || schedule->heavy_budget < 0 || !(schedule->big_threshold >= 0.0))
+        return LCBG_INVALID_ARGUMENT;
+    batch->schedule = *schedule;
+    return LCBG_OK;
+}
+
+extern "C" lcbg_status lcbg_batch_get_schedule(const lcbg_batch* batch, lcbg_schedule* schedule)
+{
+    if (!batch || !schedule) return LCBG_INVALID_ARGUMENT;
+    *schedule = batch->schedule;
+    return LCBG_OK;
+}
+
+extern "C" lcbg_status lcbg_batch_autotune(lcbg_batch* batch, const int64_t* offsets, const double* times,
+                                           int64_t n_epochs, double* mag, int32_t* method, void* stream,
+                                           int32_t max_runs, lcbg_schedule* best, float* best_ms)
+{
+    if (!batch || !best || !best_ms || max_runs < 1) return LCBG_INVALID_ARGUMENT;
+    const hipStream_t stream_ = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    lcbg_status st = map_hip(hipEventCreate(&e0));
+    if (st == LCBG_OK) st = map_hip(hipEventCreate(&e1));
+    int runs = 0;
+    // one run of the workload under the batch's current schedule
+    auto timed = [&](float* ms) -> lcbg_status {
+        lcbg_status r = map_hip(hipEventRecord(e0, stream_));
+        if (r == LCBG_OK) r = lcbg_batch_finite_source_device(batch, offsets, times, n_epochs, mag, method, stream);
+        if (r == LCBG_OK) r = map_hip(hipEventRecord(e1, stream_));
+        if (r == LCBG_OK) r = map_hip(hipEventSynchronize(e1));
+        if (r == LCBG_OK) r = map_hip(hipEventElapsedTime(ms, e0, e1));
+        ++runs;
+        return r;
+    };
+    if (st == LCBG_OK) st = timed(best_ms);  // warm-up: caches, workspace
+    lcbg_schedule cur = batch->schedule;
+    if (st == LCBG_OK && runs < max_runs) st = timed(best_ms);
+    // candidates per field, tried one field at a time with the others fixed
+    const int32_t waves[] = {1, 2, 3, 4};
+    const int32_t budgets[] = {4, 8, 16, 32, 64};
+    const double thresholds[] = {2048.0, 4096.0, 8192.0, 16384.0, 65536.0};
+    for (int field = 0; field < 5 && st == LCBG_OK; ++field) {
+        const int n_cand = field == 3 ? 5 : (field == 4 ? 5 : 4);
+        for (int k = 0; k < n_cand && st == LCBG_OK && runs < max_runs; ++k) {
+            lcbg_schedule trial = cur;
+            switch (field) {
+            case 0: trial.seed_waves_per_cu = waves[k]; break;
+            case 1: trial.cart_waves_per_cu = waves[k]; break;
+            case 2: trial.polar_waves_per_cu = waves[k]; break;
+            case 3: trial.heavy_budget = budgets[k]; break;
+            default: trial.big_threshold = thresholds[k]; break;
+            }
+            batch->schedule = trial;
+            float ms = 0.0f;
+            st = timed(&ms);
+            if (st == LCBG_OK && ms < *best_ms) {
+                *best_ms = ms;
+                cur = trial;
+            }
+        }
+    }
+    batch->schedule = cur;
+    *best = cur;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return st;
 }
 
 extern "C" lcbg_status lcbg_batch_capture_finite_source(lcbg_batch* batch, const int64_t* offsets,

@@ -1614,7 +1614,7 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
         init_ctx(C, cv, ep.src[e], slot_seeds);
         C.hints = opt.ps_images != 0;
         if (pass == 0) {
-            C.defer_budget = kHeavyBudget;
+            C.defer_budget = opt.heavy_budget;
             C.heavy_count = &ep.counters[kQHeavyCount];
         } else {
             C.farm = hv.flags + static_cast<size_t>(task) * kScanPositions * kProbesPerPosition;
@@ -1653,7 +1653,7 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
             const bool polar = uni(ep.method[e]) == LCBG_METHOD_INVERSE_RAY_POLAR;
             // an image of |A| times the source's area scans about 2 |A| nbin
             // rows; such epochs start on a large slot right away
-            const bool large = fabs(ep.a_ps[e]) * static_cast<double>(curve_nbin(cv, opt)) > 8192.0;
+            const bool large = fabs(ep.a_ps[e]) * static_cast<double>(curve_nbin(cv, opt)) > opt.big_threshold;
             const int qword = polar ? (large ? kQPolarBigCount : kQPolarCount)
                                     : (large ? kQCartBigCount : kQCartCount);
             int* qarr = polar ? (large ? L.polar_big : L.polar_q) : (large ? L.cart_big : L.cart_q);

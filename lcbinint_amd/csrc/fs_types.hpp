@@ -88,7 +88,11 @@ struct FsOptions {
     // (sum over binary_images, seeds filtered by append_valid_seed), used for
     // curves with parallax or orbital motion; 0: binary_mag0 of the static path
     int ps_images;
-    int pad;
+    // schedule (lcbg_schedule): inside positions a seed search probes before the
+    // probe farm takes it over, and the |A_ps| nbin above which an epoch starts
+    // on a large slot
+    int heavy_budget;
+    double big_threshold;
 };
 
 struct FsEpochs {

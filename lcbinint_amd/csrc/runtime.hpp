@@ -57,6 +57,7 @@ struct lcbg_batch {
     std::vector<lcbg_curve> h_params;    // the curves as passed (for host trajectories)
     bool dynamic_images = false;         // point-source sum and seeds of the LensModel path
     lcbg_batch* dyn_child = nullptr;     // per-epoch pseudo-curves of the dynamic path
+    lcbg_schedule schedule{};            // launch schedule (0 fields: defaults)
     hipGraphExec_t graph = nullptr;      // captured finite-source run
     hipStream_t capture_stream = nullptr;
 };

@@ -65,3 +65,41 @@ def test_graph_replays_match_eager(gpu):
         torch.cuda.synchronize()
         assert np.array_equal(em.cpu().numpy().view(np.int64), ref[0].view(np.int64)), rep
     g.close()
+
+
+def test_autotuner_keeps_results(gpu):
+    # the schedule changes which waves run which epochs, never the values
+    import torch
+    from lcbinint_amd import capi
+    cs, off, t, nbin = _subset(128, 32)
+    n = len(t)
+    stream = torch.cuda.current_stream().cuda_stream
+    d_off = torch.from_numpy(off).cuda()
+    d_t = torch.from_numpy(t).cuda()
+    b = capi.Batch([capi.Curve(**c) for c in cs])
+    b.set_source_bins(nbin)
+    m0 = torch.empty(n, dtype=torch.float64, device="cuda")
+    k0 = torch.empty(n, dtype=torch.int32, device="cuda")
+    capi.finite_source(b, d_off.data_ptr(), d_t.data_ptr(), n, m0.data_ptr(), k0.data_ptr(), stream)
+    torch.cuda.synchronize()
+    m1 = torch.empty_like(m0)
+    k1 = torch.empty_like(k0)
+    print("autotuning", flush=True)
+    best, ms = capi.autotune(b, d_off.data_ptr(), d_t.data_ptr(), n, m1.data_ptr(), k1.data_ptr(), stream,
+                             max_runs=10)
+    torch.cuda.synchronize()
+    print("schedule", best.as_dict(), "run ms", ms, flush=True)
+    assert ms > 0.0
+    # the tuned schedule stays on the batch; its results equal the default's
+    capi.finite_source(b, d_off.data_ptr(), d_t.data_ptr(), n, m1.data_ptr(), k1.data_ptr(), stream)
+    torch.cuda.synchronize()
+    assert torch.equal(k0, k1)
+    assert np.array_equal(m0.cpu().numpy().view(np.int64), m1.cpu().numpy().view(np.int64))
+    # an extreme schedule: one wave per CU everywhere, the farm taking every scan
+    capi.set_schedule(b, capi.Schedule(seed_waves_per_cu=1, cart_waves_per_cu=1, polar_waves_per_cu=1,
+                                       heavy_budget=1, big_threshold=1.0))
+    capi.finite_source(b, d_off.data_ptr(), d_t.data_ptr(), n, m1.data_ptr(), k1.data_ptr(), stream)
+    torch.cuda.synchronize()
+    assert torch.equal(k0, k1)
+    assert np.array_equal(m0.cpu().numpy().view(np.int64), m1.cpu().numpy().view(np.int64))
+    b.close()
