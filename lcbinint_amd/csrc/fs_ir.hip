@@ -696,14 +696,19 @@ __device__ __forceinline__ double seq_lanes2(double xa, double sa, double xb, do
 
 // The reference keeps, per seed, an unordered_map from row key to the rows
 // pushed with that key (finite_source_magnifier.cpp:917, 1989-1999).  Here the
-// map is a chained hash: head[key & (kRowHash-1)] -> newest row, next[] links.
+// map is a chained hash: head[key & (kRowHash-1)] -> newest node, node.next ->
+// older.  A node also holds the row's final xmin/xmax (a pushed row's record
+// never changes afterwards), so a duplicate check reads one node per step.
 // `dirty` (in memory, so it survives an abandoned epoch) counts the keys whose
 // heads are set; they are cleared before the next seed's scan.
+struct RowNode {
+    int key, row, next, pad;
+    double xmin, xmax;
+};
+
 struct Rows {
     double* r;     // [row*5 + field]
-    int* key;      // [row_cap] pushed row keys
-    int* idx;      // [row_cap] pushed row indices
-    int* next;     // [row_cap] older push with the same hash, -1 at the end
+    RowNode* node; // [row_cap] pushed rows in push order
     int* head;     // [kRowHash]
     int* dirty;    // keys pushed since the heads were last cleared (<0: none)
     int size;      // rows allocated (ensure)
@@ -714,7 +719,7 @@ __device__ __forceinline__ void rows_reset(Ctx& C, Rows& R)
 {
     wave_sync();
     const int nd = uni(*R.dirty);
-    for (int k = C.lane; k < nd; k += kWave) R.head[R.key[k] & (kRowHash - 1)] = -1;
+    for (int k = C.lane; k < nd; k += kWave) R.head[R.node[k].key & (kRowHash - 1)] = -1;
     wave_sync();
     wave_store(R.dirty, 0);
     wave_sync();
@@ -817,6 +822,10 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
     double cur_xmax = uni(rget(R, yi, XMAX));
     double prev_xmin = yi != 0 ? uni(rget(R, yi - 1, XMIN)) : 0.0;
     for (;;) {
+        // the row's key and its chain head, loaded now for the duplicate check at
+        // the row's close (only this row's own push changes the heads)
+        const int row_key = static_cast<int>(y * K.inv_incr + 2000000);
+        const int head_pf = R.head[row_key & (kRowHash - 1)];
         // ---------------------------------------------------------- right scan
         int stash_n = 0;      // left-scan cells held in lanes [64 - stash_n, 64)
         double sx = 0.0, sw = 0.0;
@@ -949,14 +958,15 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
         rset(C, R, yi, YY, y);
         rset(C, R, yi, DYS, countx == 0.0 ? -dy : dy);
         if (countx == 0.0) return countall;
-        const int key = static_cast<int>(y * K.inv_incr + 2000000);
+        const int key = row_key;
+        const int newest = uni(head_pf);
         {
             bool dup = false;
-            for (int p = uni(R.head[key & (kRowHash - 1)]); p >= 0 && !dup; p = uni(R.next[p])) {
-                if (uni(R.key[p]) == key) {
-                    const int e = uni(R.idx[p]);
-                    dup = cur_xmin + incr < rget(R, e, XMAX) && cur_xmax - incr > rget(R, e, XMIN);
-                }
+            for (int p = newest; p >= 0 && !dup;) {
+                const RowNode nd = R.node[p];
+                if (uni(nd.key) == key)
+                    dup = cur_xmin + incr < uni(nd.xmax) && cur_xmax - incr > uni(nd.xmin);
+                p = uni(nd.next);
             }
             if (dup) return countall - countx;
         }
@@ -967,12 +977,15 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
         }
         wave_sync();
         {
-            const int h = key & (kRowHash - 1);
-            const int older = uni(R.head[h]);
-            wave_store(&R.key[R.pushed], key);
-            wave_store(&R.idx[R.pushed], yi);
-            wave_store(&R.next[R.pushed], older);
-            wave_store(&R.head[h], R.pushed);
+            RowNode nd;
+            nd.key = key;
+            nd.row = yi;
+            nd.next = newest;
+            nd.pad = 0;
+            nd.xmin = cur_xmin;
+            nd.xmax = cur_xmax;
+            if (C.lane == 0) R.node[R.pushed] = nd;
+            wave_store(&R.head[key & (kRowHash - 1)], R.pushed);
             wave_store(R.dirty, R.pushed + 1);
         }
         R.pushed = uni(R.pushed + 1);
@@ -1025,11 +1038,9 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* 
         if (flags[i] & 1) continue;
         Rows R;
         R.r = rows_mem;
-        R.key = rows_key;
-        R.idx = rows_key + C.row_cap;
-        R.next = rows_key + 2 * C.row_cap;
-        R.head = rows_key + 3 * C.row_cap;
-        R.dirty = rows_key + 3 * C.row_cap + kRowHash;
+        R.node = reinterpret_cast<RowNode*>(rows_key);
+        R.head = rows_key + 8 * C.row_cap;
+        R.dirty = rows_key + 8 * C.row_cap + kRowHash;
         rows_reset(C, R);
         if (!rows_ensure(C, R, 1)) return __builtin_nan("");
         int yi = 0;

@@ -44,7 +44,9 @@ constexpr int kPhiCap = 1 << 30;
 constexpr int kPhiHash = 1 << 16;
 // ints per slot of CartScratch::row_key: key, idx, next per row, the hash
 // heads, and the count of keys whose heads are still set (padded to 64)
-constexpr long long row_ints_per_slot(int row_cap) { return 3ll * row_cap + kRowHash + 64; }
+// Row-key nodes (8 ints each: key, row, next, pad, xmin, xmax as doubles),
+// the chain heads, the dirty count.
+constexpr long long row_ints_per_slot(int row_cap) { return 8ll * row_cap + kRowHash + 64; }
 // cost buckets of the inverse-ray work order (largest a_ps * nbin^2 first)
 constexpr int kCostBuckets = 32;
 
@@ -146,7 +148,7 @@ struct CartScratch {
     double* areaimage; // [slot][kSeedCap]
     int* overlap;      // [slot][kSeedCap]
     double* rows;      // [slot][row_cap*5]: xmin, xmax, ax, y, dys
-    int* row_key;      // [slot][row_ints_per_slot(row_cap)]: key, idx, next, heads, dirty
+    int* row_key;      // [slot][row_ints_per_slot(row_cap)]: nodes, heads, dirty
 };
 
 struct PolarScratch {
