@@ -55,6 +55,11 @@ FLOPS_POLAR_CELL = 37.0
 # 2.0 and WRITE_SIZE by 1.0, the factors that bring each counter's reading for
 # a 256 MiB device copy to 256 MiB (bench.py --copy-calibration).
 K_IR_CART_TRAFFIC_BYTES = (2.0 * (567958.125 + 52.0) + 1.0 * (1088506.21875 + 4.0)) * 1024 / 3
+# FP64 lane operations the hardware issues per k_ir_cart launch on the same
+# workload (rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64, 64 lanes per
+# wave instruction, FMA as 2; profiles/k_ir_batched_fs.md): an upper bound on
+# the executed FP64 operations, reported beside the algorithmic rate.
+K_IR_CART_HW_FP64_FLOP = 3.964839e+11
 COPY_BYTES = 1 << 28
 
 
@@ -450,6 +455,8 @@ def main() -> int:
             "launches_per_step": 3,
             "algorithmic_flops_per_launch": cart_flops,
             "achieved_hbm_gbps": K_IR_CART_TRAFFIC_BYTES / (cart_ms * 1e-3) / 1e9,
+            "hw_fp64_tflops": K_IR_CART_HW_FP64_FLOP / (cart_ms * 1e-3) / 1e12,
+            "hw_fp64_frac": K_IR_CART_HW_FP64_FLOP / (cart_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
             "inverse_ray_stage_ms": ir_ms,
             "inverse_ray_stage_flops": ir_flops,
             "launch_ms": lm,
