@@ -85,3 +85,72 @@ class FiniteSourceLightCurves:
 
     def close(self):
         self.batch.close()
+
+
+DYNAMIC_PARAM_NAMES = capi.DYN_GRAD_PARAMS  # t0, tE, u0, alpha, s, q, rho, ld_c, piEN, piEE, g1, g2
+
+
+class _DynamicLightCurveFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, params, owner):
+        host = params.detach().to("cpu", torch.float64).numpy()
+        curves = [capi.Curve(t0=r[0], tE=r[1], u0=r[2], alpha=r[3], s=r[4], q=r[5], rho=r[6], ld_c=r[7],
+                             ld_d=0.0) for r in host]
+        dyn = [capi.Dynamics(piEN=r[8], piEE=r[9], g1=r[10], g2=r[11], **fixed)
+               for r, fixed in zip(host, owner.fixed)]
+        owner.batch.update_curves(curves)
+        mag, method, _, dmag = capi.light_curves_dynamic_grad(owner.batch, dyn, owner.orbit_mode,
+                                                              owner.offsets, owner.times)
+        owner.last_method = method
+        d = torch.from_numpy(dmag).to(params.device)
+        ctx.save_for_backward(d)
+        ctx.owner = owner
+        return torch.from_numpy(mag).to(params.device)
+
+    @staticmethod
+    def backward(ctx, grad_mag):
+        (dmag,) = ctx.saved_tensors
+        owner = ctx.owner
+        contrib = grad_mag.to(torch.float64)[:, None] * dmag
+        grad = torch.zeros((owner.n_curves, len(DYNAMIC_PARAM_NAMES)), dtype=torch.float64,
+                           device=dmag.device)
+        grad.index_add_(0, owner.curve_of.to(dmag.device), contrib)
+        return grad, None
+
+
+class DynamicLightCurves:
+    """Light curves with annual parallax and/or orbital motion as a
+    differentiable function of the (n_curves, 12) parameter tensor (columns
+    DYNAMIC_PARAM_NAMES: t0, tE, u0, alpha, s, q, rho, u_LD, piEN, piEE, g1, g2).
+    `fixed` holds per curve the parameters that are not differentiated (ra,
+    dec, tfix, g3, lom_szs, lom_ar).  Values and derivatives come from
+    lcbg_batch_light_curves_dynamic_grad (host trajectories, device
+    magnifications); parallax needs the Earth table
+    (capi.load_earth_ephemeris)."""
+
+    def __init__(self, offsets, times, fixed, orbit_mode=capi.ORBIT_CIRCULAR, nbin=None,
+                 options: capi.Options | None = None):
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.times = np.ascontiguousarray(times, dtype=np.float64)
+        self.n_curves = len(self.offsets) - 1
+        if len(fixed) != self.n_curves:
+            raise ValueError(f"fixed needs {self.n_curves} entries, got {len(fixed)}")
+        keys = ("ra", "dec", "tfix", "g3", "lom_szs", "lom_ar")
+        self.fixed = [{k: float(f.get(k, 1.0 if k == "lom_ar" else 0.0)) for k in keys} for f in fixed]
+        self.orbit_mode = int(orbit_mode)
+        placeholder = [capi.Curve(0.0, 1.0, 0.1, 0.0, 1.0, 1.0, 1e-3, 0.0, 0.0)] * self.n_curves
+        self.batch = capi.Batch(placeholder, options)
+        if nbin is not None:
+            self.batch.set_source_bins(np.broadcast_to(np.asarray(nbin, dtype=np.int32), (self.n_curves,)))
+        self.curve_of = torch.repeat_interleave(torch.arange(self.n_curves),
+                                                torch.from_numpy(np.diff(self.offsets)))
+        self.last_method = None
+
+    def __call__(self, params: torch.Tensor) -> torch.Tensor:
+        if params.shape != (self.n_curves, len(DYNAMIC_PARAM_NAMES)):
+            raise ValueError(f"params must be ({self.n_curves}, {len(DYNAMIC_PARAM_NAMES)}), "
+                             f"got {tuple(params.shape)}")
+        return _DynamicLightCurveFn.apply(params, self)
+
+    def close(self):
+        self.batch.close()

@@ -145,3 +145,27 @@ def test_dynamic_gradients_match_finite_differences(gpu):
             if sel.sum() >= 5:
                 assert np.median(err[sel]) <= med_bar, (name, label, np.median(err[sel]), err[sel].max())
     b.close()
+
+
+def test_dynamic_torch_bridge(gpu):
+    import torch
+    from lcbinint_amd import capi
+    from lcbinint_amd.torch_ops import DYNAMIC_PARAM_NAMES, DynamicLightCurves
+    capi.load_earth_ephemeris(TABLE)
+    curves, dyn, offsets, times = _curves(6, 50, 13, 1, width=0.3)
+    rows = [[c[k] if k in c else d[k] for k in DYNAMIC_PARAM_NAMES] for c, d in zip(curves, dyn)]
+    params = torch.tensor(rows, dtype=torch.float64, device="cuda", requires_grad=True)
+    lc = DynamicLightCurves(offsets, times, dyn, orbit_mode=capi.ORBIT_CIRCULAR)
+    mag = lc(params)
+    w = torch.linspace(0.5, 1.5, len(times), dtype=torch.float64, device="cuda")
+    (w * mag).sum().backward()
+    # the same contraction from the C-ABI derivatives
+    b = capi.Batch([capi.Curve(**c) for c in curves])
+    m, _, _, dm = capi.light_curves_dynamic_grad(b, [capi.Dynamics(**d) for d in dyn], capi.ORBIT_CIRCULAR,
+                                                 offsets, times)
+    b.close()
+    assert np.array_equal(mag.detach().cpu().numpy(), m)
+    exp = np.stack([np.add.reduceat(w.cpu().numpy() * dm[:, k], offsets[:-1]) for k in range(12)], axis=1)
+    got = params.grad.cpu().numpy()
+    assert np.allclose(got, exp, rtol=1e-12, atol=1e-14 * np.abs(exp).max())
+    lc.close()
