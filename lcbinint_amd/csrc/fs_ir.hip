@@ -1012,199 +1012,408 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
     }
 }
 
+// ----------------------------------------------------------- Cartesian images
+//
 // inverse_ray_cartesian_binary_mag (finite_source_magnifier.cpp:2470-2768)
-__device__ double cartesian_mag(Ctx& C, int source_bins, double* rows_mem, int* rows_key,
-                                double* areaimage, int* flags)
+// processes the seeds in order: an unflagged seed gets its image scanned, then
+// the overlap rule flags the later seeds that image covers and subtracts the
+// earlier images it covers.  The scan of an image depends only on its seed
+// (its rows start empty), so images can be computed anywhere, in any order,
+// and accounted in seed order afterwards; the accounting reads the image's
+// rows, its area and its work counts.  image_area() is the scan of one image,
+// account_image() its accounting.
+
+struct ImageRun {
+    double area;
+    int nyi;  // rows [0, nyi) take part in the overlap test
+};
+
+__device__ __forceinline__ void cart_params(Cart& K, const Ctx& C, int source_bins)
+{
+    const double nbin = static_cast<double>(source_bins > 1 ? source_bins : 1);
+    K.incr = C.rho / nbin;
+    K.inv_incr = 1.0 / K.incr;
+    const long long sb = source_bins;
+    K.max_steps = (100000LL > sb * sb * 2000LL) ? 100000LL : sb * sb * 2000LL;
+    K.jac_sign = 0;
+}
+
+__device__ __forceinline__ Rows slot_rows(const CartScratch& scr, int slot)
+{
+    Rows R;
+    R.r = scr.rows + static_cast<size_t>(slot) * scr.row_cap * 5;
+    int* rk = scr.row_key + static_cast<size_t>(slot) * row_ints_per_slot(scr.row_cap);
+    R.node = reinterpret_cast<RowNode*>(rk);
+    R.head = rk + 8 * scr.row_cap;
+    R.dirty = rk + 8 * scr.row_cap + kRowHash;
+    R.size = 0;
+    R.pushed = 0;
+    return R;
+}
+
+// Team of kTeamWaves waves sharing one workgroup (k_ir_cart).  Each wave runs
+// its own epochs; a wave whose epoch has run kOfferSteps wave steps offers the
+// next unflagged seeds of that epoch as image jobs, and a teammate between
+// epochs computes such an image in its own slot.  Job status words:
+//   kJobFree          unused
+//   kJobOpen          offered, not taken
+//   kJobRun + h       wave h computes it
+//   kJobDone + h      wave h has finished; its rows stay untouched until the
+//                     owner sets the word back to kJobFree
+//   kJobCancel + h    the owner no longer needs it (wave h stops at its next
+//                     poll and does not wait)
+// Only the owner takes a job back (Open -> Free), cancels or releases it; only
+// a helper claims it (Open -> Run) and finishes it (Run -> Done).  A wave only
+// ever waits for a job that another wave is computing, and a computation waits
+// for nothing, so every wait ends; each wait also has a time limit.
+enum : int { kJobFree = 0, kJobOpen = 1, kJobRun = 16, kJobDone = 32, kJobCancel = 48 };
+constexpr long long kOfferSteps = 2048;
+constexpr long long kTeamWaitTicks = 6000000000ll;  // 60 s of the 100 MHz clock
+
+struct TeamOut {
+    double area;
+    int nyi, bad;  // bad: 1 work budget or slot exceeded, 2 slot capacity
+    long long steps, n_cart, n_cart_guarded, n_rows;
+};
+
+struct Team {
+    int active;                          // waves still taking queue items
+    int item[kTeamWaves][kTeamJobs];     // queue item (epoch) of each job
+    int seed[kTeamWaves][kTeamJobs];     // its seed
+    int status[kTeamWaves][kTeamJobs];
+    TeamOut out[kTeamWaves];             // result of the job each helper finished
+};
+
+__device__ __forceinline__ int lds_get(const int* p)
+{
+    return uni(__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+
+__device__ __forceinline__ void lds_put(int* p, int v)
+{
+    __hip_atomic_store(p, uni(v), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Compare-and-swap on behalf of the wave: lane 0 swaps, the other lanes compare
+// against a value no status word holds.
+__device__ __forceinline__ bool lds_cas(int* p, int expected, int desired)
+{
+    int exp = lane_id() == 0 ? expected : -1;
+    __hip_atomic_compare_exchange_strong(p, &exp, desired, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __builtin_amdgcn_readfirstlane(exp == expected ? 1 : 0) != 0;
+}
+
+__device__ __forceinline__ void team_sleep() { __builtin_amdgcn_s_sleep(8); }
+
+// The owner drops job k: an offered job is withdrawn, a running one cancelled,
+// a finished one released.
+__device__ __forceinline__ void team_drop(Team& T, int w, int k)
+{
+    int* st = &T.status[w][k];
+    for (int tries = 0; tries < 8; ++tries) {
+        const int s = lds_get(st);
+        if (s == kJobFree || s >= kJobCancel) return;
+        if (s == kJobOpen) {
+            if (lds_cas(st, s, kJobFree)) return;
+        } else if (s < kJobDone) {
+            if (lds_cas(st, s, kJobCancel + (s - kJobRun))) return;
+        } else {
+            lds_put(st, kJobFree);
+            return;
+        }
+    }
+}
+
+// Job of the owner's seed j that is still live (offered, running or done), or -1.
+__device__ __forceinline__ int team_find(Team& T, int w, int j)
+{
+    for (int k = 0; k < kTeamJobs; ++k) {
+        const int s = lds_get(&T.status[w][k]);
+        if (s != kJobFree && s < kJobCancel && lds_get(&T.seed[w][k]) == j) return k;
+    }
+    return -1;
+}
+
+// Offers the owner's next unflagged seeds after `cur` in the free job entries.
+__device__ __forceinline__ void team_offer(Team& T, int w, int item, const int* flags, int cur, int n_img,
+                                           int& next_offer)
+{
+    if (next_offer <= cur) next_offer = cur + 1;
+    for (int k = 0; k < kTeamJobs && next_offer < n_img; ++k) {
+        const int s = lds_get(&T.status[w][k]);
+        if (s != kJobFree && s < kJobCancel) continue;
+        while (next_offer < n_img && (uni(flags[next_offer]) & 1)) ++next_offer;
+        if (next_offer >= n_img) break;
+        lds_put(&T.item[w][k], item);
+        lds_put(&T.seed[w][k], next_offer);
+        lds_put(&T.status[w][k], kJobOpen);
+        ++next_offer;
+    }
+}
+
+// Claims an offered job of a teammate: its owner p and job k.
+__device__ __forceinline__ bool team_claim(Team& T, int w, int& p, int& k)
+{
+    for (int d = 1; d < kTeamWaves; ++d) {
+        const int q = (w + d) % kTeamWaves;
+        for (int j = 0; j < kTeamJobs; ++j) {
+            if (lds_get(&T.status[q][j]) != kJobOpen) continue;
+            if (lds_cas(&T.status[q][j], kJobOpen, kJobRun + w)) {
+                p = q;
+                k = j;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// Checks between scans: a helper stops once its job is no longer its own; the
+// owner of a long epoch offers its next images.
+struct ImageHook {
+    Team* team;
+    int w;
+    const int* watch;  // helper: its job's status word, which must stay `expect`
+    int expect;
+    const int* flags;  // owner: offering state
+    int item, cur, n_img;
+    int* next_offer;
+    bool armed;
+};
+
+__device__ __forceinline__ bool hook_poll(Ctx& C, ImageHook& hk)
+{
+    if (hk.watch && lds_get(hk.watch) != hk.expect) {
+        C.overflow = true;
+        return false;
+    }
+    if (hk.armed && C.steps >= kOfferSteps) {
+        team_offer(*hk.team, hk.w, hk.item, hk.flags, hk.cur, hk.n_img, *hk.next_offer);
+        hk.armed = false;
+    }
+    return true;
+}
+
+// The scans of one image (:2520-2680): up from the seed, down from the row
+// below it, then the gap-repair scans of rows 0..nyi-1, four cases per row,
+// all through one call site of cart_scan.  A failure leaves C.overflow set.
+__device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed, ImageHook& hk)
+{
+    ImageRun out{__builtin_nan(""), 0};
+    const double incr = K.incr;
+    rows_reset(C, R);
+    if (!rows_ensure(C, R, 1)) return out;
+    int yi = 0;
+    const double J_seed = jacobian_real(C.L, seed.x, seed.y);
+    K.jac_sign = fabs(J_seed) < 0.02 ? sign_of(J_seed) : 0;
+    int stage = 0;
+    int row = -1;  // gap-repair row being examined
+    int cs = 4;    // next case of `row` (4: the row is done)
+    double dxmax = 0.0, dxmin = 0.0;
+    int nyi = 0;
+    double areai = 0.0, areabound = 0.0;
+    const double g = 1.1 * incr;
+    for (;;) {
+        if (!hook_poll(C, hk)) return out;
+        Pt start{0.0, 0.0};
+        double sdy = 0.0;
+        if (stage == 0) {
+            rset(C, R, 0, XMIN, seed.x);
+            rset(C, R, 0, XMAX, seed.x);
+            start = seed;
+            sdy = incr;
+        } else if (stage == 1) {
+            sdy = -incr;
+            if (!rows_ensure(C, R, yi)) return out;
+            start = Pt{rget(R, 0, XMAX), seed.y + sdy};
+            rset(C, R, yi, XMIN, rget(R, 0, XMIN));
+            rset(C, R, yi, XMAX, rget(R, 0, XMAX));
+            rset(C, R, yi, YY, rget(R, 0, YY));
+            rset(C, R, yi, DYS, sdy);
+            ++yi;
+        } else {
+            bool have = false;
+            while (!have) {
+                if (cs >= 4) {
+                    if (row >= 0 && row == nyi - 1 && areabound > 0.0 && yi > nyi) nyi = yi;
+                    ++row;
+                    // rows before the last whose four cases all fail change
+                    // nothing: skip them 64 at a time (their records and the
+                    // next row's are final while the loop runs)
+                    while (row < nyi - 1) {
+                        const int rr = row + C.lane;
+                        bool act = rr == nyi - 1;
+                        if (rr < nyi - 1 && rget(R, rr + 1, AX) > 0.0) {
+                            const double ex = rget(R, rr + 1, XMAX) - rget(R, rr, XMAX);
+                            const double en = rget(R, rr + 1, XMIN) - rget(R, rr, XMIN);
+                            act = ex > g || en > g || en < -g || ex < -g;
+                        }
+                        const uint64_t mm = ballot(act);
+                        if (mm) {
+                            row += first_set(mm);
+                            break;
+                        }
+                        row += kWave;
+                    }
+                    if (row >= nyi) break;
+                    if (!rows_ensure(C, R, row + 1)) return out;
+                    dxmax = rget(R, row + 1, XMAX) - rget(R, row, XMAX);
+                    dxmin = rget(R, row + 1, XMIN) - rget(R, row, XMIN);
+                    cs = rget(R, row + 1, AX) > 0.0 ? 0 : 4;
+                    continue;
+                }
+                const int k = cs++;
+                double nxmin = 0.0, nxmax = 0.0;
+                if (k == 0 && dxmax > g) {
+                    start = Pt{rget(R, row + 1, XMAX), rget(R, row, YY)};
+                    nxmin = rget(R, row, XMAX);
+                    nxmax = rget(R, row + 1, XMAX);
+                    sdy = -rget(R, row, DYS);
+                    have = true;
+                } else if (k == 1 && dxmin > g) {
+                    start = Pt{rget(R, row + 1, XMIN) - incr, rget(R, row + 1, YY)};
+                    nxmin = rget(R, row, XMIN);
+                    nxmax = rget(R, row + 1, XMIN);
+                    sdy = rget(R, row, DYS);
+                    have = true;
+                } else if (k == 2 && dxmin < -g) {
+                    start = Pt{rget(R, row, XMIN) - incr, rget(R, row, YY)};
+                    nxmin = rget(R, row + 1, XMIN);
+                    nxmax = rget(R, row, XMIN);
+                    sdy = -rget(R, row, DYS);
+                    have = true;
+                } else if (k == 3 && dxmax < -g) {
+                    start = Pt{rget(R, row + 1, XMAX) + incr, rget(R, row + 1, YY)};
+                    nxmin = rget(R, row + 1, XMAX);
+                    nxmax = rget(R, row, XMAX);
+                    sdy = rget(R, row, DYS);
+                    have = true;
+                }
+                if (have) {
+                    start.x = uni(start.x);
+                    start.y = uni(start.y);
+                    sdy = uni(sdy);
+                    nxmin = uni(nxmin);
+                    nxmax = uni(nxmax);
+                    if (!rows_ensure(C, R, yi)) return out;
+                    rset(C, R, yi, XMIN, nxmin);
+                    rset(C, R, yi, XMAX, nxmax);
+                    rset(C, R, yi, DYS, sdy);
+                    ++yi;
+                }
+            }
+            if (!have) break;
+        }
+        const double r = cart_scan(C, K, R, start, sdy, yi);
+        if (C.overflow) return out;
+        if (stage == 0) {
+            areai = r;
+            stage = 1;
+        } else if (stage == 1) {
+            areai += r;
+            nyi = yi;
+            stage = 2;
+        } else {
+            areai += r;
+            areabound += r;
+            if (r <= 0.0) --yi;
+        }
+    }
+    out.area = areai;
+    out.nyi = nyi;
+    return out;
+}
+
+// Accounting of image i (:2674-2739): its area is added, then each other seed
+// the image's rows cover is flagged (a later one is skipped; an earlier one's
+// area is subtracted once).  A seed flagged to be skipped has its job dropped.
+__device__ __forceinline__ void account_image(Ctx& C, const Cart& K0, int i, const Rows& R, ImageRun img,
+                                              int source_bins, double& area, double* areaimage, int* flags,
+                                              Team* team, int w)
+{
+    const Pt seed = C.seeds[i];
+    const double J_seed = jacobian_real(C.L, seed.x, seed.y);
+    const int jac_sign = fabs(J_seed) < 0.02 ? sign_of(J_seed) : 0;
+    const double m = 0.5 * K0.incr * 1.01;
+    const int n_img = C.n_seeds;
+    const int nyi = img.nyi;
+    area += img.area;
+    wave_sync();
+    wave_store(&areaimage[i], img.area);
+    wave_sync();
+    for (int other = 0; other < n_img; ++other) {
+        if (other == i) continue;
+        const Pt pos = C.seeds[other];
+        if (jac_sign != 0) {
+            const double Jo = jacobian_real(C.L, pos.x, pos.y);
+            if (sign_of(Jo) == -jac_sign) continue;
+            if (C.rho >= 4.0e-3 && source_bins >= 35 && other > i && fabs(J_seed) < 1.0e-3 && fabs(Jo) < 0.02)
+                continue;
+        }
+        bool hit = false;
+        for (int base = 0; base < nyi && !hit; base += kWave) {
+            const int row = base + C.lane;
+            bool h = false;
+            if (row < nyi && rget(R, row, AX) > 0.0) {
+                const double yy = rget(R, row, YY);
+                h = pos.y >= yy - m && pos.y <= yy + m && pos.x >= rget(R, row, XMIN) - m &&
+                    pos.x <= rget(R, row, XMAX) + m;
+            }
+            hit = ballot(h) != 0;
+        }
+        if (hit) {
+            const int fl = flags[other];
+            wave_sync();
+            if (other < i) {
+                if (!(fl & 2)) {
+                    area -= areaimage[other];
+                    wave_store(&flags[other], fl | 2);
+                }
+            } else {
+                wave_store(&flags[other], fl | 1);
+            }
+            wave_sync();
+            if (other > i && team) {
+                const int k = team_find(*team, w, other);
+                if (k >= 0) team_drop(*team, w, k);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double cart_scale(const Ctx& C, int source_bins)
+{
+    const double nbin = static_cast<double>(source_bins > 1 ? source_bins : 1);
+    return source_flux(C.rho, C.ld_c, C.ld_d) / (C.rho * C.rho) * nbin * nbin;
+}
+
+// The whole Cartesian area of one epoch by one wave, images in seed order
+// (the epoch-at-a-time form of k_ir_cart's loop, for single-epoch kernels).
+__device__ double cartesian_mag(Ctx& C, int source_bins, const CartScratch& scr, int slot)
 {
     const int n_img = C.n_seeds;
     if (n_img == 0 || !(C.rho > 0.0)) return __builtin_nan("");
-    const double nbin = static_cast<double>(source_bins > 1 ? source_bins : 1);
     Cart K;
-    K.incr = C.rho / nbin;
-    K.inv_incr = 1.0 / K.incr;
-    const int sb = source_bins;
-    K.max_steps = (100000LL > static_cast<long long>(sb) * sb * 2000LL) ? 100000LL
-                                                                         : static_cast<long long>(sb) * sb * 2000LL;
-    const double incr = K.incr;
-    const double m = 0.5 * incr * 1.01;
+    cart_params(K, C, source_bins);
+    double* areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
+    int* flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
     wave_sync();
     for (int k = C.lane; k < n_img; k += kWave) {
         areaimage[k] = 0.0;
         flags[k] = 0;  // bit 0: overlap, bit 1: subtracted
     }
     wave_sync();
+    Rows R = slot_rows(scr, slot);
+    ImageHook hk{nullptr, 0, nullptr, 0, nullptr, 0, 0, 0, nullptr, false};
     double area = 0.0;
     for (int i = 0; i < n_img; ++i) {
         if (flags[i] & 1) continue;
-        Rows R;
-        R.r = rows_mem;
-        R.node = reinterpret_cast<RowNode*>(rows_key);
-        R.head = rows_key + 8 * C.row_cap;
-        R.dirty = rows_key + 8 * C.row_cap + kRowHash;
-        rows_reset(C, R);
-        if (!rows_ensure(C, R, 1)) return __builtin_nan("");
-        int yi = 0;
-        const Pt seed = C.seeds[i];
-        const double J_seed = jacobian_real(C.L, seed.x, seed.y);
-        K.jac_sign = fabs(J_seed) < 0.02 ? sign_of(J_seed) : 0;
-        // The image's scans in the reference's order, through one call site:
-        // stage 0 up from the seed, stage 1 down from the row below it, stage
-        // 2 the gap-repair scans of rows 0..nyi-1, four cases per row.
-        int stage = 0;
-        int row = -1;      // gap-repair row being examined
-        int cs = 4;        // next case of `row` (4: the row is done)
-        double dxmax = 0.0, dxmin = 0.0;
-        int nyi = 0;
-        double areai = 0.0, areabound = 0.0;
-        const double g = 1.1 * incr;
-        for (;;) {
-            Pt start{0.0, 0.0};
-            double sdy = 0.0;
-            if (stage == 0) {
-                rset(C, R, 0, XMIN, seed.x);
-                rset(C, R, 0, XMAX, seed.x);
-                start = seed;
-                sdy = incr;
-            } else if (stage == 1) {
-                sdy = -incr;
-                if (!rows_ensure(C, R, yi)) return __builtin_nan("");
-                start = Pt{rget(R, 0, XMAX), seed.y + sdy};
-                rset(C, R, yi, XMIN, rget(R, 0, XMIN));
-                rset(C, R, yi, XMAX, rget(R, 0, XMAX));
-                rset(C, R, yi, YY, rget(R, 0, YY));
-                rset(C, R, yi, DYS, sdy);
-                ++yi;
-            } else {
-                bool have = false;
-                while (!have) {
-                    if (cs >= 4) {
-                        if (row >= 0 && row == nyi - 1 && areabound > 0.0 && yi > nyi) nyi = yi;
-                        ++row;
-                        // rows before the last whose four cases all fail change
-                        // nothing: skip them 64 at a time (their records and the
-                        // next row's are final while the loop runs)
-                        while (row < nyi - 1) {
-                            const int rr = row + C.lane;
-                            bool act = rr == nyi - 1;
-                            if (rr < nyi - 1 && rget(R, rr + 1, AX) > 0.0) {
-                                const double ex = rget(R, rr + 1, XMAX) - rget(R, rr, XMAX);
-                                const double en = rget(R, rr + 1, XMIN) - rget(R, rr, XMIN);
-                                act = ex > g || en > g || en < -g || ex < -g;
-                            }
-                            const uint64_t mm = ballot(act);
-                            if (mm) {
-                                row += first_set(mm);
-                                break;
-                            }
-                            row += kWave;
-                        }
-                        if (row >= nyi) break;
-                        if (!rows_ensure(C, R, row + 1)) return __builtin_nan("");
-                        dxmax = rget(R, row + 1, XMAX) - rget(R, row, XMAX);
-                        dxmin = rget(R, row + 1, XMIN) - rget(R, row, XMIN);
-                        cs = rget(R, row + 1, AX) > 0.0 ? 0 : 4;
-                        continue;
-                    }
-                    const int k = cs++;
-                    double nxmin = 0.0, nxmax = 0.0;
-                    if (k == 0 && dxmax > g) {
-                        start = Pt{rget(R, row + 1, XMAX), rget(R, row, YY)};
-                        nxmin = rget(R, row, XMAX);
-                        nxmax = rget(R, row + 1, XMAX);
-                        sdy = -rget(R, row, DYS);
-                        have = true;
-                    } else if (k == 1 && dxmin > g) {
-                        start = Pt{rget(R, row + 1, XMIN) - incr, rget(R, row + 1, YY)};
-                        nxmin = rget(R, row, XMIN);
-                        nxmax = rget(R, row + 1, XMIN);
-                        sdy = rget(R, row, DYS);
-                        have = true;
-                    } else if (k == 2 && dxmin < -g) {
-                        start = Pt{rget(R, row, XMIN) - incr, rget(R, row, YY)};
-                        nxmin = rget(R, row + 1, XMIN);
-                        nxmax = rget(R, row, XMIN);
-                        sdy = -rget(R, row, DYS);
-                        have = true;
-                    } else if (k == 3 && dxmax < -g) {
-                        start = Pt{rget(R, row + 1, XMAX) + incr, rget(R, row + 1, YY)};
-                        nxmin = rget(R, row + 1, XMAX);
-                        nxmax = rget(R, row, XMAX);
-                        sdy = rget(R, row, DYS);
-                        have = true;
-                    }
-                    if (have) {
-                        start.x = uni(start.x);
-                        start.y = uni(start.y);
-                        sdy = uni(sdy);
-                        nxmin = uni(nxmin);
-                        nxmax = uni(nxmax);
-                        if (!rows_ensure(C, R, yi)) return __builtin_nan("");
-                        rset(C, R, yi, XMIN, nxmin);
-                        rset(C, R, yi, XMAX, nxmax);
-                        rset(C, R, yi, DYS, sdy);
-                        ++yi;
-                    }
-                }
-                if (!have) break;
-            }
-            const double r = cart_scan(C, K, R, start, sdy, yi);
-            if (C.overflow) return __builtin_nan("");
-            if (stage == 0) {
-                areai = r;
-                stage = 1;
-            } else if (stage == 1) {
-                areai += r;
-                nyi = yi;
-                stage = 2;
-            } else {
-                areai += r;
-                areabound += r;
-                if (r <= 0.0) --yi;
-            }
-        }
-        area += areai;
-        wave_sync();
-        wave_store(&areaimage[i], areai);
-        wave_sync();
-        // overlap bookkeeping (:2681-2739)
-        for (int other = 0; other < n_img; ++other) {
-            if (other == i) continue;
-            const Pt pos = C.seeds[other];
-            if (K.jac_sign != 0) {
-                const double Jo = jacobian_real(C.L, pos.x, pos.y);
-                if (sign_of(Jo) == -K.jac_sign) continue;
-                if (C.rho >= 4.0e-3 && source_bins >= 35 && other > i && fabs(J_seed) < 1.0e-3 &&
-                    fabs(Jo) < 0.02)
-                    continue;
-            }
-            bool hit = false;
-            for (int base = 0; base < nyi && !hit; base += kWave) {
-                const int row = base + C.lane;
-                bool h = false;
-                if (row < nyi && rget(R, row, AX) > 0.0) {
-                    const double yy = rget(R, row, YY);
-                    h = pos.y >= yy - m && pos.y <= yy + m && pos.x >= rget(R, row, XMIN) - m &&
-                        pos.x <= rget(R, row, XMAX) + m;
-                }
-                hit = ballot(h) != 0;
-            }
-            if (hit) {
-                const int fl = flags[other];
-                wave_sync();
-                if (other < i) {
-                    if (!(fl & 2)) {
-                        area -= areaimage[other];
-                        wave_store(&flags[other], fl | 2);
-                    }
-                } else {
-                    wave_store(&flags[other], fl | 1);
-                }
-                wave_sync();
-            }
-        }
+        const ImageRun img = image_area(C, K, R, C.seeds[i], hk);
+        if (C.overflow) return __builtin_nan("");
+        account_image(C, K, i, R, img, source_bins, area, areaimage, flags, nullptr, 0);
     }
-    const double scale = source_flux(C.rho, C.ld_c, C.ld_d) / (C.rho * C.rho) * nbin * nbin;
-    return area / scale;
+    return area / cart_scale(C, source_bins);
 }
 
 // ---------------------------------------------------------------- polar fill
@@ -1547,12 +1756,8 @@ __device__ __forceinline__ double cart_epoch(Ctx& C, const CurveRec& cv, const F
                                              const CartScratch& scr, int slot)
 {
     if (C.overflow) return __builtin_nan("");
-    double* areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
-    int* flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
-    double* rows = scr.rows + static_cast<size_t>(slot) * scr.row_cap * 5;
-    int* row_key = scr.row_key + static_cast<size_t>(slot) * row_ints_per_slot(scr.row_cap);
     C.row_cap = scr.row_cap;
-    return cartesian_mag(C, curve_nbin(cv, opt), rows, row_key, areaimage, flags);
+    return cartesian_mag(C, curve_nbin(cv, opt), scr, slot);
 }
 
 // Cost bucket of an inverse-ray epoch: the lattice cells of an epoch scale as
@@ -1777,45 +1982,203 @@ __global__ void __launch_bounds__(kWave) k_ir_farm(const CurveRec* __restrict__ 
     if (solved) atomicAdd(&work_totals(ep)[0], static_cast<unsigned long long>(solved));
 }
 
-// Cartesian row scans of the queued epochs, in one of three passes: small
-// slots (an epoch that runs out of rows is queued for the retry pass), large
-// slots for the largest epochs, and the retry pass on large slots.
-__global__ void __launch_bounds__(kWave) k_ir_cart(const CurveRec* __restrict__ curves,
-                                                    FsOptions opt, FsEpochs ep, IrLists L,
-                                                    CartScratch scr, int pass)
+// Cartesian areas of the queued epochs, in three passes (small slots; large
+// slots for the largest epochs, beside the small pass; large slots for the
+// epochs that outgrew a small slot).  A workgroup is a team of kTeamWaves
+// waves (Team above), each with its own slot blockIdx.x * kTeamWaves + w.
+// Every wave loops: help a teammate if a job is offered, else run its own
+// epoch, whose images it computes itself or takes from the teammate that
+// computed them, accounting them in seed order.
+__global__ void __launch_bounds__(kTeamWaves * kWave) k_ir_cart(const CurveRec* __restrict__ curves,
+                                                                FsOptions opt, FsEpochs ep, IrLists L,
+                                                                CartScratch scr, int pass)
 {
+    __shared__ Team team;
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+    const int slot = static_cast<int>(blockIdx.x) * kTeamWaves + w;
+    for (int k = 0; k < kTeamJobs; ++k) {
+        wave_store(&team.item[w][k], 0);
+        wave_store(&team.seed[w][k], 0);
+        wave_store(&team.status[w][k], static_cast<int>(kJobFree));
+    }
+    if (w == 0) wave_store(&team.active, kTeamWaves);
+    __syncthreads();
+
     const bool large = pass != kPassSmall;
     const int* queue = pass == kPassSmall ? L.cart_q : pass == kPassBig ? L.cart_big : L.cart_retry;
     const int qword = pass == kPassSmall ? kQCartCount : pass == kPassBig ? kQCartBigCount : kQCartRetryCount;
     unsigned* cursor = &ep.counters[qword + 1];
     const int count = uni(static_cast<int>(ep.counters[qword]));
+    double* areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
+    int* flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
+    const Rows mine = slot_rows(scr, slot);
+    Ctx C;
+    Cart K;
+    bool have = false;       // an own epoch is in progress
+    bool exhausted = false;  // the queue is empty
+    int item = -1, nbin = 1, cur = -1, next_offer = 0;
+    int64_t e = 0;
+    long long t_begin = 0, idle_from = 0;
+    double area = 0.0;
     for (;;) {
-        const int qi = next_item(cursor);
-        if (qi >= count) break;
-        const int item = uni(queue[qi]);
-        const int64_t e = L.ir_sorted[item];
-        const long long t_begin = static_cast<long long>(wall_clock64());
-        const CurveRec cv = curves[ep.curve_of[e]];
-        Ctx C;
-        init_ctx(C, cv, ep.src[e], L.seed_pool + uni(L.seed_off[item]));
-        C.n_seeds = uni(L.seed_n[item]);
-        const double mag = cart_epoch(C, cv, opt, scr, blockIdx.x);
-        if (C.cap_overflow && !large) {
-            const int pos = wave_fetch_inc(&ep.counters[kQCartRetryCount]);
-            wave_store(&L.cart_retry[pos], item);
+        int mode = 1;  // 1: own image `cur`, 2: a teammate's job, 3: helper h's image `cur`
+        int p = -1, jk = -1, h = -1;
+        if (!have) {
+            if (team_claim(team, w, p, jk)) {
+                mode = 2;
+            } else if (!exhausted) {
+                const int qi = next_item(cursor);
+                if (qi >= count) {
+                    exhausted = true;
+                    atomicAdd(&team.active, lane_id() == 0 ? -1 : 0);
+                    idle_from = static_cast<long long>(wall_clock64());
+                    continue;
+                }
+                item = uni(queue[qi]);
+                e = L.ir_sorted[item];
+                t_begin = static_cast<long long>(wall_clock64());
+                const CurveRec cv = curves[ep.curve_of[e]];
+                init_ctx(C, cv, ep.src[e], L.seed_pool + uni(L.seed_off[item]));
+                C.n_seeds = uni(L.seed_n[item]);
+                C.row_cap = scr.row_cap;
+                nbin = curve_nbin(cv, opt);
+                cart_params(K, C, nbin);
+                wave_sync();
+                for (int k = C.lane; k < C.n_seeds; k += kWave) {
+                    areaimage[k] = 0.0;
+                    flags[k] = 0;  // bit 0: overlap, bit 1: subtracted
+                }
+                wave_sync();
+                area = 0.0;
+                cur = -1;
+                next_offer = 0;
+                have = true;
+                continue;
+            } else {
+                if (lds_get(&team.active) == 0) break;
+                if (static_cast<long long>(wall_clock64()) - idle_from > kTeamWaitTicks) break;
+                team_sleep();
+                continue;
+            }
         } else {
-            wave_store(&ep.mag[e], mag);
-            if (C.overflow) atomicOr(&ep.counters[kQFlags], C.cap_overflow ? kErrRowCap : kErrBudget);
+            int i = cur + 1;
+            while (i < C.n_seeds && (uni(flags[i]) & 1)) ++i;
+            const bool degenerate = C.n_seeds == 0 || !(C.rho > 0.0);
+            if (i >= C.n_seeds || C.overflow || degenerate) {
+                // the epoch's result
+                const double mag = (C.overflow || degenerate) ? __builtin_nan("") : area / cart_scale(C, nbin);
+                for (int k = 0; k < kTeamJobs; ++k) team_drop(team, w, k);
+                if (C.cap_overflow && !large) {
+                    const int pos = wave_fetch_inc(&ep.counters[kQCartRetryCount]);
+                    wave_store(&L.cart_retry[pos], item);
+                } else {
+                    wave_store(&ep.mag[e], mag);
+                    if (C.overflow) atomicOr(&ep.counters[kQFlags], C.cap_overflow ? kErrRowCap : kErrBudget);
+                }
+                flush_work(C, ep);
+                const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
+                add_total(&work_totals(ep)[6], ticks);
+                if (ep.cost) {
+                    wave_store(&ep.cost[ep.n + e], ep.cost[ep.n + e] + static_cast<int64_t>(ticks));
+                    wave_store(&ep.cost[3 * ep.n + e], ep.cost[3 * ep.n + e] + static_cast<int64_t>(C.n_rows));
+                    wave_store(&ep.cost[4 * ep.n + e], ep.cost[4 * ep.n + e] + static_cast<int64_t>(C.steps));
+                }
+                wave_sync();
+                have = false;
+                continue;
+            }
+            cur = i;
+            const int k = team_find(team, w, i);
+            if (k >= 0 && !lds_cas(&team.status[w][k], kJobOpen, kJobFree)) {
+                // a teammate has taken this image: wait for its result
+                const long long t0 = static_cast<long long>(wall_clock64());
+                for (;;) {
+                    const int s = lds_get(&team.status[w][k]);
+                    if (s >= kJobDone && s < kJobCancel) {
+                        h = s - kJobDone;
+                        break;
+                    }
+                    if (static_cast<long long>(wall_clock64()) - t0 > kTeamWaitTicks) break;
+                    team_sleep();
+                }
+                if (h < 0) {
+                    C.overflow = true;
+                    atomicOr(&ep.counters[kQFlags], kErrTeam);
+                    team_drop(team, w, k);
+                    continue;
+                }
+                jk = k;
+                mode = 3;
+            }
+            if (mode == 1 && C.steps >= kOfferSteps) team_offer(team, w, item, flags, cur, C.n_seeds, next_offer);
         }
-        flush_work(C, ep);
-        const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
-        add_total(&work_totals(ep)[6], ticks);
-        if (ep.cost) {
-            wave_store(&ep.cost[ep.n + e], ep.cost[ep.n + e] + static_cast<int64_t>(ticks));
-            wave_store(&ep.cost[3 * ep.n + e], ep.cost[3 * ep.n + e] + static_cast<int64_t>(C.n_rows));
-            wave_store(&ep.cost[4 * ep.n + e], ep.cost[4 * ep.n + e] + static_cast<int64_t>(C.steps));
+        ImageRun img{__builtin_nan(""), 0};
+        Rows R = mine;
+        if (mode != 3) {
+            int j = cur;
+            ImageHook hk{&team, w, nullptr, 0, flags, item, cur, C.n_seeds, &next_offer, mode == 1};
+            if (mode == 2) {
+                // the teammate's epoch, in this wave's context and slot; the
+                // job's words are read while it is still this wave's (an entry
+                // is rewritten only after its status has changed)
+                const int pitem = lds_get(&team.item[p][jk]);
+                j = lds_get(&team.seed[p][jk]);
+                const int64_t pe = L.ir_sorted[pitem];
+                const CurveRec cv = curves[ep.curve_of[pe]];
+                init_ctx(C, cv, ep.src[pe], L.seed_pool + uni(L.seed_off[pitem]));
+                C.n_seeds = uni(L.seed_n[pitem]);
+                C.row_cap = scr.row_cap;
+                cart_params(K, C, curve_nbin(cv, opt));
+                hk.watch = &team.status[p][jk];
+                hk.expect = kJobRun + w;
+                hk.armed = false;
+                if (j >= C.n_seeds) C.overflow = true;
+            }
+            if (!C.overflow) img = image_area(C, K, R, C.seeds[j], hk);
+            if (mode == 2) {
+                TeamOut& o = team.out[w];
+                wave_store(&o.area, img.area);
+                wave_store(&o.nyi, img.nyi);
+                wave_store(&o.bad, (C.overflow ? 1 : 0) | (C.cap_overflow ? 2 : 0));
+                wave_store(&o.steps, C.steps);
+                wave_store(&o.n_cart, C.n_cart);
+                wave_store(&o.n_cart_guarded, C.n_cart_guarded);
+                wave_store(&o.n_rows, C.n_rows);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (lds_cas(&team.status[p][jk], kJobRun + w, kJobDone + w)) {
+                    // the owner reads this slot's rows until it sets the word back
+                    const long long t0 = static_cast<long long>(wall_clock64());
+                    while (lds_get(&team.status[p][jk]) == kJobDone + w) {
+                        if (static_cast<long long>(wall_clock64()) - t0 > kTeamWaitTicks) {
+                            atomicOr(&ep.counters[kQFlags], kErrTeam);
+                            return;
+                        }
+                        team_sleep();
+                    }
+                }
+                continue;
+            }
+            if (C.overflow) continue;
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const TeamOut& o = team.out[h];
+            img.area = uni(o.area);
+            img.nyi = uni(o.nyi);
+            const int bad = uni(o.bad);
+            C.steps = uni(C.steps + uni(o.steps));
+            C.n_cart = uni(C.n_cart + uni(o.n_cart));
+            C.n_cart_guarded = uni(C.n_cart_guarded + uni(o.n_cart_guarded));
+            C.n_rows = uni(C.n_rows + uni(o.n_rows));
+            if (C.steps > kStepBudget || (bad & 1)) C.overflow = true;
+            if (bad & 2) C.cap_overflow = true;
+            R = slot_rows(scr, static_cast<int>(blockIdx.x) * kTeamWaves + h);
+            if (C.overflow) {
+                team_drop(team, w, jk);
+                continue;
+            }
         }
-        wave_sync();
+        account_image(C, K, cur, R, img, nbin, area, areaimage, flags, &team, w);
+        if (mode == 3) team_drop(team, w, jk);
     }
 }
 
@@ -1930,7 +2293,8 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     // the largest epochs on large slots and the polar epochs beside the
     // Cartesian small-slot pass; launched first so their waves are resident
     mark(2 * kIrCartBig, ss.big);
-    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots)), dim3(kWave), 0,
+    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots / kTeamWaves)),
+                       dim3(kTeamWaves * kWave), 0,
                        ss.big, d_curves, opt, ep, lists, scr.cart_large, int{kPassBig});
     mark(2 * kIrCartBig + 1, ss.big);
     mark(2 * kIrPolarBig, ss.big);
@@ -1946,7 +2310,8 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_polar, ss.polar));
     if (status != LCBG_OK) return status;
     mark(2 * kIrCartSmall, st);
-    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart.n_slots)), dim3(kWave), 0, st,
+    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart.n_slots / kTeamWaves)),
+                       dim3(kTeamWaves * kWave), 0, st,
                        d_curves, opt, ep, lists, scr.cart, int{kPassSmall});
     mark(2 * kIrCartSmall + 1, st);
     status = check_launch();
@@ -1955,7 +2320,8 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     if (status != LCBG_OK) return status;
     // the epochs that outgrew a small slot, on the (now free) large slots
     mark(2 * kIrCartRetry, st);
-    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots)), dim3(kWave), 0,
+    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots / kTeamWaves)),
+                       dim3(kTeamWaves * kWave), 0,
                        st, d_curves, opt, ep, lists, scr.cart_large, int{kPassRetry});
     mark(2 * kIrCartRetry + 1, st);
     mark(2 * kIrPolarRetry, st);
@@ -1971,9 +2337,10 @@ int ir_resident_per_cu(int kind)
                    : kind == 1 ? reinterpret_cast<const void*>(k_ir_cart)
                    : kind == 2 ? reinterpret_cast<const void*>(k_ir_polar)
                                : reinterpret_cast<const void*>(k_ir_farm);
+    const int threads = kind == 1 ? kTeamWaves * kWave : kWave;  // waves per workgroup
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kWave, 0) != hipSuccess || n < 1) n = 1;
-    return n;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, 0) != hipSuccess || n < 1) n = 1;
+    return n * (threads / kWave);
 }
 
 lcbg_status launch_ir_epoch_debug(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,

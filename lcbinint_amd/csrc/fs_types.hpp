@@ -40,6 +40,9 @@ constexpr int kRowCapLarge = 1 << 18;
 constexpr int kRunCapSmall = 1 << 16;
 constexpr int kRunCapLarge = 1 << 20;
 constexpr int kRowHash = 1 << 16;
+// Cartesian workgroups: waves per team, image jobs each wave can offer
+constexpr int kTeamWaves = 4;
+constexpr int kTeamJobs = 8;
 constexpr int kPhiCap = 1 << 30;
 constexpr int kPhiHash = 1 << 16;
 // ints per slot of CartScratch::row_key: key, idx, next per row, the hash
@@ -213,6 +216,7 @@ constexpr unsigned kErrPolarCap = 4u;
 constexpr unsigned kErrUnsupported = 8u;
 constexpr unsigned kErrBudget = 16u;
 constexpr unsigned kErrSeedPool = 32u;
+constexpr unsigned kErrTeam = 64u;  // a Cartesian team wait ran out of time
 
 // Work budget of one inverse-ray epoch in wave steps (64-cell chunks, probe
 // merges, polar runs); an epoch that exceeds it ends with an error flag, so
