@@ -279,7 +279,15 @@ lcbg_status lcbg_batch_autotune(lcbg_batch* batch, const int64_t* offsets, const
  * epoch by epoch and has no counterpart.  All workspace is allocated here, so
  * a replay allocates nothing.  lcbg_batch_launch_graph replays it on `stream`
  * with the same buffers: a caller updates the curves (lcbg_batch_update_curves
- * on the same stream) and the times in place between replays. */
+ * on the same stream) and the times in place between replays.  A later run of
+ * the batch with more epochs than any before reallocates the workspace and
+ * drops the graph (lcbg_batch_launch_graph then returns LCBG_INVALID_ARGUMENT).
+ *
+ * Queue order: every run keeps each inverse-ray epoch's device time (seed
+ * search + area); the next run over the same number of epochs starts the
+ * epochs that took longest first, so that none of them starts last.  The
+ * first run, and any run over a different epoch count, orders them by the
+ * estimate |A_ps| nbin^2.  The order changes no result. */
 lcbg_status lcbg_batch_capture_finite_source(lcbg_batch* batch, const int64_t* offsets,
                                              const double* times, int64_t n_epochs, double* mag,
                                              int32_t* method, void* stream);

@@ -31,6 +31,10 @@ struct FsState {
     unsigned* counters = nullptr;
     int stage2_waves = 2048;
     int64_t last_n = 0;  // epochs of the last run (their workspace is kept)
+    // per-epoch ticks of the last run, the next run's queue order (FsEpochs::sched)
+    float* sched = nullptr;
+    int64_t sched_cap = 0;
+    int64_t sched_n = -1;  // epoch count they were measured on
     IrStreams ir{};  // side streams and events of the inverse-ray stage
     float ir_kernel_ms[kIrLaunches] = {};  // per-launch times of the last profiled run
 };
@@ -89,6 +93,37 @@ lcbg_status ensure_epochs(FsState* fs, int64_t n)
         return st;
     }
     fs->epoch_cap = cap;
+    return LCBG_OK;
+}
+
+// The scheduling keys for an n-epoch run: kept from the last run over the same
+// epoch count, zeroed (no key: the static estimate) otherwise.  A run being
+// captured into a graph finds them prepared (sync = true before the capture).
+lcbg_status ensure_sched(FsState* fs, int64_t n, hipStream_t stream, bool sync)
+{
+    if (n > fs->sched_cap || !fs->sched) {
+        if (fs->sched) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(fs->sched);
+            fs->sched = nullptr;
+        }
+        int64_t cap = 1024;
+        while (cap < n) cap *= 2;
+        lcbg_status st = map_hip(hipMalloc(&fs->sched, cap * sizeof(float)));
+        if (st != LCBG_OK) {
+            fs->sched = nullptr;
+            fs->sched_cap = 0;
+            return st;
+        }
+        fs->sched_cap = cap;
+        fs->sched_n = -1;
+    }
+    if (fs->sched_n != n) {
+        lcbg_status st = sync ? map_hip(hipMemset(fs->sched, 0, n * sizeof(float)))
+                              : map_hip(hipMemsetAsync(fs->sched, 0, n * sizeof(float), stream));
+        if (st != LCBG_OK) return st;
+        fs->sched_n = n;
+    }
     return LCBG_OK;
 }
 
@@ -269,6 +304,7 @@ void fs_destroy(FsState* fs)
     if (fs->table_mem) (void)hipFree(fs->table_mem);
     if (fs->cache_mem) (void)hipFree(fs->cache_mem);
     if (fs->epoch_mem) (void)hipFree(fs->epoch_mem);
+    if (fs->sched) (void)hipFree(fs->sched);
     if (fs->ir.polar) (void)hipStreamDestroy(fs->ir.polar);
     if (fs->ir.big) (void)hipStreamDestroy(fs->ir.big);
     for (hipEvent_t e : {fs->ir.fork, fs->ir.join_polar, fs->ir.join_big})
@@ -392,11 +428,25 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     lcbg_status st = fs_prepare(b);
     if (st != LCBG_OK) return st;
     FsState* fs = b->fs;
+    const void* epoch_mem = fs->epoch_mem;
+    const float* sched = fs->sched;
     st = ensure_epochs(fs, n_epochs > 0 ? n_epochs : 1);
     if (st != LCBG_OK) return st;
     FsEpochs ep = epoch_view(fs, n_epochs, offsets, times, mag, method);
     ep.cost = b->d_cost;
     ep.dint = dint_view(fs);
+    // the pseudo-curve batch of the dynamic path runs different epochs under
+    // the same count, so it keeps no keys
+    if (!capture && !b->dynamic_images) {
+        st = ensure_sched(fs, n_epochs, stream, false);
+        if (st != LCBG_OK) return st;
+    }
+    ep.sched = (!b->dynamic_images && fs->sched_n == n_epochs) ? fs->sched : nullptr;
+    // a graph recorded over workspace that has just been reallocated is void
+    if (b->graph && !capture && (fs->epoch_mem != epoch_mem || fs->sched != sched)) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
     fs->last_n = n_epochs;
     // timing events of a profiled run, released on every return path
     struct RunEvents {
@@ -588,6 +638,7 @@ extern "C" lcbg_status lcbg_batch_capture_finite_source(lcbg_batch* batch, const
     // everything the run allocates, allocated before the capture
     lcbg_status st = fs_prepare(batch);
     if (st == LCBG_OK) st = ensure_epochs(batch->fs, n_epochs);
+    if (st == LCBG_OK) st = ensure_sched(batch->fs, n_epochs, nullptr, true);
     if (st == LCBG_OK) {
         ScratchPool& pool = g_pool[batch->device % kMaxDevices];
         std::lock_guard<std::mutex> lock(pool.mu);

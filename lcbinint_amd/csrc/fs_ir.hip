@@ -1760,12 +1760,15 @@ __device__ __forceinline__ double cart_epoch(Ctx& C, const CurveRec& cv, const F
     return cartesian_mag(C, curve_nbin(cv, opt), scr, slot);
 }
 
-// Cost bucket of an inverse-ray epoch: the lattice cells of an epoch scale as
-// |A_ps| nbin^2, and the largest epochs are started first so that none of
-// them starts last.
-__device__ __forceinline__ int cost_bucket(double a_ps, int nbin)
+// Cost bucket of an inverse-ray epoch, largest first.  The key is the epoch's
+// measured ticks in the previous run when there is one (a fit evaluates the
+// same epochs again and again at nearby parameters), else an estimate: the
+// lattice cells of an epoch scale as |A_ps| nbin^2 (times about 32 ticks).
+__device__ __forceinline__ int cost_bucket(const FsEpochs& ep, int64_t e, int nbin)
 {
-    const double est = fabs(a_ps) * static_cast<double>(nbin) * static_cast<double>(nbin);
+    const float learned = ep.sched ? ep.sched[e] : 0.0f;
+    const double est = learned > 0.0f ? static_cast<double>(learned)
+                                      : 32.0 * fabs(ep.a_ps[e]) * static_cast<double>(nbin) * static_cast<double>(nbin);
     int lg = est >= 1.0 ? ilogb(est) : 0;
     lg = lg < kCostBuckets - 1 ? lg : kCostBuckets - 1;
     return kCostBuckets - 1 - lg;
@@ -1778,7 +1781,7 @@ __global__ void k_ir_bucket_count(const CurveRec* __restrict__ curves, FsOptions
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int64_t e = ep.ir_list[i];
         const CurveRec cv = curves[ep.curve_of[e]];
-        const int b = cost_bucket(ep.a_ps[e], curve_nbin(cv, opt));
+        const int b = cost_bucket(ep, e, curve_nbin(cv, opt));
         L.bucket_of[i] = b;
         atomicAdd(&ep.counters[kBucketOffset + b], 1u);
     }
@@ -1848,6 +1851,7 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
                 wave_store(&ep.cost[e], static_cast<int64_t>(ticks));
                 wave_store(&ep.cost[2 * ep.n + e], static_cast<int64_t>(C.n_quintic));
             }
+            if (ep.sched) wave_store(&ep.sched[e], static_cast<float>(ticks));
             wave_sync();
             continue;
         }
@@ -1889,6 +1893,8 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
             wave_store(&ep.cost[e], prior + static_cast<int64_t>(ticks));
             wave_store(&ep.cost[2 * ep.n + e], prior_q + static_cast<int64_t>(C.n_quintic));
         }
+        if (ep.sched)
+            wave_store(&ep.sched[e], (pass == 0 ? 0.0f : ep.sched[e]) + static_cast<float>(ticks));
         wave_sync();
     }
 }
@@ -1982,6 +1988,44 @@ __global__ void __launch_bounds__(kWave) k_ir_farm(const CurveRec* __restrict__ 
     if (solved) atomicAdd(&work_totals(ep)[0], static_cast<unsigned long long>(solved));
 }
 
+// The area queues in item order, i.e. largest cost key first: the seed passes
+// append items in the order their searches end (deferred searches last).  One
+// workgroup per queue, a bitonic sort in LDS; a queue longer than kSortCap
+// keeps its order.
+constexpr int kSortCap = 16384;
+
+__global__ void __launch_bounds__(1024) k_ir_sort_queues(FsEpochs ep, IrLists L)
+{
+    __shared__ int key[kSortCap];
+    int* q = blockIdx.x == 0 ? L.cart_q : blockIdx.x == 1 ? L.polar_q : blockIdx.x == 2 ? L.cart_big : L.polar_big;
+    const int word = blockIdx.x == 0 ? kQCartCount
+                   : blockIdx.x == 1 ? kQPolarCount
+                   : blockIdx.x == 2 ? kQCartBigCount : kQPolarBigCount;
+    const int n = static_cast<int>(ep.counters[word]);
+    if (n < 2 || n > kSortCap) return;
+    int m = 2;
+    while (m < n) m <<= 1;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) key[i] = i < n ? q[i] : 0x7fffffff;
+    __syncthreads();
+    for (int size = 2; size <= m; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const int a = key[i], b = key[j];
+                    if ((a > b) == up) {
+                        key[i] = b;
+                        key[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) q[i] = key[i];
+}
+
 // Cartesian areas of the queued epochs, in three passes (small slots; large
 // slots for the largest epochs, beside the small pass; large slots for the
 // epochs that outgrew a small slot).  A workgroup is a team of kTeamWaves
@@ -2009,6 +2053,11 @@ __global__ void __launch_bounds__(kTeamWaves * kWave) k_ir_cart(const CurveRec* 
     const int qword = pass == kPassSmall ? kQCartCount : pass == kPassBig ? kQCartBigCount : kQCartRetryCount;
     unsigned* cursor = &ep.counters[qword + 1];
     const int count = uni(static_cast<int>(ep.counters[qword]));
+    // the first gridDim.x items (the largest) go one per team, to its wave 0,
+    // so that each has teammates running smaller epochs, free to help; all
+    // other items come from the queue cursor
+    const int reserved = count < static_cast<int>(gridDim.x) ? count : static_cast<int>(gridDim.x);
+    int first_item = (w == 0 && static_cast<int>(blockIdx.x) < reserved) ? static_cast<int>(blockIdx.x) : -1;
     double* areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
     int* flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
     const Rows mine = slot_rows(scr, slot);
@@ -2027,7 +2076,11 @@ __global__ void __launch_bounds__(kTeamWaves * kWave) k_ir_cart(const CurveRec* 
             if (team_claim(team, w, p, jk)) {
                 mode = 2;
             } else if (!exhausted) {
-                const int qi = next_item(cursor);
+                int qi = first_item;
+                if (qi >= 0)
+                    first_item = -1;
+                else
+                    qi = reserved + next_item(cursor);
                 if (qi >= count) {
                     exhausted = true;
                     atomicAdd(&team.active, lane_id() == 0 ? -1 : 0);
@@ -2083,6 +2136,7 @@ __global__ void __launch_bounds__(kTeamWaves * kWave) k_ir_cart(const CurveRec* 
                     wave_store(&ep.cost[3 * ep.n + e], ep.cost[3 * ep.n + e] + static_cast<int64_t>(C.n_rows));
                     wave_store(&ep.cost[4 * ep.n + e], ep.cost[4 * ep.n + e] + static_cast<int64_t>(C.steps));
                 }
+                if (ep.sched) wave_store(&ep.sched[e], ep.sched[e] + static_cast<float>(ticks));
                 wave_sync();
                 have = false;
                 continue;
@@ -2221,6 +2275,7 @@ __global__ void __launch_bounds__(kWave) k_ir_polar(const CurveRec* __restrict__
             wave_store(&ep.cost[3 * ep.n + e], ep.cost[3 * ep.n + e] + static_cast<int64_t>(C.n_rows));
             wave_store(&ep.cost[4 * ep.n + e], ep.cost[4 * ep.n + e] + static_cast<int64_t>(C.steps));
         }
+        if (ep.sched) wave_store(&ep.sched[e], ep.sched[e] + static_cast<float>(ticks));
         wave_sync();
     }
 }
@@ -2284,6 +2339,7 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
                        d_curves, cache, tab, ep, lists, scr.seed.heavy);
     hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
                        d_curves, cache, tab, opt, ep, lists, scr.seed, 1);
+    hipLaunchKernelGGL(k_ir_sort_queues, dim3(4), dim3(1024), 0, st, ep, lists);
     mark(2 * kIrSeeds + 1, st);
     lcbg_status status = check_launch();
     if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.fork, st));

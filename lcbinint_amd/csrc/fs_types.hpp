@@ -115,6 +115,8 @@ struct FsEpochs {
     int64_t* ir_list;
     unsigned* counters; // [0] near count, [1] near cursor, [2] ir count, [3] ir cursor, [4] error flags
     int64_t* cost;      // [epoch] wall-clock ticks of each inverse-ray epoch, or null
+    float* sched;       // [epoch] ticks of the epoch's seed search and area in the last
+                        // run (0: none), the cost key of the next run's queues; or null
     double* dint;       // [epoch][6] tangents of the magnification (gradient runs)
 };
 

@@ -34,7 +34,8 @@ def test_point_source_curve_and_errors(gpu, ref):
     assert np.array_equal(got, exp)
     with pytest.raises(ValueError):
         lc(np.zeros((3, 3)), {})
+    # a binary source is outside this library (orbital motion is inside it)
     with pytest.raises(RuntimeError, match="unsupported"):
-        lcbinint.LightCurve(orbital_motion_mode=lcbinint.OrbitalMotionMode.CIRCULAR)(t, {"g1": 0.1})
+        lcbinint.LightCurve(source="binary")(t, {"u0": 0.1})
     with pytest.raises(RuntimeError, match="numerical error"):
         lc(t, {"tE": 0.0})
