@@ -804,7 +804,11 @@ __device__ __forceinline__ void cart_count(Ctx& C, const Cart& K, long long cell
 // scan state; only the decisions taken on it do, and they are taken in the
 // reference's order).  `split` follows the right-scan length of the previous
 // row.
-__device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi)
+//
+// `watch` (LDS, or null): the scan stops with C.overflow set at the first row
+// close that finds it no longer equal to `expect`.
+__device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi,
+                                            const int* watch, int expect)
 {
     const double incr = K.incr;
     double acc = 0.0;     // this lane's share of the row's brightness sum (countx)
@@ -951,6 +955,10 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
             break;
         }
         // ------------------------------------------------------- close the row
+        if (watch && uni(__hip_atomic_load(watch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) != expect) {
+            C.overflow = true;
+            return __builtin_nan("");
+        }
         C.n_rows = uni(C.n_rows + 1);
         const double countx = uni(wave_sum(acc));
         countall += countx;
@@ -1305,7 +1313,7 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
             }
             if (!have) break;
         }
-        const double r = cart_scan(C, K, R, start, sdy, yi);
+        const double r = cart_scan(C, K, R, start, sdy, yi, hk.watch, hk.expect);
         if (C.overflow) return out;
         if (stage == 0) {
             areai = r;
