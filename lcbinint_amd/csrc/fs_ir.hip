@@ -1086,6 +1086,7 @@ struct TeamOut {
 
 struct Team {
     int active;                          // waves still taking queue items
+    int ready;                           // wave 0 has started its first epoch
     int item[kTeamWaves][kTeamJobs];     // queue item (epoch) of each job
     int seed[kTeamWaves][kTeamJobs];     // its seed
     int status[kTeamWaves][kTeamJobs];
@@ -2053,7 +2054,10 @@ __global__ void __launch_bounds__(kTeamWaves * kWave) k_ir_cart(const CurveRec* 
         wave_store(&team.seed[w][k], 0);
         wave_store(&team.status[w][k], static_cast<int>(kJobFree));
     }
-    if (w == 0) wave_store(&team.active, kTeamWaves);
+    if (w == 0) {
+        wave_store(&team.active, kTeamWaves);
+        wave_store(&team.ready, 0);
+    }
     __syncthreads();
 
     const bool large = pass != kPassSmall;
@@ -2066,6 +2070,9 @@ __global__ void __launch_bounds__(kTeamWaves * kWave) k_ir_cart(const CurveRec* 
     // other items come from the queue cursor
     const int reserved = count < static_cast<int>(gridDim.x) ? count : static_cast<int>(gridDim.x);
     int first_item = (w == 0 && static_cast<int>(blockIdx.x) < reserved) ? static_cast<int>(blockIdx.x) : -1;
+    // the other waves take their first item once wave 0 has offered the images
+    // of its (reserved) first epoch
+    bool waited = w == 0;
     double* areaimage = scr.areaimage + static_cast<size_t>(slot) * kSeedCap;
     int* flags = scr.overlap + static_cast<size_t>(slot) * kSeedCap;
     const Rows mine = slot_rows(scr, slot);
@@ -2080,6 +2087,12 @@ __global__ void __launch_bounds__(kTeamWaves * kWave) k_ir_cart(const CurveRec* 
     for (;;) {
         int mode = 1;  // 1: own image `cur`, 2: a teammate's job, 3: helper h's image `cur`
         int p = -1, jk = -1, h = -1;
+        if (!waited) {
+            const long long t0 = static_cast<long long>(wall_clock64());
+            while (lds_get(&team.ready) == 0 && static_cast<long long>(wall_clock64()) - t0 < kTeamWaitTicks)
+                team_sleep();
+            waited = true;
+        }
         if (!have) {
             if (team_claim(team, w, p, jk)) {
                 mode = 2;
@@ -2091,6 +2104,7 @@ __global__ void __launch_bounds__(kTeamWaves * kWave) k_ir_cart(const CurveRec* 
                     qi = reserved + next_item(cursor);
                 if (qi >= count) {
                     exhausted = true;
+                    if (w == 0) lds_put(&team.ready, 1);
                     atomicAdd(&team.active, lane_id() == 0 ? -1 : 0);
                     idle_from = static_cast<long long>(wall_clock64());
                     continue;
@@ -2114,6 +2128,10 @@ __global__ void __launch_bounds__(kTeamWaves * kWave) k_ir_cart(const CurveRec* 
                 cur = -1;
                 next_offer = 0;
                 have = true;
+                // a reserved item is among the largest: its later images are
+                // offered at once (the first is the owner's)
+                if (qi < reserved) team_offer(team, w, item, flags, 0, C.n_seeds, next_offer);
+                if (w == 0) lds_put(&team.ready, 1);
                 continue;
             } else {
                 if (lds_get(&team.active) == 0) break;
