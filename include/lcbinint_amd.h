@@ -251,14 +251,16 @@ lcbg_status lcbg_batch_chi2_grad_device(lcbg_batch* batch, const int64_t* offset
  * resident waves per CU for the seed search and the small-slot Cartesian and
  * polar scans (default: as many as the slots the device's occupancy gives),
  * the inside positions a first-crossing scan probes before the probe farm
- * takes it over (default 16), and the |A_ps| * nbin above which an epoch starts
- * on a large slot (default 8192).  Results do not depend on the schedule. */
+ * takes it over (default 16), and the number of leading inverse-ray epochs (in
+ * queue order, i.e. largest cost first) whose seeds are searched first and whose
+ * area scans start on large slots while the other searches run (default 16).
+ * Results do not depend on the schedule. */
 typedef struct lcbg_schedule {
     int32_t seed_waves_per_cu;
     int32_t cart_waves_per_cu;
     int32_t polar_waves_per_cu;
     int32_t heavy_budget;
-    double big_threshold;
+    int32_t lead_items;
 } lcbg_schedule;
 
 lcbg_status lcbg_batch_set_schedule(lcbg_batch* batch, const lcbg_schedule* schedule);

@@ -324,7 +324,7 @@ class Schedule(ctypes.Structure):
     """lcbg_schedule: launch schedule of the inverse-ray stage (0 = default)."""
     _fields_ = [("seed_waves_per_cu", ctypes.c_int32), ("cart_waves_per_cu", ctypes.c_int32),
                 ("polar_waves_per_cu", ctypes.c_int32), ("heavy_budget", ctypes.c_int32),
-                ("big_threshold", ctypes.c_double)]
+                ("lead_items", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

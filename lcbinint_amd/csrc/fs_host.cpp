@@ -307,7 +307,7 @@ void fs_destroy(FsState* fs)
     if (fs->sched) (void)hipFree(fs->sched);
     if (fs->ir.polar) (void)hipStreamDestroy(fs->ir.polar);
     if (fs->ir.big) (void)hipStreamDestroy(fs->ir.big);
-    for (hipEvent_t e : {fs->ir.fork, fs->ir.join_polar, fs->ir.join_big})
+    for (hipEvent_t e : {fs->ir.fork, fs->ir.fork_lead, fs->ir.join_polar, fs->ir.join_big})
         if (e) (void)hipEventDestroy(e);
     delete fs;
 }
@@ -332,7 +332,7 @@ lcbg_status fs_prepare(lcbg_batch* b)
     fs->opt.adaptive_hex = o.adaptive_hex_threshold;
     fs->opt.ps_images = b->dynamic_images ? 1 : 0;
     fs->opt.heavy_budget = kHeavyBudget;
-    fs->opt.big_threshold = 8192.0;
+    fs->opt.lead_items = kLeadItems;
     const int bins = o.caustic_bins > 32 ? o.caustic_bins : 32;
 
     // angle tables, each expression as the reference writes it
@@ -392,7 +392,7 @@ lcbg_status fs_prepare(lcbg_batch* b)
     b->caches_stale = true;
     st = map_hip(hipStreamCreateWithFlags(&fs->ir.polar, hipStreamNonBlocking));
     if (st == LCBG_OK) st = map_hip(hipStreamCreateWithFlags(&fs->ir.big, hipStreamNonBlocking));
-    for (hipEvent_t* e : {&fs->ir.fork, &fs->ir.join_polar, &fs->ir.join_big})
+    for (hipEvent_t* e : {&fs->ir.fork, &fs->ir.fork_lead, &fs->ir.join_polar, &fs->ir.join_big})
         if (st == LCBG_OK) st = map_hip(hipEventCreateWithFlags(e, hipEventDisableTiming));
     return st;
 }
@@ -509,7 +509,7 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     scr.cart.n_slots = waves(sc.cart_waves_per_cu, scr.cart.n_slots);
     scr.polar.n_slots = waves(sc.polar_waves_per_cu, scr.polar.n_slots);
     fs->opt.heavy_budget = sc.heavy_budget > 0 ? sc.heavy_budget : kHeavyBudget;
-    fs->opt.big_threshold = sc.big_threshold > 0.0 ? sc.big_threshold : 8192.0;
+    fs->opt.lead_items = sc.lead_items > 0 ? sc.lead_items : kLeadItems;
     if (st == LCBG_OK)
         st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, lists_view(fs), scr, stream, fs->ir);
     stage_done("stage 3 (inverse ray)");
@@ -561,7 +561,7 @@ extern "C" lcbg_status lcbg_batch_set_schedule(lcbg_batch* batch, const lcbg_sch
 {
     if (!batch || !schedule) return LCBG_INVALID_ARGUMENT;
     if (schedule->seed_waves_per_cu < 0 || schedule->cart_waves_per_cu < 0 ||
-        schedule->polar_waves_per_cu < 0 || schedule->heavy_budget < 0 || !(schedule->big_threshold >= 0.0))
+        schedule->polar_waves_per_cu < 0 || schedule->heavy_budget < 0 || schedule->lead_items < 0)
         return LCBG_INVALID_ARGUMENT;
     batch->schedule = *schedule;
     return LCBG_OK;
@@ -600,7 +600,7 @@ extern "C" lcbg_status lcbg_batch_autotune(lcbg_batch* batch, const int64_t* off
     // candidates per field, tried one field at a time with the others fixed
     const int32_t waves[] = {1, 2, 3, 4};
     const int32_t budgets[] = {4, 8, 16, 32, 64};
-    const double thresholds[] = {2048.0, 4096.0, 8192.0, 16384.0, 65536.0};
+    const int32_t leads[] = {4, 8, 16, 32, 64};
     for (int field = 0; field < 5 && st == LCBG_OK; ++field) {
         const int n_cand = field == 3 ? 5 : (field == 4 ? 5 : 4);
         for (int k = 0; k < n_cand && st == LCBG_OK && runs < max_runs; ++k) {
@@ -610,7 +610,7 @@ extern "C" lcbg_status lcbg_batch_autotune(lcbg_batch* batch, const int64_t* off
             case 1: trial.cart_waves_per_cu = waves[k]; break;
             case 2: trial.polar_waves_per_cu = waves[k]; break;
             case 3: trial.heavy_budget = budgets[k]; break;
-            default: trial.big_threshold = thresholds[k]; break;
+            default: trial.lead_items = leads[k]; break;
             }
             batch->schedule = trial;
             float ms = 0.0f;

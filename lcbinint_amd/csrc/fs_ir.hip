@@ -1827,13 +1827,16 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
     Pt* slot_seeds = scr.seeds + static_cast<size_t>(blockIdx.x) * kSeedCap;
     const HeavyScratch& hv = scr.heavy;
     const int n_heavy = uni(static_cast<int>(ep.counters[kQHeavyCount]));
-    const int count = pass == 0 ? uni(static_cast<int>(ep.counters[kQIrCount]))
-                                : (n_heavy < kHeavyCap ? n_heavy : kHeavyCap);
-    unsigned* cursor = &ep.counters[pass == 0 ? kQIrCursor : kQHeavyCursor];
+    const int n_ir = uni(static_cast<int>(ep.counters[kQIrCount]));
+    const int lead = opt.lead_items < n_ir ? opt.lead_items : n_ir;
+    const int count = pass == kSeedLead ? lead
+                    : pass == kSeedMain ? n_ir - lead
+                                        : (n_heavy < kHeavyCap ? n_heavy : kHeavyCap);
+    unsigned* cursor = &ep.counters[pass == kSeedLead ? kQLeadCursor : pass == kSeedMain ? kQIrCursor : kQHeavyCursor];
     for (;;) {
         const int task = next_item(cursor);
         if (task >= count) break;
-        const int item = pass == 0 ? task : uni(hv.item[task]);
+        const int item = pass == kSeedLead ? task : pass == kSeedMain ? lead + task : uni(hv.item[task]);
         const int64_t e = L.ir_sorted[item];
         const long long t_begin = static_cast<long long>(wall_clock64());
         const int c = ep.curve_of[e];
@@ -1841,7 +1844,7 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
         Ctx C;
         init_ctx(C, cv, ep.src[e], slot_seeds);
         C.hints = opt.ps_images != 0;
-        if (pass == 0) {
+        if (pass != kSeedResume) {
             C.defer_budget = opt.heavy_budget;
             C.heavy_count = &ep.counters[kQHeavyCount];
         } else {
@@ -1880,9 +1883,10 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
             wave_store(&L.seed_off[item], off);
             wave_store(&L.seed_n[item], n);
             const bool polar = uni(ep.method[e]) == LCBG_METHOD_INVERSE_RAY_POLAR;
-            // an image of |A| times the source's area scans about 2 |A| nbin
-            // rows; such epochs start on a large slot right away
-            const bool large = fabs(ep.a_ps[e]) * static_cast<double>(curve_nbin(cv, opt)) > opt.big_threshold;
+            // the leading items start on large slots while the other searches
+            // run; the rest wait for every search to end (an epoch that outgrows
+            // a small slot is retried on a large one)
+            const bool large = pass == kSeedLead;
             const int qword = polar ? (large ? kQPolarBigCount : kQPolarCount)
                                     : (large ? kQCartBigCount : kQCartCount);
             int* qarr = polar ? (large ? L.polar_big : L.polar_q) : (large ? L.cart_big : L.cart_q);
@@ -1897,13 +1901,13 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
         const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
         add_total(&work_totals(ep)[5], ticks);
         if (ep.cost) {
-            const int64_t prior = pass == 0 ? 0 : ep.cost[e];
-            const int64_t prior_q = pass == 0 ? 0 : ep.cost[2 * ep.n + e];
+            const int64_t prior = pass != kSeedResume ? 0 : ep.cost[e];
+            const int64_t prior_q = pass != kSeedResume ? 0 : ep.cost[2 * ep.n + e];
             wave_store(&ep.cost[e], prior + static_cast<int64_t>(ticks));
             wave_store(&ep.cost[2 * ep.n + e], prior_q + static_cast<int64_t>(C.n_quintic));
         }
         if (ep.sched)
-            wave_store(&ep.sched[e], (pass == 0 ? 0.0f : ep.sched[e]) + static_cast<float>(ticks));
+            wave_store(&ep.sched[e], (pass != kSeedResume ? 0.0f : ep.sched[e]) + static_cast<float>(ticks));
         wave_sync();
     }
 }
@@ -1999,17 +2003,17 @@ __global__ void __launch_bounds__(kWave) k_ir_farm(const CurveRec* __restrict__ 
 
 // The area queues in item order, i.e. largest cost key first: the seed passes
 // append items in the order their searches end (deferred searches last).  One
-// workgroup per queue, a bitonic sort in LDS; a queue longer than kSortCap
-// keeps its order.
+// workgroup per queue (queues q0, q0 + 1, ...: small Cartesian, small polar,
+// large Cartesian, large polar), a bitonic sort in LDS; a queue longer than
+// kSortCap keeps its order.
 constexpr int kSortCap = 16384;
 
-__global__ void __launch_bounds__(1024) k_ir_sort_queues(FsEpochs ep, IrLists L)
+__global__ void __launch_bounds__(1024) k_ir_sort_queues(FsEpochs ep, IrLists L, int q0)
 {
     __shared__ int key[kSortCap];
-    int* q = blockIdx.x == 0 ? L.cart_q : blockIdx.x == 1 ? L.polar_q : blockIdx.x == 2 ? L.cart_big : L.polar_big;
-    const int word = blockIdx.x == 0 ? kQCartCount
-                   : blockIdx.x == 1 ? kQPolarCount
-                   : blockIdx.x == 2 ? kQCartBigCount : kQPolarBigCount;
+    const int qn = q0 + static_cast<int>(blockIdx.x);
+    int* q = qn == 0 ? L.cart_q : qn == 1 ? L.polar_q : qn == 2 ? L.cart_big : L.polar_big;
+    const int word = qn == 0 ? kQCartCount : qn == 1 ? kQPolarCount : qn == 2 ? kQCartBigCount : kQPolarBigCount;
     const int n = static_cast<int>(ep.counters[word]);
     if (n < 2 || n > kSortCap) return;
     int m = 2;
@@ -2356,39 +2360,52 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     hipLaunchKernelGGL(k_ir_bucket_scan, dim3(1), dim3(64), 0, st, ep);
     hipLaunchKernelGGL(k_ir_bucket_scatter, dim3(256), dim3(256), 0, st, ep, lists);
     mark(2 * kIrSeeds, st);
+    // the leading items' seeds, then their areas on large slots (big stream)
+    // while the other searches run
     hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
-                       d_curves, cache, tab, opt, ep, lists, scr.seed, 0);
-    // the deferred first-crossing scans: list, farm, resume
-    hipLaunchKernelGGL(k_ir_heavy_list, dim3(kHeavyCap), dim3(kWave), 0, st, d_curves, cache, ep, lists,
-                       scr.seed.heavy);
-    hipLaunchKernelGGL(k_ir_farm, dim3(static_cast<unsigned>(scr.farm_slots)), dim3(kWave), 0, st,
-                       d_curves, cache, tab, ep, lists, scr.seed.heavy);
-    hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
-                       d_curves, cache, tab, opt, ep, lists, scr.seed, 1);
-    hipLaunchKernelGGL(k_ir_sort_queues, dim3(4), dim3(1024), 0, st, ep, lists);
-    mark(2 * kIrSeeds + 1, st);
+                       d_curves, cache, tab, opt, ep, lists, scr.seed, int{kSeedLead});
+    hipLaunchKernelGGL(k_ir_sort_queues, dim3(2), dim3(1024), 0, st, ep, lists, 2);
     lcbg_status status = check_launch();
-    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.fork, st));
-    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.polar, ss.fork, 0));
-    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.big, ss.fork, 0));
+    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.fork_lead, st));
+    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.big, ss.fork_lead, 0));
     if (status != LCBG_OK) return status;
-    // the largest epochs on large slots and the polar epochs beside the
-    // Cartesian small-slot pass; launched first so their waves are resident
     mark(2 * kIrCartBig, ss.big);
     hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots / kTeamWaves)),
                        dim3(kTeamWaves * kWave), 0,
                        ss.big, d_curves, opt, ep, lists, scr.cart_large, int{kPassBig});
     mark(2 * kIrCartBig + 1, ss.big);
-    mark(2 * kIrPolarBig, ss.big);
+    status = check_launch();
+    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_big, ss.big));
+    // the leading polar epochs on large slots (polar stream)
+    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.polar, ss.fork_lead, 0));
+    if (status != LCBG_OK) return status;
+    mark(2 * kIrPolarBig, ss.polar);
     hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar_large.n_slots)), dim3(kWave), 0,
-                       ss.big, d_curves, opt, ep, lists, scr.polar_large, int{kPassBig});
-    mark(2 * kIrPolarBig + 1, ss.big);
+                       ss.polar, d_curves, opt, ep, lists, scr.polar_large, int{kPassBig});
+    mark(2 * kIrPolarBig + 1, ss.polar);
+    status = check_launch();
+    if (status != LCBG_OK) return status;
+    // all other searches; the deferred first-crossing scans: list, farm, resume
+    hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
+                       d_curves, cache, tab, opt, ep, lists, scr.seed, int{kSeedMain});
+    hipLaunchKernelGGL(k_ir_heavy_list, dim3(kHeavyCap), dim3(kWave), 0, st, d_curves, cache, ep, lists,
+                       scr.seed.heavy);
+    hipLaunchKernelGGL(k_ir_farm, dim3(static_cast<unsigned>(scr.farm_slots)), dim3(kWave), 0, st,
+                       d_curves, cache, tab, ep, lists, scr.seed.heavy);
+    hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
+                       d_curves, cache, tab, opt, ep, lists, scr.seed, int{kSeedResume});
+    hipLaunchKernelGGL(k_ir_sort_queues, dim3(2), dim3(1024), 0, st, ep, lists, 0);
+    mark(2 * kIrSeeds + 1, st);
+    status = check_launch();
+    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.fork, st));
+    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.polar, ss.fork, 0));
+    if (status != LCBG_OK) return status;
+    // the polar small-slot pass beside the Cartesian one
     mark(2 * kIrPolarSmall, ss.polar);
     hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar.n_slots)), dim3(kWave), 0,
                        ss.polar, d_curves, opt, ep, lists, scr.polar, int{kPassSmall});
     mark(2 * kIrPolarSmall + 1, ss.polar);
     status = check_launch();
-    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_big, ss.big));
     if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_polar, ss.polar));
     if (status != LCBG_OK) return status;
     mark(2 * kIrCartSmall, st);

@@ -30,7 +30,7 @@ lcbg_status launch_fs_stage2(const CurveRec* d_curves, const FsCache& cache, con
 struct IrStreams {
     hipStream_t polar;   // polar small-slot pass
     hipStream_t big;     // large-slot pass of the largest epochs
-    hipEvent_t fork, join_polar, join_big;
+    hipEvent_t fork, fork_lead, join_polar, join_big;
     // per-launch timing (profiled runs only, else null): events 2k and 2k+1
     // bracket launch k on the stream it runs on, in the order of IrLaunch
     hipEvent_t* kev;
@@ -40,7 +40,7 @@ struct IrStreams {
 enum IrLaunch {
     kIrSeeds = 0,      // k_ir_seeds, main stream
     kIrCartBig = 1,    // k_ir_cart, large slots, big-epoch stream
-    kIrPolarBig = 2,   // k_ir_polar, large slots, big-epoch stream
+    kIrPolarBig = 2,   // k_ir_polar, large slots, polar stream
     kIrPolarSmall = 3, // k_ir_polar, small slots, polar stream
     kIrCartSmall = 4,  // k_ir_cart, small slots, main stream
     kIrCartRetry = 5,  // k_ir_cart, large slots, main stream

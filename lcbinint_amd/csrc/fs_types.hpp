@@ -94,10 +94,10 @@ struct FsOptions {
     // curves with parallax or orbital motion; 0: binary_mag0 of the static path
     int ps_images;
     // schedule (lcbg_schedule): inside positions a seed search probes before the
-    // probe farm takes it over, and the |A_ps| nbin above which an epoch starts
-    // on a large slot
+    // probe farm takes it over, and the leading items (largest cost key) whose
+    // seeds are searched first and whose areas start on large slots at once
     int heavy_budget;
-    double big_threshold;
+    int lead_items;
 };
 
 struct FsEpochs {
@@ -194,8 +194,8 @@ struct IrLists {
 // Queue words: near and inverse-ray epoch lists, error flags, the Cartesian
 // and polar queues of the small-slot pass, of the large-slot pass for the
 // largest epochs, and of the large-slot retry pass, the seed-pool cursor.
-constexpr int kWorkOffset = 24;
-constexpr int kBucketOffset = 40;
+constexpr int kWorkOffset = 26;
+constexpr int kBucketOffset = 42;
 constexpr int kCounterWords = kBucketOffset + 2 * kCostBuckets;
 enum : int {
     kQNearCount = 0, kQNearCursor = 1, kQIrCount = 2, kQIrCursor = 3, kQFlags = 4,
@@ -203,13 +203,18 @@ enum : int {
     kQCartRetryCount = 9, kQCartRetryCursor = 10, kQPolarRetryCount = 11,
     kQPolarRetryCursor = 12, kQPoolCursor = 13, kQCartBigCount = 14, kQCartBigCursor = 15,
     kQPolarBigCount = 16, kQPolarBigCursor = 17, kQGradCursor = 18, kQGradRetryCount = 19,
-    kQGradRetryCursor = 20, kQHeavyCount = 21, kQHeavyCursor = 22, kQFarmCursor = 23
+    kQGradRetryCursor = 20, kQHeavyCount = 21, kQHeavyCursor = 22, kQFarmCursor = 23,
+    kQLeadCursor = 24
 };
 // derivatives per epoch: t0, tE, u0, alpha, s, q, rho, u_LD
 constexpr int kGradParams = 8;
+// leading inverse-ray items searched first and scanned on large slots at once
+constexpr int kLeadItems = 16;
 // inverse-ray area passes: small slots, large slots for the largest epochs
 // (beside the small pass), large slots for the epochs a small slot could not hold
 enum : int { kPassSmall = 0, kPassBig = 1, kPassRetry = 2 };
+// seed-search passes: all remaining searches, the deferred ones resumed, the leading items
+enum : int { kSeedMain = 0, kSeedResume = 1, kSeedLead = 2 };
 
 // error flags in counters[4]
 constexpr unsigned kErrSeedCap = 1u;

@@ -97,7 +97,7 @@ def test_autotuner_keeps_results(gpu):
     assert np.array_equal(m0.cpu().numpy().view(np.int64), m1.cpu().numpy().view(np.int64))
     # an extreme schedule: one wave per CU everywhere, the farm taking every scan
     capi.set_schedule(b, capi.Schedule(seed_waves_per_cu=1, cart_waves_per_cu=1, polar_waves_per_cu=1,
-                                       heavy_budget=1, big_threshold=1.0))
+                                       heavy_budget=1, lead_items=1))
     capi.finite_source(b, d_off.data_ptr(), d_t.data_ptr(), n, m1.data_ptr(), k1.data_ptr(), stream)
     torch.cuda.synchronize()
     assert torch.equal(k0, k1)
