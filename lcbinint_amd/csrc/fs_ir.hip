@@ -645,7 +645,7 @@ __device__ __forceinline__ double seq_lanes(double x0, double step)
             cand = base;
         else if (lane > b)
             cand = base + static_cast<double>(lane - b) * d;
-        const double prev = __shfl_up(cand, 1, kWave);
+        const double prev = lane_prev(cand);
         const uint64_t bad = ballot(lane > b && !(cand == prev + step));
         if (!bad) break;
         const int f = first_set(bad);
@@ -675,7 +675,7 @@ __device__ __forceinline__ double seq_lanes2(double xa, double sa, double xb, do
             cand = base;
         else if (lane > b)
             cand = base + static_cast<double>(lane - b) * d;
-        const double prev = __shfl_up(cand, 1, kWave);
+        const double prev = lane_prev(cand);
         const uint64_t bad = ballot(lane > b && !(cand == prev + step));
         if (!bad) break;
         const uint64_t bad0 = bad & lo_mask;
@@ -2046,7 +2046,7 @@ __global__ void __launch_bounds__(1024) k_ir_sort_queues(FsEpochs ep, IrLists L,
 // Every wave loops: help a teammate if a job is offered, else run its own
 // epoch, whose images it computes itself or takes from the teammate that
 // computed them, accounting them in seed order.
-__global__ void __launch_bounds__(kTeamWaves * kWave) k_ir_cart(const CurveRec* __restrict__ curves,
+__global__ void __launch_bounds__(kTeamWaves * kWave, 3) k_ir_cart(const CurveRec* __restrict__ curves,
                                                                 FsOptions opt, FsEpochs ep, IrLists L,
                                                                 CartScratch scr, int pass)
 {

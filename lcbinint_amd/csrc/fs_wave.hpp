@@ -46,11 +46,37 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
+// A double moved across lanes by a DPP control (GFX9 encodings: quad_perm
+// 0x00-0xff, wave_shr:1 0x138, row_mirror 0x140, row_half_mirror 0x141,
+// row_bcast:15 0x142, row_bcast:31 0x143); lanes that receive nothing (rows
+// outside kRowMask, lane 0 of a shift) get 0.  These moves stay in the VALU,
+// where __shfl goes through the LDS crossbar.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(u & 0xffffffffull), kCtrl, kRowMask, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(u >> 32), kCtrl, kRowMask, 0xf, false);
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                                       static_cast<unsigned>(lo)));
+}
+
+// Value of lane - 1 (lane 0: 0).
+__device__ __forceinline__ double lane_prev(double v) { return dpp_move<0x138>(v); }
+
+// Sum over the wave, as a scalar: pairs, quads, half rows, rows by DPP, then
+// row 1 += row 0 and row 3 += row 2 (row_bcast:15), rows 2-3 += rows 0-1
+// (row_bcast:31); lane 63 holds the total.
+__device__ __forceinline__ double read_lane(double v, int l);
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
+    v += dpp_move<0xB1>(v);       // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E>(v);       // quad_perm [2,3,0,1]
+    v += dpp_move<0x141>(v);      // row_half_mirror
+    v += dpp_move<0x140>(v);      // row_mirror
+    v += dpp_move<0x142, 0xa>(v); // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v); // row_bcast:31 into rows 2 and 3
+    return read_lane(v, kWave - 1);
 }
 
 // v of lane l (l uniform), as a scalar.
