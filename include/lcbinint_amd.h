@@ -253,7 +253,7 @@ lcbg_status lcbg_batch_chi2_grad_device(lcbg_batch* batch, const int64_t* offset
  * the inside positions a first-crossing scan probes before the probe farm
  * takes it over (default 16), and the number of leading inverse-ray epochs (in
  * queue order, i.e. largest cost first) whose seeds are searched first and whose
- * area scans start on large slots while the other searches run (default 16).
+ * area scans start on large slots while the other searches run (default 32).
  * Results do not depend on the schedule. */
 typedef struct lcbg_schedule {
     int32_t seed_waves_per_cu;

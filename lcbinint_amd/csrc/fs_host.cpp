@@ -225,7 +225,7 @@ lcbg_status ensure_scratch(ScratchPool& pool)
     const int cart_occ = ir_resident_per_cu(1);
     const int cart_slots = cus * (cart_occ < 8 ? cart_occ : 8);
     const int polar_slots = cus;  // few polar epochs; they share the CUs with the Cartesian pass
-    const int large_slots = 64;
+    const int large_slots = 128;  // 32 Cartesian teams for the leading items
     IrScratch& s = pool.scr;
     pool.cus = cus;
     s.seed.n_slots = seed_slots;
@@ -600,7 +600,7 @@ extern "C" lcbg_status lcbg_batch_autotune(lcbg_batch* batch, const int64_t* off
     // candidates per field, tried one field at a time with the others fixed
     const int32_t waves[] = {1, 2, 3, 4};
     const int32_t budgets[] = {4, 8, 16, 32, 64};
-    const int32_t leads[] = {4, 8, 16, 32, 64};
+    const int32_t leads[] = {8, 16, 32, 48, 64};
     for (int field = 0; field < 5 && st == LCBG_OK; ++field) {
         const int n_cand = field == 3 ? 5 : (field == 4 ? 5 : 4);
         for (int k = 0; k < n_cand && st == LCBG_OK && runs < max_runs; ++k) {

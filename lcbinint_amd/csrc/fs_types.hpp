@@ -209,7 +209,7 @@ enum : int {
 // derivatives per epoch: t0, tE, u0, alpha, s, q, rho, u_LD
 constexpr int kGradParams = 8;
 // leading inverse-ray items searched first and scanned on large slots at once
-constexpr int kLeadItems = 16;
+constexpr int kLeadItems = 32;
 // inverse-ray area passes: small slots, large slots for the largest epochs
 // (beside the small pass), large slots for the epochs a small slot could not hold
 enum : int { kPassSmall = 0, kPassBig = 1, kPassRetry = 2 };
