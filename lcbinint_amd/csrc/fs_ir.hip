@@ -782,14 +782,6 @@ __device__ __forceinline__ bool cart_cell(const Ctx& C, const Cart& K, double x,
     return in;
 }
 
-__device__ __forceinline__ void cart_count(Ctx& C, const Cart& K, long long cells)
-{
-    if (K.jac_sign == 0)
-        C.n_cart = uni(C.n_cart + cells);
-    else
-        C.n_cart_guarded = uni(C.n_cart_guarded + cells);
-}
-
 // cartesian_image_area_impl (finite_source_magnifier.cpp:1892-2014).  The
 // scan keeps the current row's xmin/xmax and the previous row's xmin in
 // registers (written through to the row records), and each lane keeps its
@@ -807,8 +799,13 @@ __device__ __forceinline__ void cart_count(Ctx& C, const Cart& K, long long cell
 //
 // `watch` (LDS, or null): the scan stops with C.overflow set at the first row
 // close that finds it no longer equal to `expect`.
-__device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi,
-                                            const int* watch, int expect)
+//
+// The scan counts its cells (guard), wave steps and rows in locals; cart_scan
+// adds them to the epoch's counts when it returns.  `room` is the epoch's
+// remaining step budget.
+__device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi,
+                                                 const int* watch, int expect, long long& guard, int& steps,
+                                                 int& rows, int room)
 {
     const double incr = K.incr;
     double acc = 0.0;     // this lane's share of the row's brightness sum (countx)
@@ -818,7 +815,6 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
     double x = seed.x;    // first cell of the right scan
     double y = seed.y;
     double x0 = seed.x;
-    long long guard = 0;
     const long long budget = K.max_steps - 1;
     int split = kWave / 2;
     if (!rows_ensure(C, R, yi)) return __builtin_nan("");
@@ -836,7 +832,10 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
         bool s_in = false;
         int right_cells = 0;
         for (bool first = true;; first = false) {
-            if (!step_ok(C)) return __builtin_nan("");
+            if (++steps > room) {
+                C.overflow = true;
+                return __builtin_nan("");
+            }
             const long long remaining = budget - guard;
             if (remaining <= 0) return countall;
             const bool combined = first && remaining >= kWave;
@@ -859,7 +858,6 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
                 any_w = any_w || ballot(wv != 0.0) != 0;
                 guard += n_eval;
                 right_cells += n_eval;
-                cart_count(C, K, n_eval);
                 last_in = true;
                 x = read_lane(xk, n_eval - 1) + incr;
                 continue;
@@ -869,7 +867,6 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
             any_w = any_w || ballot(wl != 0.0) != 0;
             guard += s + 1;
             right_cells += s + 1;
-            cart_count(C, K, s + 1);
             const double xs = read_lane(xk, s);
             const bool prev_in = s > 0 ? true : last_in;
             if (prev_in) {
@@ -887,7 +884,10 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
         const double thr = yi != 0 ? prev_xmin - (-incr) : 0.0;
         bool use_stash = stash_n > 0 && budget - guard >= stash_n;
         for (;;) {
-            if (!step_ok(C)) return __builtin_nan("");
+            if (++steps > room) {
+                C.overflow = true;
+                return __builtin_nan("");
+            }
             const long long remaining = budget - guard;
             if (remaining <= 0) return countall;
             int lo, n_eval;
@@ -939,13 +939,11 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
             any_w = any_w || ballot(wl != 0.0) != 0;
             if (s < 0) {
                 guard += n_eval;
-                cart_count(C, K, n_eval);
                 last_in = ((in_m >> (end - 1)) & 1ull) != 0;
                 x = read_lane(xk, end - 1) + (-incr);
                 continue;
             }
             guard += s - lo + 1;
-            cart_count(C, K, s - lo + 1);
             const double xs = read_lane(xk, s);
             const bool prev_in = s > lo ? (((in_m >> (s - 1)) & 1ull) != 0) : last_in;
             if (prev_in) {
@@ -959,7 +957,7 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
             C.overflow = true;
             return __builtin_nan("");
         }
-        C.n_rows = uni(C.n_rows + 1);
+        ++rows;
         const double countx = uni(wave_sum(acc));
         countall += countx;
         rset(C, R, yi, AX, countx);
@@ -1018,6 +1016,23 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
         x = x + incr;
         split = iclamp(right_cells + 4, 8, 32);
     }
+}
+
+__device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi,
+                                            const int* watch, int expect)
+{
+    long long guard = 0;
+    int steps = 0, rows = 0;
+    const long long left = kStepBudget - C.steps;
+    const int room = left < 0 ? 0 : static_cast<int>(left);
+    const double r = cart_scan_rows(C, K, R, seed, dy, yi, watch, expect, guard, steps, rows, room);
+    C.steps = uni(C.steps + steps);
+    if (K.jac_sign == 0)
+        C.n_cart = uni(C.n_cart + guard);
+    else
+        C.n_cart_guarded = uni(C.n_cart_guarded + guard);
+    C.n_rows = uni(C.n_rows + rows);
+    return r;
 }
 
 // ----------------------------------------------------------- Cartesian images
