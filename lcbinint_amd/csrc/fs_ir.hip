@@ -804,8 +804,8 @@ __device__ __forceinline__ bool cart_cell(const Ctx& C, const Cart& K, double x,
 // adds them to the epoch's counts when it returns.  `room` is the epoch's
 // remaining step budget.
 __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi,
-                                                 const int* watch, int expect, long long& guard, int& steps,
-                                                 int& rows, int room)
+                                                 double prev_xmin_in, const int* watch, int expect,
+                                                 long long& guard, int& steps, int& rows, int room)
 {
     const double incr = K.incr;
     double acc = 0.0;     // this lane's share of the row's brightness sum (countx)
@@ -818,9 +818,14 @@ __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R,
     const long long budget = K.max_steps - 1;
     int split = kWave / 2;
     if (!rows_ensure(C, R, yi)) return __builtin_nan("");
-    double cur_xmin = uni(rget(R, yi, XMIN));
-    double cur_xmax = uni(rget(R, yi, XMAX));
-    double prev_xmin = yi != 0 ? uni(rget(R, yi - 1, XMIN)) : 0.0;
+    // a row's xmin is set once its right scan ends, and its xmax before any
+    // read whenever the row counts (the right scan sets it at its end unless
+    // its first cell is out, and the left scan then sets it at its first
+    // inside cell); a row that counts nothing returns before reading either.
+    // So the record's old values are never needed: the scan starts them at 0.
+    double cur_xmin = 0.0;
+    double cur_xmax = 0.0;
+    double prev_xmin = prev_xmin_in;  // row yi - 1's xmin (0 for yi = 0)
     for (;;) {
         // the row's key and its chain head, loaded now for the duplicate check at
         // the row's close (only this row's own push changes the heads)
@@ -1001,13 +1006,9 @@ __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R,
         if (!rows_ensure(C, R, yi)) return __builtin_nan("");
         prev_xmin = cur_xmin;
         x0 = cur_xmax;
-        if (fresh) {
-            cur_xmin = 0.0;
-            cur_xmax = 0.0;
-        } else {
-            cur_xmin = uni(rget(R, yi, XMIN));
-            cur_xmax = uni(rget(R, yi, XMAX));
-        }
+        (void)fresh;
+        cur_xmin = 0.0;
+        cur_xmax = 0.0;
         x = x0 - incr;
         y = y + dy;
         acc = 0.0;
@@ -1019,13 +1020,13 @@ __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R,
 }
 
 __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi,
-                                            const int* watch, int expect)
+                                            double prev_xmin, const int* watch, int expect)
 {
     long long guard = 0;
     int steps = 0, rows = 0;
     const long long left = kStepBudget - C.steps;
     const int room = left < 0 ? 0 : static_cast<int>(left);
-    const double r = cart_scan_rows(C, K, R, seed, dy, yi, watch, expect, guard, steps, rows, room);
+    const double r = cart_scan_rows(C, K, R, seed, dy, yi, prev_xmin, watch, expect, guard, steps, rows, room);
     C.steps = uni(C.steps + steps);
     if (K.jac_sign == 0)
         C.n_cart = uni(C.n_cart + guard);
@@ -1106,6 +1107,13 @@ struct Team {
     int seed[kTeamWaves][kTeamJobs];     // its seed
     int status[kTeamWaves][kTeamJobs];
     TeamOut out[kTeamWaves];             // result of the job each helper finished
+    // each wave's epoch-loop state while it scans an image (the scan does not
+    // use it, so it is kept here rather than in registers)
+    struct {
+        double area;
+        long long e, t_begin, idle_from;
+        int item, nbin, first_item, count, reserved, bits;
+    } park[kTeamWaves];
 };
 
 __device__ __forceinline__ int lds_get(const int* p)
@@ -1242,6 +1250,7 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
         if (!hook_poll(C, hk)) return out;
         Pt start{0.0, 0.0};
         double sdy = 0.0;
+        double prev_xmin = 0.0;  // the xmin just written to row yi - 1 (stages 1, 2)
         if (stage == 0) {
             rset(C, R, 0, XMIN, seed.x);
             rset(C, R, 0, XMAX, seed.x);
@@ -1251,7 +1260,8 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
             sdy = -incr;
             if (!rows_ensure(C, R, yi)) return out;
             start = Pt{rget(R, 0, XMAX), seed.y + sdy};
-            rset(C, R, yi, XMIN, rget(R, 0, XMIN));
+            prev_xmin = uni(rget(R, 0, XMIN));
+            rset(C, R, yi, XMIN, prev_xmin);
             rset(C, R, yi, XMAX, rget(R, 0, XMAX));
             rset(C, R, yi, YY, rget(R, 0, YY));
             rset(C, R, yi, DYS, sdy);
@@ -1321,6 +1331,7 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
                     nxmin = uni(nxmin);
                     nxmax = uni(nxmax);
                     if (!rows_ensure(C, R, yi)) return out;
+                    prev_xmin = nxmin;
                     rset(C, R, yi, XMIN, nxmin);
                     rset(C, R, yi, XMAX, nxmax);
                     rset(C, R, yi, DYS, sdy);
@@ -1329,7 +1340,7 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
             }
             if (!have) break;
         }
-        const double r = cart_scan(C, K, R, start, sdy, yi, hk.watch, hk.expect);
+        const double r = cart_scan(C, K, R, start, sdy, yi, prev_xmin, hk.watch, hk.expect);
         if (C.overflow) return out;
         if (stage == 0) {
             areai = r;
@@ -2083,11 +2094,11 @@ __global__ void __launch_bounds__(kTeamWaves * kWave, 3) k_ir_cart(const CurveRe
     const int* queue = pass == kPassSmall ? L.cart_q : pass == kPassBig ? L.cart_big : L.cart_retry;
     const int qword = pass == kPassSmall ? kQCartCount : pass == kPassBig ? kQCartBigCount : kQCartRetryCount;
     unsigned* cursor = &ep.counters[qword + 1];
-    const int count = uni(static_cast<int>(ep.counters[qword]));
+    int count = uni(static_cast<int>(ep.counters[qword]));
     // the first gridDim.x items (the largest) go one per team, to its wave 0,
     // so that each has teammates running smaller epochs, free to help; all
     // other items come from the queue cursor
-    const int reserved = count < static_cast<int>(gridDim.x) ? count : static_cast<int>(gridDim.x);
+    int reserved = count < static_cast<int>(gridDim.x) ? count : static_cast<int>(gridDim.x);
     int first_item = (w == 0 && static_cast<int>(blockIdx.x) < reserved) ? static_cast<int>(blockIdx.x) : -1;
     // the other waves take their first item once wave 0 has offered the images
     // of its (reserved) first epoch
@@ -2233,7 +2244,36 @@ __global__ void __launch_bounds__(kTeamWaves * kWave, 3) k_ir_cart(const CurveRe
                 hk.armed = false;
                 if (j >= C.n_seeds) C.overflow = true;
             }
+            {
+                auto& pk = team.park[w];
+                wave_store(&pk.area, area);
+                wave_store(&pk.e, static_cast<long long>(e));
+                wave_store(&pk.t_begin, t_begin);
+                wave_store(&pk.idle_from, idle_from);
+                wave_store(&pk.item, item);
+                wave_store(&pk.nbin, nbin);
+                wave_store(&pk.first_item, first_item);
+                wave_store(&pk.count, count);
+                wave_store(&pk.reserved, reserved);
+                wave_store(&pk.bits, (have ? 1 : 0) | (exhausted ? 2 : 0) | (waited ? 4 : 0));
+            }
             if (!C.overflow) img = image_area(C, K, R, C.seeds[j], hk);
+            {
+                const auto& pk = team.park[w];
+                area = uni(pk.area);
+                e = uni(pk.e);
+                t_begin = uni(pk.t_begin);
+                idle_from = uni(pk.idle_from);
+                item = uni(pk.item);
+                nbin = uni(pk.nbin);
+                first_item = uni(pk.first_item);
+                count = uni(pk.count);
+                reserved = uni(pk.reserved);
+                const int bits = uni(pk.bits);
+                have = (bits & 1) != 0;
+                exhausted = (bits & 2) != 0;
+                waited = (bits & 4) != 0;
+            }
             if (mode == 2) {
                 TeamOut& o = team.out[w];
                 wave_store(&o.area, img.area);
