@@ -29,7 +29,7 @@ struct FsState {
     int64_t epoch_cap = 0;
     void* epoch_mem = nullptr;
     unsigned* counters = nullptr;
-    int stage2_waves = 2048;
+    int stage2_waves = 8192;
     int64_t last_n = 0;  // epochs of the last run (their workspace is kept)
     // per-epoch ticks of the last run, the next run's queue order (FsEpochs::sched)
     float* sched = nullptr;
@@ -86,7 +86,8 @@ lcbg_status ensure_epochs(FsState* fs, int64_t n)
                               carve_size(kCounterAlloc * sizeof(unsigned)) +
                               carve_size(cap * sizeof(int64_t)) + carve_size(cap * sizeof(int)) * 9 +
                               carve_size(pool_cap_for(cap) * sizeof(Pt2)) +
-                              carve_size(cap * kTangentsHost * sizeof(double));
+                              carve_size(cap * kTangentsHost * sizeof(double)) +
+                              carve_size(cap * sizeof(int64_t)) * 3;
     lcbg_status st = map_hip(hipMalloc(&fs->epoch_mem, bytes));
     if (st != LCBG_OK) {
         fs->epoch_mem = nullptr;
@@ -194,6 +195,30 @@ double* dint_view(FsState* fs)
     for (int k = 0; k < 9; ++k) (void)carve<int>(p, cap);
     (void)carve<Pt2>(p, pool_cap_for(cap));
     return carve<double>(p, cap * kTangentsHost);
+}
+
+// The hexadecapole queue of stage 2 and the per-curve queue of stage 1,
+// after the tangents.
+int64_t* hex_view(FsState* fs)
+{
+    char* p = reinterpret_cast<char*>(dint_view(fs));
+    (void)carve<double>(p, fs->epoch_cap * kTangentsHost);
+    return carve<int64_t>(p, fs->epoch_cap);
+}
+
+int64_t* near_curve_view(FsState* fs)
+{
+    char* p = reinterpret_cast<char*>(hex_view(fs));
+    (void)carve<int64_t>(p, fs->epoch_cap);
+    return carve<int64_t>(p, fs->epoch_cap);
+}
+
+// Work items of the stage-2 distances (at most one per queued epoch).
+int64_t* near_chunks_view(FsState* fs)
+{
+    char* p = reinterpret_cast<char*>(near_curve_view(fs));
+    (void)carve<int64_t>(p, fs->epoch_cap);
+    return carve<int64_t>(p, fs->epoch_cap);
 }
 
 // Inverse-ray scratch, one pool per device shared by every batch: its use is
@@ -375,8 +400,10 @@ lcbg_status fs_prepare(lcbg_batch* b)
     const std::size_t per = static_cast<std::size_t>(nc);
     const std::size_t bytes = carve_size(per * bins * 4 * sizeof(Pt2)) * 2 +
                               carve_size(per * bins * 4 * sizeof(int)) * 3 +
+                              carve_size(per * (kGrid + 1) * sizeof(int)) +
                               carve_size(per * kSeedPhases * 4 * sizeof(Pt2)) +
-                              carve_size(per * kSeedPhases * 4) + carve_size(per * sizeof(CausticScalars));
+                              carve_size(per * kSeedPhases * 4) + carve_size(per * sizeof(CausticScalars)) +
+                              carve_size(per * sizeof(int));
     st = map_hip(hipMalloc(&fs->cache_mem, bytes));
     if (st != LCBG_OK) return st;
     char* p = static_cast<char*>(fs->cache_mem);
@@ -386,9 +413,11 @@ lcbg_status fs_prepare(lcbg_batch* b)
     fs->cache.pt_cell = carve<int>(p, per * bins * 4);
     fs->cache.br_cell = carve<int>(p, per * bins * 4);
     fs->cache.br_order = carve<int>(p, per * bins * 4);
+    fs->cache.br_row = carve<int>(p, per * (kGrid + 1));
     fs->cache.crit = carve<Pt2>(p, per * kSeedPhases * 4);
     fs->cache.assign = carve<uint8_t>(p, per * kSeedPhases * 4);
     fs->cache.sc = carve<CausticScalars>(p, per);
+    fs->cache.near_n = carve<int>(p, per);
     b->caches_stale = true;
     st = map_hip(hipStreamCreateWithFlags(&fs->ir.polar, hipStreamNonBlocking));
     if (st == LCBG_OK) st = map_hip(hipStreamCreateWithFlags(&fs->ir.big, hipStreamNonBlocking));
@@ -435,6 +464,9 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     FsEpochs ep = epoch_view(fs, n_epochs, offsets, times, mag, method);
     ep.cost = b->d_cost;
     ep.dint = dint_view(fs);
+    ep.hex_list = hex_view(fs);
+    ep.near_curve = near_curve_view(fs);
+    ep.near_chunks = near_chunks_view(fs);
     // the pseudo-curve batch of the dynamic path runs different epochs under
     // the same count, so it keeps no keys
     if (!capture && !b->dynamic_images) {
@@ -494,7 +526,7 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     if (st == LCBG_OK) st = launch_fs_stage1(b->d_curves, b->n_curves, fs->cache, fs->opt, ep, stream);
     stage_done("stage 1 (point source, bounding box)");
     if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[2], stream);
-    if (st == LCBG_OK) st = launch_fs_stage2(b->d_curves, fs->cache, fs->opt, ep, fs->stage2_waves, stream);
+    if (st == LCBG_OK) st = launch_fs_stage2(b->d_curves, pool.cus, fs->cache, fs->opt, ep, fs->stage2_waves, stream);
     stage_done("stage 2 (caustic distance, hexadecapole)");
     if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[3], stream);
     // the batch's schedule: fewer resident waves than slots, the deferral

@@ -16,11 +16,13 @@ lcbg_status launch_curve_prep(const CurveRec* d_curves, int n_curves, const FsCa
 lcbg_status launch_fs_stage1(const CurveRec* d_curves, int n_curves, const FsCache& cache,
                              const FsOptions& opt, const FsEpochs& ep, hipStream_t st);
 
-// Stage 2 (persistent waves over the queue): caustic distances, the
-// point-source derivative test and the hexadecapole test; queues the epochs
-// that need inverse-ray shooting.
-lcbg_status launch_fs_stage2(const CurveRec* d_curves, const FsCache& cache, const FsOptions& opt,
-                             const FsEpochs& ep, int n_waves, hipStream_t st);
+// Stage 2 (three launches over the queue: caustic distances, one resident
+// workgroup per CU (n_cus) with a curve's caustic in LDS, or one wave per epoch
+// for caches larger than LDS; the point-source derivative test, one thread per epoch; the
+// hexadecapole test, one lane per sample): queues the epochs that need
+// inverse-ray shooting.  n_waves: waves of the per-epoch launches.
+lcbg_status launch_fs_stage2(const CurveRec* d_curves, int n_cus, const FsCache& cache,
+                             const FsOptions& opt, const FsEpochs& ep, int n_waves, hipStream_t st);
 
 // Stage 3: the inverse-ray epochs ordered by estimated cost, their image seeds
 // (persistent waves), then the Cartesian row scans on `st` beside the polar

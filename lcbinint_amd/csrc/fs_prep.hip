@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
 
 // The branch positions of each curve in (grid cell, position) order, the
 // order in which augment_seeds_from_caustic_branches (:4845-4926) visits its
-// candidate segments: a counting sort over the 128x128 cells in LDS, scattered
+// candidate segments, and where each grid row starts in that order: a counting sort over the 128x128 cells in LDS, scattered
 // 256 positions at a time with each position's rank among the earlier ones of
 // its cell, which keeps positions of one cell in index order.
 constexpr int kOrderBlock = 256;
@@ -240,6 +240,10 @@ __global__ void __launch_bounds__(kOrderBlock) k_branch_order(const CurveRec* __
         run += h;
     }
     __syncthreads();
+    // first sorted position of each grid row (read before the scatter below
+    // advances the cell starts)
+    int* row = cache.br_row + static_cast<size_t>(c) * (kGrid + 1);
+    for (int iy = tid; iy <= kGrid; iy += kOrderBlock) row[iy] = iy < kGrid ? hist[iy * kGrid] : n;
     for (int base = 0; base < n; base += kOrderBlock) {
         const int k = base + tid;
         const int my = k < n ? iclamp(cell[k], 0, kCells - 1) : -1;

@@ -74,9 +74,11 @@ struct FsCache {
     int* pt_cell;      // [curve][bins*4]   grid cell (iy*128+ix) of each point
     int* br_cell;      // [curve][4*bins]
     int* br_order;     // [curve][4*bins]   branch positions sorted by (br_cell, index)
+    int* br_row;       // [curve][129]      first br_order entry of each grid row; 4*bins
     Pt2* crit;         // [curve][1400*4]   critical sources of the seed scan
     uint8_t* assign;   // [curve][1400*4]   Phase-2 branch -> root index
     CausticScalars* sc;// [curve]
+    int* near_n;       // [curve]            epochs stage 1 queued for stage 2 in the current run
 };
 
 struct FsOptions {
@@ -113,6 +115,11 @@ struct FsEpochs {
     double* dist;       // [epoch] refined caustic distance (IR epochs)
     int64_t* near_list;
     int64_t* ir_list;
+    int64_t* hex_list;  // epochs queued for the hexadecapole test
+    int64_t* near_curve;// [epoch] each curve's queued epochs in order, from the curve's
+                        // first epoch on (FsCache::near_n of them)
+    int64_t* near_chunks;// [chunk] (curve << 32) | first index of up to kNearChunk queued
+                         // epochs in near_curve, one entry per chunk
     unsigned* counters; // [0] near count, [1] near cursor, [2] ir count, [3] ir cursor, [4] error flags
     int64_t* cost;      // [epoch] wall-clock ticks of each inverse-ray epoch, or null
     float* sched;       // [epoch] ticks of the epoch's seed search and area in the last
@@ -194,8 +201,8 @@ struct IrLists {
 // Queue words: near and inverse-ray epoch lists, error flags, the Cartesian
 // and polar queues of the small-slot pass, of the large-slot pass for the
 // largest epochs, and of the large-slot retry pass, the seed-pool cursor.
-constexpr int kWorkOffset = 26;
-constexpr int kBucketOffset = 42;
+constexpr int kWorkOffset = 28;
+constexpr int kBucketOffset = 44;
 constexpr int kCounterWords = kBucketOffset + 2 * kCostBuckets;
 enum : int {
     kQNearCount = 0, kQNearCursor = 1, kQIrCount = 2, kQIrCursor = 3, kQFlags = 4,
@@ -204,10 +211,12 @@ enum : int {
     kQPolarRetryCursor = 12, kQPoolCursor = 13, kQCartBigCount = 14, kQCartBigCursor = 15,
     kQPolarBigCount = 16, kQPolarBigCursor = 17, kQGradCursor = 18, kQGradRetryCount = 19,
     kQGradRetryCursor = 20, kQHeavyCount = 21, kQHeavyCursor = 22, kQFarmCursor = 23,
-    kQLeadCursor = 24
+    kQLeadCursor = 24, kQHexCount = 25, kQNearChunkCount = 26, kQNearChunkCursor = 27
 };
 // derivatives per epoch: t0, tE, u0, alpha, s, q, rho, u_LD
 constexpr int kGradParams = 8;
+// queued epochs of one curve per stage-2 distance work item
+constexpr int kNearChunk = 128;
 // leading inverse-ray items searched first and scanned on large slots at once
 constexpr int kLeadItems = 32;
 // inverse-ray area passes: small slots, large slots for the largest epochs

@@ -74,6 +74,22 @@ def test_plain_curve_matches_reference(gpu, ref, mode):
     _compare(got, meth, exp, emeth, f"plain curve mode {mode}")
 
 
+@pytest.mark.parametrize("bins", [1000, 1800])
+def test_caustic_bins_matches_reference(gpu, ref, bins):
+    # 1800 caustic samples per branch no longer fit in LDS: the caustic
+    # distances then run one wave per epoch from global memory
+    from lcbinint_amd.workloads import plain_curve
+    cv, t = plain_curve()
+    o = _options(gpu, caustic_bins=bins)
+    got, meth, prof = _gpu_curves(gpu, [cv], np.array([0, len(t)]), t, o)
+    assert prof[2] == 0, prof
+    exp, emeth, st = ref.light_curve(_settings_for(o, cv["ld_c"]), t, cv["t0"], cv["tE"], cv["u0"],
+                                     cv["alpha"], cv["s"], cv["q"], cv["rho"])
+    assert st == 0
+    assert (emeth == 1).any() and (emeth >= 2).any()
+    _compare(got, meth, exp, emeth, f"plain curve, {bins} caustic bins")
+
+
 def test_forced_inverse_ray_positions(gpu, ref):
     # bbox exit, derivative and hexadecapole switched off as binary_inverse_ray
     # does (python/lcbinint_pybind.cpp:3826-3847)
