@@ -128,18 +128,29 @@ __device__ __forceinline__ int wave_append(unsigned* count, bool want)
 }
 
 // True if neither branch segment at position pos can come closer to p than
-// the distance whose square bounds reach2 = ((nearest + max_seg_len)(1 + 1e-9))^2:
-// for 0 < pos < n - 1 both segments are shorter than max_seg_len, so their
-// distance is at least |p - a| - max_seg_len, which exceeds nearest by far
-// more than the rounding of either value.  Such segments leave the minimum
-// (started at nearest) unchanged.  The wrap-around segments of positions 0 and
-// n - 1, which max_seg_len does not bound, are always evaluated.
-__device__ __forceinline__ bool segments_beyond(Pt p, Pt a, int pos, int n, double reach2)
+// `nearest`, the value the minimum starts at.  reach2 is
+// ((nearest + max_seg_len)(1 + 1e-9))^2 and reach2_wrap the same with the
+// larger of max_seg_len and the longest wrap-around segment of the curve
+// (positions n - 1 -> 0), which max_seg_len does not bound: a segment of length l from a lies at least
+// |p - a| - l from p, so beyond these radii it exceeds nearest by far more
+// than the rounding of either value and leaves the minimum unchanged.
+__device__ __forceinline__ bool segments_beyond(Pt p, Pt a, int pos, int n, double reach2, double reach2_wrap)
 {
-    if (pos == 0 || pos == n - 1) return false;
     const double dx = p.x - a.x;
     const double dy = p.y - a.y;
-    return dx * dx + dy * dy > reach2;
+    const double d2 = dx * dx + dy * dy;
+    return d2 > ((pos == 0 || pos == n - 1) ? reach2_wrap : reach2);
+}
+
+// Longest wrap-around segment (position n - 1 back to 0) of the four branches.
+__device__ __forceinline__ double wrap_segment_len(const Pt* br, int bins, int n)
+{
+    double l2 = 0.0;
+    for (int b = 0; b < 4; ++b) {
+        const Pt* bb = br + static_cast<size_t>(b) * bins;
+        l2 = dmax(l2, dist2(bb[n - 1], bb[0]));
+    }
+    return __builtin_sqrt(l2);
 }
 
 // Stage 2a, one wave per queued epoch (a static stride over the queue): the
@@ -211,10 +222,13 @@ __global__ void __launch_bounds__(kWave) k_fs_near_dist(const CurveRec* __restri
         const int jx1 = cell_coord(src.x + seg_radius, sc.min_x, sc.step_x);
         const int jy0 = cell_coord(src.y - seg_radius, sc.min_y, sc.step_y);
         const int jy1 = cell_coord(src.y + seg_radius, sc.min_y, sc.step_y);
+        const int n = sc.n;
         const double reach = seg_radius * (1.0 + 1.0e-9);
         const double reach2 = reach * reach;
+        const double reach_wrap =
+            (nearest + dmax(sc.max_seg_len, n >= 2 ? wrap_segment_len(br, bins, n) : 0.0)) * (1.0 + 1.0e-9);
+        const double reach2_wrap = reach_wrap * reach_wrap;
         double dseg = nearest;
-        const int n = sc.n;
         if (n >= 2) {
             const int lo = row[jy0], hi = row[jy1 + 1];
             for (int k = lo + lane; k < hi; k += kWave) {
@@ -225,7 +239,7 @@ __global__ void __launch_bounds__(kWave) k_fs_near_dist(const CurveRec* __restri
                 const int b = at / bins;
                 const int pos = at - b * bins;
                 const Pt* bb = br + static_cast<size_t>(b) * bins;
-                if (segments_beyond(src, bb[pos], pos, n, reach2)) continue;
+                if (segments_beyond(src, bb[pos], pos, n, reach2, reach2_wrap)) continue;
                 const int prev = pos > 0 ? pos - 1 : n - 1;
                 const int next = (pos + 1) % n;
                 dseg = dmin(dseg, point_segment_distance(src, bb[prev], bb[pos]));
@@ -252,6 +266,7 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
 {
     __shared__ Pt lds_pt[4 * kLdsBins];
     __shared__ int lds_cell[4 * kLdsBins];
+    __shared__ float lds_seg[4 * kLdsBins];  // longer segment at each point, rounded up
     __shared__ int s_item;
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = lane_id();
@@ -268,6 +283,8 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
         const int64_t desc = ep.near_chunks[item];
         const int c = static_cast<int>(desc >> 32);
         const int chunk = static_cast<int>(desc & 0xffffffff);
+        const CausticScalars sc = cache.sc[c];
+        const int n = sc.n;
         if (c != loaded) {
             const size_t base = static_cast<size_t>(c) * nk;
             for (int k = tid; k < nk; k += kNearCurveBlock) {
@@ -276,12 +293,21 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
             }
             loaded = c;
             __syncthreads();
+            if (n >= 2) {
+                for (int k = tid; k < nk; k += kNearCurveBlock) {
+                    const int b = k / bins;
+                    const int pos = k - b * bins;
+                    const Pt* bb = lds_pt + b * bins;
+                    const Pt a = bb[pos];
+                    const double l2 = dmax(dist2(a, bb[pos > 0 ? pos - 1 : n - 1]), dist2(a, bb[pos + 1 < n ? pos + 1 : 0]));
+                    lds_seg[k] = __double2float_ru(__builtin_sqrt(l2));
+                }
+            }
+            __syncthreads();
         }
         const int64_t begin = ep.offsets[c];
         const int count = cache.near_n[c];
-        const CausticScalars sc = cache.sc[c];
         const double margin = opt.kinji * curves[c].rho;
-        const int n = sc.n;
         const int stop = chunk + kNearChunk < count ? chunk + kNearChunk : count;
         for (int j = chunk + wave; j < stop; j += kNearCurveBlock / kWave) {
             const int64_t e = uni(static_cast<long long>(ep.near_curve[begin + j]));
@@ -292,6 +318,7 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
             const int iy1 = cell_coord(src.y + margin, sc.min_y, sc.step_y);
             double d2 = __builtin_inf();
             double d2all = __builtin_inf();
+#pragma unroll 2
             for (int k = lane; k < nk; k += kWave) {
                 const int cell = lds_cell[k];
                 const int iy = cell / kGrid;
@@ -311,8 +338,11 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
             const int jx1 = cell_coord(src.x + seg_radius, sc.min_x, sc.step_x);
             const int jy0 = cell_coord(src.y - seg_radius, sc.min_y, sc.step_y);
             const int jy1 = cell_coord(src.y + seg_radius, sc.min_y, sc.step_y);
-            const double reach = seg_radius * (1.0 + 1.0e-9);
-            const double reach2 = reach * reach;
+            // A segment of length l from point a lies at least |src - a| - l
+            // from src; beyond (best + l)(1 + 1e-9), best being this lane's
+            // minimum so far (never below the final one), both segments at a
+            // exceed best by far more than the rounding of either value and
+            // cannot change the minimum.
             double dseg = nearest;
             if (n >= 2) {
                 for (int k = lane; k < nk; k += kWave) {
@@ -323,7 +353,8 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
                     const int b = k >= 2 * bins ? (k >= 3 * bins ? 3 : 2) : (k >= bins ? 1 : 0);
                     const int pos = k - b * bins;
                     const Pt* bb = lds_pt + b * bins;
-                    if (segments_beyond(src, bb[pos], pos, n, reach2)) continue;
+                    const double reach = (dseg + static_cast<double>(lds_seg[k])) * (1.0 + 1.0e-9);
+                    if (dist2(src, bb[pos]) > reach * reach) continue;
                     const int prev = pos > 0 ? pos - 1 : n - 1;
                     const int next = pos + 1 < n ? pos + 1 : 0;
                     dseg = dmin(dseg, point_segment_distance(src, bb[prev], bb[pos]));
