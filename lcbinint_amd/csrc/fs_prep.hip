@@ -32,6 +32,13 @@ __device__ __forceinline__ int grid_index(double v, double lo, double step)
     return iclamp(static_cast<int>((v - lo) / step), 0, kGrid - 1);
 }
 
+// The two (cos, sin) pairs are the same bits.
+__device__ __forceinline__ bool same_phase(Pt a, Pt b)
+{
+    return __double_as_longlong(a.x) == __double_as_longlong(b.x) &&
+           __double_as_longlong(a.y) == __double_as_longlong(b.y);
+}
+
 __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __restrict__ curves,
                                                            FsCache cache, FsTables tab)
 {
@@ -55,15 +62,23 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
     }
     const Lens L = make_lens(cv.s, cv.q_eff);
 
+    // A seed-scan phase whose cos and sin round to the caustic sample's (752 of
+    // 1400 at 1400 bins) has the same quartic, hence the same roots: one solve
+    // gives both.
     for (int i = tid; i < bins; i += kPrepBlock) {
         const Pt ph = tab.caus_trig[i];
         cx roots[4];
         critical_roots(L, cx{ph.x, ph.y}, roots);
 #pragma unroll
         for (int r = 0; r < 4; ++r) pts[4 * i + r] = lens_equation_complex(L, roots[r]);
+        if (i < kSeedPhases && same_phase(tab.seed_trig[i], ph)) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) crit[4 * i + r] = map_real(L, roots[r].re, roots[r].im);
+        }
     }
     for (int k = tid; k < kSeedPhases; k += kPrepBlock) {
         const Pt ph = tab.seed_trig[k];
+        if (k < bins && same_phase(tab.caus_trig[k], ph)) continue;
         cx roots[4];
         critical_roots(L, cx{ph.x, ph.y}, roots);
 #pragma unroll
