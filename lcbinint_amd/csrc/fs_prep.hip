@@ -11,8 +11,9 @@
 //    which the reference recomputes for every inverse-ray epoch although they
 //    depend on (s, q) only.
 //
-// One 256-thread workgroup per curve; root solves are one thread each, the
-// sequential branch tracking runs on wave 0 with the 24 permutations across lanes.
+// One 256-thread workgroup per curve for the root solves (one thread each),
+// then one wave per curve for the sequential branch tracking (the 24
+// permutations across lanes), then the branch positions sorted by cell.
 #include "fs_launch.hpp"
 #include "fs_math.hpp"
 #include "fs_wave.hpp"
@@ -39,6 +40,53 @@ __device__ __forceinline__ bool same_phase(Pt a, Pt b)
            __double_as_longlong(a.y) == __double_as_longlong(b.y);
 }
 
+// One of four values by the bits of k (0..3): selects between registers.  The
+// four points are separate values, not an array, so that a per-lane k cannot
+// become an indexed access (which would go through scratch memory).
+__device__ __forceinline__ double pick4d(double a, double b, double c, double d, int k)
+{
+    const double ab = (k & 1) ? b : a;
+    const double cd = (k & 1) ? d : c;
+    return (k & 2) ? cd : ab;
+}
+
+__device__ __forceinline__ Pt pick4(Pt c0, Pt c1, Pt c2, Pt c3, int k)
+{
+    return Pt{pick4d(c0.x, c1.x, c2.x, c3.x, k), pick4d(c0.y, c1.y, c2.y, c3.y, k)};
+}
+
+// (cost, index) pairs ordered by cost, then index, NaN costs last: the order
+// of wave_argmin, a total order, so any reduction tree gives its result.
+__device__ __forceinline__ void argmin_take(double& v, int& idx, double ov, int oi)
+{
+    const bool take = (ov < v) || (ov == v && oi < idx) || (v != v && ov == ov);
+    v = take ? ov : v;
+    idx = take ? oi : idx;
+}
+
+template <int kCtrl>
+__device__ __forceinline__ void argmin_dpp(double& v, int& idx)
+{
+    const double ov = dpp_move<kCtrl>(v);
+    const int oi = __builtin_amdgcn_update_dpp(0, idx, kCtrl, 0xf, 0xf, false);
+    argmin_take(v, idx, ov, oi);
+}
+
+// The smallest (cost, index) of lanes 0-31, as scalars: quads, half rows and
+// rows by DPP (every lane of a row has a source), then rows 0 and 1.
+__device__ __forceinline__ void argmin_rows01(double& v, int& idx)
+{
+    argmin_dpp<0xB1>(v, idx);   // quad_perm [1,0,3,2]
+    argmin_dpp<0x4E>(v, idx);   // quad_perm [2,3,0,1]
+    argmin_dpp<0x141>(v, idx);  // row_half_mirror
+    argmin_dpp<0x140>(v, idx);  // row_mirror
+    const double v1 = read_lane(v, 16);
+    const int i1 = __builtin_amdgcn_readlane(idx, 16);
+    v = read_lane(v, 0);
+    idx = __builtin_amdgcn_readlane(idx, 0);
+    argmin_take(v, idx, v1, i1);
+}
+
 __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __restrict__ curves,
                                                            FsCache cache, FsTables tab)
 {
@@ -47,9 +95,7 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
     const CurveRec cv = curves[c];
     const int bins = cache.bins;
     Pt* pts = cache.pts + static_cast<size_t>(c) * bins * 4;
-    Pt* br = cache.br + static_cast<size_t>(c) * bins * 4;
     int* pt_cell = cache.pt_cell + static_cast<size_t>(c) * bins * 4;
-    int* br_cell = cache.br_cell + static_cast<size_t>(c) * bins * 4;
     Pt* crit = cache.crit + static_cast<size_t>(c) * kSeedPhases * 4;
     uint8_t* assign = cache.assign + static_cast<size_t>(c) * kSeedPhases * 4;
     CausticScalars* sc = cache.sc + c;
@@ -120,9 +166,40 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
     }
     __syncthreads();
 
-    // branch tracking on wave 0 (append_tracked_caustic_points :569-611)
-    if (tid < kWave) {
-        const int lane = tid;
+    if (tid == 0) {
+        sc->min_x = min_x;
+        sc->max_x = max_x;
+        sc->min_y = min_y;
+        sc->max_y = max_y;
+        sc->step_x = step_x;
+        sc->step_y = step_y;
+    }
+    // seed Phase-2 greedy nearest-neighbour matching (:1813-1840): the
+    // reference starts every branch at (1e30, 1e30) with a best distance of
+    // 1e60, and each squared distance to that start is about 2e60 (or NaN), so
+    // no branch is ever matched and every start stays put: all 255
+    for (int k = tid; k < kSeedPhases * 4; k += kPrepBlock) assign[k] = 255;
+}
+
+// Branch tracking (append_tracked_caustic_points :569-611), one wave per curve
+// so that every curve's sequential chain of phases runs at once: 64 phases at a
+// time, lane j loads phase base + j and keeps the points chosen for it (stored
+// once the chunk is done), a step reads its phase's points from lane i - base,
+// lanes 0-23 price the 24 permutations.  Then the branch grid cells and the
+// longest non-wrap segment (:4065-4095); the cache counts as built (n = bins)
+// from here on.
+__global__ void __launch_bounds__(kWave) k_branch_track(const CurveRec* __restrict__ curves, FsCache cache)
+{
+    const int c = blockIdx.x;
+    const CurveRec cv = curves[c];
+    if (!(cv.rho > 0.0) || cv.s == 0.0 || !(cv.q_eff > 0.0)) return;  // no caustic cache
+    const int lane = lane_id();
+    const int bins = cache.bins;
+    const Pt* pts = cache.pts + static_cast<size_t>(c) * bins * 4;
+    Pt* br = cache.br + static_cast<size_t>(c) * bins * 4;
+    int* br_cell = cache.br_cell + static_cast<size_t>(c) * bins * 4;
+    CausticScalars* sc = cache.sc + c;
+    {
         Pt prev[4];
         {
             // first phase: order by atan2 (std::sort of 4 elements = insertion sort)
@@ -152,63 +229,64 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
         }
         const int pl = lane < 24 ? lane : 0;
         const int p0 = c_perm[pl][0], p1 = c_perm[pl][1], p2 = c_perm[pl][2], p3 = c_perm[pl][3];
-        for (int i = 1; i < bins; ++i) {
-            const Pt* cur = pts + 4 * i;
-            double cost = 0.0;
-            cost += dist2(prev[0], cur[p0]);
-            cost += dist2(prev[1], cur[p1]);
-            cost += dist2(prev[2], cur[p2]);
-            cost += dist2(prev[3], cur[p3]);
-            if (lane >= 24) cost = __builtin_inf();
-            int idx = lane;
-            wave_argmin(cost, idx);
-            cost = uni(cost);
-            idx = uni(idx);
-            // the reference keeps the identity when no cost is below +inf
-            if (!(cost < __builtin_inf())) idx = 0;
-            const int b0 = c_perm[idx][0], b1 = c_perm[idx][1], b2 = c_perm[idx][2], b3 = c_perm[idx][3];
-            prev[0] = cur[b0];
-            prev[1] = cur[b1];
-            prev[2] = cur[b2];
-            prev[3] = cur[b3];
-            wave_store(&br[0 * bins + i], prev[0]);
-            wave_store(&br[1 * bins + i], prev[1]);
-            wave_store(&br[2 * bins + i], prev[2]);
-            wave_store(&br[3 * bins + i], prev[3]);
+        for (int base = 0; base < bins; base += kWave) {
+            Pt ch[4], keep[4];
+            const int ph = base + lane;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                ch[r] = ph < bins ? pts[4 * ph + r] : Pt{0.0, 0.0};
+                keep[r] = Pt{0.0, 0.0};
+            }
+            const int i0 = base > 0 ? base : 1;
+            const int i1 = base + kWave < bins ? base + kWave : bins;
+            for (int i = i0; i < i1; ++i) {
+                const int j = i - base;
+                const Pt c0{read_lane(ch[0].x, j), read_lane(ch[0].y, j)};
+                const Pt c1{read_lane(ch[1].x, j), read_lane(ch[1].y, j)};
+                const Pt c2{read_lane(ch[2].x, j), read_lane(ch[2].y, j)};
+                const Pt c3{read_lane(ch[3].x, j), read_lane(ch[3].y, j)};
+                double cost = 0.0;
+                cost += dist2(prev[0], pick4(c0, c1, c2, c3, p0));
+                cost += dist2(prev[1], pick4(c0, c1, c2, c3, p1));
+                cost += dist2(prev[2], pick4(c0, c1, c2, c3, p2));
+                cost += dist2(prev[3], pick4(c0, c1, c2, c3, p3));
+                if (lane >= 24) cost = __builtin_inf();
+                int idx = lane;
+                argmin_rows01(cost, idx);
+                // the reference keeps the identity when no cost is below +inf
+                if (!(cost < __builtin_inf())) idx = 0;
+                const int b0 = __builtin_amdgcn_readlane(p0, idx), b1 = __builtin_amdgcn_readlane(p1, idx);
+                const int b2 = __builtin_amdgcn_readlane(p2, idx), b3 = __builtin_amdgcn_readlane(p3, idx);
+                prev[0] = pick4(c0, c1, c2, c3, b0);
+                prev[1] = pick4(c0, c1, c2, c3, b1);
+                prev[2] = pick4(c0, c1, c2, c3, b2);
+                prev[3] = pick4(c0, c1, c2, c3, b3);
+                const bool mine = lane == j;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    keep[r].x = mine ? prev[r].x : keep[r].x;
+                    keep[r].y = mine ? prev[r].y : keep[r].y;
+                }
+            }
+            if (ph >= i0 && ph < i1) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) br[r * bins + ph] = keep[r];
+            }
         }
     }
     __syncthreads();
-
-    // branch grid cells and the longest non-wrap segment (:4065-4095)
+    const double min_x = sc->min_x, min_y = sc->min_y, step_x = sc->step_x, step_y = sc->step_y;
     double seg2 = 0.0;
-    for (int k = tid; k < 4 * bins; k += kPrepBlock) {
+    for (int k = lane; k < 4 * bins; k += kWave) {
         const int b = k / bins;
         const int j = k - b * bins;
         const Pt pt = br[k];
         if (j < bins - 1) seg2 = dmax(seg2, dist2(pt, br[k + 1]));
         br_cell[k] = grid_index(pt.y, min_y, step_y) * kGrid + grid_index(pt.x, min_x, step_x);
     }
-    red[0][tid] = seg2;
-    __syncthreads();
-    for (int off = kPrepBlock / 2; off >= 1; off >>= 1) {
-        if (tid < off) red[0][tid] = dmax(red[0][tid], red[0][tid + off]);
-        __syncthreads();
-    }
-    if (tid == 0) {
-        sc->min_x = min_x;
-        sc->max_x = max_x;
-        sc->min_y = min_y;
-        sc->max_y = max_y;
-        sc->step_x = step_x;
-        sc->step_y = step_y;
-        sc->max_seg_len = __builtin_sqrt(red[0][0]);
-        sc->n = bins;
-    }
-    // seed Phase-2 greedy nearest-neighbour matching (:1813-1840): the
-    // reference starts every branch at (1e30, 1e30) with a best distance of
-    // 1e60, and each squared distance to that start is about 2e60 (or NaN), so
-    // no branch is ever matched and every start stays put: all 255
-    for (int k = tid; k < kSeedPhases * 4; k += kPrepBlock) assign[k] = 255;
+    seg2 = wave_max(seg2);
+    wave_store(&sc->max_seg_len, __builtin_sqrt(seg2));
+    wave_store(&sc->n, bins);
 }
 
 // The branch positions of each curve in (grid cell, position) order, the
@@ -282,6 +360,8 @@ lcbg_status launch_curve_prep(const CurveRec* d_curves, int n_curves, const FsCa
     if (n_curves <= 0) return LCBG_OK;
     hipLaunchKernelGGL(k_curve_prep, dim3(static_cast<unsigned>(n_curves)), dim3(kPrepBlock), 0,
                        st, d_curves, cache, tab);
+    hipLaunchKernelGGL(k_branch_track, dim3(static_cast<unsigned>(n_curves)), dim3(kWave), 0, st, d_curves,
+                       cache);
     hipLaunchKernelGGL(k_branch_order, dim3(static_cast<unsigned>(n_curves)), dim3(kOrderBlock), 0,
                        st, d_curves, cache);
     return check_launch();
