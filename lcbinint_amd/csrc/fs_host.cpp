@@ -331,8 +331,9 @@ void fs_destroy(FsState* fs)
     if (fs->epoch_mem) (void)hipFree(fs->epoch_mem);
     if (fs->sched) (void)hipFree(fs->sched);
     if (fs->ir.polar) (void)hipStreamDestroy(fs->ir.polar);
+    if (fs->ir.polar_small) (void)hipStreamDestroy(fs->ir.polar_small);
     if (fs->ir.big) (void)hipStreamDestroy(fs->ir.big);
-    for (hipEvent_t e : {fs->ir.fork, fs->ir.fork_lead, fs->ir.join_polar, fs->ir.join_big})
+    for (hipEvent_t e : {fs->ir.fork, fs->ir.fork_lead, fs->ir.join_polar, fs->ir.join_polar_small, fs->ir.join_big})
         if (e) (void)hipEventDestroy(e);
     delete fs;
 }
@@ -420,8 +421,10 @@ lcbg_status fs_prepare(lcbg_batch* b)
     fs->cache.near_n = carve<int>(p, per);
     b->caches_stale = true;
     st = map_hip(hipStreamCreateWithFlags(&fs->ir.polar, hipStreamNonBlocking));
+    if (st == LCBG_OK) st = map_hip(hipStreamCreateWithFlags(&fs->ir.polar_small, hipStreamNonBlocking));
     if (st == LCBG_OK) st = map_hip(hipStreamCreateWithFlags(&fs->ir.big, hipStreamNonBlocking));
-    for (hipEvent_t* e : {&fs->ir.fork, &fs->ir.fork_lead, &fs->ir.join_polar, &fs->ir.join_big})
+    for (hipEvent_t* e : {&fs->ir.fork, &fs->ir.fork_lead, &fs->ir.join_polar, &fs->ir.join_polar_small,
+                          &fs->ir.join_big})
         if (st == LCBG_OK) st = map_hip(hipEventCreateWithFlags(e, hipEventDisableTiming));
     return st;
 }

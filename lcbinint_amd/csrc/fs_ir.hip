@@ -2439,6 +2439,7 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
                        ss.polar, d_curves, opt, ep, lists, scr.polar_large, int{kPassBig});
     mark(2 * kIrPolarBig + 1, ss.polar);
     status = check_launch();
+    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_polar, ss.polar));
     if (status != LCBG_OK) return status;
     // all other searches; the deferred first-crossing scans: list, farm, resume
     hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
@@ -2453,15 +2454,16 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     mark(2 * kIrSeeds + 1, st);
     status = check_launch();
     if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.fork, st));
-    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.polar, ss.fork, 0));
+    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.polar_small, ss.fork, 0));
     if (status != LCBG_OK) return status;
-    // the polar small-slot pass beside the Cartesian one
-    mark(2 * kIrPolarSmall, ss.polar);
+    // the polar small-slot pass beside the Cartesian one and the polar
+    // large-slot pass (its own stream: it need not wait for the largest epoch)
+    mark(2 * kIrPolarSmall, ss.polar_small);
     hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar.n_slots)), dim3(kWave), 0,
-                       ss.polar, d_curves, opt, ep, lists, scr.polar, int{kPassSmall});
-    mark(2 * kIrPolarSmall + 1, ss.polar);
+                       ss.polar_small, d_curves, opt, ep, lists, scr.polar, int{kPassSmall});
+    mark(2 * kIrPolarSmall + 1, ss.polar_small);
     status = check_launch();
-    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_polar, ss.polar));
+    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_polar_small, ss.polar_small));
     if (status != LCBG_OK) return status;
     mark(2 * kIrCartSmall, st);
     hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart.n_slots / kTeamWaves)),
@@ -2471,6 +2473,7 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     status = check_launch();
     if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(st, ss.join_big, 0));
     if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(st, ss.join_polar, 0));
+    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(st, ss.join_polar_small, 0));
     if (status != LCBG_OK) return status;
     // the epochs that outgrew a small slot, on the (now free) large slots
     mark(2 * kIrCartRetry, st);

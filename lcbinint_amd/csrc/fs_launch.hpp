@@ -30,9 +30,10 @@ lcbg_status launch_fs_stage2(const CurveRec* d_curves, int n_cus, const FsCache&
 // streams (forked and joined through the events), then a large-slot retry
 // pass for the epochs that outgrew a small slot.
 struct IrStreams {
-    hipStream_t polar;   // polar small-slot pass
-    hipStream_t big;     // large-slot pass of the largest epochs
-    hipEvent_t fork, fork_lead, join_polar, join_big;
+    hipStream_t polar;        // polar large-slot pass of the largest polar epochs
+    hipStream_t polar_small;  // polar small-slot pass
+    hipStream_t big;          // Cartesian large-slot pass of the largest epochs
+    hipEvent_t fork, fork_lead, join_polar, join_polar_small, join_big;
     // per-launch timing (profiled runs only, else null): events 2k and 2k+1
     // bracket launch k on the stream it runs on, in the order of IrLaunch
     hipEvent_t* kev;
@@ -43,7 +44,7 @@ enum IrLaunch {
     kIrSeeds = 0,      // k_ir_seeds, main stream
     kIrCartBig = 1,    // k_ir_cart, large slots, big-epoch stream
     kIrPolarBig = 2,   // k_ir_polar, large slots, polar stream
-    kIrPolarSmall = 3, // k_ir_polar, small slots, polar stream
+    kIrPolarSmall = 3, // k_ir_polar, small slots, polar small-slot stream
     kIrCartSmall = 4,  // k_ir_cart, small slots, main stream
     kIrCartRetry = 5,  // k_ir_cart, large slots, main stream
     kIrPolarRetry = 6, // k_ir_polar, large slots, main stream
