@@ -307,7 +307,18 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
         }
         const int64_t begin = ep.offsets[c];
         const int count = cache.near_n[c];
-        const double margin = opt.kinji * curves[c].rho;
+        const double rho = curves[c].rho;
+        const double margin = opt.kinji * rho;
+        // The near test reads the distance only through d < hex_threshold rho
+        // and, for rho >= 1e-3, the safety factor max(1, 30 (hex_threshold /
+        // max(d / rho, hex_threshold))^3); both are the same for every
+        // d >= 3.2 hex_threshold rho, as 30 / 3.2^3 < 1.  Past `settle` (that
+        // bound, slightly raised) the search stops refining: it returns some
+        // value at or above it, which changes no decision.
+        const double settle = opt.hex_threshold > 0.0 ? 3.2 * opt.hex_threshold * rho * (1.0 + 1.0e-10)
+                                                      : __builtin_inf();
+        const double wrap_len = n >= 2 ? wrap_segment_len(lds_pt, bins, n) : 0.0;
+        const double any_len = dmax(sc.max_seg_len, wrap_len);
         const int stop = chunk + kNearChunk < count ? chunk + kNearChunk : count;
         for (int j = chunk + wave; j < stop; j += kNearCurveBlock / kWave) {
             const int64_t e = uni(static_cast<long long>(ep.near_curve[begin + j]));
@@ -317,32 +328,65 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
             const int iy0 = cell_coord(src.y - margin, sc.min_y, sc.step_y);
             const int iy1 = cell_coord(src.y + margin, sc.min_y, sc.step_y);
             double d2 = __builtin_inf();
-            double d2all = __builtin_inf();
-#pragma unroll 2
             for (int k = lane; k < nk; k += kWave) {
                 const int cell = lds_cell[k];
                 const int iy = cell / kGrid;
                 const int ix = cell - iy * kGrid;
+                if (ix < ix0 || ix > ix1 || iy < iy0 || iy > iy1) continue;
                 const Pt p = lds_pt[k];
                 const double dx = src.x - p.x;
                 const double dy = src.y - p.y;
                 const double cand = dx * dx + dy * dy;
-                if (ix >= ix0 && ix <= ix1 && iy >= iy0 && iy <= iy1 && cand < d2) d2 = cand;
-                if (cand < d2all) d2all = cand;
+                if (cand < d2) d2 = cand;
             }
             d2 = uni(wave_min(d2));
-            const double nearest = d2 != __builtin_inf() ? __builtin_sqrt(d2)
-                                                         : uni(__builtin_sqrt(wave_min(d2all)));
-            const double seg_radius = nearest + sc.max_seg_len;
+            // With no caustic point in the margin box every point lies beyond
+            // margin.  If margin is past settle and no wrap-around segment (which
+            // max_seg_len does not bound) comes within settle, the nearest
+            // point's exact distance cannot matter: only segments closer than
+            // settle can, and both ends of such a segment lie within settle +
+            // max_seg_len < nearest + max_seg_len, inside the reference's box.
+            bool exact = true;
+            double nearest = 0.0;
+            if (d2 != __builtin_inf()) {
+                nearest = __builtin_sqrt(d2);
+            } else {
+                // (the slack covers the rounding of src -/+ margin in the box)
+                if (n >= 2 && settle + 1.0e-12 * (1.0 + fabs(src.x) + fabs(src.y)) < margin * (1.0 - 1.0e-9)) {
+                    double wrap_min = __builtin_inf();
+                    for (int b = 0; b < 4; ++b) {
+                        const Pt* bb = lds_pt + b * bins;
+                        wrap_min = dmin(wrap_min, point_segment_distance(src, bb[n - 1], bb[0]));
+                    }
+                    exact = !(wrap_min >= settle);
+                }
+                if (exact) {
+                    double d2all = __builtin_inf();
+#pragma unroll 2
+                    for (int k = lane; k < nk; k += kWave) {
+                        const Pt p = lds_pt[k];
+                        const double dx = src.x - p.x;
+                        const double dy = src.y - p.y;
+                        const double cand = dx * dx + dy * dy;
+                        if (cand < d2all) d2all = cand;
+                    }
+                    nearest = uni(__builtin_sqrt(wave_min(d2all)));
+                } else {
+                    nearest = settle;
+                }
+            }
+            // the reference's box (nearest + max_seg_len), or the box that holds
+            // every point within settle + the longest segment
+            const double seg_radius = exact ? nearest + sc.max_seg_len : (settle + any_len) * (1.0 + 1.0e-8);
             const int jx0 = cell_coord(src.x - seg_radius, sc.min_x, sc.step_x);
             const int jx1 = cell_coord(src.x + seg_radius, sc.min_x, sc.step_x);
             const int jy0 = cell_coord(src.y - seg_radius, sc.min_y, sc.step_y);
             const int jy1 = cell_coord(src.y + seg_radius, sc.min_y, sc.step_y);
             // A segment of length l from point a lies at least |src - a| - l
             // from src; beyond (best + l)(1 + 1e-9), best being this lane's
-            // minimum so far (never below the final one), both segments at a
-            // exceed best by far more than the rounding of either value and
-            // cannot change the minimum.
+            // minimum so far capped at settle, both segments at a exceed best
+            // by far more than the rounding of either value and cannot change
+            // the minimum below settle.
             double dseg = nearest;
             if (n >= 2) {
                 for (int k = lane; k < nk; k += kWave) {
@@ -353,7 +397,7 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
                     const int b = k >= 2 * bins ? (k >= 3 * bins ? 3 : 2) : (k >= bins ? 1 : 0);
                     const int pos = k - b * bins;
                     const Pt* bb = lds_pt + b * bins;
-                    const double reach = (dseg + static_cast<double>(lds_seg[k])) * (1.0 + 1.0e-9);
+                    const double reach = (dmin(dseg, settle) + static_cast<double>(lds_seg[k])) * (1.0 + 1.0e-9);
                     if (dist2(src, bb[pos]) > reach * reach) continue;
                     const int prev = pos > 0 ? pos - 1 : n - 1;
                     const int next = pos + 1 < n ? pos + 1 : 0;
