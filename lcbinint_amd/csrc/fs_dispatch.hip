@@ -253,9 +253,10 @@ __global__ void __launch_bounds__(kWave) k_fs_near_dist(const CurveRec* __restri
 // Stage 2a with the curve's caustic in LDS: one resident 1024-thread
 // workgroup per CU takes work items (a curve and up to kNearChunk of its
 // queued epochs) from the list stage 1 wrote; it loads the curve's branch
-// points and their cells into LDS when the curve changes (the caustic points
-// reordered: the same values and cells), and its waves take the item's epochs
-// in turn.  Both distances are minima over LDS, exact in any order.  Used when
+// points, their cells, their order by cell and the grid-row starts into LDS
+// when the curve changes (the caustic points reordered: the same values and
+// cells), and its waves take the item's epochs in turn, each visiting only the
+// grid rows of its boxes.  Both distances are minima over LDS, exact in any order.  Used when
 // 4 * bins points fit kLdsBins.
 constexpr int kLdsBins = 1400;
 constexpr int kNearCurveBlock = 1024;
@@ -267,6 +268,8 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
     __shared__ Pt lds_pt[4 * kLdsBins];
     __shared__ int lds_cell[4 * kLdsBins];
     __shared__ float lds_seg[4 * kLdsBins];  // longer segment at each point, rounded up
+    __shared__ int lds_order[4 * kLdsBins];  // positions sorted by (cell, position)
+    __shared__ int lds_row[kGrid + 1];       // first entry of each grid row in lds_order
     __shared__ int s_item;
     const int tid = static_cast<int>(threadIdx.x);
     const int lane = lane_id();
@@ -290,7 +293,10 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
             for (int k = tid; k < nk; k += kNearCurveBlock) {
                 lds_pt[k] = cache.br[base + k];
                 lds_cell[k] = cache.br_cell[base + k];
+                lds_order[k] = cache.br_order[base + k];
             }
+            for (int r = tid; r <= kGrid; r += kNearCurveBlock)
+                lds_row[r] = cache.br_row[static_cast<size_t>(c) * (kGrid + 1) + r];
             loaded = c;
             __syncthreads();
             if (n >= 2) {
@@ -328,11 +334,12 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
             const int iy0 = cell_coord(src.y - margin, sc.min_y, sc.step_y);
             const int iy1 = cell_coord(src.y + margin, sc.min_y, sc.step_y);
             double d2 = __builtin_inf();
-            for (int k = lane; k < nk; k += kWave) {
+            // the points of the box's grid rows, in (cell, position) order
+            for (int t = lds_row[iy0] + lane; t < lds_row[iy1 + 1]; t += kWave) {
+                const int k = lds_order[t];
                 const int cell = lds_cell[k];
-                const int iy = cell / kGrid;
-                const int ix = cell - iy * kGrid;
-                if (ix < ix0 || ix > ix1 || iy < iy0 || iy > iy1) continue;
+                const int ix = cell - (cell / kGrid) * kGrid;
+                if (ix < ix0 || ix > ix1) continue;
                 const Pt p = lds_pt[k];
                 const double dx = src.x - p.x;
                 const double dy = src.y - p.y;
@@ -389,11 +396,11 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
             // the minimum below settle.
             double dseg = nearest;
             if (n >= 2) {
-                for (int k = lane; k < nk; k += kWave) {
+                for (int t = lds_row[jy0] + lane; t < lds_row[jy1 + 1]; t += kWave) {
+                    const int k = lds_order[t];
                     const int cell = lds_cell[k];
-                    const int iy = cell / kGrid;
-                    const int ix = cell - iy * kGrid;
-                    if (ix < jx0 || ix > jx1 || iy < jy0 || iy > jy1) continue;
+                    const int ix = cell - (cell / kGrid) * kGrid;
+                    if (ix < jx0 || ix > jx1) continue;
                     const int b = k >= 2 * bins ? (k >= 3 * bins ? 3 : 2) : (k >= bins ? 1 : 0);
                     const int pos = k - b * bins;
                     const Pt* bb = lds_pt + b * bins;
