@@ -90,6 +90,33 @@ def test_caustic_bins_matches_reference(gpu, ref, bins):
     _compare(got, meth, exp, emeth, f"plain curve, {bins} caustic bins")
 
 
+@pytest.mark.parametrize("kinji,hex_thr", [(5.0, 3.0), (20.0, 1.0), (20.0, 0.0), (2.0, 6.0)])
+def test_dispatch_thresholds_match_reference(gpu, ref, kinji, hex_thr):
+    # The caustic-distance search stops refining past 3.2 hex_threshold rho and
+    # skips the nearest-point scan only when the bounding-box margin
+    # (point_source_threshold rho) lies beyond that: (5, 3) and (2, 6) never
+    # skip it, (20, 1) skips it more often than the defaults, and (20, 0)
+    # (no distance test at all) never caps.
+    from lcbinint_amd.workloads import finite_source_batch, plain_curve
+    curves, offsets, times, _ = finite_source_batch()
+    sizes = np.diff(offsets)
+    pick = [i for i in range(len(curves)) if sizes[i] <= 300 and curves[i]["rho"] <= 3e-3][:6]
+    cv0, t0 = plain_curve()
+    sel = [cv0] + [curves[i] for i in pick]
+    ts = [t0] + [times[offsets[i]:offsets[i + 1]] for i in pick]
+    off = np.concatenate([[0], np.cumsum([len(t) for t in ts])]).astype(np.int64)
+    tt = np.concatenate(ts)
+    o = _options(gpu, point_source_threshold=kinji, hexadecapole_threshold=hex_thr)
+    got, meth, prof = _gpu_curves(gpu, sel, off, tt, o)
+    assert prof[2] == 0, prof
+    for ci, cv in enumerate(sel):
+        sl = slice(off[ci], off[ci + 1])
+        exp, emeth, st = ref.light_curve(_settings_for(o, cv["ld_c"]), tt[sl], cv["t0"], cv["tE"],
+                                         cv["u0"], cv["alpha"], cv["s"], cv["q"], cv["rho"])
+        assert st == 0
+        _compare(got[sl], meth[sl], exp, emeth, f"kinji {kinji}, hex {hex_thr}, curve {ci}")
+
+
 def test_forced_inverse_ray_positions(gpu, ref):
     # bbox exit, derivative and hexadecapole switched off as binary_inverse_ray
     # does (python/lcbinint_pybind.cpp:3826-3847)
