@@ -333,7 +333,8 @@ void fs_destroy(FsState* fs)
     if (fs->ir.polar) (void)hipStreamDestroy(fs->ir.polar);
     if (fs->ir.polar_small) (void)hipStreamDestroy(fs->ir.polar_small);
     if (fs->ir.big) (void)hipStreamDestroy(fs->ir.big);
-    for (hipEvent_t e : {fs->ir.fork, fs->ir.fork_lead, fs->ir.join_polar, fs->ir.join_polar_small, fs->ir.join_big})
+    for (hipEvent_t e : {fs->ir.fork, fs->ir.fork_lead, fs->ir.join_polar, fs->ir.join_polar_small, fs->ir.join_big,
+                         fs->ir.cache_fork, fs->ir.cache_join})
         if (e) (void)hipEventDestroy(e);
     delete fs;
 }
@@ -424,16 +425,30 @@ lcbg_status fs_prepare(lcbg_batch* b)
     if (st == LCBG_OK) st = map_hip(hipStreamCreateWithFlags(&fs->ir.polar_small, hipStreamNonBlocking));
     if (st == LCBG_OK) st = map_hip(hipStreamCreateWithFlags(&fs->ir.big, hipStreamNonBlocking));
     for (hipEvent_t* e : {&fs->ir.fork, &fs->ir.fork_lead, &fs->ir.join_polar, &fs->ir.join_polar_small,
-                          &fs->ir.join_big})
+                          &fs->ir.join_big, &fs->ir.cache_fork, &fs->ir.cache_join})
         if (st == LCBG_OK) st = map_hip(hipEventCreateWithFlags(e, hipEventDisableTiming));
     return st;
 }
 
 // Rebuilds the per-curve caches on `stream` if the curves changed since.
-lcbg_status fs_refresh(lcbg_batch* b, hipStream_t stream)
+// With `beside` (finite-source runs) the branch part is forked onto the large
+// pass's stream, and *joined is set: the caller makes `stream` wait for
+// ir.cache_join before anything reads the branches.
+lcbg_status fs_refresh(lcbg_batch* b, hipStream_t stream, bool beside = false, bool* joined = nullptr)
 {
+    if (joined) *joined = false;
     if (!b->caches_stale) return LCBG_OK;
-    lcbg_status st = launch_curve_prep(b->d_curves, b->n_curves, b->fs->cache, b->fs->tab, stream);
+    FsState* fs = b->fs;
+    lcbg_status st = launch_curve_prep(b->d_curves, b->n_curves, fs->cache, fs->tab, stream);
+    if (st == LCBG_OK && beside) {
+        st = map_hip(hipEventRecord(fs->ir.cache_fork, stream));
+        if (st == LCBG_OK) st = map_hip(hipStreamWaitEvent(fs->ir.big, fs->ir.cache_fork, 0));
+        if (st == LCBG_OK) st = launch_branch_cache(b->d_curves, b->n_curves, fs->cache, fs->ir.big);
+        if (st == LCBG_OK) st = map_hip(hipEventRecord(fs->ir.cache_join, fs->ir.big));
+        if (st == LCBG_OK && joined) *joined = true;
+    } else if (st == LCBG_OK) {
+        st = launch_branch_cache(b->d_curves, b->n_curves, fs->cache, stream);
+    }
     if (st == LCBG_OK) b->caches_stale = false;
     return st;
 }
@@ -505,7 +520,18 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
         fs->ir.kev = kev;
         (void)hipEventRecord(ev[0], stream);
     }
-    st = fs_refresh(b, stream);
+    bool cache_joined = false;
+    st = fs_refresh(b, stream, true, &cache_joined);
+    // every return path leaves `stream` ordered after the branch caches
+    struct CacheJoin {
+        hipStream_t stream;
+        hipEvent_t join;
+        bool armed;
+        ~CacheJoin()
+        {
+            if (armed) (void)hipStreamWaitEvent(stream, join, 0);
+        }
+    } cache_join{stream, fs->ir.cache_join, cache_joined};
     if (st != LCBG_OK) return st;
     if (stage_ms) (void)hipEventRecord(ev[1], stream);
     ScratchPool& pool = g_pool[b->device % kMaxDevices];
@@ -529,6 +555,11 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     if (st == LCBG_OK) st = launch_fs_stage1(b->d_curves, b->n_curves, fs->cache, fs->opt, ep, stream);
     stage_done("stage 1 (point source, bounding box)");
     if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[2], stream);
+    // stage 2 reads the branch caches built beside stage 1
+    if (st == LCBG_OK && cache_join.armed) {
+        cache_join.armed = false;
+        st = map_hip(hipStreamWaitEvent(stream, fs->ir.cache_join, 0));
+    }
     if (st == LCBG_OK) st = launch_fs_stage2(b->d_curves, pool.cus, fs->cache, fs->opt, ep, fs->stage2_waves, stream);
     stage_done("stage 2 (caustic distance, hexadecapole)");
     if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[3], stream);

@@ -8,8 +8,12 @@
 
 namespace lcbg {
 
+// Caustic caches: critical-curve solves, bounding box and point cells (the
+// part stage 1 reads), then the tracked branches, their cells and their order
+// by cell (the part stage 2 and the seed search read).
 lcbg_status launch_curve_prep(const CurveRec* d_curves, int n_curves, const FsCache& cache,
                               const FsTables& tab, hipStream_t st);
+lcbg_status launch_branch_cache(const CurveRec* d_curves, int n_curves, const FsCache& cache, hipStream_t st);
 
 // Stage 1 (one workgroup per curve): trajectory, point-source solve, the
 // bounding-box exit; queues the remaining epochs for stage 2.
@@ -34,6 +38,7 @@ struct IrStreams {
     hipStream_t polar_small;  // polar small-slot pass
     hipStream_t big;          // Cartesian large-slot pass of the largest epochs
     hipEvent_t fork, fork_lead, join_polar, join_polar_small, join_big;
+    hipEvent_t cache_fork, cache_join;  // branch caches on `big` beside stage 1
     // per-launch timing (profiled runs only, else null): events 2k and 2k+1
     // bracket launch k on the stream it runs on, in the order of IrLaunch
     hipEvent_t* kev;

@@ -173,6 +173,7 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
         sc->max_y = max_y;
         sc->step_x = step_x;
         sc->step_y = step_y;
+        sc->n = bins;
     }
     // seed Phase-2 greedy nearest-neighbour matching (:1813-1840): the
     // reference starts every branch at (1e30, 1e30) with a best distance of
@@ -186,8 +187,8 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
 // time, lane j loads phase base + j and keeps the points chosen for it (stored
 // once the chunk is done), a step reads its phase's points from lane i - base,
 // lanes 0-23 price the 24 permutations.  Then the branch grid cells and the
-// longest non-wrap segment (:4065-4095); the cache counts as built (n = bins)
-// from here on.
+// longest non-wrap segment (:4065-4095).  Stage 1 needs only what
+// k_curve_prep writes (bounding box, n), so it may run beside this kernel.
 __global__ void __launch_bounds__(kWave) k_branch_track(const CurveRec* __restrict__ curves, FsCache cache)
 {
     const int c = blockIdx.x;
@@ -286,7 +287,6 @@ __global__ void __launch_bounds__(kWave) k_branch_track(const CurveRec* __restri
     }
     seg2 = wave_max(seg2);
     wave_store(&sc->max_seg_len, __builtin_sqrt(seg2));
-    wave_store(&sc->n, bins);
 }
 
 // The branch positions of each curve in (grid cell, position) order, the
@@ -360,6 +360,12 @@ lcbg_status launch_curve_prep(const CurveRec* d_curves, int n_curves, const FsCa
     if (n_curves <= 0) return LCBG_OK;
     hipLaunchKernelGGL(k_curve_prep, dim3(static_cast<unsigned>(n_curves)), dim3(kPrepBlock), 0,
                        st, d_curves, cache, tab);
+    return check_launch();
+}
+
+lcbg_status launch_branch_cache(const CurveRec* d_curves, int n_curves, const FsCache& cache, hipStream_t st)
+{
+    if (n_curves <= 0) return LCBG_OK;
     hipLaunchKernelGGL(k_branch_track, dim3(static_cast<unsigned>(n_curves)), dim3(kWave), 0, st, d_curves,
                        cache);
     hipLaunchKernelGGL(k_branch_order, dim3(static_cast<unsigned>(n_curves)), dim3(kOrderBlock), 0,
