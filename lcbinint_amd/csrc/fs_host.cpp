@@ -29,7 +29,7 @@ struct FsState {
     int64_t epoch_cap = 0;
     void* epoch_mem = nullptr;
     unsigned* counters = nullptr;
-    int stage2_waves = 8192;
+    int stage2_waves = 8192;  // waves of the per-epoch stage-2 launches (hexadecapole, derivative test)
     int64_t last_n = 0;  // epochs of the last run (their workspace is kept)
     // per-epoch ticks of the last run, the next run's queue order (FsEpochs::sched)
     float* sched = nullptr;
