@@ -11,10 +11,13 @@ marks the per-curve caustic caches stale), then one finite-source run rebuilds
 the caches and runs point source, dispatcher, hexadecapole and inverse-ray
 shooting over all epochs.  Epoch times are resident in HBM.
 
-Also reported: the plain GPU path on the single README curve (200 epochs,
-nbin 50), called once per curve and once per epoch through the host-pointer
-entry point; the 4096 x 256 point-source batch; the reference core compiled
-from source (oracle/_ref) timed on the host cores on a bounded sample.
+Also reported: the plain GPU method on the single README curve (200 epochs,
+nbin 50): one epoch per launch on a reused one-curve batch, value and value +
+gradient, and the host-pointer drop-in entry point called once per curve and
+once per epoch; the 4096 x 256 point-source batch; the value + gradient
+workload (256 x 1000); the parallax / orbital-motion workload; the reference
+core compiled from source (oracle/_ref) timed on the host cores, on a bounded
+sample of the headline workload and on the README curve (one process).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -174,13 +177,13 @@ def bench_batched(capi, torch, steps, warmup, autotune=True):
                 schedule=schedule.as_dict() if schedule is not None else None, tuned_ms=tuned_ms)
 
 
-def bench_value_grad(capi, torch, steps, warmup, per_epoch_calls=64):
+def bench_value_grad(capi, torch, steps, warmup):
     """chi^2 and its gradient w.r.t. (t0, tE, u0, alpha, s, q, rho, u_LD) of
     256 curves x 1000 epochs (lcbinint_amd.workloads.differentiation_batch),
     against observed fluxes made from the model (fs = 1, fb = 0.1, 1% noise).
     A step uploads the curve parameters and evaluates magnifications, their
     derivatives and the per-curve chi^2 gradient.  The plain per-epoch path
-    evaluates one epoch per call (its own batch, host round trip)."""
+    is plain_per_epoch(grad=True)."""
     from lcbinint_amd.workloads import differentiation_batch, synthetic_fluxes
     curves, offsets, times = differentiation_batch()
     n, nc = len(times), len(curves)
@@ -218,25 +221,10 @@ def bench_value_grad(capi, torch, steps, warmup, per_epoch_calls=64):
     batch.close()
     if not finite:
         raise RuntimeError("non-finite chi^2 gradient")
-    # plain per-epoch path: one epoch per call
-    P = capi.c_double_p
-    rng = np.random.default_rng(0)
-    picks = rng.choice(n, per_epoch_calls, replace=False)
-    curve_of = np.repeat(np.arange(nc), np.diff(offsets))
-    one_t = torch.empty(1, dtype=torch.float64, device="cuda")
-    one_off = torch.tensor([0, 1], dtype=torch.int64, device="cuda")
-    one_m = torch.empty(1, dtype=torch.float64, device="cuda")
-    one_k = torch.empty(1, dtype=torch.int32, device="cuda")
-    one_d = torch.empty(8, dtype=torch.float64, device="cuda")
-    t0 = time.perf_counter()
-    for e in picks:
-        b1 = capi.Batch([capi.Curve(**curves[curve_of[e]])])
-        one_t.fill_(float(times[e]))
-        capi.finite_source_grad(b1, one_off.data_ptr(), one_t.data_ptr(), 1, one_m.data_ptr(),
-                                one_k.data_ptr(), 0, one_d.data_ptr(), sptr)
-        _ = one_d.cpu()
-        b1.close()
-    per_epoch = len(picks) / (time.perf_counter() - t0)
+    # plain per-epoch path: one epoch per launch on a reused one-curve handle
+    # (no allocation, caustic cache rebuilt only when the curve changes), the
+    # epochs of a few curves in order, as the reference's epoch loop runs them
+    per_epoch, per_epoch_meth = plain_per_epoch(capi, torch, grad=True)
     value_grad = n * steps / wall
     return {
         "workload": "chi^2 and gradient w.r.t. (t0,tE,u0,alpha,s,q,rho,u_LD): 256 curves x 1000 "
@@ -245,8 +233,50 @@ def bench_value_grad(capi, torch, steps, warmup, per_epoch_calls=64):
                     "fluxes F = A + 0.1 with 1% noise",
         "value_grad_per_s": value_grad, "unit": "epochs/s", "ms_per_step": 1e3 * wall / steps,
         "methods": methods, "plain_per_epoch_value_grad_per_s": per_epoch,
+        "plain_per_epoch_methods": per_epoch_meth,
+        "plain_per_epoch": "README curve (200 epochs, nbin 50, auto grid), one epoch per launch of "
+                           "lcbg_batch_finite_source_grad_device on a reused one-curve batch (no "
+                           "allocation, caustic cache built once), result read back per epoch",
         "batched_over_plain_per_epoch": value_grad / per_epoch,
     }
+
+
+def plain_per_epoch(capi, torch, grad):
+    """The plain GPU method on the README curve (200 epochs, nbin 50, auto
+    grid): one epoch per launch, in time order, on one reused one-curve batch
+    (no allocation; the caustic cache is built once), the result copied back
+    after every launch, as the reference's epoch loop evaluates a curve.
+    Returns (epochs/s, method counts)."""
+    from lcbinint_amd.workloads import plain_curve
+    cv, times = plain_curve()
+    one_t = torch.empty(1, dtype=torch.float64, device="cuda")
+    one_off = torch.tensor([0, 1], dtype=torch.int64, device="cuda")
+    one_m = torch.empty(1, dtype=torch.float64, device="cuda")
+    one_k = torch.empty(1, dtype=torch.int32, device="cuda")
+    one_d = torch.empty(8, dtype=torch.float64, device="cuda")
+    sptr = torch.cuda.current_stream().cuda_stream
+    batch = capi.Batch([capi.Curve(**cv)])
+    meth = np.zeros(4, dtype=np.int64)
+
+    def one(t):
+        one_t.fill_(float(t))
+        if grad:
+            capi.finite_source_grad(batch, one_off.data_ptr(), one_t.data_ptr(), 1, one_m.data_ptr(),
+                                    one_k.data_ptr(), 0, one_d.data_ptr(), sptr)
+            _ = one_d.cpu()
+        else:
+            capi.finite_source(batch, one_off.data_ptr(), one_t.data_ptr(), 1, one_m.data_ptr(),
+                               one_k.data_ptr(), sptr)
+        return int(one_k.cpu()[0])
+
+    one(times[0])  # warm-up: workspace, scratch and the caustic cache
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in times:
+        meth[one(t)] += 1
+    elapsed = time.perf_counter() - t0
+    batch.close()
+    return len(times) / elapsed, meth.tolist()
 
 
 def bench_dynamic(capi, n_curves=128, steps=2):
@@ -315,6 +345,29 @@ def bench_plain(capi, calls_per_epoch=50):
         run(t[i:i + 1], one, one_m)
     per_epoch = len(idx) / (time.perf_counter() - t0)
     return per_curve, per_epoch, np.bincount(meth, minlength=4).tolist()
+
+
+def cpu_readme_curve():
+    """The reference core (oracle/_ref) on the README curve, one process, as
+    LightCurve(...)(times, params) runs it: ms per epoch."""
+    from lcbinint_amd.workloads import plain_curve
+    from oracle.bindings import default_settings, load_ref
+    path = ROOT / "oracle" / "_ref" / "liblcbinint_ref.so"
+    if not path.exists():
+        return None
+    ref = load_ref(path)
+    cv, t = plain_curve()
+    st = default_settings(source_bins=50, limb_darkening_c=cv["ld_c"])
+    args = (cv["t0"], cv["tE"], cv["u0"], cv["alpha"], cv["s"], cv["q"], cv["rho"])
+    ref.light_curve(st, t, *args)  # warm-up
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, meth, status = ref.light_curve(st, t, *args)
+    dt = (time.perf_counter() - t0) / reps
+    return {"ms_per_epoch": 1e3 * dt / len(t), "mags_per_s": len(t) / dt, "cores": 1, "kind": "reference",
+            "sample": f"README curve, {len(t)} epochs, nbin 50, auto grid, {reps} repeats, one process",
+            "methods": np.bincount(meth, minlength=4).tolist()}
 
 
 def bench_point_source(capi, torch, steps=10):
@@ -393,13 +446,20 @@ def main() -> int:
     if not args.no_extras:
         print("[bench] plain path and point-source batch", file=sys.stderr, flush=True)
         per_curve, per_epoch, plain_meth = bench_plain(capi)
+        launch_rate, launch_meth = plain_per_epoch(capi, torch, grad=False)
         extras["plain_gpu"] = {
             "workload": "README curve: t0=0 tE=1 u0=-0.01 alpha=0.5 s=0.95 q=0.01 rho=5e-3, "
                         "linear LD 0.5, 200 epochs, nbin 50, auto grid",
             "per_curve_call_mags_per_s": per_curve, "per_epoch_call_mags_per_s": per_epoch,
             "methods": plain_meth,
-            "batched_over_per_epoch": value / per_epoch,
-            "batched_over_per_curve": value / per_curve,
+            "batched_over_per_epoch_call": value / per_epoch,
+            "batched_over_per_curve_call": value / per_curve,
+            "per_epoch_launch_mags_per_s": launch_rate,
+            "per_epoch_launch_methods": launch_meth,
+            "per_epoch_launch": "the same curve, one epoch per launch of lcbg_batch_finite_source_device "
+                                "on a reused one-curve batch (no allocation, caustic cache built once), "
+                                "result read back per epoch",
+            "batched_over_per_epoch_launch": value / launch_rate,
         }
         extras["point_source_batch"] = bench_point_source(capi, torch)
         print("[bench] value and gradient workload", file=sys.stderr, flush=True)
@@ -412,6 +472,7 @@ def main() -> int:
         from lcbinint_amd.workloads import finite_source_batch
         curves, offsets, times, nbin = finite_source_batch()
         cpu = cpu_baseline(curves, offsets, times, nbin)
+        extras["cpu_readme_curve"] = cpu_readme_curve()
     out = {
         "metric": METRIC,
         "value": value,
@@ -442,9 +503,9 @@ def main() -> int:
                              "bitwise_equal_to_eager": b["graph_same"]},
         },
         "roofline": {
-            # FP64 compute roof: MI355X's FP64 vector and FP64 matrix peaks are
-            # the same 78.6 TFLOP/s; the lattice scan issues FP64 VALU work
-            "bound": "mfma",
+            # the lattice scan issues FP64 VALU work (no matrix instructions);
+            # the roof is MI355X's FP64 vector peak
+            "bound": "fp64_valu",
             "kernel": "k_ir_cart",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
@@ -459,6 +520,11 @@ def main() -> int:
             "hw_fp64_frac": K_IR_CART_HW_FP64_FLOP / (cart_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
             "inverse_ray_stage_ms": ir_ms,
             "inverse_ray_stage_flops": ir_flops,
+            # the three k_ir_cart launches overlap each other and the polar
+            # and seed launches: the whole stage's algorithmic rate over its
+            # wall time, beside the per-launch figure
+            "inverse_ray_stage_tflops": ir_flops / (ir_ms * 1e-3) / 1e12,
+            "inverse_ray_stage_frac": ir_flops / (ir_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
             "launch_ms": lm,
             "work": w,
         },

@@ -64,7 +64,35 @@ typedef struct lcbg_options {
     double point_source_threshold;/* bbox margin in rho (default 20) */
     double hexadecapole_threshold;/* caustic-distance threshold in rho (default 3) */
     double adaptive_hex_threshold;/* hexadecapole tolerance (default 1e-3) */
+    double finite_source_tol;     /* absolute tolerance of the converged flag (default 0) */
+    double finite_source_reltol;  /* relative tolerance of the converged flag (default 0) */
 } lcbg_options;
+
+/* Per-epoch result record, the fields of lcbi_result
+ * (include/lcbinint/lcbinint.h:106-116) plus the finite-source method:
+ * magnification, the point-source magnification (binary_mag0 on the static
+ * path, the sum over binary_images on the LensModel path), the finite-source
+ * magnification (= magnification), the source position the magnifier
+ * received, the point-source image count, the lcbg_method, the method's error
+ * estimate (FiniteSourceResult::error_estimate: 0 for point source and polar,
+ * the derivative or hexadecapole relative error times max(|A|, 1), the
+ * Cartesian area error indicator of finite_source_magnifier.cpp:1065-1106 with
+ * the high-magnification floor of :2745-2756), the refinement level (always
+ * 0: the plain method does not refine) and the converged flag
+ * (:3290-3294: a Cartesian epoch converged iff its error estimate is at most
+ * 0.999 (tol + reltol max(|A|, 1)); other methods iff A is finite). */
+typedef struct lcbg_result {
+    double magnification;
+    double point_source_magnification;
+    double finite_source_magnification;
+    double source_x;
+    double source_y;
+    int32_t image_count;
+    int32_t method;
+    double error_estimate;
+    int32_t refinement_level;
+    int32_t converged;
+} lcbg_result;
 
 /* Defaults of lcbi_default_options (src/lcbinint/lcbinint.cpp:36-54). */
 lcbg_options lcbg_default_options(void);
@@ -82,9 +110,13 @@ lcbg_status lcbg_binary_mag0_device(const double* s, const double* q, const doub
 
 /* A batch of light-curve models prepared on the device of the calling thread.
  * Creation copies the per-curve records (with cos/sin(alpha) rounded by the
- * host libm, as the reference's std::cos/std::sin are) and allocates the
- * batch's device workspace; the run entry points allocate nothing.  A batch
- * is used by one host thread at a time.  `curves` is host memory. */
+ * host libm, as the reference's std::cos/std::sin are).  The device workspace
+ * is allocated by the first finite-source run on the batch (caustic caches,
+ * angle tables, side streams, and per device the inverse-ray scratch kept
+ * until the process ends), and again, after a device-wide synchronisation,
+ * by a run with more epochs than any earlier one; other runs allocate
+ * nothing.  lcbg_batch_capture_finite_source allocates before it records.
+ * A batch is used by one host thread at a time.  `curves` is host memory. */
 typedef struct lcbg_batch lcbg_batch;
 
 lcbg_status lcbg_batch_create(const lcbg_curve* curves, int32_t n_curves,
@@ -186,6 +218,17 @@ lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const lcbg_dynami
                                             const double* times, int64_t n_epochs, double* mag,
                                             int32_t* method);
 
+/* lcbg_batch_light_curves_dynamic writing one lcbg_result per epoch (host
+ * array of n_epochs), the fields LensModel::magnification fills
+ * (src/lcbinint/model/lens_model.cpp:186-234): the source position in the
+ * instantaneous binary frame, the point-source magnification and image count
+ * of binary_images, the finite-source method, error estimate and converged
+ * flag.  Returns LCBG_NUMERICAL_ERROR if a magnification is not finite. */
+lcbg_status lcbg_batch_light_curves_dynamic_results(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                                    int32_t orbital_motion_mode, const int64_t* offsets,
+                                                    const double* times, int64_t n_epochs,
+                                                    lcbg_result* results);
+
 /* lcbg_batch_light_curves_dynamic with, for every epoch, the derivatives of
  * the magnification with respect to (t0, tE, u0, alpha, s, q, rho, u_LD, piEN,
  * piEE, g1, g2): dmag[12 i + k] (host, n_epochs x 12; no counterpart in the
@@ -223,8 +266,9 @@ lcbg_status lcbg_batch_finite_source_device(lcbg_batch* batch, const int64_t* of
  * q and alpha as in lcbg_curve.  Point-source and hexadecapole epochs are
  * differentiated exactly (implicit differentiation of the lens equation per
  * image); inverse-ray epochs through an antialiased quadrature on a lattice
- * anchored at the lens-frame origin (cell side rho/nbin, the source-disk
- * indicator replaced by the covered fraction of each cell).  mag_aa (may be
+ * anchored at the lens-frame origin (cell side the power of two at or below
+ * rho/nbin, the source-disk indicator replaced by the covered fraction of
+ * each cell).  mag_aa (may be
  * NULL) receives the value of the differentiated function: equal to mag up to
  * rounding for point-source and hexadecapole epochs, the antialiased
  * quadrature for inverse-ray epochs. */
@@ -264,6 +308,14 @@ typedef struct lcbg_schedule {
 } lcbg_schedule;
 
 lcbg_status lcbg_batch_set_schedule(lcbg_batch* batch, const lcbg_schedule* schedule);
+
+/* Work budget of one inverse-ray epoch, in wave steps (a step is one 64-cell
+ * chunk of a row, one probe merge or one polar run; 0 = the default 2^22, also
+ * the maximum).  An epoch that exceeds it is abandoned: its magnification is
+ * NaN and lcbg_batch_last_run_status reports it (flag 4).  The reference has
+ * no such limit; the budget bounds how long any one epoch can hold the
+ * device. */
+lcbg_status lcbg_batch_set_work_budget(lcbg_batch* batch, int64_t steps);
 lcbg_status lcbg_batch_get_schedule(const lcbg_batch* batch, lcbg_schedule* schedule);
 
 /* Autotuner: runs lcbg_batch_finite_source_device on the given workload with
@@ -312,9 +364,13 @@ lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const int64_t* o
  * cost[n + e] = ticks of its area scans (added to the value found there, so
  * the caller zeroes it), cost[2n + e] = quintic probe solves of the seed
  * search, cost[3n + e] = Cartesian rows or polar runs closed and cost[4n + e]
- * = wave steps of the area scans (both added likewise).  cost is a device
- * pointer with 5 n_epochs entries; other epochs are left untouched.  NULL
- * turns it off.
+ * = wave steps of the area scans (both added likewise), and for each Cartesian
+ * epoch the fields of the reference's LegacyAreaDiagnostics
+ * (finite_source_magnifier.cpp:1006-1015): cost[5n + e] processed images,
+ * cost[6n + e] fold-guarded images, cost[7n + e] counted rows (boundary_rows),
+ * cost[8n + e] gap repairs, cost[9n + e] overlap hits and cost[10n + e] the
+ * bits of the double max_jump_cells.  cost is a device pointer with 11
+ * n_epochs entries; other epochs are left untouched.  NULL turns it off.
  * lcbg_wall_clock_rate_khz() gives the tick rate of the current device. */
 lcbg_status lcbg_batch_set_epoch_cost(lcbg_batch* batch, int64_t* cost);
 int64_t lcbg_wall_clock_rate_khz(void);
@@ -368,10 +424,42 @@ lcbg_status lcbg_batch_debug_critical_sources(lcbg_batch* batch, int32_t curve, 
 /* Host-pointer light curve of one curve with the given options (NULL =
  * defaults): the drop-in path behind LightCurve(...)(times, params).  Returns
  * LCBG_NUMERICAL_ERROR if any magnification is not finite (the reference raises
- * RuntimeError("numerical error"), python/lcbinint_pybind.cpp:1153-1155) and
- * LCBG_UNSUPPORTED if a workspace capacity was exceeded. */
+ * RuntimeError("numerical error"), python/lcbinint_pybind.cpp:1153-1155),
+ * including epochs this library abandons at a workspace capacity or work
+ * budget; mag[i] of such an epoch is NaN. */
 lcbg_status lcbg_light_curve(const lcbg_curve* curve, const lcbg_options* options,
                              const double* times, int64_t n, double* mag, int32_t* method);
+
+/* lcbi_magnification_array (src/lcbinint/lcbinint.cpp:89-121) for the static
+ * VBM binary lens: one lcbg_result per epoch (host arrays).  Every epoch is
+ * evaluated; the status is that of the first epoch whose magnification is not
+ * finite (LCBG_NUMERICAL_ERROR, or LCBG_UNSUPPORTED for a method outside this
+ * library), where the reference's loop stops, and *first_failed_epoch (may be
+ * NULL) receives that epoch's index, -1 if none. */
+lcbg_status lcbg_light_curve_results(const lcbg_curve* curve, const lcbg_options* options,
+                                     const double* times, int64_t n, lcbg_result* results,
+                                     int64_t* first_failed_epoch);
+
+/* Outcome of the batch's last finite-source run (any of the run entry points
+ * or a graph replay): waits for that run, then writes the error flags of
+ * epochs abandoned by this library's kernels (error_flags: bit 0 seed list
+ * full, 1 Cartesian row records full, 2 polar run records full, 3 a method
+ * outside this library, 4 work budget, 5 seed pool full, 6 a Cartesian team
+ * wait ran out of time) and the index of the first epoch whose magnification
+ * is not finite (-1 if none), where lcbi_magnification_array stops
+ * (src/lcbinint/lcbinint.cpp:110-119).  Either pointer may be NULL.  Returns
+ * LCBG_OK if every epoch is finite, else LCBG_UNSUPPORTED (flag 3),
+ * LCBG_DEVICE_ERROR (flag 6) or LCBG_NUMERICAL_ERROR. */
+lcbg_status lcbg_batch_last_run_status(lcbg_batch* batch, uint32_t* error_flags,
+                                       int64_t* first_failed_epoch);
+
+/* Per-epoch result records of the batch's last finite-source run (device
+ * pointer to n_epochs records; n_epochs must be that run's epoch count),
+ * enqueued on `stream`, which must be ordered after the run: the lcbg_result
+ * fields, converged with the tolerances of the batch's options.  The run's
+ * mag and method buffers are read, so they must still hold its results. */
+lcbg_status lcbg_batch_epoch_results_device(lcbg_batch* batch, int64_t n_epochs, lcbg_result* results,
+                                            void* stream);
 
 /* Device time in milliseconds of the last kernel launched by the calling host
  * thread through a *_device entry point, measured with HIP events recorded on

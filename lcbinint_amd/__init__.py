@@ -12,8 +12,8 @@ The evaluation runs in the HIP library (liblcbinint_amd.so).  If the extension
 has not been built, every API name raises ImportError when it is used; the
 build module stays importable (``python -m lcbinint_amd.build``).
 """
-_API = ("LightCurve", "Options", "LimbDarkening", "OrbitalMotionMode", "FiniteSourceMethod",
-        "EventCoordinates")
+_API = ("LightCurve", "LightCurveFunc", "LightCurveInfo", "Options", "LimbDarkening",
+        "OrbitalMotionMode", "FiniteSourceMethod", "EventCoordinates")
 
 # torch first, so that the extension binds to torch's HIP runtime (one HIP
 # runtime per process; see capi.py)
@@ -27,6 +27,8 @@ try:
         EventCoordinates,
         FiniteSourceMethod,
         LightCurve,
+        LightCurveFunc,
+        LightCurveInfo,
         LimbDarkening,
         OrbitalMotionMode,
         Options,

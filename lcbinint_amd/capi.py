@@ -54,7 +54,26 @@ class Options(ctypes.Structure):
         ("point_source_threshold", ctypes.c_double),
         ("hexadecapole_threshold", ctypes.c_double),
         ("adaptive_hex_threshold", ctypes.c_double),
+        ("finite_source_tol", ctypes.c_double),
+        ("finite_source_reltol", ctypes.c_double),
     ]
+
+
+class Result(ctypes.Structure):
+    """lcbg_result: one epoch's record (the fields of the reference's lcbi_result)."""
+    _fields_ = [
+        ("magnification", ctypes.c_double), ("point_source_magnification", ctypes.c_double),
+        ("finite_source_magnification", ctypes.c_double), ("source_x", ctypes.c_double),
+        ("source_y", ctypes.c_double), ("image_count", ctypes.c_int32), ("method", ctypes.c_int32),
+        ("error_estimate", ctypes.c_double), ("refinement_level", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+    ]
+
+
+# numpy view of an array of lcbg_result records
+RESULT_DTYPE = np.dtype([(name, np.float64 if typ is ctypes.c_double else np.int32)
+                         for name, typ in Result._fields_], align=True)
+assert RESULT_DTYPE.itemsize == ctypes.sizeof(Result)
 
 
 def _sig(name, restype, *argtypes):
@@ -414,3 +433,43 @@ def chi2_grad(batch: Batch, offsets_ptr: int, times_ptr: int, n_epochs: int, flu
                                       sigma_ptr, fs, fb, mag_ptr, method_ptr, mag_aa_ptr, dmag_ptr,
                                       chi2_ptr, grad_ptr, stream),
           "lcbg_batch_chi2_grad_device")
+
+
+lcbg_light_curve_results = _sig(
+    "lcbg_light_curve_results", ctypes.c_int, ctypes.POINTER(Curve), ctypes.POINTER(Options),
+    c_double_p, ctypes.c_int64, ctypes.c_void_p, c_int64_p)
+lcbg_batch_last_run_status = _sig(
+    "lcbg_batch_last_run_status", ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32),
+    c_int64_p)
+lcbg_batch_epoch_results_device = _sig(
+    "lcbg_batch_epoch_results_device", ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+    ctypes.c_void_p)
+
+
+def light_curve_results(curve: Curve, times, options: Options | None = None):
+    """(records, status, first failed epoch) of one curve on the host path:
+    records is a structured array of RESULT_DTYPE, status the lcbg_status."""
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    out = np.zeros(len(t), dtype=RESULT_DTYPE)
+    first = ctypes.c_int64(-1)
+    st = lcbg_light_curve_results(ctypes.byref(curve), ctypes.byref(options) if options is not None else None,
+                                  t.ctypes.data_as(c_double_p), len(t), out.ctypes.data, ctypes.byref(first))
+    return out, st, first.value
+
+
+def last_run_status(batch: "Batch"):
+    """(status, error flags, first failed epoch or -1) of the batch's last run."""
+    flags = ctypes.c_uint32()
+    first = ctypes.c_int64()
+    st = lcbg_batch_last_run_status(batch.handle, ctypes.byref(flags), ctypes.byref(first))
+    return st, flags.value, first.value
+
+
+def epoch_results(batch: "Batch", n_epochs: int, results_ptr: int, stream: int = 0):
+    """Enqueues the per-epoch records of the batch's last run into a device buffer
+    of n_epochs * RESULT_DTYPE.itemsize bytes."""
+    check(lcbg_batch_epoch_results_device(batch.handle, n_epochs, results_ptr, stream),
+          "lcbg_batch_epoch_results_device")
+
+lcbg_batch_set_work_budget = _sig("lcbg_batch_set_work_budget", ctypes.c_int, ctypes.c_void_p, ctypes.c_int64)
+

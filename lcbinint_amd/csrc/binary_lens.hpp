@@ -189,6 +189,26 @@ __device__ __forceinline__ double ps_from_images(const Geom& g, const cx* roots)
     return mag;
 }
 
+// Number of images binary_images keeps (point_source_magnifier.cpp:723-778):
+// three if r[3] * 1e-4 > r[2] + 1e-12 over the residuals sorted ascending, else five.
+__device__ __forceinline__ int binary_images_count(const Geom& g, const cx* roots)
+{
+    double res[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) res[i] = __builtin_sqrt(residual_squared(g, roots[i]));
+    // the third and fourth smallest residuals of five
+    for (int i = 1; i < 5; ++i) {
+        const double v = res[i];
+        int j = i;
+        while (j > 0 && v < res[j - 1]) {
+            res[j] = res[j - 1];
+            --j;
+        }
+        res[j] = v;
+    }
+    return (res[3] * 1.0e-4 > res[2] + 1.0e-12) ? 3 : 5;
+}
+
 // binary_mag0 (point_source_magnifier.cpp:567 -> 652-693), no root cache.
 __device__ __forceinline__ PointMag binary_mag0(double s, double q, double x, double y)
 {

@@ -43,6 +43,7 @@ __global__ void __launch_bounds__(kStage1Block) k_fs_stage1(const CurveRec* __re
                          tau * cv.sin_alpha + cv.u0 * cv.cos_alpha};
             ep.curve_of[e] = c;
             ep.src[e] = src;
+            ep.err[e] = 0.0;  // point source: no error estimate (:4668-4698)
             if (cv.s == 0.0 || cv.q_eff <= 0.0) {
                 ep.mag[e] = 0.0;
                 ep.method[e] = LCBG_METHOD_POINT_SOURCE;
@@ -485,6 +486,7 @@ __global__ void __launch_bounds__(kNearTestBlock) k_fs_near_test(const CurveRec*
                 if (isfinite(dre) && dre <= thr) {
                     ep.mag[e] = a_d;
                     ep.method[e] = LCBG_METHOD_POINT_SOURCE;
+                    ep.err[e] = dre * dmax(fabs(a_d), 1.0);  // :4742-4751
                 } else {
                     hex = true;
                 }
@@ -567,6 +569,7 @@ __global__ void __launch_bounds__(kWave) k_fs_hex(const CurveRec* __restrict__ c
             if (rel <= opt.adaptive_hex) {
                 ep.mag[e] = hmag;
                 ep.method[e] = LCBG_METHOD_HEXADECAPOLE;
+                ep.err[e] = rel * dmax(fabs(hmag), 1.0);  // :4792-4802
             } else {
                 ir = mark_inverse_ray(opt, ep, e, a_ps);
             }
@@ -574,6 +577,91 @@ __global__ void __launch_bounds__(kWave) k_fs_hex(const CurveRec* __restrict__ c
         const int is = wave_append(&ep.counters[kQIrCount], ir);
         if (ir) ep.ir_list[is] = e;
     }
+}
+
+// The first epoch whose magnification is not finite: counters[kFirstFailWord]
+// = max over such epochs e of n - e (0: none).  The reference's array call stops
+// at that epoch (src/lcbinint/lcbinint.cpp:110-119).
+constexpr int kScanBlock = 256;
+
+__global__ void __launch_bounds__(kScanBlock) k_first_nonfinite(const double* __restrict__ mag, int64_t n,
+                                                                 unsigned* counters)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kScanBlock;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kScanBlock; base < n; base += stride) {
+        const int64_t e = base + static_cast<int64_t>(threadIdx.x);
+        const bool bad = e < n && !isfinite(mag[e]);
+        const uint64_t m = ballot(bad);
+        if (m) {
+            const int64_t first = base + (static_cast<int64_t>(threadIdx.x) & ~int64_t{kWave - 1}) + first_set(m);
+            if (lane_id() == 0) atomicMax(&counters[kFirstFailWord], static_cast<unsigned>(n - first));
+            return;  // later epochs of this thread's range come after it
+        }
+    }
+}
+
+lcbg_status launch_first_nonfinite(const double* mag, int64_t n, unsigned* counters, hipStream_t st)
+{
+    if (n <= 0) return LCBG_OK;
+    const int64_t blocks = (n + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_first_nonfinite, dim3(static_cast<unsigned>(blocks < 2048 ? blocks : 2048)),
+                       dim3(kScanBlock), 0, st, mag, n, counters);
+    return check_launch();
+}
+
+// Per-epoch records of a finished run, the fields of lcbi_result
+// (include/lcbinint/lcbinint.h:106-116) as evaluate_binary_light_curve_info
+// (python/lcbinint_pybind.cpp:1211-1253) and LensModel::magnification
+// (src/lcbinint/model/lens_model.cpp:186-234) fill them.
+__global__ void __launch_bounds__(kScanBlock) k_epoch_results(const CurveRec* __restrict__ curves,
+                                                               FsOptions opt, FsEpochs ep, double tol,
+                                                               double reltol, lcbg_result* out)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kScanBlock;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kScanBlock + threadIdx.x; e < ep.n; e += stride) {
+        const CurveRec cv = curves[ep.curve_of[e]];
+        const Pt src = ep.src[e];
+        const double mag = ep.mag[e];
+        const int method = ep.method[e];
+        int images = 0;
+        if (!(cv.s == 0.0 || cv.q_eff <= 0.0)) {
+            const Geom g = make_geom(cv.s, cv.q_eff, src.x, src.y);
+            cx roots[5];
+            const double* rr = ep.roots + 10 * e;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) roots[k] = cx{rr[2 * k], rr[2 * k + 1]};
+            // the LensModel path counts binary_images; the static path binary_mag0's
+            images = (opt.ps_images && cv.rho != 0.0) ? binary_images_count(g, roots)
+                                                       : popcount64(select_physical(g, roots));
+        }
+        const double err = ep.err[e];
+        bool converged = isfinite(mag);
+        // fixed_inverse_ray_binary (finite_source_magnifier.cpp:3292-3294)
+        if (method == LCBG_METHOD_INVERSE_RAY_CARTESIAN)
+            converged = converged && err <= 0.999 * (tol + reltol * dmax(fabs(mag), 1.0));
+        lcbg_result r;
+        r.magnification = mag;
+        r.point_source_magnification = ep.a_ps[e];
+        r.finite_source_magnification = mag;
+        r.source_x = src.x;
+        r.source_y = src.y;
+        r.image_count = images;
+        r.method = method;
+        r.error_estimate = err;
+        r.refinement_level = 0;
+        r.converged = converged ? 1 : 0;
+        out[e] = r;
+    }
+}
+
+lcbg_status launch_epoch_results(const CurveRec* d_curves, const FsOptions& opt, const FsEpochs& ep,
+                                 double tol, double reltol, lcbg_result* out, hipStream_t st)
+{
+    if (ep.n <= 0) return LCBG_OK;
+    const int64_t blocks = (ep.n + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_epoch_results, dim3(static_cast<unsigned>(blocks < 4096 ? blocks : 4096)),
+                       dim3(kScanBlock), 0, st, d_curves, opt, ep, tol, reltol, out);
+    return check_launch();
 }
 
 __global__ void k_reset_counters(unsigned* counters, int n)

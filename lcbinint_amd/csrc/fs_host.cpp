@@ -31,6 +31,8 @@ struct FsState {
     unsigned* counters = nullptr;
     int stage2_waves = 8192;  // waves of the per-epoch stage-2 launches (hexadecapole, derivative test)
     int64_t last_n = 0;  // epochs of the last run (their workspace is kept)
+    double* last_mag = nullptr;     // the last run's output buffers (caller-owned)
+    int32_t* last_method = nullptr;
     // per-epoch ticks of the last run, the next run's queue order (FsEpochs::sched)
     float* sched = nullptr;
     int64_t sched_cap = 0;
@@ -87,7 +89,7 @@ lcbg_status ensure_epochs(FsState* fs, int64_t n)
                               carve_size(cap * sizeof(int64_t)) + carve_size(cap * sizeof(int)) * 9 +
                               carve_size(pool_cap_for(cap) * sizeof(Pt2)) +
                               carve_size(cap * kTangentsHost * sizeof(double)) +
-                              carve_size(cap * sizeof(int64_t)) * 3;
+                              carve_size(cap * sizeof(int64_t)) * 3 + carve_size(cap * sizeof(double));
     lcbg_status st = map_hip(hipMalloc(&fs->epoch_mem, bytes));
     if (st != LCBG_OK) {
         fs->epoch_mem = nullptr;
@@ -221,6 +223,26 @@ int64_t* near_chunks_view(FsState* fs)
     return carve<int64_t>(p, fs->epoch_cap);
 }
 
+// Per-epoch error estimates, after the stage-2 work items.
+double* err_view(FsState* fs)
+{
+    char* p = reinterpret_cast<char*>(near_chunks_view(fs));
+    (void)carve<int64_t>(p, fs->epoch_cap);
+    return carve<double>(p, fs->epoch_cap);
+}
+
+// The status a run's outcome maps to: a method outside this library is
+// unsupported, a team wait that ran out of time a device failure, and any
+// other abandoned epoch (its magnification is NaN) a numerical error, as a
+// non-finite magnification is in the reference (src/lcbinint/lcbinint.cpp:113-118).
+lcbg_status status_of_run(unsigned flags, int64_t first_failed)
+{
+    if (flags & kErrUnsupported) return LCBG_UNSUPPORTED;
+    if (flags & kErrTeam) return LCBG_DEVICE_ERROR;
+    if (flags || first_failed >= 0) return LCBG_NUMERICAL_ERROR;
+    return LCBG_OK;
+}
+
 // Inverse-ray scratch, one pool per device shared by every batch: its use is
 // serialised by a mutex held while a run is enqueued and by an event that the
 // next run waits for.
@@ -323,6 +345,45 @@ lcbg_status ensure_scratch(ScratchPool& pool)
 
 }  // namespace
 
+void angle_tables(int bins, std::vector<Pt2>& host)
+{
+    // each expression as the reference writes it
+    host.clear();
+    host.reserve(static_cast<std::size_t>(angle_table_count(bins)));
+    for (int i = 0; i < bins; ++i) {
+        const double phase = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(bins);
+        host.push_back({std::cos(phase), std::sin(phase)});
+    }
+    const double phase_step = 2.0 * kPiHost / static_cast<double>(kSeedPhases);
+    for (int k = 0; k < kSeedPhases; ++k) {
+        const double phi = phase_step * static_cast<double>(k);
+        host.push_back({std::cos(phi), std::sin(phi)});
+    }
+    for (int i = 0; i < kBoundaryProbes; ++i) {
+        const double phi = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(kBoundaryProbes);
+        host.push_back({std::cos(phi), std::sin(phi)});
+    }
+    for (int i = 0; i < kAngularProbes; ++i) {
+        const double th = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(kAngularProbes);
+        host.push_back({std::cos(th), std::sin(th)});
+    }
+    for (int i = 0; i < kInteriorProbes; ++i) {
+        const double phi = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(kInteriorProbes);
+        host.push_back({std::cos(phi), std::sin(phi)});
+    }
+}
+
+FsTables table_view(const Pt2* t, int bins)
+{
+    FsTables tab;
+    tab.caus_trig = t;
+    tab.seed_trig = t + bins;
+    tab.boundary_trig = tab.seed_trig + kSeedPhases;
+    tab.angular_trig = tab.boundary_trig + kBoundaryProbes;
+    tab.interior_trig = tab.angular_trig + kAngularProbes;
+    return tab;
+}
+
 void fs_destroy(FsState* fs)
 {
     if (!fs) return;
@@ -360,43 +421,16 @@ lcbg_status fs_prepare(lcbg_batch* b)
     fs->opt.ps_images = b->dynamic_images ? 1 : 0;
     fs->opt.heavy_budget = kHeavyBudget;
     fs->opt.lead_items = kLeadItems;
+    fs->opt.step_budget = kStepBudget;
     const int bins = o.caustic_bins > 32 ? o.caustic_bins : 32;
 
-    // angle tables, each expression as the reference writes it
     std::vector<Pt2> host;
-    const std::size_t n_caus = static_cast<std::size_t>(bins);
-    host.reserve(n_caus + kSeedPhases + kBoundaryProbes + kAngularProbes + kInteriorProbes);
-    for (int i = 0; i < bins; ++i) {
-        const double phase = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(bins);
-        host.push_back({std::cos(phase), std::sin(phase)});
-    }
-    const double phase_step = 2.0 * kPiHost / static_cast<double>(kSeedPhases);
-    for (int k = 0; k < kSeedPhases; ++k) {
-        const double phi = phase_step * static_cast<double>(k);
-        host.push_back({std::cos(phi), std::sin(phi)});
-    }
-    for (int i = 0; i < kBoundaryProbes; ++i) {
-        const double phi = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(kBoundaryProbes);
-        host.push_back({std::cos(phi), std::sin(phi)});
-    }
-    for (int i = 0; i < kAngularProbes; ++i) {
-        const double th = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(kAngularProbes);
-        host.push_back({std::cos(th), std::sin(th)});
-    }
-    for (int i = 0; i < kInteriorProbes; ++i) {
-        const double phi = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(kInteriorProbes);
-        host.push_back({std::cos(phi), std::sin(phi)});
-    }
+    angle_tables(bins, host);
     lcbg_status st = map_hip(hipMalloc(&fs->table_mem, host.size() * sizeof(Pt2)));
     if (st != LCBG_OK) return st;
     st = map_hip(hipMemcpy(fs->table_mem, host.data(), host.size() * sizeof(Pt2), hipMemcpyHostToDevice));
     if (st != LCBG_OK) return st;
-    const Pt2* t = static_cast<const Pt2*>(fs->table_mem);
-    fs->tab.caus_trig = t;
-    fs->tab.seed_trig = t + n_caus;
-    fs->tab.boundary_trig = fs->tab.seed_trig + kSeedPhases;
-    fs->tab.angular_trig = fs->tab.boundary_trig + kBoundaryProbes;
-    fs->tab.interior_trig = fs->tab.angular_trig + kAngularProbes;
+    fs->tab = table_view(static_cast<const Pt2*>(fs->table_mem), bins);
 
     const int nc = b->n_curves > 0 ? b->n_curves : 1;
     const std::size_t per = static_cast<std::size_t>(nc);
@@ -485,6 +519,7 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     ep.hex_list = hex_view(fs);
     ep.near_curve = near_curve_view(fs);
     ep.near_chunks = near_chunks_view(fs);
+    ep.err = err_view(fs);
     // the pseudo-curve batch of the dynamic path runs different epochs under
     // the same count, so it keeps no keys
     if (!capture && !b->dynamic_images) {
@@ -498,6 +533,8 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
         b->graph = nullptr;
     }
     fs->last_n = n_epochs;
+    fs->last_mag = mag;
+    fs->last_method = method;
     // timing events of a profiled run, released on every return path
     struct RunEvents {
         hipEvent_t ev[5] = {};
@@ -576,12 +613,14 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     scr.polar.n_slots = waves(sc.polar_waves_per_cu, scr.polar.n_slots);
     fs->opt.heavy_budget = sc.heavy_budget > 0 ? sc.heavy_budget : kHeavyBudget;
     fs->opt.lead_items = sc.lead_items > 0 ? sc.lead_items : kLeadItems;
+    fs->opt.step_budget = b->work_budget > 0 ? b->work_budget : kStepBudget;
     if (st == LCBG_OK)
         st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, lists_view(fs), scr, stream, fs->ir);
     stage_done("stage 3 (inverse ray)");
     if (ga && st == LCBG_OK)
         st = launch_fs_grad(b->d_curves, b->n_curves, fs->opt, ep, lists_view(fs), pool.scr, ga->mag_aa,
                             ga->dmag, ga->flux, ga->sigma, ga->fs, ga->fb, ga->chi2, ga->grad, stream);
+    if (st == LCBG_OK) st = launch_first_nonfinite(mag, n_epochs, ep.counters, stream);
     if (st == LCBG_OK && !capture) st = map_hip(hipEventRecord(pool.last_use, stream));
     if (stage_ms && st == LCBG_OK) {
         (void)hipEventRecord(ev[4], stream);
@@ -593,17 +632,21 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     return st;
 }
 
-lcbg_status fs_error_flags(lcbg_batch* b, unsigned* flags, unsigned* counts)
+lcbg_status fs_error_flags(lcbg_batch* b, unsigned* flags, unsigned* counts, int64_t* first_failed = nullptr)
 {
     if (!b->fs || !b->fs->epoch_mem) return LCBG_INVALID_ARGUMENT;
     const FsEpochs ep = epoch_view(b->fs, 0, nullptr, nullptr, nullptr, nullptr);
-    unsigned h[8];
+    unsigned h[kCounterWords];
     lcbg_status st = map_hip(hipMemcpy(h, ep.counters, sizeof(h), hipMemcpyDeviceToHost));
     if (st != LCBG_OK) return st;
-    if (flags) *flags = h[4];
+    if (flags) *flags = h[kQFlags];
     if (counts) {
-        counts[0] = h[0];
-        counts[1] = h[2];
+        counts[0] = h[kQNearCount];
+        counts[1] = h[kQIrCount];
+    }
+    if (first_failed) {
+        const unsigned w = h[kFirstFailWord];
+        *first_failed = w == 0 ? -1 : b->fs->last_n - static_cast<int64_t>(w);
     }
     return LCBG_OK;
 }
@@ -630,6 +673,13 @@ extern "C" lcbg_status lcbg_batch_set_schedule(lcbg_batch* batch, const lcbg_sch
         schedule->polar_waves_per_cu < 0 || schedule->heavy_budget < 0 || schedule->lead_items < 0)
         return LCBG_INVALID_ARGUMENT;
     batch->schedule = *schedule;
+    return LCBG_OK;
+}
+
+extern "C" lcbg_status lcbg_batch_set_work_budget(lcbg_batch* batch, int64_t steps)
+{
+    if (!batch || steps < 0 || steps > kStepBudget) return LCBG_INVALID_ARGUMENT;
+    batch->work_budget = steps;
     return LCBG_OK;
 }
 
@@ -768,7 +818,7 @@ constexpr int kTrajParams = 9;  // t0, tE, u0, alpha, s, piEN, piEE, g1, g2
 // lcbg_batch_light_curves_dynamic_grad (dmag != nullptr).
 lcbg_status dynamic_run(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t orbital_motion_mode,
                         const int64_t* offsets, const double* times, int64_t n_epochs, double* mag,
-                        int32_t* method, double* mag_aa, double* dmag)
+                        int32_t* method, double* mag_aa, double* dmag, lcbg_result* results = nullptr)
 {
     if (!batch || n_epochs < 0) return LCBG_INVALID_ARGUMENT;
     if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER)
@@ -804,9 +854,10 @@ lcbg_status dynamic_run(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t orb
     lcbg_batch* ch = batch->dyn_child;
     const int32_t chunk = ch->n_curves;
     int64_t* d_off = nullptr;
-    double* d_buf = nullptr;  // times, mag, mag_aa, dmag (8 per epoch)
+    double* d_buf = nullptr;  // times, mag, mag_aa, dmag (8 per epoch), result records
     int32_t* d_meth = nullptr;
-    const std::size_t buf_doubles = static_cast<std::size_t>(chunk) * (grad ? 3 + kGradParams : 2);
+    const std::size_t rec_doubles = results ? static_cast<std::size_t>(chunk) * sizeof(lcbg_result) / sizeof(double) : 0;
+    const std::size_t buf_doubles = static_cast<std::size_t>(chunk) * (grad ? 3 + kGradParams : 2) + rec_doubles;
     st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_off), (chunk + 1) * sizeof(int64_t)));
     if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_buf), buf_doubles * sizeof(double)));
     if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_meth), chunk * sizeof(int32_t)));
@@ -814,6 +865,9 @@ lcbg_status dynamic_run(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t orb
     double* d_mag = d_buf + chunk;
     double* d_aa = grad ? d_buf + 2 * chunk : nullptr;
     double* d_dm = grad ? d_buf + 3 * chunk : nullptr;
+    lcbg_result* d_res =
+        results ? reinterpret_cast<lcbg_result*>(d_buf + static_cast<std::size_t>(chunk) * (grad ? 3 + kGradParams : 2))
+                : nullptr;
     std::vector<int64_t> h_off(static_cast<std::size_t>(chunk) + 1);
     std::vector<double> h_t(static_cast<std::size_t>(chunk));
     std::vector<double> h_dm(grad ? static_cast<std::size_t>(chunk) * kGradParams : 0);
@@ -862,7 +916,12 @@ lcbg_status dynamic_run(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t orb
                 st = fs_run(ch, d_off, d_t, m, d_mag, d_meth, nullptr, nullptr);
             }
         }
+        // the pseudo-curve's source is the magnifier's source position, its
+        // separation the epoch's s(t): the records read as LensModel fills them
+        if (st == LCBG_OK && results) st = lcbg_batch_epoch_results_device(ch, m, d_res, nullptr);
         if (st == LCBG_OK) st = map_hip(hipStreamSynchronize(nullptr));
+        if (st == LCBG_OK && results)
+            st = map_hip(hipMemcpy(results + base, d_res, m * sizeof(lcbg_result), hipMemcpyDeviceToHost));
         if (st == LCBG_OK)
             st = map_hip(hipMemcpy(mag + base, d_mag, m * sizeof(double), hipMemcpyDeviceToHost));
         if (st == LCBG_OK)
@@ -903,6 +962,9 @@ lcbg_status dynamic_run(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t orb
     if (d_off) (void)hipFree(d_off);
     if (d_buf) (void)hipFree(d_buf);
     if (d_meth) (void)hipFree(d_meth);
+    // a non-finite magnification is a numerical error (lens_model.cpp:228-231)
+    for (int64_t i = 0; st == LCBG_OK && i < n_epochs; ++i)
+        if (!std::isfinite(mag[i])) st = LCBG_NUMERICAL_ERROR;
     return st;
 }
 
@@ -915,6 +977,18 @@ extern "C" lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const 
 {
     return dynamic_run(batch, dyn, orbital_motion_mode, offsets, times, n_epochs, mag, method, nullptr,
                        nullptr);
+}
+
+extern "C" lcbg_status lcbg_batch_light_curves_dynamic_results(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                                               int32_t orbital_motion_mode,
+                                                               const int64_t* offsets, const double* times,
+                                                               int64_t n_epochs, lcbg_result* results)
+{
+    if (n_epochs > 0 && !results) return LCBG_INVALID_ARGUMENT;
+    std::vector<double> mag(static_cast<std::size_t>(n_epochs > 0 ? n_epochs : 0));
+    std::vector<int32_t> meth(mag.size());
+    return dynamic_run(batch, dyn, orbital_motion_mode, offsets, times, n_epochs, mag.data(), meth.data(),
+                       nullptr, nullptr, results);
 }
 
 extern "C" lcbg_status lcbg_batch_light_curves_dynamic_grad(lcbg_batch* batch, const lcbg_dynamics* dyn,
@@ -1062,48 +1136,159 @@ extern "C" int64_t lcbg_wall_clock_rate_khz(void)
     return khz;
 }
 
+namespace {
+
+// The single-curve host path keeps one batch and its device buffers per host
+// thread and device, so that a loop over curves (a fit calling
+// LightCurve(...)(times, params) again and again) reuses the workspace: a
+// call with the options of the previous call only uploads the curve and the
+// times.  A call with other options, or more epochs than any before, replaces
+// them.  The objects live until the process ends.
+struct SingleCurve {
+    lcbg_batch* batch = nullptr;
+    lcbg_options options{};
+    int device = -1;
+    int64_t cap = 0;
+    char* mem = nullptr;  // times, mag, method, offsets, results
+};
+
+thread_local SingleCurve* g_single = nullptr;
+
+bool same_options(const lcbg_options& a, const lcbg_options& b)
+{
+    return std::memcmp(&a, &b, sizeof(lcbg_options)) == 0;
+}
+
+lcbg_status single_curve_run(const lcbg_curve* curve, const lcbg_options* options, const double* times,
+                             int64_t n, double* mag, int32_t* method, lcbg_result* results,
+                             int64_t* first_failed)
+{
+    if (first_failed) *first_failed = -1;
+    if (!curve || n < 0 || (n > 0 && !times) || (n > 0 && !mag && !results)) return LCBG_INVALID_ARGUMENT;
+    if (n == 0) return LCBG_OK;
+    lcbg_options opt = options ? *options : lcbg_default_options();
+    int dev = 0;
+    lcbg_status st = map_hip(hipGetDevice(&dev));
+    if (st != LCBG_OK) return st;
+    if (!g_single) g_single = new (std::nothrow) SingleCurve();
+    if (!g_single) return LCBG_DEVICE_ERROR;
+    SingleCurve& sc = *g_single;
+    if (sc.batch && (sc.device != dev || !same_options(sc.options, opt))) {
+        lcbg_batch_destroy(sc.batch);
+        sc.batch = nullptr;
+        if (sc.device != dev && sc.mem) {
+            int cur = dev;
+            (void)hipSetDevice(sc.device);
+            (void)hipFree(sc.mem);
+            (void)hipSetDevice(cur);
+            sc.mem = nullptr;
+            sc.cap = 0;
+        }
+    }
+    if (!sc.batch) {
+        st = lcbg_batch_create(curve, 1, &opt, &sc.batch);
+        if (st != LCBG_OK) {
+            sc.batch = nullptr;
+            return st;
+        }
+        sc.options = opt;
+        sc.device = dev;
+    } else {
+        st = lcbg_batch_update_curves(sc.batch, curve, nullptr);
+        if (st != LCBG_OK) return st;
+    }
+    const std::size_t per = sizeof(double) * 2 + sizeof(int32_t) + sizeof(lcbg_result) + 3 * 256;
+    if (n > sc.cap) {
+        if (sc.mem) (void)hipFree(sc.mem);
+        sc.mem = nullptr;
+        sc.cap = 0;
+        int64_t cap = 256;
+        while (cap < n) cap *= 2;
+        st = map_hip(hipMalloc(reinterpret_cast<void**>(&sc.mem), static_cast<std::size_t>(cap) * per + 1024));
+        if (st != LCBG_OK) {
+            sc.mem = nullptr;
+            return st;
+        }
+        sc.cap = cap;
+    }
+    char* p = sc.mem;
+    double* d_times = carve<double>(p, sc.cap);
+    double* d_mag = carve<double>(p, sc.cap);
+    int32_t* d_method = carve<int32_t>(p, sc.cap);
+    lcbg_result* d_res = carve<lcbg_result>(p, sc.cap);
+    int64_t* d_off = carve<int64_t>(p, 2);
+    const int64_t off[2] = {0, n};
+    const std::size_t bytes = static_cast<std::size_t>(n) * sizeof(double);
+    st = map_hip(hipMemcpy(d_times, times, bytes, hipMemcpyHostToDevice));
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(d_off, off, sizeof(off), hipMemcpyHostToDevice));
+    if (st == LCBG_OK)
+        st = lcbg_batch_finite_source_device(sc.batch, d_off, d_times, n, d_mag, d_method, nullptr);
+    if (st == LCBG_OK && results) st = lcbg_batch_epoch_results_device(sc.batch, n, d_res, nullptr);
+    if (st == LCBG_OK && mag) st = map_hip(hipMemcpy(mag, d_mag, bytes, hipMemcpyDeviceToHost));
+    if (st == LCBG_OK && method)
+        st = map_hip(hipMemcpy(method, d_method, static_cast<std::size_t>(n) * sizeof(int32_t),
+                               hipMemcpyDeviceToHost));
+    if (st == LCBG_OK && results)
+        st = map_hip(hipMemcpy(results, d_res, static_cast<std::size_t>(n) * sizeof(lcbg_result),
+                               hipMemcpyDeviceToHost));
+    unsigned flags = 0;
+    int64_t first = -1;
+    if (st == LCBG_OK) st = fs_error_flags(sc.batch, &flags, nullptr, &first);
+    if (st == LCBG_OK) {
+        if (first_failed) *first_failed = first;
+        st = status_of_run(flags, first);
+    }
+    return st;
+}
+
+}  // namespace
+
 extern "C" lcbg_status lcbg_light_curve(const lcbg_curve* curve, const lcbg_options* options,
                                         const double* times, int64_t n, double* mag,
                                         int32_t* method)
 {
-    if (!curve || n < 0 || (n > 0 && (!times || !mag))) return LCBG_INVALID_ARGUMENT;
-    if (n == 0) return LCBG_OK;
-    lcbg_batch* b = nullptr;
-    lcbg_status st = lcbg_batch_create(curve, 1, options, &b);
-    if (st != LCBG_OK) return st;
-    double* d_times = nullptr;
-    double* d_mag = nullptr;
-    int32_t* d_method = nullptr;
-    int64_t* d_off = nullptr;
-    const int64_t off[2] = {0, n};
-    const std::size_t bytes = static_cast<std::size_t>(n) * sizeof(double);
-    st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_times), bytes));
-    if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_mag), bytes));
-    if (st == LCBG_OK)
-        st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_method), static_cast<std::size_t>(n) * sizeof(int32_t)));
-    if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_off), sizeof(off)));
-    if (st == LCBG_OK) st = map_hip(hipMemcpy(d_times, times, bytes, hipMemcpyHostToDevice));
-    if (st == LCBG_OK) st = map_hip(hipMemcpy(d_off, off, sizeof(off), hipMemcpyHostToDevice));
-    if (st == LCBG_OK) st = lcbg_batch_finite_source_device(b, d_off, d_times, n, d_mag, d_method, nullptr);
-    if (st == LCBG_OK) st = map_hip(hipMemcpy(mag, d_mag, bytes, hipMemcpyDeviceToHost));
-    if (st == LCBG_OK && method)
-        st = map_hip(hipMemcpy(method, d_method, static_cast<std::size_t>(n) * sizeof(int32_t),
-                               hipMemcpyDeviceToHost));
-    unsigned flags = 0;
-    if (st == LCBG_OK) st = fs_error_flags(b, &flags, nullptr);
-    if (st == LCBG_OK && flags) st = LCBG_UNSUPPORTED;
-    if (st == LCBG_OK) {
-        for (int64_t i = 0; i < n; ++i) {
-            if (!std::isfinite(mag[i])) {
-                st = LCBG_NUMERICAL_ERROR;  // python/lcbinint_pybind.cpp:1153-1155
-                break;
-            }
+    if (n > 0 && !mag) return LCBG_INVALID_ARGUMENT;
+    return single_curve_run(curve, options, times, n, mag, method, nullptr, nullptr);
+}
+
+extern "C" lcbg_status lcbg_light_curve_results(const lcbg_curve* curve, const lcbg_options* options,
+                                                const double* times, int64_t n, lcbg_result* results,
+                                                int64_t* first_failed_epoch)
+{
+    if (n > 0 && !results) return LCBG_INVALID_ARGUMENT;
+    return single_curve_run(curve, options, times, n, nullptr, nullptr, results, first_failed_epoch);
+}
+
+extern "C" lcbg_status lcbg_batch_last_run_status(lcbg_batch* batch, uint32_t* error_flags,
+                                                  int64_t* first_failed_epoch)
+{
+    if (!batch || !batch->fs || !batch->fs->epoch_mem) return LCBG_INVALID_ARGUMENT;
+    {
+        ScratchPool& pool = g_pool[batch->device % kMaxDevices];
+        std::lock_guard<std::mutex> lock(pool.mu);
+        if (pool.last_use) {
+            const lcbg_status st = map_hip(hipEventSynchronize(pool.last_use));
+            if (st != LCBG_OK) return st;
         }
     }
-    if (d_times) (void)hipFree(d_times);
-    if (d_mag) (void)hipFree(d_mag);
-    if (d_method) (void)hipFree(d_method);
-    if (d_off) (void)hipFree(d_off);
-    lcbg_batch_destroy(b);
-    return st;
+    unsigned flags = 0;
+    int64_t first = -1;
+    const lcbg_status st = fs_error_flags(batch, &flags, nullptr, &first);
+    if (st != LCBG_OK) return st;
+    if (error_flags) *error_flags = flags;
+    if (first_failed_epoch) *first_failed_epoch = first;
+    return status_of_run(flags, first);
+}
+
+extern "C" lcbg_status lcbg_batch_epoch_results_device(lcbg_batch* batch, int64_t n_epochs, lcbg_result* results,
+                                                       void* stream)
+{
+    if (!batch || !results || n_epochs < 0) return LCBG_INVALID_ARGUMENT;
+    if (n_epochs == 0) return LCBG_OK;
+    FsState* fs = batch->fs;
+    if (!fs || !fs->epoch_mem || fs->last_n != n_epochs) return LCBG_INVALID_ARGUMENT;
+    FsEpochs ep = epoch_view(fs, n_epochs, nullptr, nullptr, fs->last_mag, fs->last_method);
+    ep.err = err_view(fs);
+    return launch_epoch_results(batch->d_curves, fs->opt, ep, batch->options.finite_source_tol,
+                                batch->options.finite_source_reltol, results, static_cast<hipStream_t>(stream));
 }

@@ -34,6 +34,7 @@ struct Ctx {
     int n_seeds;
     bool overflow;
     long long steps;  // wave steps spent on this epoch
+    long long budget; // wave steps the epoch may take (FsOptions::step_budget)
     int last_phase;   // seed phases run: 0 images, 1 first crossing, 2 branches,
                       // 3 boundary, 4 interior, 5 caustic segments (diagnostics)
     // algorithmic work of the epoch (uniform): quintic and quartic solves, the
@@ -53,13 +54,18 @@ struct Ctx {
     unsigned* heavy_count;
     bool seeds_changed;   // a probe added or replaced a seed
     bool hints;           // center images are seed hints (LensModel path)
+    // LegacyAreaDiagnostics of the Cartesian scan (finite_source_magnifier.cpp:1006-1015):
+    // processed images, fold-guarded images, counted rows, gap repairs,
+    // overlap hits, largest row-edge jump in cells
+    int d_processed, d_fold, d_boundary_rows, d_gaps, d_overlaps;
+    double d_max_jump;
 };
 
 // Counts one wave step; past the budget the epoch is abandoned with an error.
 __device__ __forceinline__ bool step_ok(Ctx& C)
 {
     C.steps = uni(C.steps + 1);
-    if (C.steps > kStepBudget) C.overflow = true;
+    if (C.steps > C.budget) C.overflow = true;
     return !C.overflow;
 }
 
@@ -1024,7 +1030,7 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
 {
     long long guard = 0;
     int steps = 0, rows = 0;
-    const long long left = kStepBudget - C.steps;
+    const long long left = C.budget - C.steps;
     const int room = left < 0 ? 0 : static_cast<int>(left);
     const double r = cart_scan_rows(C, K, R, seed, dy, yi, prev_xmin, watch, expect, guard, steps, rows, room);
     C.steps = uni(C.steps + steps);
@@ -1050,6 +1056,8 @@ __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt s
 struct ImageRun {
     double area;
     int nyi;  // rows [0, nyi) take part in the overlap test
+    int gaps;         // gap-repair scans started (diagnostics->gap_repairs, :2578-2644)
+    double max_jump;  // largest |row edge step| / incr over counted rows (:2571-2573)
 };
 
 __device__ __forceinline__ void cart_params(Cart& K, const Ctx& C, int source_bins)
@@ -1095,8 +1103,9 @@ constexpr long long kOfferSteps = 2048;
 constexpr long long kTeamWaitTicks = 6000000000ll;  // 60 s of the 100 MHz clock
 
 struct TeamOut {
-    double area;
+    double area, max_jump;
     int nyi, bad;  // bad: 1 work budget or slot exceeded, 2 slot capacity
+    int gaps;
     long long steps, n_cart, n_cart_guarded, n_rows;
 };
 
@@ -1232,8 +1241,10 @@ __device__ __forceinline__ bool hook_poll(Ctx& C, ImageHook& hk)
 // all through one call site of cart_scan.  A failure leaves C.overflow set.
 __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed, ImageHook& hk)
 {
-    ImageRun out{__builtin_nan(""), 0};
+    ImageRun out{__builtin_nan(""), 0, 0, 0.0};
     const double incr = K.incr;
+    int gaps = 0;
+    double max_jump = 0.0;
     rows_reset(C, R);
     if (!rows_ensure(C, R, 1)) return out;
     int yi = 0;
@@ -1278,11 +1289,17 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
                     while (row < nyi - 1) {
                         const int rr = row + C.lane;
                         bool act = rr == nyi - 1;
+                        double jump = 0.0;
                         if (rr < nyi - 1 && rget(R, rr + 1, AX) > 0.0) {
                             const double ex = rget(R, rr + 1, XMAX) - rget(R, rr, XMAX);
                             const double en = rget(R, rr + 1, XMIN) - rget(R, rr, XMIN);
                             act = ex > g || en > g || en < -g || ex < -g;
+                            jump = dmax(fabs(ex), fabs(en)) / incr;
                         }
+                        // every row below nyi - 1 is examined by the reference's
+                        // loop and its records are final here: the maximum may
+                        // take in rows past the first active one
+                        max_jump = dmax(max_jump, uni(wave_max(jump)));
                         const uint64_t mm = ballot(act);
                         if (mm) {
                             row += first_set(mm);
@@ -1295,6 +1312,7 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
                     dxmax = rget(R, row + 1, XMAX) - rget(R, row, XMAX);
                     dxmin = rget(R, row + 1, XMIN) - rget(R, row, XMIN);
                     cs = rget(R, row + 1, AX) > 0.0 ? 0 : 4;
+                    if (cs == 0) max_jump = dmax(max_jump, uni(dmax(fabs(dxmax), fabs(dxmin)) / incr));
                     continue;
                 }
                 const int k = cs++;
@@ -1325,6 +1343,7 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
                     have = true;
                 }
                 if (have) {
+                    ++gaps;
                     start.x = uni(start.x);
                     start.y = uni(start.y);
                     sdy = uni(sdy);
@@ -1357,6 +1376,8 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
     }
     out.area = areai;
     out.nyi = nyi;
+    out.gaps = gaps;
+    out.max_jump = max_jump;
     return out;
 }
 
@@ -1374,6 +1395,19 @@ __device__ __forceinline__ void account_image(Ctx& C, const Cart& K0, int i, con
     const int n_img = C.n_seeds;
     const int nyi = img.nyi;
     area += img.area;
+    // diagnostics of this image (:2536-2540, 2673-2680)
+    {
+        int counted = 0;
+        for (int base = 0; base < nyi; base += kWave) {
+            const int row = base + C.lane;
+            counted += popcount64(ballot(row < nyi && rget(R, row, AX) > 0.0));
+        }
+        C.d_processed = uni(C.d_processed + 1);
+        C.d_fold = uni(C.d_fold + (jac_sign != 0 ? 1 : 0));
+        C.d_boundary_rows = uni(C.d_boundary_rows + counted);
+        C.d_gaps = uni(C.d_gaps + img.gaps);
+        C.d_max_jump = uni(dmax(C.d_max_jump, img.max_jump));
+    }
     wave_sync();
     wave_store(&areaimage[i], img.area);
     wave_sync();
@@ -1398,6 +1432,7 @@ __device__ __forceinline__ void account_image(Ctx& C, const Cart& K0, int i, con
             hit = ballot(h) != 0;
         }
         if (hit) {
+            C.d_overlaps = uni(C.d_overlaps + 1);
             const int fl = flags[other];
             wave_sync();
             if (other < i) {
@@ -1421,6 +1456,60 @@ __device__ __forceinline__ double cart_scale(const Ctx& C, int source_bins)
 {
     const double nbin = static_cast<double>(source_bins > 1 ? source_bins : 1);
     return source_flux(C.rho, C.ld_c, C.ld_d) / (C.rho * C.rho) * nbin * nbin;
+}
+
+// high_magnification_floor_coefficient (finite_source_magnifier.cpp:1017-1063).
+__device__ double cart_floor_coefficient(const Ctx& C, double mag)
+{
+    const double r = C.rho;
+    const double am = fabs(mag);
+    const int seeds = C.n_seeds, gaps = C.d_gaps, ov = C.d_overlaps;
+    const double jump = C.d_max_jump;
+    if (r >= 0.1 && (gaps > 0 || jump > 50.0)) return 0.06;
+    if (r < 1.0e-2 && seeds <= 5 && am <= 10.0 && gaps > 0 && jump > 20.0) return 0.035;
+    if (r < 1.0e-2 && seeds <= 5 && am <= 10.0 && jump > 1000.0) return 0.04;
+    if (r < 1.0e-2 && seeds >= 16 && ov >= 8 && gaps >= 100 && jump > 1000.0 && am <= 50.0) return 0.035;
+    if (r < 1.0e-2 && am <= 300.0) return 0.02;
+    if (r < 1.0e-2 && am <= 1000.0) return 0.04;
+    if (r < 1.0e-2 && seeds <= 5 && gaps > 0 && jump > 20.0) return 0.04;
+    if (am <= 80.0 || gaps <= 1000) return 0.0;
+    return r >= 1.0e-2 ? 0.04 : 0.05;
+}
+
+// The error estimate of a Cartesian epoch: cartesian_area_error_indicator
+// (:1065-1106) raised to the high-magnification floor
+// (inverse_ray_cartesian_binary_mag, :2745-2756).
+__device__ double cart_error_estimate(const Ctx& C, int source_bins, double mag)
+{
+    const int bins = source_bins > 1 ? source_bins : 1;
+    const double r = C.rho;
+    const double flux = source_flux(r, C.ld_c, C.ld_d);
+    double est = 0.0;
+    if (isfinite(flux) && flux > 0.0 && r > 0.0) {
+        const int seeds = C.n_seeds;
+        const double gap_w = r >= 2.0e-2 ? 0.005 : ((r < 1.0e-2 && seeds >= 16) ? 0.015 : 0.03);
+        const double ov_w = r >= 2.0e-2 ? 0.005 : 0.02;
+        const double bnd_w = r >= 2.0e-2 ? 0.012 : 0.03;
+        double cells = bnd_w * static_cast<double>(C.d_boundary_rows) + gap_w * static_cast<double>(C.d_gaps) +
+                       ov_w * static_cast<double>(C.d_overlaps) +
+                       0.02 * static_cast<double>(C.d_fold > 0 ? C.d_fold : 0) +
+                       0.002 * static_cast<double>(seeds - 5 > 0 ? seeds - 5 : 0);
+        if (r >= 1.0e-3 && seeds >= 16 && C.d_overlaps >= 8) {
+            const double island_w = r < 1.0e-2 ? 0.001 : 0.0005;
+            cells += island_w * static_cast<double>(seeds) * static_cast<double>(C.d_overlaps);
+        }
+        if (r >= 1.0e-3 && C.d_max_jump > 10.0) {
+            const double jump_w = r >= 2.0e-2 ? 0.2 : 4.0;
+            cells += jump_w * log10(C.d_max_jump / 10.0);
+        }
+        const double nb = static_cast<double>(bins);
+        const double cell_area = (r / nb) * (r / nb);
+        const double e = 1.25 * cells * cell_area / flux;
+        est = isfinite(e) ? e : 0.0;
+    }
+    const double fc = cart_floor_coefficient(C, mag);
+    if (fc > 0.0) est = dmax(est, fabs(mag) * (fc / static_cast<double>(bins)));
+    return est;
 }
 
 // The whole Cartesian area of one epoch by one wave, images in seed order
@@ -1713,6 +1802,7 @@ __device__ __forceinline__ void init_ctx(Ctx& C, const CurveRec& cv, Pt src, Pt*
     C.n_seeds = 0;
     C.overflow = false;
     C.steps = 0;
+    C.budget = kStepBudget;
     C.last_phase = 5;
     C.n_quintic = 0;
     C.n_quartic = 0;
@@ -1731,6 +1821,12 @@ __device__ __forceinline__ void init_ctx(Ctx& C, const CurveRec& cv, Pt src, Pt*
     C.heavy_count = nullptr;
     C.seeds_changed = false;
     C.hints = false;
+    C.d_processed = 0;
+    C.d_fold = 0;
+    C.d_boundary_rows = 0;
+    C.d_gaps = 0;
+    C.d_overlaps = 0;
+    C.d_max_jump = 0.0;
 }
 
 __device__ __forceinline__ unsigned long long* work_totals(const FsEpochs& ep)
@@ -1869,6 +1965,7 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
         const CurveRec cv = curves[c];
         Ctx C;
         init_ctx(C, cv, ep.src[e], slot_seeds);
+        C.budget = opt.step_budget;
         C.hints = opt.ps_images != 0;
         if (pass != kSeedResume) {
             C.defer_budget = opt.heavy_budget;
@@ -1921,7 +2018,7 @@ __global__ void __launch_bounds__(kWave) k_ir_seeds(const CurveRec* __restrict__
         } else {
             wave_store(&L.seed_n[item], -1);
             wave_store(&ep.mag[e], __builtin_nan(""));
-            if (C.overflow) atomicOr(&ep.counters[kQFlags], kErrSeedCap);
+            if (C.overflow) atomicOr(&ep.counters[kQFlags], C.steps > C.budget ? kErrBudget : kErrSeedCap);
         }
         flush_work(C, ep);
         const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
@@ -2144,6 +2241,7 @@ __global__ void __launch_bounds__(kTeamWaves * kWave, 3) k_ir_cart(const CurveRe
                 t_begin = static_cast<long long>(wall_clock64());
                 const CurveRec cv = curves[ep.curve_of[e]];
                 init_ctx(C, cv, ep.src[e], L.seed_pool + uni(L.seed_off[item]));
+                C.budget = opt.step_budget;
                 C.n_seeds = uni(L.seed_n[item]);
                 C.row_cap = scr.row_cap;
                 nbin = curve_nbin(cv, opt);
@@ -2182,6 +2280,16 @@ __global__ void __launch_bounds__(kTeamWaves * kWave, 3) k_ir_cart(const CurveRe
                     wave_store(&L.cart_retry[pos], item);
                 } else {
                     wave_store(&ep.mag[e], mag);
+                    wave_store(&ep.err[e], isfinite(mag) ? cart_error_estimate(C, nbin, mag) : __builtin_nan(""));
+                    if (ep.cost) {
+                        const int64_t n = ep.n;
+                        wave_store(&ep.cost[5 * n + e], static_cast<int64_t>(C.d_processed));
+                        wave_store(&ep.cost[6 * n + e], static_cast<int64_t>(C.d_fold));
+                        wave_store(&ep.cost[7 * n + e], static_cast<int64_t>(C.d_boundary_rows));
+                        wave_store(&ep.cost[8 * n + e], static_cast<int64_t>(C.d_gaps));
+                        wave_store(&ep.cost[9 * n + e], static_cast<int64_t>(C.d_overlaps));
+                        wave_store(&ep.cost[10 * n + e], static_cast<int64_t>(__double_as_longlong(C.d_max_jump)));
+                    }
                     if (C.overflow) atomicOr(&ep.counters[kQFlags], C.cap_overflow ? kErrRowCap : kErrBudget);
                 }
                 flush_work(C, ep);
@@ -2222,7 +2330,7 @@ __global__ void __launch_bounds__(kTeamWaves * kWave, 3) k_ir_cart(const CurveRe
             }
             if (mode == 1 && C.steps >= kOfferSteps) team_offer(team, w, item, flags, cur, C.n_seeds, next_offer);
         }
-        ImageRun img{__builtin_nan(""), 0};
+        ImageRun img{__builtin_nan(""), 0, 0, 0.0};
         Rows R = mine;
         if (mode != 3) {
             int j = cur;
@@ -2236,6 +2344,7 @@ __global__ void __launch_bounds__(kTeamWaves * kWave, 3) k_ir_cart(const CurveRe
                 const int64_t pe = L.ir_sorted[pitem];
                 const CurveRec cv = curves[ep.curve_of[pe]];
                 init_ctx(C, cv, ep.src[pe], L.seed_pool + uni(L.seed_off[pitem]));
+                C.budget = opt.step_budget;
                 C.n_seeds = uni(L.seed_n[pitem]);
                 C.row_cap = scr.row_cap;
                 cart_params(K, C, curve_nbin(cv, opt));
@@ -2278,6 +2387,8 @@ __global__ void __launch_bounds__(kTeamWaves * kWave, 3) k_ir_cart(const CurveRe
                 TeamOut& o = team.out[w];
                 wave_store(&o.area, img.area);
                 wave_store(&o.nyi, img.nyi);
+                wave_store(&o.gaps, img.gaps);
+                wave_store(&o.max_jump, img.max_jump);
                 wave_store(&o.bad, (C.overflow ? 1 : 0) | (C.cap_overflow ? 2 : 0));
                 wave_store(&o.steps, C.steps);
                 wave_store(&o.n_cart, C.n_cart);
@@ -2303,12 +2414,14 @@ __global__ void __launch_bounds__(kTeamWaves * kWave, 3) k_ir_cart(const CurveRe
             const TeamOut& o = team.out[h];
             img.area = uni(o.area);
             img.nyi = uni(o.nyi);
+            img.gaps = uni(o.gaps);
+            img.max_jump = uni(o.max_jump);
             const int bad = uni(o.bad);
             C.steps = uni(C.steps + uni(o.steps));
             C.n_cart = uni(C.n_cart + uni(o.n_cart));
             C.n_cart_guarded = uni(C.n_cart_guarded + uni(o.n_cart_guarded));
             C.n_rows = uni(C.n_rows + uni(o.n_rows));
-            if (C.steps > kStepBudget || (bad & 1)) C.overflow = true;
+            if (C.steps > C.budget || (bad & 1)) C.overflow = true;
             if (bad & 2) C.cap_overflow = true;
             R = slot_rows(scr, static_cast<int>(blockIdx.x) * kTeamWaves + h);
             if (C.overflow) {
@@ -2342,6 +2455,7 @@ __global__ void __launch_bounds__(kWave) k_ir_polar(const CurveRec* __restrict__
         const CurveRec cv = curves[ep.curve_of[e]];
         Ctx C;
         init_ctx(C, cv, ep.src[e], L.seed_pool + uni(L.seed_off[item]));
+        C.budget = opt.step_budget;
         C.n_seeds = uni(L.seed_n[item]);
         C.run_cap = scr.run_cap;
         const double mag = polar_epoch(C, cv, opt, head, runs);
@@ -2350,6 +2464,8 @@ __global__ void __launch_bounds__(kWave) k_ir_polar(const CurveRec* __restrict__
             wave_store(&L.polar_retry[pos], item);
         } else {
             wave_store(&ep.mag[e], mag);
+            // the polar method reports no error estimate (:3169-3175)
+            wave_store(&ep.err[e], isfinite(mag) ? 0.0 : __builtin_nan(""));
             if (C.overflow) atomicOr(&ep.counters[kQFlags], C.cap_overflow ? kErrPolarCap : kErrBudget);
         }
         flush_work(C, ep);
@@ -2382,6 +2498,7 @@ __global__ void __launch_bounds__(kWave) k_ir_epoch_debug(const CurveRec* __rest
     const CurveRec cv = curves[c];
     Ctx C;
     init_ctx(C, cv, ep.src[e], slot_seeds);
+    C.budget = opt.step_budget;
     C.hints = opt.ps_images != 0;
     if (last_phase >= 0) C.last_phase = last_phase;
     if (n_in < 0) {
@@ -2406,10 +2523,17 @@ __global__ void __launch_bounds__(kWave) k_ir_epoch_debug(const CurveRec* __rest
 
 lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
                          const FsOptions& opt, const FsEpochs& ep, const IrLists& lists,
-                         const IrScratch& scr, hipStream_t st, const IrStreams& ss)
+                         const IrScratch& scr, hipStream_t st, const IrStreams& ss, bool farm)
 {
     auto mark = [&](int k, hipStream_t s) {
         if (ss.kev) (void)hipEventRecord(ss.kev[k], s);
+    };
+    // `to` continues after the work enqueued so far on `from`
+    auto fork = [&](hipEvent_t ev, hipStream_t from, hipStream_t to) -> lcbg_status {
+        if (from == to) return LCBG_OK;
+        lcbg_status r = map_hip(hipEventRecord(ev, from));
+        if (r == LCBG_OK) r = map_hip(hipStreamWaitEvent(to, ev, 0));
+        return r;
     };
     hipLaunchKernelGGL(k_ir_bucket_count, dim3(256), dim3(256), 0, st, d_curves, opt, ep, lists);
     hipLaunchKernelGGL(k_ir_bucket_scan, dim3(1), dim3(64), 0, st, ep);
@@ -2421,8 +2545,7 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
                        d_curves, cache, tab, opt, ep, lists, scr.seed, int{kSeedLead});
     hipLaunchKernelGGL(k_ir_sort_queues, dim3(2), dim3(1024), 0, st, ep, lists, 2);
     lcbg_status status = check_launch();
-    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.fork_lead, st));
-    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.big, ss.fork_lead, 0));
+    if (status == LCBG_OK) status = fork(ss.fork_lead, st, ss.big);
     if (status != LCBG_OK) return status;
     mark(2 * kIrCartBig, ss.big);
     hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots / kTeamWaves)),
@@ -2430,31 +2553,30 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
                        ss.big, d_curves, opt, ep, lists, scr.cart_large, int{kPassBig});
     mark(2 * kIrCartBig + 1, ss.big);
     status = check_launch();
-    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_big, ss.big));
     // the leading polar epochs on large slots (polar stream)
-    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.polar, ss.fork_lead, 0));
+    if (status == LCBG_OK && ss.polar != st) status = map_hip(hipStreamWaitEvent(ss.polar, ss.fork_lead, 0));
     if (status != LCBG_OK) return status;
     mark(2 * kIrPolarBig, ss.polar);
     hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar_large.n_slots)), dim3(kWave), 0,
                        ss.polar, d_curves, opt, ep, lists, scr.polar_large, int{kPassBig});
     mark(2 * kIrPolarBig + 1, ss.polar);
     status = check_launch();
-    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_polar, ss.polar));
     if (status != LCBG_OK) return status;
     // all other searches; the deferred first-crossing scans: list, farm, resume
     hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
                        d_curves, cache, tab, opt, ep, lists, scr.seed, int{kSeedMain});
-    hipLaunchKernelGGL(k_ir_heavy_list, dim3(kHeavyCap), dim3(kWave), 0, st, d_curves, cache, ep, lists,
-                       scr.seed.heavy);
-    hipLaunchKernelGGL(k_ir_farm, dim3(static_cast<unsigned>(scr.farm_slots)), dim3(kWave), 0, st,
-                       d_curves, cache, tab, ep, lists, scr.seed.heavy);
-    hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
-                       d_curves, cache, tab, opt, ep, lists, scr.seed, int{kSeedResume});
+    if (farm) {
+        hipLaunchKernelGGL(k_ir_heavy_list, dim3(kHeavyCap), dim3(kWave), 0, st, d_curves, cache, ep, lists,
+                           scr.seed.heavy);
+        hipLaunchKernelGGL(k_ir_farm, dim3(static_cast<unsigned>(scr.farm_slots)), dim3(kWave), 0, st,
+                           d_curves, cache, tab, ep, lists, scr.seed.heavy);
+        hipLaunchKernelGGL(k_ir_seeds, dim3(static_cast<unsigned>(scr.seed.n_slots)), dim3(kWave), 0, st,
+                           d_curves, cache, tab, opt, ep, lists, scr.seed, int{kSeedResume});
+    }
     hipLaunchKernelGGL(k_ir_sort_queues, dim3(2), dim3(1024), 0, st, ep, lists, 0);
     mark(2 * kIrSeeds + 1, st);
     status = check_launch();
-    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.fork, st));
-    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(ss.polar_small, ss.fork, 0));
+    if (status == LCBG_OK) status = fork(ss.fork, st, ss.polar_small);
     if (status != LCBG_OK) return status;
     // the polar small-slot pass beside the Cartesian one and the polar
     // large-slot pass (its own stream: it need not wait for the largest epoch)
@@ -2463,7 +2585,6 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
                        ss.polar_small, d_curves, opt, ep, lists, scr.polar, int{kPassSmall});
     mark(2 * kIrPolarSmall + 1, ss.polar_small);
     status = check_launch();
-    if (status == LCBG_OK) status = map_hip(hipEventRecord(ss.join_polar_small, ss.polar_small));
     if (status != LCBG_OK) return status;
     mark(2 * kIrCartSmall, st);
     hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart.n_slots / kTeamWaves)),
@@ -2471,9 +2592,9 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
                        d_curves, opt, ep, lists, scr.cart, int{kPassSmall});
     mark(2 * kIrCartSmall + 1, st);
     status = check_launch();
-    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(st, ss.join_big, 0));
-    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(st, ss.join_polar, 0));
-    if (status == LCBG_OK) status = map_hip(hipStreamWaitEvent(st, ss.join_polar_small, 0));
+    if (status == LCBG_OK) status = fork(ss.join_big, ss.big, st);
+    if (status == LCBG_OK) status = fork(ss.join_polar, ss.polar, st);
+    if (status == LCBG_OK) status = fork(ss.join_polar_small, ss.polar_small, st);
     if (status != LCBG_OK) return status;
     // the epochs that outgrew a small slot, on the (now free) large slots
     mark(2 * kIrCartRetry, st);

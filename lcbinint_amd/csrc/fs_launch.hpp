@@ -6,7 +6,20 @@
 #include "fs_types.hpp"
 #include "runtime.hpp"
 
+#include <vector>
+
 namespace lcbg {
+
+// The angle tables of a caustic_bins setting (bins caustic phases, the 1400
+// seed-scan phases, the boundary, angular and interior probe angles), rounded
+// by the host libm as the reference's cos/sin calls are; their count; the
+// table pointers of a device copy.
+inline int angle_table_count(int bins)
+{
+    return bins + kSeedPhases + kBoundaryProbes + kAngularProbes + kInteriorProbes;
+}
+void angle_tables(int bins, std::vector<Pt2>& host);
+FsTables table_view(const Pt2* device_tables, int bins);
 
 // Caustic caches: critical-curve solves, bounding box and point cells (the
 // part stage 1 reads), then the tracked branches, their cells and their order
@@ -56,9 +69,13 @@ enum IrLaunch {
     kIrLaunches = 7
 };
 
+// With every stream of `ss` equal to `st` the passes run in order on `st`
+// and no event is used.  farm = false leaves out the probe-farm launches
+// (heavy list, farm, resumed searches): valid when opt.heavy_budget < 0, so
+// that no search defers.
 lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const FsTables& tab,
                          const FsOptions& opt, const FsEpochs& ep, const IrLists& lists,
-                         const IrScratch& scr, hipStream_t st, const IrStreams& ss);
+                         const IrScratch& scr, hipStream_t st, const IrStreams& ss, bool farm = true);
 
 // One inverse-ray epoch of the last run on slot 0 (diagnostics): seed search
 // (n_in < 0) or given seeds, then the magnification with `method`.
@@ -79,6 +96,14 @@ lcbg_status launch_fs_grad(const CurveRec* d_curves, int n_curves, const FsOptio
 // Workgroups of one inverse-ray kernel resident per CU (kind 0 seeds,
 // 1 Cartesian, 2 polar), from its register and LDS use.
 int ir_resident_per_cu(int kind);
+
+// Records the first epoch of mag[0, n) that is not finite in
+// counters[kFirstFailWord] (n - e; 0: none).
+lcbg_status launch_first_nonfinite(const double* mag, int64_t n, unsigned* counters, hipStream_t st);
+
+// Per-epoch result records (lcbg_result) of the run whose arrays `ep` views.
+lcbg_status launch_epoch_results(const CurveRec* d_curves, const FsOptions& opt, const FsEpochs& ep,
+                                 double tol, double reltol, lcbg_result* out, hipStream_t st);
 
 // Zeroes the queue counters in-stream with a kernel (graph-capturable).
 lcbg_status launch_reset_counters(unsigned* counters, int n, hipStream_t st);

@@ -100,6 +100,9 @@ struct FsOptions {
     // seeds are searched first and whose areas start on large slots at once
     int heavy_budget;
     int lead_items;
+    // wave steps one inverse-ray epoch may take before it is abandoned
+    // (lcbg_batch_set_work_budget; default kStepBudget)
+    long long step_budget;
 };
 
 struct FsEpochs {
@@ -125,6 +128,8 @@ struct FsEpochs {
     float* sched;       // [epoch] ticks of the epoch's seed search and area in the last
                         // run (0: none), the cost key of the next run's queues; or null
     double* dint;       // [epoch][6] tangents of the magnification (gradient runs)
+    double* err;        // [epoch] error estimate of the method that settled the epoch
+                        // (FiniteSourceResult::error_estimate, finite_source_magnifier.hpp:47-54)
 };
 
 // Scratch of the resident waves of each inverse-ray kernel.
@@ -203,7 +208,10 @@ struct IrLists {
 // largest epochs, and of the large-slot retry pass, the seed-pool cursor.
 constexpr int kWorkOffset = 28;
 constexpr int kBucketOffset = 44;
-constexpr int kCounterWords = kBucketOffset + 2 * kCostBuckets;
+// counters[kFirstFailWord]: n - e of the first epoch e whose magnification is
+// not finite (0: none), written by the run's last kernel.
+constexpr int kFirstFailWord = kBucketOffset + 2 * kCostBuckets;
+constexpr int kCounterWords = kFirstFailWord + 1;
 enum : int {
     kQNearCount = 0, kQNearCursor = 1, kQIrCount = 2, kQIrCursor = 3, kQFlags = 4,
     kQCartCount = 5, kQCartCursor = 6, kQPolarCount = 7, kQPolarCursor = 8,

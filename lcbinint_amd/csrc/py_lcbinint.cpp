@@ -124,6 +124,26 @@ void raise_status(lcbg_status st)
     }
 }
 
+// LightCurveInfo (python/lcbinint_pybind.cpp:2790-2804): per-epoch records.
+struct PyLightCurveInfo {
+    std::vector<double> times, magnifications, point_source_magnifications, finite_source_magnifications,
+        finite_source_error_estimates, source_x, source_y;
+    std::vector<int> image_counts, finite_source_methods, finite_source_refinement_levels;
+    std::vector<std::string> finite_source_method_names;
+    std::vector<bool> finite_source_converged;
+};
+
+const char* method_name(int m)
+{
+    switch (m) {
+    case LCBG_METHOD_POINT_SOURCE: return "point_source";
+    case LCBG_METHOD_HEXADECAPOLE: return "hexadecapole";
+    case LCBG_METHOD_INVERSE_RAY_CARTESIAN: return "inverse_ray_cartesian";
+    case LCBG_METHOD_INVERSE_RAY_POLAR: return "inverse_ray_polar";
+    }
+    return "unknown";
+}
+
 // EventCoordinates (python/lcbinint_pybind.cpp:95-101, 2848-2866)
 struct PyEvent {
     double ra = 0.0, dec = 0.0, tfix = 0.0, obs_lat = 0.0, obs_lon = 0.0;
@@ -143,18 +163,10 @@ public:
             throw std::invalid_argument("source must be 'single' or 'binary'");
     }
 
-    py::array_t<double> evaluate(py::array_t<double, py::array::c_style | py::array::forcecast> times,
-                                 double t0, double tE, double u0, double alpha, double s, double q,
-                                 double rho, double piEN, double piEE, double g1, double g2,
-                                 double g3, double lom_szs, double lom_ar) const
+    // the binary lens in VBM coordinates, static or with annual parallax and
+    // orbital motion; other models are outside this library
+    void check_supported(double tE, double s, double q) const
     {
-        const py::buffer_info in = times.request();
-        if (in.ndim != 1) throw std::invalid_argument("times must be one-dimensional");
-        const auto n = static_cast<py::ssize_t>(in.shape[0]);
-        std::vector<double> t(static_cast<const double*>(in.ptr), static_cast<const double*>(in.ptr) + n);
-        auto out = py::array_t<double>({n}, {static_cast<py::ssize_t>(sizeof(double))});
-        // the binary lens in VBM coordinates, static or with annual parallax and
-        // orbital motion; other models are outside this library
         const bool supported = lens_ == "binary_lens" && source_ == "single" && !terrestrial_ &&
                                options_.vbm_compatible != 0 && options_.center_of_mass == 0 &&
                                options_.xallarap == "none" &&
@@ -163,21 +175,107 @@ public:
         if (!supported) throw std::runtime_error("unsupported");
         if (!std::isfinite(tE) || tE == 0.0 || !std::isfinite(q) || !std::isfinite(s))
             throw std::runtime_error("numerical error");
+    }
+
+    static std::vector<double> times_of(const py::array_t<double, py::array::c_style | py::array::forcecast>& times)
+    {
+        const py::buffer_info in = times.request();
+        if (in.ndim != 1) throw std::invalid_argument("times must be one-dimensional");
+        const auto n = static_cast<py::ssize_t>(in.shape[0]);
+        return std::vector<double>(static_cast<const double*>(in.ptr), static_cast<const double*>(in.ptr) + n);
+    }
+
+    lcbg_options run_options() const
+    {
+        lcbg_options opts = options_.o;
+        opts.finite_source_tol = options_.finite_source_tol;
+        opts.finite_source_reltol = options_.finite_source_reltol;
+        return opts;
+    }
+
+    // evaluate_binary_light_curve_info (python/lcbinint_pybind.cpp:1176-1290)
+    PyLightCurveInfo info(py::array_t<double, py::array::c_style | py::array::forcecast> times, double t0,
+                          double tE, double u0, double alpha, double s, double q, double rho, double piEN,
+                          double piEE, double g1, double g2, double g3, double lom_szs, double lom_ar) const
+    {
+        std::vector<double> t = times_of(times);
+        check_supported(tE, s, q);
+        const lcbg_curve cv{t0, tE, u0, alpha, s, q, rho, ld_.c, ld_.d};
+        const lcbg_options opts = run_options();
+        const double pN = parallax_ ? piEN : 0.0;
+        const double pE = parallax_ ? piEE : 0.0;
+        const int64_t n = static_cast<int64_t>(t.size());
+        std::vector<lcbg_result> rec(t.size());
+        lcbg_status st;
+        {
+            py::gil_scoped_release release;
+            if (orbit_ == 0 && pN == 0.0 && pE == 0.0) {
+                int64_t first = -1;
+                st = lcbg_light_curve_results(&cv, &opts, t.data(), n, rec.data(), &first);
+            } else {
+                const lcbg_dynamics dyn{pN, pE, event_.ra, event_.dec, event_.tfix, g1, g2, g3, lom_szs, lom_ar};
+                lcbg_batch* b = nullptr;
+                st = lcbg_batch_create(&cv, 1, &opts, &b);
+                const int64_t off[2] = {0, n};
+                if (st == LCBG_OK)
+                    st = lcbg_batch_light_curves_dynamic_results(b, &dyn, orbit_, off, t.data(), n, rec.data());
+                lcbg_batch_destroy(b);
+            }
+        }
+        raise_status(st);
+        PyLightCurveInfo out;
+        out.times = t;
+        for (const lcbg_result& r : rec) {
+            out.magnifications.push_back(r.magnification);
+            out.point_source_magnifications.push_back(r.point_source_magnification);
+            out.finite_source_magnifications.push_back(r.finite_source_magnification);
+            out.finite_source_error_estimates.push_back(r.error_estimate);
+            out.source_x.push_back(r.source_x);
+            out.source_y.push_back(r.source_y);
+            out.image_counts.push_back(r.image_count);
+            out.finite_source_methods.push_back(r.method);
+            out.finite_source_method_names.push_back(method_name(r.method));
+            out.finite_source_refinement_levels.push_back(r.refinement_level);
+            out.finite_source_converged.push_back(r.converged != 0);
+        }
+        return out;
+    }
+
+    PyLightCurveInfo info_from_dict(py::array_t<double, py::array::c_style | py::array::forcecast> times,
+                                    const py::dict& p) const
+    {
+        return info(times, dict_get(p, "t0", 0.0), dict_get(p, "tE", 1.0), dict_get(p, "u0", 0.0),
+                    dict_get(p, "alpha", 0.0), dict_get(p, "s", 1.0), dict_get(p, "q", 1.0),
+                    dict_get(p, "rho", 0.0), parallax_ ? dict_get(p, "piEN", 0.0) : 0.0,
+                    parallax_ ? dict_get(p, "piEE", 0.0) : 0.0, dict_get(p, "g1", 0.0), dict_get(p, "g2", 0.0),
+                    dict_get(p, "g3", 0.0), dict_get(p, "lom_szs", 0.0), dict_get(p, "lom_ar", 1.0));
+    }
+
+    py::array_t<double> evaluate(py::array_t<double, py::array::c_style | py::array::forcecast> times,
+                                 double t0, double tE, double u0, double alpha, double s, double q,
+                                 double rho, double piEN, double piEE, double g1, double g2,
+                                 double g3, double lom_szs, double lom_ar) const
+    {
+        std::vector<double> t = times_of(times);
+        const auto n = static_cast<py::ssize_t>(t.size());
+        auto out = py::array_t<double>({n}, {static_cast<py::ssize_t>(sizeof(double))});
+        check_supported(tE, s, q);
         lcbg_curve cv{t0, tE, u0, alpha, s, q, rho, ld_.c, ld_.d};
+        const lcbg_options opts = run_options();
         const double pN = parallax_ ? piEN : 0.0;
         const double pE = parallax_ ? piEE : 0.0;
         lcbg_status st;
         if (orbit_ == 0 && pN == 0.0 && pE == 0.0) {
             // direct static binary (python/lcbinint_pybind.cpp:1103-1158)
             py::gil_scoped_release release;
-            st = lcbg_light_curve(&cv, &options_.o, t.data(), static_cast<int64_t>(n),
+            st = lcbg_light_curve(&cv, &opts, t.data(), static_cast<int64_t>(n),
                                   out.mutable_data(), nullptr);
         } else {
             // LensModel path (python/lcbinint_pybind.cpp:1159-1173)
             const lcbg_dynamics dyn{pN, pE, event_.ra, event_.dec, event_.tfix, g1, g2, g3, lom_szs, lom_ar};
             py::gil_scoped_release release;
             lcbg_batch* b = nullptr;
-            st = lcbg_batch_create(&cv, 1, &options_.o, &b);
+            st = lcbg_batch_create(&cv, 1, &opts, &b);
             const int64_t off[2] = {0, static_cast<int64_t>(n)};
             std::vector<int32_t> meth(static_cast<std::size_t>(n));
             if (st == LCBG_OK)
@@ -337,6 +435,32 @@ PYBIND11_MODULE(_lcbinint, m)
         .def_property("param_type", &coordinate_name, &apply_param_type)
         .def_property_readonly("xallarap_param_type", [](const PyOptions& p) { return p.xallarap; });
 
+    py::class_<PyLightCurveInfo>(m, "LightCurveInfo")
+        .def_readonly("times", &PyLightCurveInfo::times)
+        .def_readonly("magnifications", &PyLightCurveInfo::magnifications)
+        .def_readonly("point_source_magnifications", &PyLightCurveInfo::point_source_magnifications)
+        .def_readonly("finite_source_magnifications", &PyLightCurveInfo::finite_source_magnifications)
+        .def_readonly("finite_source_error_estimates", &PyLightCurveInfo::finite_source_error_estimates)
+        .def_readonly("source_x", &PyLightCurveInfo::source_x)
+        .def_readonly("source_y", &PyLightCurveInfo::source_y)
+        .def_readonly("image_counts", &PyLightCurveInfo::image_counts)
+        .def_readonly("finite_source_methods", &PyLightCurveInfo::finite_source_methods)
+        .def_readonly("finite_source_method_names", &PyLightCurveInfo::finite_source_method_names)
+        .def_readonly("finite_source_refinement_levels", &PyLightCurveInfo::finite_source_refinement_levels)
+        .def_readonly("finite_source_converged", &PyLightCurveInfo::finite_source_converged)
+        .def_property_readonly("all_converged",
+                               [](const PyLightCurveInfo& c) {
+                                   for (bool v : c.finite_source_converged)
+                                       if (!v) return false;
+                                   return true;
+                               })
+        .def_property_readonly("unconverged_indices", [](const PyLightCurveInfo& c) {
+            std::vector<int> idx;
+            for (std::size_t i = 0; i < c.finite_source_converged.size(); ++i)
+                if (!c.finite_source_converged[i]) idx.push_back(static_cast<int>(i));
+            return idx;
+        });
+
     py::class_<PyEvent>(m, "EventCoordinates")
         .def(py::init([](double ra, double dec, double tfix, double obs_lat, double obs_lon) {
                  return PyEvent{ra, dec, tfix, obs_lat, obs_lon};
@@ -379,5 +503,14 @@ PYBIND11_MODULE(_lcbinint, m)
              py::arg("s") = 1.0, py::arg("q") = 1.0, py::arg("rho") = 0.0, py::arg("piEN") = 0.0,
              py::arg("piEE") = 0.0, py::arg("g1") = 0.0, py::arg("g2") = 0.0, py::arg("g3") = 0.0,
              py::arg("lom_szs") = 0.0, py::arg("lom_ar") = 1.0)
-        .def("light_curve", &PyLightCurve::from_dict, py::arg("times"), py::arg("params"));
+        .def("light_curve", &PyLightCurve::from_dict, py::arg("times"), py::arg("params"))
+        .def("info", &PyLightCurve::info_from_dict, py::arg("times"), py::arg("params"))
+        .def("info", &PyLightCurve::info, py::arg("times"), py::kw_only(), py::arg("t0") = 0.0,
+             py::arg("tE") = 1.0, py::arg("u0") = 0.0, py::arg("alpha") = 0.0, py::arg("s") = 1.0,
+             py::arg("q") = 1.0, py::arg("rho") = 0.0, py::arg("piEN") = 0.0, py::arg("piEE") = 0.0,
+             py::arg("g1") = 0.0, py::arg("g2") = 0.0, py::arg("g3") = 0.0, py::arg("lom_szs") = 0.0,
+             py::arg("lom_ar") = 1.0);
+
+    // python/lcbinint_pybind.cpp:3405
+    m.attr("LightCurveFunc") = m.attr("LightCurve");
 }

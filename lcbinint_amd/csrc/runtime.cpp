@@ -98,6 +98,8 @@ extern "C" lcbg_options lcbg_default_options(void)
     o.point_source_threshold = 20.0;
     o.hexadecapole_threshold = 3.0;
     o.adaptive_hex_threshold = 0.001;
+    o.finite_source_tol = 0.0;
+    o.finite_source_reltol = 0.0;
     return o;
 }
 

@@ -58,6 +58,7 @@ struct lcbg_batch {
     bool dynamic_images = false;         // point-source sum and seeds of the LensModel path
     lcbg_batch* dyn_child = nullptr;     // per-epoch pseudo-curves of the dynamic path
     lcbg_schedule schedule{};            // launch schedule (0 fields: defaults)
+    int64_t work_budget = 0;             // wave steps per inverse-ray epoch (0: default)
     hipGraphExec_t graph = nullptr;      // captured finite-source run
     hipStream_t capture_stream = nullptr;
 };

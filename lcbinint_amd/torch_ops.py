@@ -31,14 +31,16 @@ class _LightCurveFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, params, owner):
         n = owner.n_epochs
-        owner.batch.update_curves(_curves_from(params))
         dev = params.device
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        # the curve upload and the run share the caller's stream, so that the
+        # kernels read the new curves under any torch.cuda.stream context
+        owner.batch.update_curves(_curves_from(params), stream)
         mag = torch.empty(n, dtype=torch.float64, device=dev)
         method = torch.empty(n, dtype=torch.int32, device=dev)
         dmag = torch.empty((n, 8), dtype=torch.float64, device=dev)
         capi.finite_source_grad(owner.batch, owner.d_off.data_ptr(), owner.d_t.data_ptr(), n,
-                                mag.data_ptr(), method.data_ptr(), 0, dmag.data_ptr(),
-                                torch.cuda.current_stream(dev).cuda_stream)
+                                mag.data_ptr(), method.data_ptr(), 0, dmag.data_ptr(), stream)
         ctx.save_for_backward(dmag)
         ctx.owner = owner
         owner.last_method = method
@@ -98,7 +100,9 @@ class _DynamicLightCurveFn(torch.autograd.Function):
                              ld_d=0.0) for r in host]
         dyn = [capi.Dynamics(piEN=r[8], piEE=r[9], g1=r[10], g2=r[11], **fixed)
                for r, fixed in zip(host, owner.fixed)]
-        owner.batch.update_curves(curves)
+        # host-synchronous path: the curves are read on the host
+        owner.batch.update_curves(curves, torch.cuda.current_stream(params.device).cuda_stream
+                                  if params.is_cuda else 0)
         mag, method, _, dmag = capi.light_curves_dynamic_grad(owner.batch, dyn, owner.orbit_mode,
                                                               owner.offsets, owner.times)
         owner.last_method = method

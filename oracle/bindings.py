@@ -132,6 +132,8 @@ class Ref:
                       P(I32), P(D), P(I32))
         self._lc = _f(L, "ref_light_curve_static", I32, P(RefSettings), V, ctypes.c_long,
                       D, D, D, D, D, D, D, V, V)
+        self._lci = _f(L, "ref_light_curve_static_info", I32, P(RefSettings), D, D, V, ctypes.c_long,
+                       D, D, D, D, D, D, D, V)
         self._lcd = _f(L, "ref_light_curve_dynamic", I32, P(RefSettings), V, ctypes.c_long, V, I32,
                        V, V, V)
         self._ir = _f(L, "ref_binary_inverse_ray", D, P(RefSettings), D, D, D, D, D, P(I32))
@@ -201,6 +203,22 @@ class Ref:
         st = self._lc(ctypes.byref(settings), ptr(t), len(t), t0, tE, u0, alpha, s, q, rho,
                       ptr(out), ptr(meth))
         return out, meth, st
+
+    INFO_FIELDS = ("magnification", "point_source_magnification", "finite_source_magnification",
+                   "source_x", "source_y", "image_count", "method", "error_estimate",
+                   "refinement_level", "converged")
+
+    def light_curve_info(self, settings, times, t0, tE, u0, alpha, s, q, rho, tol=0.0, reltol=0.0):
+        """Per-epoch records of the static VBM info loop: dict of arrays keyed
+        by INFO_FIELDS, and the status (0, or 2 if an epoch was not finite)."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        rec = np.full((len(t), 10), np.nan)
+        st = self._lci(ctypes.byref(settings), tol, reltol, ptr(t), len(t), t0, tE, u0, alpha, s, q, rho,
+                       ptr(rec))
+        out = {k: rec[:, i] for i, k in enumerate(self.INFO_FIELDS)}
+        for k in ("image_count", "method", "refinement_level", "converged"):
+            out[k] = np.nan_to_num(out[k], nan=-1).astype(np.int32)
+        return out, st
 
     DYNAMIC_KEYS = ("t0", "tE", "u0", "alpha", "s", "q", "rho", "piEN", "piEE", "ra", "dec", "tfix",
                     "g1", "g2", "g3", "lom_szs", "lom_ar")

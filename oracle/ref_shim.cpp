@@ -220,6 +220,54 @@ int ref_light_curve_static(const RefSettings* rs, const double* times, long n, d
     return 0;
 }
 
+// evaluate_binary_light_curve_info (lcbinint_pybind.cpp:1176-1253), static VBM
+// branch: per epoch the row (magnification, point-source magnification,
+// finite-source magnification, source x, source y, image count, method, error
+// estimate, refinement level, converged) in rec[10 i .. 10 i + 9], with the
+// finite-source tolerances tol / reltol of the settings (they decide
+// `converged`).  Returns 0, or 2 at the first non-finite magnification (rows
+// from there on are left unwritten), where the reference raises.
+int ref_light_curve_static_info(const RefSettings* rs, double tol, double reltol, const double* times,
+                                long n, double t0, double tE, double u0, double alpha, double s,
+                                double q, double rho, double* rec)
+{
+    PointSourceMagnifier pm;
+    FiniteSourceSettings settings = to_settings(*rs);
+    settings.finite_source_tol = tol;
+    settings.finite_source_reltol = reltol;
+    FiniteSourceMagnifier fm(settings);
+    const double cos_alpha = std::cos(alpha);
+    const double sin_alpha = std::sin(alpha);
+    const double inverse_tE = 1.0 / tE;
+    const double effective_q = q != 0.0 ? 1.0 / q : q;
+    for (long i = 0; i < n; ++i) {
+        const double tau = (times[i] - t0) * inverse_tE;
+        const SourcePosition src {tau * cos_alpha - u0 * sin_alpha, tau * sin_alpha + u0 * cos_alpha};
+        const auto point = pm.binary_mag0(s, effective_q, src);
+        double* r = rec + 10 * i;
+        r[1] = point.magnification;
+        r[3] = src.x;
+        r[4] = src.y;
+        r[5] = point.image_count;
+        if (rho == 0.0) {
+            r[0] = r[2] = point.magnification;
+            r[6] = 0.0;
+            r[7] = 0.0;
+            r[8] = 0.0;
+            r[9] = 1.0;
+            continue;
+        }
+        const auto fs = fm.binary_mag(s, effective_q, src, std::abs(rho), point.magnification, nullptr, true);
+        if (!std::isfinite(fs.magnification)) return 2;
+        r[0] = r[2] = fs.magnification;
+        r[6] = static_cast<double>(static_cast<int>(fs.decision.method));
+        r[7] = fs.error_estimate;
+        r[8] = fs.refinement_level;
+        r[9] = fs.converged ? 1.0 : 0.0;
+    }
+    return 0;
+}
+
 // Forced inverse-ray, as lcbinint.binary_inverse_ray (lcbinint_pybind.cpp:3826-3847):
 // bbox exit, derivative check and hexadecapole are switched off.
 double ref_binary_inverse_ray(const RefSettings* rs, double s, double q, double x, double y,

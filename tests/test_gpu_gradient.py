@@ -253,3 +253,28 @@ def test_torch_autograd_bridge(gpu):
     got = params.grad.cpu().numpy()
     assert np.allclose(got, exp, rtol=1e-10, atol=1e-12 * np.abs(exp).max())
     lc.close()
+
+
+def test_torch_bridge_on_a_side_stream(gpu):
+    """Under `with torch.cuda.stream(s)` the curve upload and the run share s:
+    a second forward with new parameters on a non-default stream gives the
+    values of the same parameters on the default stream."""
+    import torch
+    from lcbinint_amd.torch_ops import FiniteSourceLightCurves
+    from lcbinint_amd.workloads import differentiation_batch
+    curves, offsets, times = differentiation_batch(n_curves=16, n_epochs=100)
+    p0 = torch.tensor([[c[k] for k in PARAMS] for c in curves], dtype=torch.float64, device="cuda")
+    p1 = p0.clone()
+    p1[:, 2] += 0.01  # u0
+    lc = FiniteSourceLightCurves(offsets, times, nbin=50)
+    ref_lc = FiniteSourceLightCurves(offsets, times, nbin=50)
+    exp = ref_lc(p1).detach().cpu().numpy()
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        _ = lc(p0)
+        got = lc(p1)
+    s.synchronize()
+    assert np.array_equal(got.detach().cpu().numpy(), exp)
+    lc.close()
+    ref_lc.close()

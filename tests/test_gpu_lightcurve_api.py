@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_readme_light_curve(gpu, ref):
-    import lcbinint_amd as lcbinint
+    import lcbinint  # the reference's module name
     t = np.linspace(-0.5, 0.5, 200)
     params = {"t0": 0.0, "tE": 1.0, "u0": -0.01, "alpha": 0.5, "s": 0.95, "q": 1e-2, "rho": 5e-3}
     lc = lcbinint.LightCurve(options=lcbinint.Options(coordinates="vbm", nbin=50),
@@ -39,3 +39,36 @@ def test_point_source_curve_and_errors(gpu, ref):
         lcbinint.LightCurve(source="binary")(t, {"u0": 0.1})
     with pytest.raises(RuntimeError, match="numerical error"):
         lc(t, {"tE": 0.0})
+
+
+def test_info_matches_reference_info(gpu, ref):
+    """LightCurve.info against evaluate_binary_light_curve_info
+    (python/lcbinint_pybind.cpp:1176-1253), static VBM curve, with a relative
+    tolerance that leaves some Cartesian epochs converged and some not."""
+    import lcbinint
+    t = np.linspace(-0.5, 0.5, 200)
+    params = {"t0": 0.0, "tE": 1.0, "u0": -0.01, "alpha": 0.5, "s": 0.95, "q": 1e-2, "rho": 5e-3}
+    lc = lcbinint.LightCurve(options=lcbinint.Options(coordinates="vbm", nbin=50, reltol=2e-3),
+                             limb_darkening=lcbinint.LimbDarkening.linear(0.5))
+    info = lc.info(t, params)
+    exp, st = ref.light_curve_info(default_settings(limb_darkening_c=0.5), t, 0.0, 1.0, -0.01, 0.5,
+                                   0.95, 1e-2, 5e-3, 0.0, 2e-3)
+    assert st == 0
+    meth = np.array(info.finite_source_methods)
+    assert np.array_equal(meth, exp["method"])
+    assert info.finite_source_method_names[int(np.argmax(meth == 2))] == "inverse_ray_cartesian"
+    assert np.array_equal(np.array(info.image_counts), exp["image_count"])
+    assert np.array_equal(np.array(info.point_source_magnifications), exp["point_source_magnification"])
+    assert np.array_equal(np.array(info.source_x), exp["source_x"])
+    assert np.array_equal(np.array(info.source_y), exp["source_y"])
+    got = np.array(info.magnifications)
+    assert np.max(np.abs(got - exp["magnification"]) / exp["magnification"]) <= 1e-9
+    err = np.array(info.finite_source_error_estimates)
+    cart = meth == 2
+    assert np.array_equal(err[~cart], exp["error_estimate"][~cart])
+    assert np.max(np.abs(err[cart] - exp["error_estimate"][cart]) / exp["error_estimate"][cart]) <= 1e-12
+    conv = np.array(info.finite_source_converged, dtype=np.int32)
+    assert np.array_equal(conv, exp["converged"])
+    assert info.unconverged_indices == [int(i) for i in np.nonzero(conv == 0)[0]]
+    assert info.all_converged == bool(conv.all())
+    assert np.array_equal(np.array(lc(t, params)), got)
