@@ -461,6 +461,30 @@ lcbg_status lcbg_batch_last_run_status(lcbg_batch* batch, uint32_t* error_flags,
 lcbg_status lcbg_batch_epoch_results_device(lcbg_batch* batch, int64_t n_epochs, lcbg_result* results,
                                             void* stream);
 
+/* No counterpart in the reference: an entry point that allocates nothing, for
+ * callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  It runs what
+ * lcbg_batch_finite_source_device runs on a one-curve batch, for one curve
+ * (host record, as for lcbg_batch_create) with options (NULL = defaults):
+ * times, mag and method are device pointers of n epochs, and the work is
+ * enqueued on `stream` only.  Every piece of workspace lies in one static
+ * device array of 128 MiB, which each device that runs this entry point
+ * holds until the process ends; no device memory is allocated or freed.
+ * Limits: n <= 8192 (else LCBG_INVALID_ARGUMENT), caustic_bins <= 2048
+ * (else LCBG_INVALID_ARGUMENT), 2^17 Cartesian rows per image and 2^20 polar
+ * runs per epoch (an epoch past them, like one past the work budget, gets a
+ * NaN magnification; there is no batch to query, so callers test mag).
+ * The first call on a device, and a call whose caustic_bins differs from the
+ * previous call's on that device, upload a small angle table synchronously:
+ * a caller that captures the call into a HIP graph makes one call with the
+ * same caustic_bins first.  All calls on one device share the array, so
+ * calls that overlap in time on one device (other streams, other threads)
+ * race and give wrong results; serialise them.  Results equal those of the
+ * batch entry point (the launch schedule changes no result). */
+lcbg_status lcbg_light_curve_device_noalloc(const lcbg_curve* curve, const lcbg_options* options,
+                                            const double* times, int64_t n, double* mag, int32_t* method,
+                                            void* stream);
+
 /* Device time in milliseconds of the last kernel launched by the calling host
  * thread through a *_device entry point, measured with HIP events recorded on
  * the launch stream around that kernel.  Returns a negative value if none. */

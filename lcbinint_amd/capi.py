@@ -473,3 +473,6 @@ def epoch_results(batch: "Batch", n_epochs: int, results_ptr: int, stream: int =
 
 lcbg_batch_set_work_budget = _sig("lcbg_batch_set_work_budget", ctypes.c_int, ctypes.c_void_p, ctypes.c_int64)
 
+lcbg_light_curve_device_noalloc = _sig(
+    "lcbg_light_curve_device_noalloc", ctypes.c_int, ctypes.POINTER(Curve), ctypes.POINTER(Options),
+    ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
