@@ -20,7 +20,7 @@
 //    g = rho^2 - |w(z) - y|^2 the source brightness is
 //    (1 - c) [g > 0] + c sqrt(max(g, 0)) / rho; each cell weighs it averaged
 //    over its footprint, g linearised over the cell and smeared by a Gaussian
-//    of one cell (the antialiased edge), and A = sum W h^2 / (pi rho^2
+//    of kSmear cells (the antialiased edge), and A = sum W h^2 / (pi rho^2
 //    (1 - c/3)).  The cells with nonzero weight are enumerated by a scan-line
 //    flood fill from the value run's image seeds.  This is a differentiable
 //    stand-in for the plain quadrature, which it matches to its
@@ -39,6 +39,13 @@ namespace lcbg {
 namespace {
 
 constexpr double kSqrtHalfG = 0.70710678118654752440;
+// Width of the antialiasing Gaussian in lattice cells.  A wider kernel
+// smooths more but moves the quadrature further from the plain one (median
+// |mag_aa / mag - 1| 1e-3 at one cell, 2e-3 at 1.5 cells on 256 x 1000
+// curves), so the width stays one cell; the roughness that was left at one
+// cell came from the critical curve, where the width sigma |grad g| vanishes,
+// and is removed by the floor of width_floor.
+constexpr double kSmear = 1.0;
 
 struct DLens {
     Dual a, m1, m2;
@@ -80,6 +87,25 @@ __device__ __forceinline__ MapOut<T> lens_map_k(const LT& L, const T& x, const T
     m.kr = L.m1 * (xa * xa - y2) * i1s + L.m2 * (x * x - y2) * i2s;
     m.ki = 2.0 * (L.m1 * xa * y * i1s + L.m2 * x * y * i2s);
     return m;
+}
+
+// Floor of the smoothing variance, in units of sigma^4: 128 phi(|kappa|^2)
+// with phi(k) = 2 k / (1 + k^2), which is 1 on the critical curve
+// (|kappa| = 1), where grad g vanishes at the points of the source edge that
+// touch the caustic and the linear width sigma |grad g| would leave the weight
+// sharp, and falls to 0 away from it, both towards the lenses (where kappa
+// diverges) and far from them.  The second-order variance of g over the
+// footprint on the critical curve is 16 sigma^4 (sigma^4 |H|_F^2 / 2 with
+// g_zzbar = -(1 + |kappa|^2) = -2); the floor is eight times that.  Measured
+// on 256 x 1000 curves against central differences of the same quadrature
+// (99th percentile of the relative error of dA/dt0 over inverse-ray epochs):
+// 9e-4 without the floor, 8e-5 at 16 sigma^4, 7e-6 at 64 sigma^4; the floor
+// changes the value by far less than the width does.
+template <class T>
+__device__ __forceinline__ T width_floor(const MapOut<T>& m)
+{
+    const T k = m.kr * m.kr + m.ki * m.ki;
+    return 256.0 * k / (1.0 + k * k);
 }
 
 // Magnification 1/|det J| of the image at z of source y, with tangents: the
@@ -235,7 +261,9 @@ __device__ __forceinline__ void sqrt_gauss(double t, double* F, double* dF)
 }
 
 // Cells the fill enumerates: those within eight Gaussian widths outside the
-// edge (g + 8 h |grad g|_1 > 0; the 1-norm bounds the 2-norm from above).
+// edge, g + 8 (sigma |grad g|_1 + sigma^2 sqrt(floor)) > 0 with
+// sigma = kSmear h (the 1-norm bounds the 2-norm from above, and the sum bounds
+// the width of aa_weight).
 __device__ __forceinline__ bool aa_in(const GradCtx& G, int i, int j)
 {
     const double rho = G.rho.v;
@@ -247,13 +275,18 @@ __device__ __forceinline__ bool aa_in(const GradCtx& G, int i, int j)
     const double g = rho * rho - (dx * dx + dy * dy);
     const double gx = -2.0 * (dx * (1.0 + m.kr) + dy * m.ki);
     const double gy = -2.0 * (dx * m.ki + dy * (1.0 - m.kr));
-    return g + 8.0 * G.h * (dabs(gx) + dabs(gy)) > 0.0;
+    const double sig = kSmear * G.h;
+    const double curv = sig * sig * __builtin_sqrt(width_floor(m));
+    return g + 8.0 * (sig * (dabs(gx) + dabs(gy)) + curv) > 0.0;
 }
 
 // Weight of cell (i, j) with tangents: the source brightness
 // I = (1 - c) [g > 0] + c sqrt(max(g, 0)) / rho  (g = rho^2 - |w - y|^2,
-// linear limb darkening) averaged over the cell's linearised footprint with a
-// Gaussian of one cell: g -> g + s Z, s = h |grad g|.  With t = g / s,
+// linear limb darkening) averaged over a Gaussian footprint of kSmear cells
+// (sigma = kSmear h per axis): g -> g + s Z, s^2 = sigma^2 |grad g|^2 +
+// sigma^4 width_floor.  The floor keeps the width positive where grad g
+// vanishes on the critical curve, so that the weight stays smooth there.
+// With t = g / s,
 // W = (1 - c) Phi(t) + (c / rho) sqrt(s) F(t).  Unlike a box footprint, the
 // Gaussian makes the lattice sum smooth at the cell scale (its ripple scales
 // with the kernel's transform at the lattice frequency, ~exp(-2 pi^2)).
@@ -267,7 +300,8 @@ __device__ Dual aa_weight(const GradCtx& G, int i, int j)
     const Dual g = G.rho * G.rho - (dx * dx + dy * dy);
     const Dual gx = -2.0 * (dx * (1.0 + m.kr) + dy * m.ki);
     const Dual gy = -2.0 * (dx * m.ki + dy * (1.0 - m.kr));
-    const Dual sc = G.h * dsqrt(gx * gx + gy * gy);
+    const double sig2 = (kSmear * G.h) * (kSmear * G.h);
+    const Dual sc = dsqrt(sig2 * (gx * gx + gy * gy) + (sig2 * sig2) * width_floor(m));
     if (!(sc.v > 0.0)) return dconst(g.v > 0.0 ? 1.0 - G.c.v + G.c.v * __builtin_sqrt(g.v) / G.rho.v : 0.0);
     const Dual t = g / sc;
     if (t.v < -9.0) return dconst(0.0);

@@ -280,3 +280,19 @@ def load_oracle(path):
 
 def load_ref(path):
     return Ref(path)
+
+
+def ref_light_curves_worker(args):
+    """Process-pool worker: the reference's static light curve of each
+    (index, curve dict, nbin, times) item with the default settings at that
+    nbin and the curve's linear limb darkening; returns (index, mag, method,
+    status) per item.  args = (library path, items)."""
+    lib, items = args
+    ref = load_ref(lib)
+    out = []
+    for c, cv, nb, t in items:
+        st = default_settings(source_bins=int(nb), limb_darkening_c=cv["ld_c"])
+        m, k, status = ref.light_curve(st, t, cv["t0"], cv["tE"], cv["u0"], cv["alpha"], cv["s"],
+                                       cv["q"], cv["rho"])
+        out.append((c, m, k, status))
+    return out
