@@ -326,6 +326,7 @@ struct AaFill {
     int run_cap;
     bool overflow;
     long long steps;
+    long long cells;  // cells of the recorded runs
 };
 
 __device__ __forceinline__ bool aa_visited(const AaFill& F, int i, int j)
@@ -343,9 +344,8 @@ __device__ __forceinline__ bool aa_step(AaFill& F)
     return !F.overflow;
 }
 
-// Extends cell (i, j) to its run along the row, records it and adds its
-// weights to this lane's accumulator.
-__device__ void aa_add_run(const GradCtx& G, AaFill& F, int i, int j, Dual& acc)
+// Extends cell (i, j) to its run along the row and records it.
+__device__ void aa_add_run(const GradCtx& G, AaFill& F, int i, int j)
 {
     int left = i;
     for (;;) {
@@ -388,21 +388,23 @@ __device__ void aa_add_run(const GradCtx& G, AaFill& F, int i, int j, Dual& acc)
         wave_store(&F.head[j & (kPhiHash - 1)], r);
     }
     F.n_runs = uni(F.n_runs + 1);
+    F.cells = uni(F.cells + static_cast<long long>(right - left + 1));
     wave_sync();
-    for (int c = left + G.lane; c <= right; c += kWave) acc += aa_weight(G, c, j);
 }
 
-// The antialiased magnification of one epoch with tangents, flood-filled from
-// the seeds.  Images smaller than the lattice can resolve (linear size
-// rho sqrt(A_i) under four cells) enter as point images with their exact
-// tangents, and the seeds near them start no fill.  Returns NaN (and
-// F.overflow) past a capacity or the budget.
-__device__ Dual aa_magnification(const GradCtx& G, AaFill& F, const Pt* seeds, int n_seeds)
+// The cells of one epoch's antialiased quadrature, flood-filled from the
+// seeds: the runs of cells with nonzero weight are left in F.runs (F.n_runs of
+// them, F.cells cells) and their weights are summed later by k_grad_cells.
+// Images smaller than the lattice can resolve (linear size rho sqrt(A_i)
+// under four cells) enter as point images with their exact tangents, the
+// returned sum, and the seeds near them start no fill.  F.overflow is set past
+// a capacity or the budget.
+__device__ Dual aa_enumerate(const GradCtx& G, AaFill& F, const Pt* seeds, int n_seeds)
 {
-    Dual acc = dconst(0.0);
     F.n_runs = 0;
     F.overflow = false;
     F.steps = 0;
+    F.cells = 0;
     const double h = G.h;
     Pt unresolved[5];
     int n_unres = 0;
@@ -455,7 +457,7 @@ __device__ Dual aa_magnification(const GradCtx& G, AaFill& F, const Pt* seeds, i
         const int si = i0 + (found % 9) - 4;
         const int sj = j0 + (found / 9) - 4;
         if (aa_visited(F, si, sj)) continue;
-        aa_add_run(G, F, si, sj, acc);
+        aa_add_run(G, F, si, sj);
     }
     // grow: each recorded run visits its two neighbour rows, diagonal
     // neighbours included (8-connected), so that a thin diagonal image stays
@@ -476,7 +478,7 @@ __device__ Dual aa_magnification(const GradCtx& G, AaFill& F, const Pt* seeds, i
                     c += kWave;
                     continue;
                 }
-                aa_add_run(G, F, c + first_set(m), nb, acc);
+                aa_add_run(G, F, c + first_set(m), nb);
                 c = F.runs[4 * (F.n_runs - 1) + 2] + 1;
             }
         }
@@ -484,15 +486,9 @@ __device__ Dual aa_magnification(const GradCtx& G, AaFill& F, const Pt* seeds, i
     wave_sync();
     for (int q = G.lane; q < F.n_runs; q += kWave) F.head[F.runs[4 * q] & (kPhiHash - 1)] = -1;
     wave_sync();
-    Dual total;
-    total.v = uni(wave_sum(acc.v));
-#pragma unroll
-    for (int k = 0; k < kTangents; ++k) total.d[k] = uni(wave_sum(acc.d[k]));
-    if (F.overflow) return dconst(__builtin_nan(""));
-    // sum W h^2 / (pi rho^2 (1 - c/3)), plus the unresolved point images
-    const Dual flux = (1.0 - G.c * (1.0 / 3.0)) * (kPi * G.rho * G.rho);
-    return total * (G.h * G.h) / flux + a_points;
+    return a_points;
 }
+
 
 __device__ __forceinline__ void epoch_inputs(const CurveRec& cv, Pt src, Dual& yx, Dual& yy, Dual& s,
                                              Dual& q, Dual& rho, Dual& c)
@@ -530,12 +526,42 @@ __global__ void __launch_bounds__(256) k_grad_point(const CurveRec* __restrict__
     store_grad(ep, mag_aa, e, A);
 }
 
-// Inverse-ray epochs of the value run (its item list and seeds), one wave
-// each; pass kPassSmall queues the epochs that outgrow a small slot's runs for
-// the kPassRetry pass on large slots.
+
+// The lattice of an epoch's antialiased quadrature and its inputs with tangents.
+__device__ __forceinline__ GradCtx grad_ctx(const CurveRec& cv, Pt src, const FsOptions& opt)
+{
+    GradCtx G;
+    G.L = make_lens(cv.s, cv.q_eff);
+    Dual s, q;
+    epoch_inputs(cv, src, G.yx, G.yy, s, q, G.rho, G.c);
+    G.Ld = make_dlens(s, q);
+    G.q_eff = cv.q_eff;
+    const int nb = cv.source_bins > 0 ? cv.source_bins : opt.source_bins;
+    // a cell side that does not move with rho: the power of two at or
+    // below rho / nbin (between one and two cells per rho / nbin)
+    G.h = exp2(floor(log2(cv.rho / static_cast<double>(nb > 1 ? nb : 1))));
+    G.lane = lane_id();
+    return G;
+}
+
+// Exclusive prefix sum over the wave of a per-lane int (Hillis-Steele).
+__device__ __forceinline__ int wave_exclusive_scan(int v, int* total)
+{
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(x, off, kWave);
+        if (lane_id() >= off) x += y;
+    }
+    *total = __shfl(x, kWave - 1, kWave);
+    return x - v;
+}
+
+// Cell enumeration of every inverse-ray item, one wave per item, persistent
+// waves over the items in cost order; the runs go to the pools of W.
 __global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ curves,
                                                     FsOptions opt, FsEpochs ep, IrLists Lst,
-                                                    PolarScratch scr, int pass, double* mag_aa)
+                                                    PolarScratch scr, int pass, GradWork W)
 {
     const bool retry = pass == kPassRetry;
     AaFill F;
@@ -551,32 +577,160 @@ __global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ 
         const int64_t e = Lst.ir_sorted[item];
         const int n_seeds = uni(Lst.seed_n[item]);
         const CurveRec cv = curves[ep.curve_of[e]];
-        GradCtx G;
-        G.L = make_lens(cv.s, cv.q_eff);
-        Dual s, q;
-        epoch_inputs(cv, ep.src[e], G.yx, G.yy, s, q, G.rho, G.c);
-        G.Ld = make_dlens(s, q);
-        G.q_eff = cv.q_eff;
-        const int nb = cv.source_bins > 0 ? cv.source_bins : opt.source_bins;
-        // a cell side that does not move with rho: the power of two at or
-        // below rho / nbin (between one and two cells per rho / nbin)
-        G.h = exp2(floor(log2(cv.rho / static_cast<double>(nb > 1 ? nb : 1))));
-        G.lane = lane_id();
-        Dual A;
-        if (n_seeds < 0) {
-            A = dconst(__builtin_nan(""));
-        } else {
-            A = aa_magnification(G, F, Lst.seed_pool + uni(Lst.seed_off[item]), n_seeds);
+        const GradCtx G = grad_ctx(cv, ep.src[e], opt);
+        bool failed = n_seeds < 0;
+        Dual pts = dconst(0.0);
+        F.overflow = false;
+        F.n_runs = 0;
+        if (!failed) {
+            pts = aa_enumerate(G, F, Lst.seed_pool + uni(Lst.seed_off[item]), n_seeds);
+            failed = F.overflow;
         }
         const bool cap = F.overflow && F.n_runs >= F.run_cap;
         if (cap && !retry) {
             const int pos = wave_fetch_inc(&ep.counters[kQGradRetryCount]);
             wave_store(&Lst.cart_retry[pos], item);
-        } else {
             wave_sync();
-            store_grad(ep, mag_aa, e, A);  // uniform values, stored from every lane
-            if (F.overflow) atomicOr(&ep.counters[kQFlags], cap ? kErrPolarCap : kErrBudget);
+            continue;
         }
+        if (F.overflow) atomicOr(&ep.counters[kQFlags], cap ? kErrPolarCap : kErrBudget);
+        int run_base = -1, chunk_base = 0;
+        const int n = uni(F.n_runs);
+        const long long cells = uni(F.cells);
+        const int n_chunks = static_cast<int>((cells + kWave - 1) / kWave);
+        if (!failed) {
+            run_base = wave_fetch_add(&ep.counters[kQGradRuns], static_cast<unsigned>(n));
+            chunk_base = wave_fetch_add(&ep.counters[kQGradChunks], static_cast<unsigned>(n_chunks));
+            // every allocated chunk inside the pool names its item, also when the
+            // item fails here (k_grad_cells then skips it)
+            const unsigned end = static_cast<unsigned>(chunk_base) + static_cast<unsigned>(n_chunks);
+            const int lim = static_cast<int>((end < W.chunk_cap ? end : W.chunk_cap) - static_cast<unsigned>(chunk_base));
+            for (int k = G.lane; k < lim; k += kWave) W.chunk_item[chunk_base + k] = item;
+            if (static_cast<unsigned>(run_base) + static_cast<unsigned>(n) > W.run_cap || end > W.chunk_cap ||
+                cells > 0x7fffffffll) {
+                atomicOr(&ep.counters[kQFlags], kErrGradPool);
+                failed = true;
+                run_base = -1;
+            }
+        }
+        if (!failed) {
+            // the runs, each with the count of the item's cells before it
+            int before = 0;
+            for (int q0 = 0; q0 < n; q0 += kWave) {
+                const int q = q0 + G.lane;
+                int len = 0;
+                if (q < n) {
+                    const int row = F.runs[4 * q + 0];
+                    const int l = F.runs[4 * q + 1];
+                    const int r = F.runs[4 * q + 2];
+                    len = r - l + 1;
+                    int* dst = W.runs + 4 * static_cast<size_t>(run_base + q);
+                    dst[0] = row;
+                    dst[1] = l;
+                    dst[2] = r;
+                    dst[3] = item;
+                }
+                int total = 0;
+                const int off = wave_exclusive_scan(len, &total);
+                if (q < n) W.run_cells[run_base + q] = before + off;
+                before = uni(before + total);
+            }
+        }
+        wave_sync();
+        wave_store(&W.run_off[item], run_base);
+        wave_store(&W.run_n[item], n);
+        wave_store(&W.chunk_off[item], chunk_base);
+        wave_store(&W.cells[item], static_cast<int>(failed ? 0 : cells));
+        wave_store(&W.h[item], G.h);
+        wave_store(&W.points[static_cast<size_t>(item) * kGradSlots], pts.v);
+#pragma unroll
+        for (int k = 0; k < kTangents; ++k)
+            wave_store(&W.points[static_cast<size_t>(item) * kGradSlots + 1 + k], pts.d[k]);
+        wave_sync();
+    }
+}
+
+// Weights of every chunk of 64 enumerated cells, one wave per chunk
+// (persistent waves): each lane finds its cell's run by binary search over the
+// item's runs, evaluates aa_weight, and the wave's sums go to W.partial.
+__global__ void __launch_bounds__(kWave) k_grad_cells(const CurveRec* __restrict__ curves, FsOptions opt,
+                                                       FsEpochs ep, IrLists Lst, GradWork W)
+{
+    const unsigned total = ep.counters[kQGradChunks];
+    const int n_chunks = uni(static_cast<int>(total < W.chunk_cap ? total : W.chunk_cap));
+    for (int c = static_cast<int>(blockIdx.x); c < n_chunks; c += static_cast<int>(gridDim.x)) {
+        const int item = uni(W.chunk_item[c]);
+        const int run_off = uni(W.run_off[item]);
+        if (run_off < 0) continue;  // the item failed after its chunks were counted
+        const int run_n = uni(W.run_n[item]);
+        const int cells = uni(W.cells[item]);
+        const int t = (c - uni(W.chunk_off[item])) * kWave + lane_id();
+        const int64_t e = Lst.ir_sorted[item];
+        const CurveRec cv = curves[ep.curve_of[e]];
+        GradCtx G = grad_ctx(cv, ep.src[e], opt);
+        Dual w = dconst(0.0);
+        if (t < cells) {
+            // the last run whose first cell index is <= t
+            int lo = 0, hi = run_n - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (W.run_cells[run_off + mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            const int* run = W.runs + 4 * static_cast<size_t>(run_off + lo);
+            const int i = run[1] + (t - W.run_cells[run_off + lo]);
+            w = aa_weight(G, i, run[0]);
+        }
+        double sums[kGradSlots];
+        sums[0] = wave_sum(w.v);
+#pragma unroll
+        for (int k = 0; k < kTangents; ++k) sums[1 + k] = wave_sum(w.d[k]);
+#pragma unroll
+        for (int k = 0; k < kGradSlots; ++k) wave_store(&W.partial[static_cast<size_t>(c) * kGradSlots + k], sums[k]);
+    }
+}
+
+// The antialiased magnification of each inverse-ray item: its chunks' sums,
+// lane-strided in chunk order and reduced over the wave (a fixed order, so
+// repeated runs agree bitwise), sum W h^2 / (pi rho^2 (1 - c/3)) plus the
+// unresolved point images.
+__global__ void __launch_bounds__(kWave) k_grad_finish(const CurveRec* __restrict__ curves, FsOptions opt,
+                                                        FsEpochs ep, IrLists Lst, GradWork W, double* mag_aa)
+{
+    const int count = uni(static_cast<int>(ep.counters[kQIrCount]));
+    for (int item = static_cast<int>(blockIdx.x); item < count; item += static_cast<int>(gridDim.x)) {
+        const int64_t e = Lst.ir_sorted[item];
+        const int run_off = uni(W.run_off[item]);
+        if (run_off < 0) {
+            wave_sync();
+            store_grad(ep, mag_aa, e, dconst(__builtin_nan("")));
+            wave_sync();
+            continue;
+        }
+        const int c0 = uni(W.chunk_off[item]);
+        const int nc = uni((W.cells[item] + kWave - 1) / kWave);
+        double acc[kGradSlots];
+#pragma unroll
+        for (int k = 0; k < kGradSlots; ++k) acc[k] = 0.0;
+        for (int c = lane_id(); c < nc; c += kWave) {
+#pragma unroll
+            for (int k = 0; k < kGradSlots; ++k) acc[k] += W.partial[static_cast<size_t>(c0 + c) * kGradSlots + k];
+        }
+        Dual total;
+        total.v = wave_sum(acc[0]);
+#pragma unroll
+        for (int k = 0; k < kTangents; ++k) total.d[k] = wave_sum(acc[1 + k]);
+        const CurveRec cv = curves[ep.curve_of[e]];
+        const GradCtx G = grad_ctx(cv, ep.src[e], opt);
+        const double h = uni(W.h[item]);
+        Dual pts;
+        pts.v = uni(W.points[static_cast<size_t>(item) * kGradSlots]);
+#pragma unroll
+        for (int k = 0; k < kTangents; ++k) pts.d[k] = uni(W.points[static_cast<size_t>(item) * kGradSlots + 1 + k]);
+        const Dual flux = (1.0 - G.c * (1.0 / 3.0)) * (kPi * G.rho * G.rho);
+        const Dual A = total * (h * h) / flux + pts;
+        wave_sync();
+        store_grad(ep, mag_aa, e, A);  // uniform values, stored from every lane
         wave_sync();
     }
 }
@@ -645,15 +799,21 @@ __global__ void __launch_bounds__(256) k_grad_chain(const CurveRec* __restrict__
 
 lcbg_status launch_fs_grad(const CurveRec* d_curves, int n_curves, const FsOptions& opt,
                            const FsEpochs& ep, const IrLists& lists, const IrScratch& scr,
-                           double* mag_aa, double* dmag, const double* flux, const double* sigma,
-                           double fs, double fb, double* chi2, double* grad, hipStream_t st)
+                           const GradWork& work, int n_cus, double* mag_aa, double* dmag,
+                           const double* flux, const double* sigma, double fs, double fb,
+                           double* chi2, double* grad, hipStream_t st)
 {
     const unsigned blocks = static_cast<unsigned>((ep.n + 255) / 256);
     hipLaunchKernelGGL(k_grad_point, dim3(blocks), dim3(256), 0, st, d_curves, ep, mag_aa);
     hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(scr.polar.n_slots)), dim3(kWave), 0, st,
-                       d_curves, opt, ep, lists, scr.polar, int{kPassSmall}, mag_aa);
+                       d_curves, opt, ep, lists, scr.polar, int{kPassSmall}, work);
     hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(scr.polar_large.n_slots)), dim3(kWave), 0,
-                       st, d_curves, opt, ep, lists, scr.polar_large, int{kPassRetry}, mag_aa);
+                       st, d_curves, opt, ep, lists, scr.polar_large, int{kPassRetry}, work);
+    // chunks across the device (16 waves per CU), then the per-item sums
+    hipLaunchKernelGGL(k_grad_cells, dim3(static_cast<unsigned>(16 * (n_cus > 0 ? n_cus : 256))), dim3(kWave),
+                       0, st, d_curves, opt, ep, lists, work);
+    hipLaunchKernelGGL(k_grad_finish, dim3(static_cast<unsigned>(4 * (n_cus > 0 ? n_cus : 256))), dim3(kWave),
+                       0, st, d_curves, opt, ep, lists, work, mag_aa);
     hipLaunchKernelGGL(k_grad_chain, dim3(static_cast<unsigned>(n_curves)), dim3(256), 0, st, d_curves,
                        ep, flux, sigma, fs, fb, dmag, chi2, grad);
     return check_launch();

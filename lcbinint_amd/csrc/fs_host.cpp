@@ -89,7 +89,9 @@ lcbg_status ensure_epochs(FsState* fs, int64_t n)
                               carve_size(cap * sizeof(int64_t)) + carve_size(cap * sizeof(int)) * 9 +
                               carve_size(pool_cap_for(cap) * sizeof(Pt2)) +
                               carve_size(cap * kTangentsHost * sizeof(double)) +
-                              carve_size(cap * sizeof(int64_t)) * 3 + carve_size(cap * sizeof(double));
+                              carve_size(cap * sizeof(int64_t)) * 3 + carve_size(cap * sizeof(double)) +
+                              carve_size(cap * sizeof(int)) * 4 + carve_size(cap * sizeof(double)) +
+                              carve_size(cap * kGradSlots * sizeof(double));
     lcbg_status st = map_hip(hipMalloc(&fs->epoch_mem, bytes));
     if (st != LCBG_OK) {
         fs->epoch_mem = nullptr;
@@ -231,6 +233,20 @@ double* err_view(FsState* fs)
     return carve<double>(p, fs->epoch_cap);
 }
 
+// Per-item fields of the gradient's quadrature work, after the error estimates.
+void grad_item_view(FsState* fs, GradWork& w)
+{
+    char* p = reinterpret_cast<char*>(err_view(fs));
+    const int64_t cap = fs->epoch_cap;
+    (void)carve<double>(p, cap);
+    w.run_off = carve<int>(p, cap);
+    w.run_n = carve<int>(p, cap);
+    w.chunk_off = carve<int>(p, cap);
+    w.cells = carve<int>(p, cap);
+    w.h = carve<double>(p, cap);
+    w.points = carve<double>(p, cap * kGradSlots);
+}
+
 // The status a run's outcome maps to: a method outside this library is
 // unsupported, a team wait that ran out of time a device failure, and any
 // other abandoned epoch (its magnification is NaN) a numerical error, as a
@@ -252,7 +268,36 @@ struct ScratchPool {
     int cus = 0;
     void* mem = nullptr;
     hipEvent_t last_use = nullptr;
+    GradWork grad{};       // pools of the gradient quadrature (first gradient run)
+    void* grad_mem = nullptr;
 };
+
+// Runs and 64-cell chunks the gradient's inverse-ray quadrature may hold in one
+// run over all its items: 2^24 each (1.3 GB).
+constexpr unsigned kGradRunCap = 1u << 24;
+constexpr unsigned kGradChunkCap = 1u << 24;
+
+lcbg_status ensure_grad_pool(ScratchPool& pool)
+{
+    if (pool.grad_mem) return LCBG_OK;
+    const std::size_t r = kGradRunCap, c = kGradChunkCap;
+    const std::size_t bytes = carve_size(r * 4 * sizeof(int)) + carve_size(r * sizeof(int)) +
+                              carve_size(c * sizeof(int)) + carve_size(c * kGradSlots * sizeof(double));
+    lcbg_status st = map_hip(hipMalloc(&pool.grad_mem, bytes));
+    if (st != LCBG_OK) {
+        pool.grad_mem = nullptr;
+        return st;
+    }
+    char* p = static_cast<char*>(pool.grad_mem);
+    GradWork& w = pool.grad;
+    w.runs = carve<int>(p, r * 4);
+    w.run_cells = carve<int>(p, r);
+    w.chunk_item = carve<int>(p, c);
+    w.partial = carve<double>(p, c * kGradSlots);
+    w.run_cap = kGradRunCap;
+    w.chunk_cap = kGradChunkCap;
+    return LCBG_OK;
+}
 
 constexpr int kMaxDevices = 64;
 ScratchPool g_pool[kMaxDevices];
@@ -617,9 +662,14 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     if (st == LCBG_OK)
         st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, lists_view(fs), scr, stream, fs->ir);
     stage_done("stage 3 (inverse ray)");
-    if (ga && st == LCBG_OK)
-        st = launch_fs_grad(b->d_curves, b->n_curves, fs->opt, ep, lists_view(fs), pool.scr, ga->mag_aa,
-                            ga->dmag, ga->flux, ga->sigma, ga->fs, ga->fb, ga->chi2, ga->grad, stream);
+    if (ga && st == LCBG_OK) st = ensure_grad_pool(pool);
+    if (ga && st == LCBG_OK) {
+        GradWork work = pool.grad;
+        grad_item_view(fs, work);
+        st = launch_fs_grad(b->d_curves, b->n_curves, fs->opt, ep, lists_view(fs), pool.scr, work, pool.cus,
+                            ga->mag_aa, ga->dmag, ga->flux, ga->sigma, ga->fs, ga->fb, ga->chi2, ga->grad,
+                            stream);
+    }
     if (st == LCBG_OK) st = launch_first_nonfinite(mag, n_epochs, ep.counters, stream);
     if (st == LCBG_OK && !capture) st = map_hip(hipEventRecord(pool.last_use, stream));
     if (stage_ms && st == LCBG_OK) {

@@ -90,8 +90,9 @@ lcbg_status launch_ir_epoch_debug(const CurveRec* d_curves, const FsCache& cache
 // dmag, flux/sigma/chi2/grad may be null.
 lcbg_status launch_fs_grad(const CurveRec* d_curves, int n_curves, const FsOptions& opt,
                            const FsEpochs& ep, const IrLists& lists, const IrScratch& scr,
-                           double* mag_aa, double* dmag, const double* flux, const double* sigma,
-                           double fs, double fb, double* chi2, double* grad, hipStream_t st);
+                           const GradWork& work, int n_cus, double* mag_aa, double* dmag,
+                           const double* flux, const double* sigma, double fs, double fb,
+                           double* chi2, double* grad, hipStream_t st);
 
 // Workgroups of one inverse-ray kernel resident per CU (kind 0 seeds,
 // 1 Cartesian, 2 polar), from its register and LDS use.

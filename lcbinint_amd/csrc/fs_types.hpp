@@ -209,9 +209,12 @@ struct IrLists {
 constexpr int kWorkOffset = 28;
 constexpr int kBucketOffset = 44;
 // counters[kFirstFailWord]: n - e of the first epoch e whose magnification is
-// not finite (0: none), written by the run's last kernel.
+// not finite (0: none), written by the run's last kernel.  Then the cursors of
+// the gradient's run and chunk pools (GradWork).
 constexpr int kFirstFailWord = kBucketOffset + 2 * kCostBuckets;
-constexpr int kCounterWords = kFirstFailWord + 1;
+constexpr int kQGradRuns = kFirstFailWord + 1;
+constexpr int kQGradChunks = kFirstFailWord + 2;
+constexpr int kCounterWords = kFirstFailWord + 3;
 enum : int {
     kQNearCount = 0, kQNearCursor = 1, kQIrCount = 2, kQIrCursor = 3, kQFlags = 4,
     kQCartCount = 5, kQCartCursor = 6, kQPolarCount = 7, kQPolarCursor = 8,
@@ -223,6 +226,28 @@ enum : int {
 };
 // derivatives per epoch: t0, tE, u0, alpha, s, q, rho, u_LD
 constexpr int kGradParams = 8;
+// value and six tangents of a magnification in the gradient kernels
+constexpr int kGradSlots = 7;
+
+// Work of the gradient's antialiased inverse-ray quadrature.  k_grad_ir
+// enumerates each inverse-ray item's cells as runs along lattice rows and
+// appends them to the run pool, with the 64-cell chunks that cover them to the
+// chunk pool; k_grad_cells evaluates the weights of every chunk across the
+// device; k_grad_finish sums each item's chunks in order.
+struct GradWork {
+    int* runs;         // [run_cap][4]: row, first cell, last cell, item
+    int* run_cells;    // [run_cap] cells of the item's earlier runs
+    int* chunk_item;   // [chunk_cap] item of each chunk
+    double* partial;   // [chunk_cap][kGradSlots] weight sums of each chunk
+    unsigned run_cap, chunk_cap;
+    // per inverse-ray item, in the epoch workspace
+    int* run_off;      // first run in the pool (-1: the item failed)
+    int* run_n;
+    int* chunk_off;    // first chunk in the pool
+    int* cells;
+    double* h;         // lattice cell side
+    double* points;    // [item][kGradSlots] unresolved point images
+};
 // queued epochs of one curve per stage-2 distance work item
 constexpr int kNearChunk = 128;
 // leading inverse-ray items searched first and scanned on large slots at once
@@ -241,6 +266,7 @@ constexpr unsigned kErrUnsupported = 8u;
 constexpr unsigned kErrBudget = 16u;
 constexpr unsigned kErrSeedPool = 32u;
 constexpr unsigned kErrTeam = 64u;  // a Cartesian team wait ran out of time
+constexpr unsigned kErrGradPool = 128u;  // the gradient's run or chunk pool was full
 
 // Work budget of one inverse-ray epoch in wave steps (64-cell chunks, probe
 // merges, polar runs); an epoch that exceeds it ends with an error flag, so
