@@ -573,7 +573,7 @@ __global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ 
     for (;;) {
         const int qi = next_item(cursor);
         if (qi >= count) break;
-        const int item = retry ? uni(Lst.cart_retry[qi]) : qi;
+        const int item = retry ? uni(Lst.grad_retry[qi]) : qi;
         const int64_t e = Lst.ir_sorted[item];
         const int n_seeds = uni(Lst.seed_n[item]);
         const CurveRec cv = curves[ep.curve_of[e]];
@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ 
         const bool cap = F.overflow && F.n_runs >= F.run_cap;
         if (cap && !retry) {
             const int pos = wave_fetch_inc(&ep.counters[kQGradRetryCount]);
-            wave_store(&Lst.cart_retry[pos], item);
+            wave_store(&Lst.grad_retry[pos], item);
             wave_sync();
             continue;
         }
@@ -797,18 +797,24 @@ __global__ void __launch_bounds__(256) k_grad_chain(const CurveRec* __restrict__
     }
 }
 
-lcbg_status launch_fs_grad(const CurveRec* d_curves, int n_curves, const FsOptions& opt,
-                           const FsEpochs& ep, const IrLists& lists, const IrScratch& scr,
-                           const GradWork& work, int n_cus, double* mag_aa, double* dmag,
-                           const double* flux, const double* sigma, double fs, double fb,
-                           double* chi2, double* grad, hipStream_t st)
+lcbg_status launch_grad_enumerate(const CurveRec* d_curves, const FsOptions& opt, const FsEpochs& ep,
+                                  const IrLists& lists, const PolarScratch& slots,
+                                  const PolarScratch& large_slots, const GradWork& work, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(slots.n_slots)), dim3(kWave), 0, st, d_curves,
+                       opt, ep, lists, slots, int{kPassSmall}, work);
+    hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(large_slots.n_slots)), dim3(kWave), 0, st,
+                       d_curves, opt, ep, lists, large_slots, int{kPassRetry}, work);
+    return check_launch();
+}
+
+lcbg_status launch_grad_rest(const CurveRec* d_curves, int n_curves, const FsOptions& opt,
+                             const FsEpochs& ep, const IrLists& lists, const GradWork& work, int n_cus,
+                             double* mag_aa, double* dmag, const double* flux, const double* sigma,
+                             double fs, double fb, double* chi2, double* grad, hipStream_t st)
 {
     const unsigned blocks = static_cast<unsigned>((ep.n + 255) / 256);
     hipLaunchKernelGGL(k_grad_point, dim3(blocks), dim3(256), 0, st, d_curves, ep, mag_aa);
-    hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(scr.polar.n_slots)), dim3(kWave), 0, st,
-                       d_curves, opt, ep, lists, scr.polar, int{kPassSmall}, work);
-    hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(scr.polar_large.n_slots)), dim3(kWave), 0,
-                       st, d_curves, opt, ep, lists, scr.polar_large, int{kPassRetry}, work);
     // chunks across the device (16 waves per CU), then the per-item sums
     hipLaunchKernelGGL(k_grad_cells, dim3(static_cast<unsigned>(16 * (n_cus > 0 ? n_cus : 256))), dim3(kWave),
                        0, st, d_curves, opt, ep, lists, work);

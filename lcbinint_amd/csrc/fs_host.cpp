@@ -86,7 +86,7 @@ lcbg_status ensure_epochs(FsState* fs, int64_t n)
                               carve_size(cap * 10 * sizeof(double)) +
                               carve_size(cap * sizeof(int64_t)) * 2 +
                               carve_size(kCounterAlloc * sizeof(unsigned)) +
-                              carve_size(cap * sizeof(int64_t)) + carve_size(cap * sizeof(int)) * 9 +
+                              carve_size(cap * sizeof(int64_t)) + carve_size(cap * sizeof(int)) * 10 +
                               carve_size(pool_cap_for(cap) * sizeof(Pt2)) +
                               carve_size(cap * kTangentsHost * sizeof(double)) +
                               carve_size(cap * sizeof(int64_t)) * 3 + carve_size(cap * sizeof(double)) +
@@ -178,6 +178,7 @@ IrLists lists_view(FsState* fs)
     L.polar_retry = carve<int>(p, cap);
     L.cart_big = carve<int>(p, cap);
     L.polar_big = carve<int>(p, cap);
+    L.grad_retry = carve<int>(p, cap);
     L.pool_cap = static_cast<unsigned>(pool_cap_for(cap));
     L.seed_pool = carve<Pt2>(p, L.pool_cap);
     return L;
@@ -196,7 +197,7 @@ double* dint_view(FsState* fs)
     (void)carve<int64_t>(p, cap);
     (void)carve<unsigned>(p, kCounterAlloc);
     (void)carve<int64_t>(p, cap);
-    for (int k = 0; k < 9; ++k) (void)carve<int>(p, cap);
+    for (int k = 0; k < 10; ++k) (void)carve<int>(p, cap);
     (void)carve<Pt2>(p, pool_cap_for(cap));
     return carve<double>(p, cap * kTangentsHost);
 }
@@ -269,6 +270,7 @@ struct ScratchPool {
     void* mem = nullptr;
     hipEvent_t last_use = nullptr;
     GradWork grad{};       // pools of the gradient quadrature (first gradient run)
+    PolarScratch grad_slots{}, grad_large{};  // its enumeration slots
     void* grad_mem = nullptr;
 };
 
@@ -281,8 +283,18 @@ lcbg_status ensure_grad_pool(ScratchPool& pool)
 {
     if (pool.grad_mem) return LCBG_OK;
     const std::size_t r = kGradRunCap, c = kGradChunkCap;
+    // enumeration slots: one per CU with 2^16 runs, 16 with 2^20
+    pool.grad_slots.n_slots = pool.cus;
+    pool.grad_slots.run_cap = kRunCapSmall;
+    pool.grad_large.n_slots = 16;
+    pool.grad_large.run_cap = kRunCapLarge;
+    auto slot_bytes = [](const PolarScratch& ps) {
+        const std::size_t n = static_cast<std::size_t>(ps.n_slots);
+        return carve_size(n * kPhiHash * sizeof(int)) + carve_size(n * ps.run_cap * 4 * sizeof(int));
+    };
     const std::size_t bytes = carve_size(r * 4 * sizeof(int)) + carve_size(r * sizeof(int)) +
-                              carve_size(c * sizeof(int)) + carve_size(c * kGradSlots * sizeof(double));
+                              carve_size(c * sizeof(int)) + carve_size(c * kGradSlots * sizeof(double)) +
+                              slot_bytes(pool.grad_slots) + slot_bytes(pool.grad_large);
     lcbg_status st = map_hip(hipMalloc(&pool.grad_mem, bytes));
     if (st != LCBG_OK) {
         pool.grad_mem = nullptr;
@@ -296,7 +308,15 @@ lcbg_status ensure_grad_pool(ScratchPool& pool)
     w.partial = carve<double>(p, c * kGradSlots);
     w.run_cap = kGradRunCap;
     w.chunk_cap = kGradChunkCap;
-    return LCBG_OK;
+    for (PolarScratch* ps : {&pool.grad_slots, &pool.grad_large}) {
+        const std::size_t n = static_cast<std::size_t>(ps->n_slots);
+        ps->phi_head = carve<int>(p, n * kPhiHash);
+        ps->runs = carve<int>(p, n * ps->run_cap * 4);
+        // empty run chains; the kernel restores them per item
+        st = map_hip(hipMemset(ps->phi_head, 0xff, n * kPhiHash * sizeof(int)));
+        if (st != LCBG_OK) return st;
+    }
+    return map_hip(hipDeviceSynchronize());
 }
 
 constexpr int kMaxDevices = 64;
@@ -439,8 +459,9 @@ void fs_destroy(FsState* fs)
     if (fs->ir.polar) (void)hipStreamDestroy(fs->ir.polar);
     if (fs->ir.polar_small) (void)hipStreamDestroy(fs->ir.polar_small);
     if (fs->ir.big) (void)hipStreamDestroy(fs->ir.big);
+    if (fs->ir.grad) (void)hipStreamDestroy(fs->ir.grad);
     for (hipEvent_t e : {fs->ir.fork, fs->ir.fork_lead, fs->ir.join_polar, fs->ir.join_polar_small, fs->ir.join_big,
-                         fs->ir.cache_fork, fs->ir.cache_join})
+                         fs->ir.cache_fork, fs->ir.cache_join, fs->ir.join_grad})
         if (e) (void)hipEventDestroy(e);
     delete fs;
 }
@@ -503,8 +524,9 @@ lcbg_status fs_prepare(lcbg_batch* b)
     st = map_hip(hipStreamCreateWithFlags(&fs->ir.polar, hipStreamNonBlocking));
     if (st == LCBG_OK) st = map_hip(hipStreamCreateWithFlags(&fs->ir.polar_small, hipStreamNonBlocking));
     if (st == LCBG_OK) st = map_hip(hipStreamCreateWithFlags(&fs->ir.big, hipStreamNonBlocking));
+    if (st == LCBG_OK) st = map_hip(hipStreamCreateWithFlags(&fs->ir.grad, hipStreamNonBlocking));
     for (hipEvent_t* e : {&fs->ir.fork, &fs->ir.fork_lead, &fs->ir.join_polar, &fs->ir.join_polar_small,
-                          &fs->ir.join_big, &fs->ir.cache_fork, &fs->ir.cache_join})
+                          &fs->ir.join_big, &fs->ir.cache_fork, &fs->ir.cache_join, &fs->ir.join_grad})
         if (st == LCBG_OK) st = map_hip(hipEventCreateWithFlags(e, hipEventDisableTiming));
     return st;
 }
@@ -619,6 +641,7 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     ScratchPool& pool = g_pool[b->device % kMaxDevices];
     std::lock_guard<std::mutex> lock(pool.mu);
     st = ensure_scratch(pool);
+    if (st == LCBG_OK && ga) st = ensure_grad_pool(pool);
     if (st != LCBG_OK) return st;
     // LCBG_STAGE_SYNC=1: wait after every stage and report it on stderr
     // (a diagnostic in the spirit of the reference's LCBININT_AREA_DIAGNOSTICS)
@@ -662,13 +685,21 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     if (st == LCBG_OK)
         st = launch_fs_ir(b->d_curves, fs->cache, fs->tab, fs->opt, ep, lists_view(fs), scr, stream, fs->ir);
     stage_done("stage 3 (inverse ray)");
-    if (ga && st == LCBG_OK) st = ensure_grad_pool(pool);
     if (ga && st == LCBG_OK) {
+        // the cell enumeration of the gradient quadrature needs only the seeds:
+        // it runs on its own stream from the end of the seed search on (the
+        // ir.fork event), beside the area passes, and the rest joins after both
         GradWork work = pool.grad;
         grad_item_view(fs, work);
-        st = launch_fs_grad(b->d_curves, b->n_curves, fs->opt, ep, lists_view(fs), pool.scr, work, pool.cus,
-                            ga->mag_aa, ga->dmag, ga->flux, ga->sigma, ga->fs, ga->fb, ga->chi2, ga->grad,
-                            stream);
+        st = map_hip(hipStreamWaitEvent(fs->ir.grad, fs->ir.fork, 0));
+        if (st == LCBG_OK)
+            st = launch_grad_enumerate(b->d_curves, fs->opt, ep, lists_view(fs), pool.grad_slots, pool.grad_large,
+                                       work, fs->ir.grad);
+        if (st == LCBG_OK) st = map_hip(hipEventRecord(fs->ir.join_grad, fs->ir.grad));
+        if (st == LCBG_OK) st = map_hip(hipStreamWaitEvent(stream, fs->ir.join_grad, 0));
+        if (st == LCBG_OK)
+            st = launch_grad_rest(b->d_curves, b->n_curves, fs->opt, ep, lists_view(fs), work, pool.cus, ga->mag_aa,
+                                  ga->dmag, ga->flux, ga->sigma, ga->fs, ga->fb, ga->chi2, ga->grad, stream);
     }
     if (st == LCBG_OK) st = launch_first_nonfinite(mag, n_epochs, ep.counters, stream);
     if (st == LCBG_OK && !capture) st = map_hip(hipEventRecord(pool.last_use, stream));

@@ -50,8 +50,10 @@ struct IrStreams {
     hipStream_t polar;        // polar large-slot pass of the largest polar epochs
     hipStream_t polar_small;  // polar small-slot pass
     hipStream_t big;          // Cartesian large-slot pass of the largest epochs
+    hipStream_t grad;         // gradient cell enumeration, beside the area passes
     hipEvent_t fork, fork_lead, join_polar, join_polar_small, join_big;
     hipEvent_t cache_fork, cache_join;  // branch caches on `big` beside stage 1
+    hipEvent_t join_grad;               // end of the gradient cell enumeration
     // per-launch timing (profiled runs only, else null): events 2k and 2k+1
     // bracket launch k on the stream it runs on, in the order of IrLaunch
     hipEvent_t* kev;
@@ -84,15 +86,20 @@ lcbg_status launch_ir_epoch_debug(const CurveRec* d_curves, const FsCache& cache
                                   int64_t e, int method, int last_phase, Pt2* seeds_io, int n_in,
                                   int* n_out, double* mag_out, hipStream_t st);
 
-// Gradient kernels after a value run: tangents of every epoch (point source,
-// hexadecapole, antialiased inverse ray), then per-epoch derivatives dmag
-// [n][8] and, if chi2 is not null, chi^2 and its gradient per curve.  mag_aa,
-// dmag, flux/sigma/chi2/grad may be null.
-lcbg_status launch_fs_grad(const CurveRec* d_curves, int n_curves, const FsOptions& opt,
-                           const FsEpochs& ep, const IrLists& lists, const IrScratch& scr,
-                           const GradWork& work, int n_cus, double* mag_aa, double* dmag,
-                           const double* flux, const double* sigma, double fs, double fb,
-                           double* chi2, double* grad, hipStream_t st);
+// Gradient kernels of a value run.  launch_grad_enumerate needs the seeds of
+// the inverse-ray items (after the seed search) and runs on its own slots
+// (small, then large ones for the items that outgrow them);
+// launch_grad_rest, after it and after the value run, computes the tangents of
+// every epoch (point source, hexadecapole, antialiased inverse ray), then
+// per-epoch derivatives dmag [n][8] and, if chi2 is not null, chi^2 and its
+// gradient per curve.  mag_aa, dmag, flux/sigma/chi2/grad may be null.
+lcbg_status launch_grad_enumerate(const CurveRec* d_curves, const FsOptions& opt, const FsEpochs& ep,
+                                  const IrLists& lists, const PolarScratch& slots,
+                                  const PolarScratch& large_slots, const GradWork& work, hipStream_t st);
+lcbg_status launch_grad_rest(const CurveRec* d_curves, int n_curves, const FsOptions& opt,
+                             const FsEpochs& ep, const IrLists& lists, const GradWork& work, int n_cus,
+                             double* mag_aa, double* dmag, const double* flux, const double* sigma,
+                             double fs, double fb, double* chi2, double* grad, hipStream_t st);
 
 // Workgroups of one inverse-ray kernel resident per CU (kind 0 seeds,
 // 1 Cartesian, 2 polar), from its register and LDS use.

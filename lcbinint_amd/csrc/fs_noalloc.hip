@@ -130,6 +130,7 @@ std::size_t na_layout(std::uintptr_t base, NaViews& v)
     L.polar_retry = bump<int>(p, cap);
     L.cart_big = bump<int>(p, cap);
     L.polar_big = bump<int>(p, cap);
+    L.grad_retry = bump<int>(p, cap);
     L.pool_cap = kNaPoolCap;
     L.seed_pool = bump<Pt2>(p, kNaPoolCap);
     IrScratch& s = v.scr;

@@ -195,6 +195,7 @@ struct IrLists {
     int* polar_retry;    // [cap] items that ran out of a small polar slot
     int* cart_big;       // [cap] largest Cartesian items, large slots from the start
     int* polar_big;      // [cap] largest polar items
+    int* grad_retry;     // [cap] items the gradient quadrature retries on large slots
     Pt2* seed_pool;      // [pool_cap]
     unsigned pool_cap;
 };
