@@ -279,39 +279,58 @@ def plain_per_epoch(capi, torch, grad):
     return len(times) / elapsed, meth.tolist()
 
 
-def bench_dynamic(capi, n_curves=128, steps=2):
-    """Light curves with annual parallax and circular orbital motion: the first
-    n_curves curves of lcbinint_amd.workloads.dynamic_batch (the 1024-curve
-    workload), substitute Earth table of tests/golden.  Value, and value plus
-    the derivatives with respect to 12 parameters; host trajectories included
-    in the timing (host arrays in and out)."""
+def bench_dynamic(capi, n_curves=1024, steps=1):
+    """Light curves with annual parallax and circular orbital motion: the
+    1024-curve workload of lcbinint_amd.workloads.dynamic_batch (500-3000
+    epochs per curve over 200 days), substitute Earth table of tests/golden.
+    Value, and value plus the derivatives with respect to 12 parameters, through
+    lcbg_batch_light_curves_dynamic_device (offsets and times resident on the
+    device; the per-curve trajectory constants computed on the host inside the
+    timed region).  The same curves with parallax alone (no orbital motion:
+    one caustic cache per curve, reused across calls) are timed beside."""
+    import torch
     from lcbinint_amd.workloads import dynamic_batch
     table = ROOT / "tests" / "golden" / "earth_ephemeris_substitute.txt"
     capi.load_earth_ephemeris(table)
-    curves, dyn, off, t = dynamic_batch()
-    curves, dyn = curves[:n_curves], dyn[:n_curves]
-    off = off[:n_curves + 1]
-    t = t[:off[-1]]
+    curves, dyn, off, t = dynamic_batch(n_curves=n_curves)
+    n = len(t)
     b = capi.Batch([capi.Curve(**c) for c in curves])
-    d = [capi.Dynamics(**x) for x in dyn]
-    capi.light_curves_dynamic(b, d, capi.ORBIT_CIRCULAR, off, t)  # warm-up
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        mag, meth = capi.light_curves_dynamic(b, d, capi.ORBIT_CIRCULAR, off, t)
-    value_rate = len(t) * steps / (time.perf_counter() - t0)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        _, _, _, dm = capi.light_curves_dynamic_grad(b, d, capi.ORBIT_CIRCULAR, off, t)
-    grad_rate = len(t) * steps / (time.perf_counter() - t0)
+    d = (capi.Dynamics * n_curves)(*[capi.Dynamics(**x) for x in dyn])
+    d_off = torch.from_numpy(off).cuda()
+    d_t = torch.from_numpy(t).cuda()
+    mag = torch.empty(n, dtype=torch.float64, device="cuda")
+    meth = torch.empty(n, dtype=torch.int32, device="cuda")
+    dm = torch.empty((n, 12), dtype=torch.float64, device="cuda")
+    sp = torch.cuda.current_stream().cuda_stream
+
+    def run(mode, grad):
+        capi.check(capi.lcbg_batch_light_curves_dynamic_device(
+            b.handle, d, mode, d_off.data_ptr(), d_t.data_ptr(), n, mag.data_ptr(), meth.data_ptr(), None,
+            dm.data_ptr() if grad else None, sp), "lcbg_batch_light_curves_dynamic_device")
+
+    out = {}
+    for mode, label in ((capi.ORBIT_CIRCULAR, "circular"), (capi.ORBIT_STATIC, "parallax_only")):
+        rates = {}
+        for grad in (False, True):
+            run(mode, grad)  # warm-up (workspace and, without orbital motion, caustic caches)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                run(mode, grad)
+            torch.cuda.synchronize()
+            rates["value_grad" if grad else "value"] = n * steps / (time.perf_counter() - t0)
+            finite = bool(torch.isfinite(mag).all()) and (not grad or bool(torch.isfinite(dm).all()))
+            if not finite:
+                raise RuntimeError(f"non-finite value or derivative in the dynamic workload ({label})")
+        out[label] = {"value_epochs_per_s": rates["value"], "value_grad_epochs_per_s": rates["value_grad"],
+                      "methods": np.bincount(meth.cpu().numpy(), minlength=4).tolist()}
     b.close()
-    if not (np.isfinite(mag).all() and np.isfinite(dm).all()):
-        raise RuntimeError("non-finite value or derivative in the dynamic workload")
     return {
-        "workload": f"annual parallax + circular orbital motion: first {n_curves} of 1024 curves "
-                    f"({len(t)} epochs over 200 days), rho log-uniform [1e-4, 1e-2], linear LD, nbin 50, "
-                    "piEN/piEE U(-0.5, 0.5), ds/dt, dalpha/dt U(-1, 1)/yr, seed 5",
-        "value_epochs_per_s": value_rate, "value_grad_epochs_per_s": grad_rate, "grad_params": 12,
-        "methods": np.bincount(meth, minlength=4).tolist(),
+        "workload": f"annual parallax + circular orbital motion: {n_curves} curves, {n} epochs over 200 days "
+                    "(500-3000 per curve), rho log-uniform [1e-4, 1e-2], linear LD U(0.1, 0.9), nbin 50, "
+                    "piEN/piEE U(-0.5, 0.5), ds/dt, dalpha/dt U(-1, 1)/yr, seed 5; device arrays",
+        "grad_params": 12,
+        **out,
     }
 
 

@@ -202,6 +202,22 @@ lcbg_status lcbg_dynamic_sources_jacobian(const lcbg_curve* curve, const lcbg_dy
                                           int32_t orbital_motion_mode, const double* times, int64_t n,
                                           double* xys, double* jac);
 
+/* lcbg_dynamic_sources for every curve of the batch, on the device: device
+ * arrays offsets (n_curves + 1), times, and outputs x, y, s (n_epochs each)
+ * and, if jac is not NULL, jac[27 i + 9 j + k] as lcbg_dynamic_sources_jacobian
+ * lays it out; dyn is a host array of n_curves records.  Per-curve constants
+ * come from the host (its libm, as lcbg_dynamic_sources); per epoch the
+ * parallax and the rotation run on the device, which reproduces the host's
+ * values bitwise without orbital motion and within a few ulp with it (the
+ * device libm's cos, sin and atan2 of the orbit phase).  Enqueued on `stream`
+ * and waited for; LCBG_NUMERICAL_ERROR if an orbit is not finite.  The device
+ * form of Trajectory::source_position and orbital_state over a batch
+ * (trajectory.cpp:436-484, orbital_motion.cpp:183-199). */
+lcbg_status lcbg_batch_dynamic_sources_device(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                              int32_t orbital_motion_mode, const int64_t* offsets,
+                                              const double* times, int64_t n_epochs, double* x, double* y,
+                                              double* s, double* jac, void* stream);
+
 /* Light curves of the batch's curves with annual parallax and/or orbital
  * motion (dyn: n_curves records), the LensModel path of
  * evaluate_binary_light_curve_numpy (python/lcbinint_pybind.cpp:1159-1173):
@@ -210,9 +226,15 @@ lcbg_status lcbg_dynamic_sources_jacobian(const lcbg_curve* curve, const lcbg_dy
  * (point_source_magnifier.cpp:723-778), those images as seed hints, and
  * FiniteSourceMagnifier::binary_mag on the caustic of that epoch's
  * separation.  Host pointers (offsets: n_curves + 1, times/mag/method:
- * n_epochs); per-curve source_bins as set on the batch.  The trajectories run
- * on the host, the magnifications on the device in chunks of up to 16384
- * epochs, each epoch with its own caustic cache. */
+ * n_epochs); per-curve source_bins as set on the batch.  The trajectories are
+ * evaluated on the device from per-curve constants the host computes (the
+ * same values as lcbg_dynamic_sources: bitwise without orbital motion, within
+ * a few ulp with it, whose per-epoch cos, sin and atan2 come from the device
+ * libm).  Without orbital motion the separation is constant and every curve
+ * keeps one caustic cache, reused by later calls while s and q are unchanged;
+ * with it, every epoch has its own caustic cache, in chunks of up to 65536
+ * epochs.  Device buffers are kept on the batch and grow only when a call
+ * holds more epochs or curves than any before (then allocated once). */
 lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const lcbg_dynamics* dyn,
                                             int32_t orbital_motion_mode, const int64_t* offsets,
                                             const double* times, int64_t n_epochs, double* mag,
@@ -232,7 +254,7 @@ lcbg_status lcbg_batch_light_curves_dynamic_results(lcbg_batch* batch, const lcb
 /* lcbg_batch_light_curves_dynamic with, for every epoch, the derivatives of
  * the magnification with respect to (t0, tE, u0, alpha, s, q, rho, u_LD, piEN,
  * piEE, g1, g2): dmag[12 i + k] (host, n_epochs x 12; no counterpart in the
- * reference).  The host trajectory is differentiated in forward mode (x, y
+ * reference).  The device trajectory is differentiated in forward mode (x, y
  * and s(t) with respect to t0, tE, u0, alpha, s, piEN, piEE, g1, g2; g3,
  * lom_szs, lom_ar, ra, dec and tfix held fixed), the magnification as in
  * lcbg_batch_finite_source_grad_device with respect to the source position,
@@ -244,6 +266,22 @@ lcbg_status lcbg_batch_light_curves_dynamic_grad(lcbg_batch* batch, const lcbg_d
                                                  int32_t orbital_motion_mode, const int64_t* offsets,
                                                  const double* times, int64_t n_epochs, double* mag,
                                                  int32_t* method, double* mag_aa, double* dmag);
+
+/* lcbg_batch_light_curves_dynamic(_grad) on device arrays (offsets, times,
+ * mag, method, mag_aa, dmag; dmag NULL: values only, mag_aa may be NULL),
+ * enqueued on `stream`; dyn is a host array of n_curves records, from which
+ * the host computes the per-curve trajectory constants.  The call waits for
+ * the trajectory kernel (a non-finite orbit returns LCBG_NUMERICAL_ERROR
+ * before any magnification is enqueued, as lens_model.cpp:117-120 raises);
+ * the rest is asynchronous, and non-finite magnifications are left in mag
+ * for the caller to test.  offsets are trusted as
+ * lcbg_batch_finite_source_device trusts them (no counterpart in the
+ * reference: the device form of the LensModel loop). */
+lcbg_status lcbg_batch_light_curves_dynamic_device(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                                   int32_t orbital_motion_mode, const int64_t* offsets,
+                                                   const double* times, int64_t n_epochs, double* mag,
+                                                   int32_t* method, double* mag_aa, double* dmag,
+                                                   void* stream);
 
 /* Finite-source light curves of every curve of the batch: the plain method of
  * FiniteSourceMagnifier::binary_mag (finite_source_magnifier.cpp:4633-4843) as

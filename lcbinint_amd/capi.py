@@ -279,6 +279,15 @@ lcbg_batch_light_curves_dynamic_grad = _sig("lcbg_batch_light_curves_dynamic_gra
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p)
+lcbg_batch_light_curves_dynamic_device = _sig("lcbg_batch_light_curves_dynamic_device", ctypes.c_int,
+                                              ctypes.c_void_p, ctypes.POINTER(Dynamics), ctypes.c_int32,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p)
+lcbg_batch_dynamic_sources_device = _sig("lcbg_batch_dynamic_sources_device", ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.POINTER(Dynamics), ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
 TRAJ_PARAMS = ("t0", "tE", "u0", "alpha", "s", "piEN", "piEE", "g1", "g2")
 DYN_GRAD_PARAMS = ("t0", "tE", "u0", "alpha", "s", "q", "rho", "ld_c", "piEN", "piEE", "g1", "g2")
 

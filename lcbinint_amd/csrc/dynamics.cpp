@@ -18,6 +18,7 @@
 //    164-178), VBM convention.
 #include "lcbinint_amd.h"
 #include "runtime.hpp"
+#include "traj.hpp"
 
 #include <algorithm>
 #include <array>
@@ -29,7 +30,6 @@
 #include <mutex>
 #include <sstream>
 #include <string>
-#include <thread>
 #include <vector>
 
 namespace lcbg {
@@ -63,6 +63,7 @@ V3 v_cross(V3 a, V3 b)
 std::mutex g_earth_mu;
 std::vector<EarthRow> g_earth;
 bool g_earth_loaded = false;
+int g_earth_generation = 0;  // tables read so far
 
 std::string trimmed(const std::string& s)
 {
@@ -120,6 +121,7 @@ const std::vector<EarthRow>& earth_table()
     if (!g_earth_loaded) {
         g_earth = read_table(std::getenv("LCBININT_EARTH_EPHEMERIS"));
         g_earth_loaded = true;
+        ++g_earth_generation;
     }
     return g_earth;
 }
@@ -308,7 +310,6 @@ public:
         *dbeta = -pi_e * dn + pi_n * de;
     }
 
-private:
     T emit_time(const T& obs) const
     {
         T emit = obs;
@@ -515,10 +516,112 @@ bool trajectory(const TrajIn<T>& P, const lcbg_dynamics& d, int orbit_mode, cons
     return ok;
 }
 
+// The per-curve constants of the device trajectory (traj.hpp), in U (double
+// or TDual<9>) from the host computation in T (double or Dual<9>).
+inline double to_u(double x, double*) { return x; }
+inline TDual<kTrajParams> to_u(const Dual<kTrajParams>& x, TDual<kTrajParams>*)
+{
+    TDual<kTrajParams> r;
+    r.v = x.v;
+    for (int k = 0; k < kTrajParams; ++k) r.d[k] = x.d[k];
+    return r;
+}
+
+template <class T, class U>
+void traj_consts(const TrajIn<T>& P, const lcbg_dynamics& d, int orbit_mode, TrajConst<U>* out)
+{
+    U* tag = nullptr;
+    TrajConst<U>& K = *out;
+    const bool parallax = d.piEN != 0.0 || d.piEE != 0.0;
+    K.t0 = to_u(P.t0, tag);
+    K.tE = to_u(P.tE, tag);
+    K.u0 = to_u(P.u0, tag);
+    K.cos_t = to_u(f_cos(P.alpha), tag);
+    K.sin_t = to_u(f_sin(P.alpha), tag);
+    K.parallax = parallax ? 1 : 0;
+    K.mode = orbit_mode;
+    K.offset = 0.0;
+    for (int i = 0; i < 3; ++i) K.event[i] = K.east[i] = K.north[i] = 0.0;
+    const U zero = to_u(T(0.0), tag);
+    K.eph_ref = K.ref_pn = K.ref_pe = K.ref_vn = K.ref_ve = zero;
+    K.piEN = to_u(P.piEN, tag);
+    K.piEE = to_u(P.piEE, tag);
+    if (parallax) {
+        const Parallax<T> px(earth_table(), d.ra, d.dec, d.tfix != 0.0 ? T(d.tfix) : P.t0);
+        K.offset = px.offset_;
+        for (int i = 0; i < 3; ++i) {
+            K.event[i] = px.event_[i];
+            K.east[i] = px.east_[i];
+            K.north[i] = px.north_[i];
+        }
+        K.eph_ref = to_u(px.eph_ref_, tag);
+        K.ref_pn = to_u(px.ref_pn_, tag);
+        K.ref_pe = to_u(px.ref_pe_, tag);
+        K.ref_vn = to_u(px.ref_vn_, tag);
+        K.ref_ve = to_u(px.ref_ve_, tag);
+    }
+    const OrbitIn<T> oin{P.g1, P.g2, d.g3, d.lom_szs, d.lom_ar};
+    const Orbit<T> orbit(orbit_mode, P.s, P.alpha, oin, d.tfix != 0.0 ? T(d.tfix) : P.t0);
+    K.sep = to_u(orbit.sep, tag);
+    K.angle = to_u(orbit.angle, tag);
+    K.t_ref = to_u(orbit.t_ref, tag);
+    K.w_orb = to_u(orbit.w_orb, tag);
+    K.phi0 = to_u(orbit.phi0, tag);
+    K.c_inc = to_u(orbit.c_inc, tag);
+    K.s_true = to_u(orbit.s_true, tag);
+    K.c_om = to_u(orbit.c_om, tag);
+    K.s_om = to_u(orbit.s_om, tag);
+    K.n = to_u(orbit.n, tag);
+    K.ecc = to_u(orbit.ecc, tag);
+    K.t_peri = to_u(orbit.t_peri, tag);
+    K.a = to_u(orbit.a, tag);
+    K.one_minus_e2 = to_u(orbit.one_minus_e2, tag);
+    for (int i = 0; i < 3; ++i) {
+        K.xa[i] = to_u(orbit.xa[i], tag);
+        K.ya[i] = to_u(orbit.ya[i], tag);
+    }
+}
+
 }  // namespace
 
-namespace {
-constexpr int kTrajParams = 9;  // t0, tE, u0, alpha, s, piEN, piEE, g1, g2
+// Per-curve constants of the device trajectory: values, and with jac the
+// derivatives with respect to t0, tE, u0, alpha, s, piEN, piEE, g1, g2.
+void trajectory_constants(const lcbg_curve& cv, const lcbg_dynamics& d, int orbit_mode, TrajConst<double>* val_out,
+                          TrajConst<TDual<kTrajParams>>* dual_out)
+{
+    if (val_out) {
+        const TrajIn<double> P{cv.t0, cv.tE, cv.u0, cv.alpha, cv.s, d.piEN, d.piEE, d.g1, d.g2};
+        traj_consts(P, d, orbit_mode, val_out);
+    }
+    if (dual_out) {
+        using D = Dual<kTrajParams>;
+        TrajIn<D> P;
+        const double seed[kTrajParams] = {cv.t0, cv.tE, cv.u0, cv.alpha, cv.s, d.piEN, d.piEE, d.g1, d.g2};
+        D* fields[kTrajParams] = {&P.t0, &P.tE, &P.u0, &P.alpha, &P.s, &P.piEN, &P.piEE, &P.g1, &P.g2};
+        for (int k = 0; k < kTrajParams; ++k) {
+            *fields[k] = D(seed[k]);
+            fields[k]->d[k] = 1.0;
+        }
+        traj_consts(P, d, orbit_mode, dual_out);
+    }
+}
+
+// The Earth table rows for the device (empty if none was read) and the
+// number of tables read so far, which tells a device copy that it is stale.
+std::vector<EarthRowD> earth_rows_device(int* generation)
+{
+    const std::vector<EarthRow>& rows = earth_table();
+    std::lock_guard<std::mutex> lock(g_earth_mu);
+    *generation = g_earth_generation;
+    std::vector<EarthRowD> out(rows.size());
+    for (std::size_t i = 0; i < rows.size(); ++i) {
+        out[i].t = rows[i].t;
+        for (int k = 0; k < 3; ++k) {
+            out[i].p[k] = rows[i].p[k];
+            out[i].v[k] = rows[i].v[k];
+        }
+    }
+    return out;
 }
 
 bool dynamic_sources(const lcbg_curve& cv, const lcbg_dynamics& d, int orbit_mode, const double* t,
@@ -550,35 +653,6 @@ bool dynamic_sources_jacobian(const lcbg_curve& cv, const lcbg_dynamics& d, int 
     return ok;
 }
 
-// The same for every curve of a batch, curves spread over host threads; jac
-// may be null.
-bool dynamic_sources_batch(const lcbg_curve* curves, const lcbg_dynamics* dyn, int32_t n_curves,
-                           int orbit_mode, const int64_t* offsets, const double* times, double* xys,
-                           double* jac)
-{
-    if (n_curves <= 0) return true;
-    bool any_parallax = false;
-    for (int32_t c = 0; c < n_curves; ++c) any_parallax |= dyn[c].piEN != 0.0 || dyn[c].piEE != 0.0;
-    if (any_parallax) (void)earth_table();  // load once, outside the threads
-    unsigned hw = std::thread::hardware_concurrency();
-    const int n_threads = static_cast<int>(std::min<unsigned>(hw ? hw : 1, 16u));
-    std::vector<char> ok(static_cast<std::size_t>(n_curves), 1);
-    auto work = [&](int k) {
-        for (int32_t c = k; c < n_curves; c += n_threads) {
-            const int64_t b = offsets[c], m = offsets[c + 1] - offsets[c];
-            ok[static_cast<std::size_t>(c)] =
-                jac ? dynamic_sources_jacobian(curves[c], dyn[c], orbit_mode, times + b, m, xys + 3 * b,
-                                               jac + 3 * kTrajParams * b)
-                    : dynamic_sources(curves[c], dyn[c], orbit_mode, times + b, m, xys + 3 * b);
-        }
-    };
-    std::vector<std::thread> pool;
-    for (int k = 1; k < n_threads; ++k) pool.emplace_back(work, k);
-    work(0);
-    for (auto& th : pool) th.join();
-    return std::all_of(ok.begin(), ok.end(), [](char v) { return v != 0; });
-}
-
 }  // namespace lcbg
 
 extern "C" lcbg_status lcbg_load_earth_ephemeris(const char* path)
@@ -589,6 +663,7 @@ extern "C" lcbg_status lcbg_load_earth_ephemeris(const char* path)
     std::lock_guard<std::mutex> lock(g_earth_mu);
     g_earth.swap(rows);
     g_earth_loaded = true;
+    ++g_earth_generation;
     return g_earth.size() >= 2 ? LCBG_OK : LCBG_INVALID_ARGUMENT;
 }
 

@@ -20,6 +20,54 @@ namespace lcbg {
 
 constexpr int kStage1Block = 256;
 
+// Stage 1 of one epoch e of curve c: source position, quintic roots and
+// point-source magnification; epochs outside the caustic's bounding box (plus
+// kinji rho) settle as point source.  Returns true if the epoch goes on to the
+// caustic-distance test.
+__device__ __forceinline__ bool stage1_epoch(const CurveRec& cv, const CausticScalars& sc, const FsOptions& opt,
+                                             const FsEpochs& ep, int c, int64_t e)
+{
+    const double tau = (ep.times[e] - cv.t0) * cv.inverse_tE;
+    const double u0 = ep.src_y ? ep.src_y[e] : cv.u0;
+    const Pt src{tau * cv.cos_alpha - u0 * cv.sin_alpha,
+                 tau * cv.sin_alpha + u0 * cv.cos_alpha};
+    ep.curve_of[e] = c;
+    ep.src[e] = src;
+    ep.err[e] = 0.0;  // point source: no error estimate (:4668-4698)
+    if (cv.s == 0.0 || cv.q_eff <= 0.0) {
+        ep.mag[e] = 0.0;
+        ep.method[e] = LCBG_METHOD_POINT_SOURCE;
+        ep.a_ps[e] = 0.0;
+        return false;
+    }
+    const Geom g = make_geom(cv.s, cv.q_eff, src.x, src.y);
+    cx roots[5];
+    solve_quintic(g, roots);
+    const unsigned mask = select_physical(g, roots);
+    // LensModel path with rho > 0 (lens_model.cpp:199-212): the sum over
+    // binary_images in their residual order; rho == 0 keeps binary_mag0
+    const double a_ps = (opt.ps_images && cv.rho != 0.0) ? ps_from_images(g, roots)
+                                                          : ps_from_roots(g, roots, mask).mag;
+    ep.a_ps[e] = a_ps;
+    double* rr = ep.roots + 10 * e;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        rr[2 * k] = roots[k].re;
+        rr[2 * k + 1] = roots[k].im;
+    }
+    // rho == 0: binary_mag0 only (python/lcbinint_pybind.cpp:1129-1139);
+    // finite_mode <= 0: point-source magnification (:4671-4674)
+    const double rho = cv.rho;
+    const double margin = opt.kinji * rho;
+    if (!(rho > 0.0) || opt.mode <= 0 || sc.n == 0 || src.x < sc.min_x - margin || src.x > sc.max_x + margin ||
+        src.y < sc.min_y - margin || src.y > sc.max_y + margin) {
+        ep.mag[e] = a_ps;
+        ep.method[e] = LCBG_METHOD_POINT_SOURCE;
+        return false;
+    }
+    return true;
+}
+
 __global__ void __launch_bounds__(kStage1Block) k_fs_stage1(const CurveRec* __restrict__ curves,
                                                             FsCache cache, FsOptions opt,
                                                             FsEpochs ep)
@@ -36,47 +84,7 @@ __global__ void __launch_bounds__(kStage1Block) k_fs_stage1(const CurveRec* __re
     int run = 0;  // queued epochs of this curve so far
     for (int64_t first = begin; first < end; first += kStage1Block) {
         const int64_t e = first + tid;
-        bool near = false;
-        if (e < end) {
-            const double tau = (ep.times[e] - cv.t0) * cv.inverse_tE;
-            const Pt src{tau * cv.cos_alpha - cv.u0 * cv.sin_alpha,
-                         tau * cv.sin_alpha + cv.u0 * cv.cos_alpha};
-            ep.curve_of[e] = c;
-            ep.src[e] = src;
-            ep.err[e] = 0.0;  // point source: no error estimate (:4668-4698)
-            if (cv.s == 0.0 || cv.q_eff <= 0.0) {
-                ep.mag[e] = 0.0;
-                ep.method[e] = LCBG_METHOD_POINT_SOURCE;
-                ep.a_ps[e] = 0.0;
-            } else {
-                const Geom g = make_geom(cv.s, cv.q_eff, src.x, src.y);
-                cx roots[5];
-                solve_quintic(g, roots);
-                const unsigned mask = select_physical(g, roots);
-                // LensModel path with rho > 0 (lens_model.cpp:199-212): the sum over
-                // binary_images in their residual order; rho == 0 keeps binary_mag0
-                const double a_ps = (opt.ps_images && cv.rho != 0.0) ? ps_from_images(g, roots)
-                                                                      : ps_from_roots(g, roots, mask).mag;
-                ep.a_ps[e] = a_ps;
-                double* rr = ep.roots + 10 * e;
-#pragma unroll
-                for (int k = 0; k < 5; ++k) {
-                    rr[2 * k] = roots[k].re;
-                    rr[2 * k + 1] = roots[k].im;
-                }
-                // rho == 0: binary_mag0 only (python/lcbinint_pybind.cpp:1129-1139);
-                // finite_mode <= 0: point-source magnification (:4671-4674)
-                const double rho = cv.rho;
-                const double margin = opt.kinji * rho;
-                if (!(rho > 0.0) || opt.mode <= 0 || sc.n == 0 || src.x < sc.min_x - margin ||
-                    src.x > sc.max_x + margin || src.y < sc.min_y - margin || src.y > sc.max_y + margin) {
-                    ep.mag[e] = a_ps;
-                    ep.method[e] = LCBG_METHOD_POINT_SOURCE;
-                } else {
-                    near = true;
-                }
-            }
-        }
+        const bool near = e < end && stage1_epoch(cv, sc, opt, ep, c, e);
         // queue this chunk's remaining epochs in order: after the curve's
         // earlier ones in near_curve, and as one block in near_list
         const uint64_t m = ballot(near);
@@ -126,6 +134,33 @@ __device__ __forceinline__ int wave_append(unsigned* count, bool want)
     if (lane_id() == leader) base = static_cast<int>(atomicAdd(count, static_cast<unsigned>(popcount64(m))));
     base = shfl(base, leader);
     return want ? base + below : -1;
+}
+
+// Stage 1 for batches whose curves hold at most one epoch each (the per-epoch
+// pseudo-curves of orbital motion): one lane per curve, the queues as
+// k_fs_stage1 leaves them for one-epoch curves (their order aside).
+__global__ void __launch_bounds__(kStage1Block) k_fs_stage1_flat(const CurveRec* __restrict__ curves,
+                                                                 int n_curves, FsCache cache, FsOptions opt,
+                                                                 FsEpochs ep)
+{
+    const int c = static_cast<int>(blockIdx.x) * kStage1Block + static_cast<int>(threadIdx.x);
+    bool near = false;
+    int64_t e = -1;
+    if (c < n_curves) {
+        const int64_t begin = ep.offsets[c];
+        if (ep.offsets[c + 1] > begin) {
+            e = begin;
+            near = stage1_epoch(curves[c], cache.sc[c], opt, ep, c, e);
+        }
+        cache.near_n[c] = near ? 1 : 0;
+    }
+    const int slot = wave_append(&ep.counters[kQNearCount], near);
+    const int chunk = wave_append(&ep.counters[kQNearChunkCount], near);
+    if (near) {
+        ep.near_list[slot] = e;
+        ep.near_curve[e] = e;  // the curve's first epoch is e itself
+        ep.near_chunks[chunk] = static_cast<int64_t>(c) << 32;
+    }
 }
 
 // True if neither branch segment at position pos can come closer to p than
@@ -676,11 +711,17 @@ lcbg_status launch_reset_counters(unsigned* counters, int n, hipStream_t st)
 }
 
 lcbg_status launch_fs_stage1(const CurveRec* d_curves, int n_curves, const FsCache& cache,
-                             const FsOptions& opt, const FsEpochs& ep, hipStream_t st)
+                             const FsOptions& opt, const FsEpochs& ep, hipStream_t st, bool one_epoch)
 {
     if (n_curves <= 0) return LCBG_OK;
-    hipLaunchKernelGGL(k_fs_stage1, dim3(static_cast<unsigned>(n_curves)), dim3(kStage1Block), 0,
-                       st, d_curves, cache, opt, ep);
+    if (one_epoch) {
+        const unsigned blocks = static_cast<unsigned>((n_curves + kStage1Block - 1) / kStage1Block);
+        hipLaunchKernelGGL(k_fs_stage1_flat, dim3(blocks), dim3(kStage1Block), 0, st, d_curves, n_curves, cache,
+                           opt, ep);
+    } else {
+        hipLaunchKernelGGL(k_fs_stage1, dim3(static_cast<unsigned>(n_curves)), dim3(kStage1Block), 0,
+                           st, d_curves, cache, opt, ep);
+    }
     return check_launch();
 }
 

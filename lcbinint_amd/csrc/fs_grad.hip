@@ -754,8 +754,9 @@ __global__ void __launch_bounds__(256) k_grad_chain(const CurveRec* __restrict__
     for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
         const double* t = ep.dint + e * kTangents;
         const double tau = (ep.times[e] - cv.t0) * cv.inverse_tE;
-        const double yx = tau * cv.cos_alpha - cv.u0 * cv.sin_alpha;
-        const double yy = tau * cv.sin_alpha + cv.u0 * cv.cos_alpha;
+        const double u0 = ep.src_y ? ep.src_y[e] : cv.u0;
+        const double yx = tau * cv.cos_alpha - u0 * cv.sin_alpha;
+        const double yy = tau * cv.sin_alpha + u0 * cv.cos_alpha;
         double g[kGradParams];
         const double along = t[kTx] * cv.cos_alpha + t[kTy] * cv.sin_alpha;
         g[0] = -along * cv.inverse_tE;                              // t0

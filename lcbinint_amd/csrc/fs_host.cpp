@@ -5,8 +5,10 @@
 // Compiled by g++: the phase and probe angle tables are rounded by glibc's
 // cos/sin exactly as the reference's std::polar / std::cos / std::sin calls are
 // (finite_source_magnifier.cpp:553, 1551-1553, 1579-1583, 1606-1610, 1748, 4025).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -14,6 +16,7 @@
 #include <vector>
 
 #include "fs_launch.hpp"
+#include "traj.hpp"
 
 namespace lcbg {
 
@@ -587,6 +590,7 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     ep.near_curve = near_curve_view(fs);
     ep.near_chunks = near_chunks_view(fs);
     ep.err = err_view(fs);
+    ep.src_y = b->src_y;
     // the pseudo-curve batch of the dynamic path runs different epochs under
     // the same count, so it keeps no keys
     if (!capture && !b->dynamic_images) {
@@ -625,7 +629,14 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
         (void)hipEventRecord(ev[0], stream);
     }
     bool cache_joined = false;
-    st = fs_refresh(b, stream, true, &cache_joined);
+    // curves of one epoch each (the pseudo-curves of orbital motion) build
+    // their branches after stage 1, for the curves it queued only; such
+    // partial caches are never reused
+    const bool lazy_branches = b->one_epoch_curves && b->caches_stale;
+    if (lazy_branches)
+        st = launch_curve_prep(b->d_curves, b->n_curves, fs->cache, fs->tab, stream);
+    else
+        st = fs_refresh(b, stream, true, &cache_joined);
     // every return path leaves `stream` ordered after the branch caches
     struct CacheJoin {
         hipStream_t stream;
@@ -657,7 +668,10 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     };
     if (!capture) st = map_hip(hipStreamWaitEvent(stream, pool.last_use, 0));
     if (st == LCBG_OK) st = launch_reset_counters(ep.counters, kCounterWords, stream);
-    if (st == LCBG_OK) st = launch_fs_stage1(b->d_curves, b->n_curves, fs->cache, fs->opt, ep, stream);
+    if (st == LCBG_OK)
+        st = launch_fs_stage1(b->d_curves, b->n_curves, fs->cache, fs->opt, ep, stream, b->one_epoch_curves);
+    if (st == LCBG_OK && lazy_branches)
+        st = launch_branch_cache(b->d_curves, b->n_curves, fs->cache, stream, true);
     stage_done("stage 1 (point source, bounding box)");
     if (stage_ms && st == LCBG_OK) (void)hipEventRecord(ev[2], stream);
     // stage 2 reads the branch caches built beside stage 1
@@ -878,175 +892,319 @@ extern "C" lcbg_status lcbg_batch_launch_graph(lcbg_batch* batch, void* stream)
 }
 
 namespace lcbg {
-bool dynamic_sources_batch(const lcbg_curve* curves, const lcbg_dynamics* dyn, int32_t n_curves,
-                           int orbit_mode, const int64_t* offsets, const double* times, double* xys,
-                           double* jac);
+void trajectory_constants(const lcbg_curve& cv, const lcbg_dynamics& d, int orbit_mode, TrajConst<double>* val_out,
+                          TrajConst<TDual<kTrajParams>>* dual_out);
+std::vector<EarthRowD> earth_rows_device(int* generation);
+
+// Device buffers of the dynamic path, kept across calls and grown on demand
+// (a call allocates only when it holds more epochs or curves than any before).
+struct DynState {
+    void* mem = nullptr;
+    int64_t cap_epochs = 0;
+    int32_t cap_curves = 0;
+    void* consts = nullptr;      // [curve] TrajConst<double> or TrajConst<TDual<9>>
+    int64_t* off = nullptr;      // [curve + 1] offsets (host calls)
+    double* t = nullptr;         // [epoch] times (host calls)
+    double *x = nullptr, *y = nullptr, *s = nullptr;  // [epoch] trajectory
+    double* jac = nullptr;       // [epoch][3][9]
+    double* mag = nullptr;       // [epoch] (host calls)
+    int32_t* meth = nullptr;     // [epoch] (host calls)
+    double* aa = nullptr;        // [epoch] (host calls)
+    double* gp = nullptr;        // [epoch][8] pseudo-curve derivatives
+    double* dmag = nullptr;      // [epoch][12] (host calls)
+    lcbg_result* res = nullptr;  // [epoch]
+    unsigned* bad = nullptr;     // non-finite orbit flag
+    int64_t* child_off = nullptr;  // [chunk + 1]
+    double* child_t = nullptr;     // [chunk]
+    EarthRowD* earth = nullptr;
+    int n_earth = 0;
+    int earth_gen = -1;
+    lcbg_batch* static_child = nullptr;  // one pseudo-curve per curve (no orbital motion)
+};
+
+void dyn_destroy(DynState* d)
+{
+    if (!d) return;
+    if (d->static_child) lcbg_batch_destroy(d->static_child);
+    if (d->mem) (void)hipFree(d->mem);
+    if (d->earth) (void)hipFree(d->earth);
+    delete d;
+}
+}  // namespace lcbg
+
+namespace {
+// Epochs per device pass of the dynamic path with orbital motion: each epoch
+// is a curve of its own (its own separation, its own caustic cache of about
+// 340 KB), so 65536 of them hold about 22 GB.
+constexpr int32_t kDynChunk = 65536;
+constexpr int kDynGradParams = 12;
+
+lcbg_status dyn_reserve(lcbg_batch* b, int64_t n_epochs)
+{
+    if (!b->dyn) b->dyn = new (std::nothrow) DynState();
+    if (!b->dyn) return LCBG_DEVICE_ERROR;
+    DynState& d = *b->dyn;
+    const int32_t nc = b->n_curves;
+    if (d.mem && d.cap_epochs >= n_epochs && d.cap_curves >= nc) return LCBG_OK;
+    int64_t cap = 1024;
+    while (cap < n_epochs) cap *= 2;
+    const std::size_t ce = static_cast<std::size_t>(cap), cc = static_cast<std::size_t>(nc);
+    const std::size_t ck = static_cast<std::size_t>(kDynChunk);
+    const std::size_t bytes = carve_size(cc * sizeof(TrajConst<TDual<kTrajParams>>)) +
+                              carve_size((cc + 1) * sizeof(int64_t)) + carve_size(ce * sizeof(double)) * 7 +
+                              carve_size(ce * sizeof(int32_t)) +
+                              carve_size(ce * 3 * kTrajParams * sizeof(double)) +
+                              carve_size(ce * kGradParams * sizeof(double)) +
+                              carve_size(ce * kDynGradParams * sizeof(double)) +
+                              carve_size(ce * sizeof(lcbg_result)) + carve_size(64) +
+                              carve_size((ck + 1) * sizeof(int64_t)) + carve_size(ck * sizeof(double));
+    if (d.mem) {
+        (void)hipFree(d.mem);
+        d.mem = nullptr;
+        d.cap_epochs = 0;
+        d.cap_curves = 0;
+    }
+    lcbg_status st = map_hip(hipMalloc(&d.mem, bytes));
+    if (st != LCBG_OK) return st;
+    char* p = static_cast<char*>(d.mem);
+    d.consts = carve<unsigned char>(p, cc * sizeof(TrajConst<TDual<kTrajParams>>));
+    d.off = carve<int64_t>(p, cc + 1);
+    d.t = carve<double>(p, ce);
+    d.x = carve<double>(p, ce);
+    d.y = carve<double>(p, ce);
+    d.s = carve<double>(p, ce);
+    d.mag = carve<double>(p, ce);
+    d.aa = carve<double>(p, ce);
+    (void)carve<double>(p, ce);  // spare
+    d.meth = carve<int32_t>(p, ce);
+    d.jac = carve<double>(p, ce * 3 * kTrajParams);
+    d.gp = carve<double>(p, ce * kGradParams);
+    d.dmag = carve<double>(p, ce * kDynGradParams);
+    d.res = carve<lcbg_result>(p, ce);
+    d.bad = carve<unsigned>(p, 16);
+    d.child_off = carve<int64_t>(p, ck + 1);
+    d.child_t = carve<double>(p, ck);
+    d.cap_epochs = cap;
+    d.cap_curves = nc;
+    return LCBG_OK;
 }
 
-namespace {
-// Epochs per device pass of the dynamic path: each epoch is a curve of its own
-// (its own separation, its own caustic cache); 16384 caches take ~5.6 GB.
-constexpr int32_t kDynChunk = 16384;
-constexpr int kDynGradParams = 12;
-}  // namespace
-
-namespace {
-
-constexpr int kTrajParams = 9;  // t0, tE, u0, alpha, s, piEN, piEE, g1, g2
-
-// The dynamic path, optionally with the derivatives of each epoch's
-// magnification with respect to the 12 parameters of
-// lcbg_batch_light_curves_dynamic_grad (dmag != nullptr).
-lcbg_status dynamic_run(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t orbital_motion_mode,
-                        const int64_t* offsets, const double* times, int64_t n_epochs, double* mag,
-                        int32_t* method, double* mag_aa, double* dmag, lcbg_result* results = nullptr)
+// The Earth table on the device, uploaded again after a new table was read.
+lcbg_status dyn_earth(DynState& d)
 {
-    if (!batch || n_epochs < 0) return LCBG_INVALID_ARGUMENT;
+    int gen = 0;
+    std::vector<EarthRowD> rows = earth_rows_device(&gen);
+    if (gen == d.earth_gen) return LCBG_OK;
+    if (d.earth) (void)hipFree(d.earth);
+    d.earth = nullptr;
+    d.n_earth = 0;
+    if (!rows.empty()) {
+        lcbg_status st = map_hip(hipMalloc(reinterpret_cast<void**>(&d.earth), rows.size() * sizeof(EarthRowD)));
+        if (st == LCBG_OK)
+            st = map_hip(hipMemcpy(d.earth, rows.data(), rows.size() * sizeof(EarthRowD), hipMemcpyHostToDevice));
+        if (st != LCBG_OK) return st;
+        d.n_earth = static_cast<int>(rows.size());
+    }
+    d.earth_gen = gen;
+    return LCBG_OK;
+}
+
+// The trajectories of every epoch of the batch's curves on the device: per
+// epoch x, y, s (and with jac the [3][9] Jacobian), from per-curve constants
+// computed on the host (its libm, as the host trajectory).  Ends with the
+// stream synchronised: LCBG_NUMERICAL_ERROR if an orbit is not finite
+// (lens_model.cpp:117-120).
+lcbg_status dyn_trajectory(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t orbit_mode, const int64_t* off,
+                           const double* times, bool with_jac, double* x, double* y, double* s, double* jac,
+                           hipStream_t stream)
+{
+    DynState& d = *batch->dyn;
+    const int32_t nc = batch->n_curves;
+    lcbg_status st = LCBG_OK;
+    bool any_parallax = false;
+    for (int32_t c = 0; c < nc; ++c) any_parallax |= dyn[c].piEN != 0.0 || dyn[c].piEE != 0.0;
+    if (any_parallax) st = dyn_earth(d);
+    if (st != LCBG_OK) return st;
+    std::vector<TrajConst<double>> kv(with_jac ? 0 : static_cast<std::size_t>(nc));
+    std::vector<TrajConst<TDual<kTrajParams>>> kd(with_jac ? static_cast<std::size_t>(nc) : 0);
+    for (int32_t c = 0; c < nc; ++c)
+        trajectory_constants(batch->h_params[static_cast<std::size_t>(c)], dyn[c], orbit_mode,
+                             with_jac ? nullptr : &kv[static_cast<std::size_t>(c)],
+                             with_jac ? &kd[static_cast<std::size_t>(c)] : nullptr);
+    const void* kh = with_jac ? static_cast<const void*>(kd.data()) : static_cast<const void*>(kv.data());
+    const std::size_t kbytes = with_jac ? kd.size() * sizeof(kd[0]) : kv.size() * sizeof(kv[0]);
+    st = map_hip(hipMemcpyAsync(d.consts, kh, kbytes, hipMemcpyHostToDevice, stream));
+    if (st == LCBG_OK) st = map_hip(hipMemsetAsync(d.bad, 0, sizeof(unsigned), stream));
+    if (st == LCBG_OK)
+        st = launch_traj(d.consts, with_jac, nc, off, times, d.earth, d.n_earth, x, y, s, jac, d.bad, stream);
+    unsigned h_bad = 0;
+    if (st == LCBG_OK) st = map_hip(hipMemcpyAsync(&h_bad, d.bad, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    // the host copies of the constants and of h_bad live until here
+    if (st == LCBG_OK) st = map_hip(hipStreamSynchronize(stream));
+    if (st != LCBG_OK) return st;
+    return h_bad ? LCBG_NUMERICAL_ERROR : LCBG_OK;
+}
+
+// Inputs and outputs of one dynamic run: host arrays (device == false, the
+// run ends synchronised) or device arrays enqueued on `stream`.
+struct DynIO {
+    const int64_t* offsets;
+    const double* times;
+    int64_t n_epochs;
+    double* mag;
+    int32_t* method;
+    double* mag_aa;         // may be null
+    double* dmag;           // [epoch][12], null: values only
+    lcbg_result* results;   // host calls only; may be null
+    bool device;
+    hipStream_t stream;
+};
+
+// The dynamic path (LensModel with parallax and orbital motion,
+// src/lcbinint/model/lens_model.cpp:106-231): the trajectory of every epoch
+// on the device (k_traj), then the batched finite-source run on pseudo-curves
+// that place each epoch's source and separation; optionally the derivatives of
+// each epoch's magnification with respect to the 12 parameters (t0, tE, u0,
+// alpha, s, q, rho, u_LD, piEN, piEE, g1, g2).  Without orbital motion the
+// separation is constant: one pseudo-curve per curve, with the impact
+// parameter per epoch, keeps one caustic cache per curve, reused across calls
+// while s and q stay; with it, every epoch is a curve of its own, in chunks.
+lcbg_status dynamic_run(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t orbital_motion_mode, const DynIO& io)
+{
+    if (!batch || io.n_epochs < 0) return LCBG_INVALID_ARGUMENT;
     if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER)
         return LCBG_UNSUPPORTED;
     const int32_t nc = batch->n_curves;
-    if (nc == 0 || n_epochs == 0) return LCBG_OK;
-    if (!dyn || !offsets || !times || !mag || !method) return LCBG_INVALID_ARGUMENT;
-    if (offsets[0] != 0 || offsets[nc] != n_epochs) return LCBG_INVALID_ARGUMENT;
-    for (int32_t c = 0; c < nc; ++c)
-        if (offsets[c + 1] < offsets[c]) return LCBG_INVALID_ARGUMENT;
-    const bool grad = dmag != nullptr;
-    std::vector<double> xys(static_cast<std::size_t>(3 * n_epochs));
-    std::vector<double> jac(grad ? static_cast<std::size_t>(3 * kTrajParams * n_epochs) : 0);
-    if (!dynamic_sources_batch(batch->h_params.data(), dyn, nc, orbital_motion_mode, offsets, times,
-                               xys.data(), grad ? jac.data() : nullptr))
-        return LCBG_NUMERICAL_ERROR;  // a non-finite orbit (lens_model.cpp:117-120)
-    lcbg_status st = LCBG_OK;
-    // pseudo-curve capacity: the run's epoch count rounded up to a power of two,
-    // at least 64 and at most kDynChunk; a larger later run replaces the child
-    int32_t want = 64;
-    while (want < kDynChunk && want < n_epochs) want *= 2;
-    if (batch->dyn_child && batch->dyn_child->n_curves < want) {
-        lcbg_batch_destroy(batch->dyn_child);
-        batch->dyn_child = nullptr;
+    const int64_t n = io.n_epochs;
+    if (nc == 0 || n == 0) return LCBG_OK;
+    if (!dyn || !io.offsets || !io.times || !io.mag || !io.method) return LCBG_INVALID_ARGUMENT;
+    if (n > INT32_MAX) return LCBG_INVALID_ARGUMENT;
+    if (!io.device) {
+        if (io.offsets[0] != 0 || io.offsets[nc] != n) return LCBG_INVALID_ARGUMENT;
+        for (int32_t c = 0; c < nc; ++c)
+            if (io.offsets[c + 1] < io.offsets[c]) return LCBG_INVALID_ARGUMENT;
     }
-    if (!batch->dyn_child) {
-        std::vector<lcbg_curve> idle(static_cast<std::size_t>(want),
-                                     lcbg_curve{0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0});
-        st = lcbg_batch_create(idle.data(), want, &batch->options, &batch->dyn_child);
-        if (st != LCBG_OK) return st;
-        batch->dyn_child->dynamic_images = true;
+    const bool grad = io.dmag != nullptr;
+    const hipStream_t stream = io.stream;
+    lcbg_status st = dyn_reserve(batch, n);
+    if (st != LCBG_OK) return st;
+    DynState& d = *batch->dyn;
+    const int64_t* off = io.offsets;
+    const double* times = io.times;
+    if (!io.device) {
+        st = map_hip(hipMemcpyAsync(d.off, io.offsets, (nc + 1) * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+        if (st == LCBG_OK)
+            st = map_hip(hipMemcpyAsync(d.t, io.times, n * sizeof(double), hipMemcpyHostToDevice, stream));
+        off = d.off;
+        times = d.t;
     }
-    lcbg_batch* ch = batch->dyn_child;
-    const int32_t chunk = ch->n_curves;
-    int64_t* d_off = nullptr;
-    double* d_buf = nullptr;  // times, mag, mag_aa, dmag (8 per epoch), result records
-    int32_t* d_meth = nullptr;
-    const std::size_t rec_doubles = results ? static_cast<std::size_t>(chunk) * sizeof(lcbg_result) / sizeof(double) : 0;
-    const std::size_t buf_doubles = static_cast<std::size_t>(chunk) * (grad ? 3 + kGradParams : 2) + rec_doubles;
-    st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_off), (chunk + 1) * sizeof(int64_t)));
-    if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_buf), buf_doubles * sizeof(double)));
-    if (st == LCBG_OK) st = map_hip(hipMalloc(reinterpret_cast<void**>(&d_meth), chunk * sizeof(int32_t)));
-    double* d_t = d_buf;
-    double* d_mag = d_buf + chunk;
-    double* d_aa = grad ? d_buf + 2 * chunk : nullptr;
-    double* d_dm = grad ? d_buf + 3 * chunk : nullptr;
-    lcbg_result* d_res =
-        results ? reinterpret_cast<lcbg_result*>(d_buf + static_cast<std::size_t>(chunk) * (grad ? 3 + kGradParams : 2))
-                : nullptr;
-    std::vector<int64_t> h_off(static_cast<std::size_t>(chunk) + 1);
-    std::vector<double> h_t(static_cast<std::size_t>(chunk));
-    std::vector<double> h_dm(grad ? static_cast<std::size_t>(chunk) * kGradParams : 0);
-    int32_t c = 0;  // curve of the chunk's first epoch
-    for (int64_t base = 0; st == LCBG_OK && base < n_epochs; base += chunk) {
-        const int32_t m = static_cast<int32_t>(std::min<int64_t>(chunk, n_epochs - base));
-        // pseudo-curve k = epoch base + k: source (x, y) = (time, u0) with
-        // t0 = 0, tE = 1, alpha = 0, the epoch's separation, the curve's q, rho, LD, nbin
-        int32_t cc = c;
-        for (int32_t k = 0; k < chunk; ++k) {
-            CurveRec& r = ch->h_curves[static_cast<std::size_t>(k)];
-            h_off[static_cast<std::size_t>(k)] = k < m ? k : m;
-            if (k >= m) {
-                r = make_curve_rec(lcbg_curve{0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0});
-                continue;
-            }
-            const int64_t e = base + k;
-            while (offsets[cc + 1] <= e) ++cc;
-            const CurveRec& pc = batch->h_curves[static_cast<std::size_t>(cc)];
+    if (st == LCBG_OK)
+        st = dyn_trajectory(batch, dyn, orbital_motion_mode, off, times, grad, d.x, d.y, d.s, grad ? d.jac : nullptr,
+                            stream);
+    if (st != LCBG_OK) return st;
+    double* mag = io.device ? io.mag : d.mag;
+    int32_t* meth = io.device ? io.method : d.meth;
+    double* aa = io.device ? io.mag_aa : d.aa;
+    double* dm = io.device ? io.dmag : d.dmag;
+    if (orbital_motion_mode == LCBG_ORBIT_STATIC) {
+        // one pseudo-curve per curve: t0 = 0, tE = 1, alpha = 0, the curve's
+        // s, q, rho, LD and lattice; per epoch "time" x and impact parameter y
+        if (d.static_child && d.static_child->n_curves != nc) {
+            lcbg_batch_destroy(d.static_child);
+            d.static_child = nullptr;
+        }
+        if (!d.static_child) {
+            st = lcbg_batch_create(batch->h_params.data(), nc, &batch->options, &d.static_child);
+            if (st != LCBG_OK) return st;
+            d.static_child->dynamic_images = true;
+            d.static_child->h_curves.assign(static_cast<std::size_t>(nc), CurveRec{});
+        }
+        lcbg_batch* ch = d.static_child;
+        std::vector<CurveRec> recs(batch->h_curves);
+        for (CurveRec& r : recs) {
             r.t0 = 0.0;
             r.inverse_tE = 1.0;
-            r.u0 = xys[static_cast<std::size_t>(3 * e + 1)];
+            r.u0 = 0.0;
             r.cos_alpha = 1.0;
             r.sin_alpha = 0.0;
-            r.s = xys[static_cast<std::size_t>(3 * e + 2)];
-            r.q_eff = pc.q_eff;
-            r.rho = pc.rho;
-            r.ld_c = pc.ld_c;
-            r.ld_d = pc.ld_d;
-            r.source_bins = pc.source_bins;
-            r.pad = 0;
-            h_t[static_cast<std::size_t>(k)] = xys[static_cast<std::size_t>(3 * e)];
         }
-        h_off[static_cast<std::size_t>(chunk)] = m;
-        ch->caches_stale = true;
-        st = map_hip(hipMemcpy(ch->d_curves, ch->h_curves.data(), chunk * sizeof(CurveRec),
-                               hipMemcpyHostToDevice));
-        if (st == LCBG_OK)
-            st = map_hip(hipMemcpy(d_off, h_off.data(), (chunk + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
-        if (st == LCBG_OK) st = map_hip(hipMemcpy(d_t, h_t.data(), m * sizeof(double), hipMemcpyHostToDevice));
+        if (std::memcmp(recs.data(), ch->h_curves.data(), recs.size() * sizeof(CurveRec)) != 0) {
+            ch->h_curves = recs;
+            st = map_hip(hipMemcpy(ch->d_curves, recs.data(), recs.size() * sizeof(CurveRec), hipMemcpyHostToDevice));
+            ch->caches_stale = true;
+        }
+        ch->src_y = d.y;
         if (st == LCBG_OK) {
             if (grad) {
-                const GradArgs ga{d_aa, d_dm, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr};
-                st = fs_run(ch, d_off, d_t, m, d_mag, d_meth, nullptr, nullptr, &ga);
+                const GradArgs ga{aa, d.gp, nullptr, nullptr, 0.0, 0.0, nullptr, nullptr};
+                st = fs_run(ch, off, d.x, n, mag, meth, stream, nullptr, &ga);
             } else {
-                st = fs_run(ch, d_off, d_t, m, d_mag, d_meth, nullptr, nullptr);
+                st = fs_run(ch, off, d.x, n, mag, meth, stream, nullptr);
             }
         }
-        // the pseudo-curve's source is the magnifier's source position, its
-        // separation the epoch's s(t): the records read as LensModel fills them
-        if (st == LCBG_OK && results) st = lcbg_batch_epoch_results_device(ch, m, d_res, nullptr);
-        if (st == LCBG_OK) st = map_hip(hipStreamSynchronize(nullptr));
-        if (st == LCBG_OK && results)
-            st = map_hip(hipMemcpy(results + base, d_res, m * sizeof(lcbg_result), hipMemcpyDeviceToHost));
-        if (st == LCBG_OK)
-            st = map_hip(hipMemcpy(mag + base, d_mag, m * sizeof(double), hipMemcpyDeviceToHost));
-        if (st == LCBG_OK)
-            st = map_hip(hipMemcpy(method + base, d_meth, m * sizeof(int32_t), hipMemcpyDeviceToHost));
-        if (st == LCBG_OK && grad && mag_aa)
-            st = map_hip(hipMemcpy(mag_aa + base, d_aa, m * sizeof(double), hipMemcpyDeviceToHost));
-        if (st == LCBG_OK && grad)
-            st = map_hip(hipMemcpy(h_dm.data(), d_dm, static_cast<std::size_t>(m) * kGradParams * sizeof(double),
-                                   hipMemcpyDeviceToHost));
-        if (st == LCBG_OK && grad) {
-            // chain: the pseudo-curve's derivatives give dA/dx = -d/dt0,
-            // dA/dy = d/du0, dA/ds, and q, rho, u_LD directly (alpha = 0, tE = 1)
-            for (int32_t k = 0; k < m; ++k) {
-                const int64_t e = base + k;
-                const double* gp = h_dm.data() + static_cast<std::size_t>(k) * kGradParams;
-                const double da[3] = {-gp[0], gp[2], gp[4]};
-                const double* J = jac.data() + static_cast<std::size_t>(3 * kTrajParams * e);
-                double tr[kTrajParams];
-                for (int j = 0; j < kTrajParams; ++j)
-                    tr[j] = da[0] * J[j] + da[1] * J[kTrajParams + j] + da[2] * J[2 * kTrajParams + j];
-                double* out = dmag + e * kDynGradParams;
-                out[0] = tr[0];   // t0
-                out[1] = tr[1];   // tE
-                out[2] = tr[2];   // u0
-                out[3] = tr[3];   // alpha
-                out[4] = tr[4];   // s
-                out[5] = gp[5];   // q
-                out[6] = gp[6];   // rho
-                out[7] = gp[7];   // u_LD
-                out[8] = tr[5];   // piEN
-                out[9] = tr[6];   // piEE
-                out[10] = tr[7];  // g1
-                out[11] = tr[8];  // g2
-            }
+        ch->src_y = nullptr;
+        if (st == LCBG_OK && grad) st = launch_dyn_chain(d.gp, d.jac, 0, static_cast<int>(n), dm, stream);
+        if (st == LCBG_OK && io.results) st = lcbg_batch_epoch_results_device(ch, n, d.res, stream);
+    } else {
+        // pseudo-curve capacity: the run's epoch count rounded up to a power of
+        // two, at least 64 and at most kDynChunk; a larger later run replaces it
+        int32_t want = 64;
+        while (want < kDynChunk && want < n) want *= 2;
+        if (batch->dyn_child && batch->dyn_child->n_curves < want) {
+            lcbg_batch_destroy(batch->dyn_child);
+            batch->dyn_child = nullptr;
         }
-        c = cc;
+        if (!batch->dyn_child) {
+            std::vector<lcbg_curve> idle(static_cast<std::size_t>(want),
+                                         lcbg_curve{0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0});
+            st = lcbg_batch_create(idle.data(), want, &batch->options, &batch->dyn_child);
+            if (st != LCBG_OK) return st;
+            batch->dyn_child->dynamic_images = true;
+            batch->dyn_child->one_epoch_curves = true;
+        }
+        lcbg_batch* ch = batch->dyn_child;
+        const int32_t chunk = ch->n_curves;
+        for (int64_t base = 0; st == LCBG_OK && base < n; base += chunk) {
+            const int32_t m = static_cast<int32_t>(std::min<int64_t>(chunk, n - base));
+            st = launch_pseudo_curves(batch->d_curves, nc, off, d.x, d.y, d.s, base, m, chunk, ch->d_curves,
+                                      d.child_off, d.child_t, stream);
+            ch->caches_stale = true;
+            if (st == LCBG_OK) {
+                if (grad) {
+                    const GradArgs ga{aa ? aa + base : nullptr, d.gp + base * kGradParams, nullptr, nullptr, 0.0,
+                                      0.0, nullptr, nullptr};
+                    st = fs_run(ch, d.child_off, d.child_t, m, mag + base, meth + base, stream, nullptr, &ga);
+                } else {
+                    st = fs_run(ch, d.child_off, d.child_t, m, mag + base, meth + base, stream, nullptr);
+                }
+            }
+            // the pseudo-curve's source is the magnifier's source position, its
+            // separation the epoch's s(t): the records read as LensModel fills them
+            if (st == LCBG_OK && io.results) st = lcbg_batch_epoch_results_device(ch, m, d.res + base, stream);
+            if (st == LCBG_OK && grad)
+                st = launch_dyn_chain(d.gp + base * kGradParams, d.jac, base, m, dm, stream);
+        }
     }
-    if (d_off) (void)hipFree(d_off);
-    if (d_buf) (void)hipFree(d_buf);
-    if (d_meth) (void)hipFree(d_meth);
+    if (st != LCBG_OK || io.device) return st;
+    st = map_hip(hipStreamSynchronize(stream));
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(io.mag, d.mag, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(io.method, d.meth, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (st == LCBG_OK && grad && io.mag_aa)
+        st = map_hip(hipMemcpy(io.mag_aa, d.aa, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (st == LCBG_OK && grad)
+        st = map_hip(hipMemcpy(io.dmag, d.dmag, n * kDynGradParams * sizeof(double), hipMemcpyDeviceToHost));
+    if (st == LCBG_OK && io.results)
+        st = map_hip(hipMemcpy(io.results, d.res, n * sizeof(lcbg_result), hipMemcpyDeviceToHost));
     // a non-finite magnification is a numerical error (lens_model.cpp:228-231)
-    for (int64_t i = 0; st == LCBG_OK && i < n_epochs; ++i)
-        if (!std::isfinite(mag[i])) st = LCBG_NUMERICAL_ERROR;
+    for (int64_t i = 0; st == LCBG_OK && i < n; ++i)
+        if (!std::isfinite(io.mag[i])) st = LCBG_NUMERICAL_ERROR;
     return st;
+}
+
+DynIO host_io(const int64_t* offsets, const double* times, int64_t n, double* mag, int32_t* method,
+              double* mag_aa = nullptr, double* dmag = nullptr, lcbg_result* results = nullptr)
+{
+    return DynIO{offsets, times, n, mag, method, mag_aa, dmag, results, false, nullptr};
 }
 
 }  // namespace
@@ -1056,8 +1214,7 @@ extern "C" lcbg_status lcbg_batch_light_curves_dynamic(lcbg_batch* batch, const 
                                                        const int64_t* offsets, const double* times,
                                                        int64_t n_epochs, double* mag, int32_t* method)
 {
-    return dynamic_run(batch, dyn, orbital_motion_mode, offsets, times, n_epochs, mag, method, nullptr,
-                       nullptr);
+    return dynamic_run(batch, dyn, orbital_motion_mode, host_io(offsets, times, n_epochs, mag, method));
 }
 
 extern "C" lcbg_status lcbg_batch_light_curves_dynamic_results(lcbg_batch* batch, const lcbg_dynamics* dyn,
@@ -1068,8 +1225,8 @@ extern "C" lcbg_status lcbg_batch_light_curves_dynamic_results(lcbg_batch* batch
     if (n_epochs > 0 && !results) return LCBG_INVALID_ARGUMENT;
     std::vector<double> mag(static_cast<std::size_t>(n_epochs > 0 ? n_epochs : 0));
     std::vector<int32_t> meth(mag.size());
-    return dynamic_run(batch, dyn, orbital_motion_mode, offsets, times, n_epochs, mag.data(), meth.data(),
-                       nullptr, nullptr, results);
+    return dynamic_run(batch, dyn, orbital_motion_mode,
+                       host_io(offsets, times, n_epochs, mag.data(), meth.data(), nullptr, nullptr, results));
 }
 
 extern "C" lcbg_status lcbg_batch_light_curves_dynamic_grad(lcbg_batch* batch, const lcbg_dynamics* dyn,
@@ -1079,8 +1236,34 @@ extern "C" lcbg_status lcbg_batch_light_curves_dynamic_grad(lcbg_batch* batch, c
                                                             double* mag_aa, double* dmag)
 {
     if (n_epochs > 0 && !dmag) return LCBG_INVALID_ARGUMENT;
-    return dynamic_run(batch, dyn, orbital_motion_mode, offsets, times, n_epochs, mag, method, mag_aa,
-                       dmag ? dmag : nullptr);
+    return dynamic_run(batch, dyn, orbital_motion_mode, host_io(offsets, times, n_epochs, mag, method, mag_aa, dmag));
+}
+
+extern "C" lcbg_status lcbg_batch_light_curves_dynamic_device(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                                              int32_t orbital_motion_mode,
+                                                              const int64_t* offsets, const double* times,
+                                                              int64_t n_epochs, double* mag, int32_t* method,
+                                                              double* mag_aa, double* dmag, void* stream)
+{
+    return dynamic_run(batch, dyn, orbital_motion_mode,
+                       DynIO{offsets, times, n_epochs, mag, method, mag_aa, dmag, nullptr, true,
+                             static_cast<hipStream_t>(stream)});
+}
+
+extern "C" lcbg_status lcbg_batch_dynamic_sources_device(lcbg_batch* batch, const lcbg_dynamics* dyn,
+                                                         int32_t orbital_motion_mode, const int64_t* offsets,
+                                                         const double* times, int64_t n_epochs, double* x,
+                                                         double* y, double* s, double* jac, void* stream)
+{
+    if (!batch || n_epochs < 0) return LCBG_INVALID_ARGUMENT;
+    if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER)
+        return LCBG_UNSUPPORTED;
+    if (batch->n_curves == 0 || n_epochs == 0) return LCBG_OK;
+    if (!dyn || !offsets || !times || !x || !y || !s) return LCBG_INVALID_ARGUMENT;
+    lcbg_status st = dyn_reserve(batch, 0);
+    if (st != LCBG_OK) return st;
+    return dyn_trajectory(batch, dyn, orbital_motion_mode, offsets, times, jac != nullptr, x, y, s, jac,
+                          static_cast<hipStream_t>(stream));
 }
 
 extern "C" lcbg_status lcbg_batch_finite_source_grad_device(lcbg_batch* batch, const int64_t* offsets,

@@ -26,12 +26,16 @@ FsTables table_view(const Pt2* device_tables, int bins);
 // by cell (the part stage 2 and the seed search read).
 lcbg_status launch_curve_prep(const CurveRec* d_curves, int n_curves, const FsCache& cache,
                               const FsTables& tab, hipStream_t st);
-lcbg_status launch_branch_cache(const CurveRec* d_curves, int n_curves, const FsCache& cache, hipStream_t st);
+// only_near: after stage 1, for the curves with queued epochs only
+// (cache.near_n[c] > 0); the others keep no branches.
+lcbg_status launch_branch_cache(const CurveRec* d_curves, int n_curves, const FsCache& cache, hipStream_t st,
+                                bool only_near = false);
 
-// Stage 1 (one workgroup per curve): trajectory, point-source solve, the
+// Stage 1 (one workgroup per curve; with one_epoch, for curves of at most one
+// epoch each, one lane per curve): trajectory, point-source solve, the
 // bounding-box exit; queues the remaining epochs for stage 2.
 lcbg_status launch_fs_stage1(const CurveRec* d_curves, int n_curves, const FsCache& cache,
-                             const FsOptions& opt, const FsEpochs& ep, hipStream_t st);
+                             const FsOptions& opt, const FsEpochs& ep, hipStream_t st, bool one_epoch = false);
 
 // Stage 2 (three launches over the queue: caustic distances, one resident
 // workgroup per CU (n_cus) with a curve's caustic in LDS, or one wave per epoch
@@ -115,5 +119,24 @@ lcbg_status launch_epoch_results(const CurveRec* d_curves, const FsOptions& opt,
 
 // Zeroes the queue counters in-stream with a kernel (graph-capturable).
 lcbg_status launch_reset_counters(unsigned* counters, int n, hipStream_t st);
+
+// Device trajectories (traj.hip).  consts: n_curves TrajConst<double>, or
+// TrajConst<TDual<9>> with with_jac (then jac [epoch][3][9] is written);
+// per epoch the source position xs, ys in the binary frame and the
+// separation ss; *bad |= 1 if an orbit is not finite.
+struct EarthRowD;
+lcbg_status launch_traj(const void* consts, bool with_jac, int n_curves, const int64_t* offsets,
+                        const double* times, const EarthRowD* earth, int n_rows, double* xs, double* ys,
+                        double* ss, double* jac, unsigned* bad, hipStream_t st);
+// Records of the pseudo-curves of epochs [base, base + m): curve k has the
+// one epoch base + k at "time" xs, with u0 = ys, s = ss, t0 = 0, tE = 1,
+// alpha = 0 and its parent's q, rho, LD and lattice size; curves m..chunk-1
+// are idle.  child_off gets chunk + 1 offsets, child_t the m times.
+lcbg_status launch_pseudo_curves(const CurveRec* parent, int n_parent, const int64_t* offsets, const double* xs,
+                                 const double* ys, const double* ss, int64_t base, int m, int chunk, CurveRec* child,
+                                 int64_t* child_off, double* child_t, hipStream_t st);
+// dmag [epoch][12] of epochs [base, base + m) from the pseudo-curves'
+// derivatives gp [k][8] and the trajectory Jacobian.
+lcbg_status launch_dyn_chain(const double* gp, const double* jac, int64_t base, int m, double* dmag, hipStream_t st);
 
 }  // namespace lcbg

@@ -189,9 +189,11 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
 // lanes 0-23 price the 24 permutations.  Then the branch grid cells and the
 // longest non-wrap segment (:4065-4095).  Stage 1 needs only what
 // k_curve_prep writes (bounding box, n), so it may run beside this kernel.
-__global__ void __launch_bounds__(kWave) k_branch_track(const CurveRec* __restrict__ curves, FsCache cache)
+__global__ void __launch_bounds__(kWave) k_branch_track(const CurveRec* __restrict__ curves, FsCache cache,
+                                                        bool only_near)
 {
     const int c = blockIdx.x;
+    if (only_near && cache.near_n[c] == 0) return;
     const CurveRec cv = curves[c];
     if (!(cv.rho > 0.0) || cv.s == 0.0 || !(cv.q_eff > 0.0)) return;  // no caustic cache
     const int lane = lane_id();
@@ -298,9 +300,10 @@ constexpr int kOrderBlock = 256;
 constexpr int kCells = kGrid * kGrid;
 
 __global__ void __launch_bounds__(kOrderBlock) k_branch_order(const CurveRec* __restrict__ curves,
-                                                              FsCache cache)
+                                                              FsCache cache, bool only_near)
 {
     const int c = blockIdx.x;
+    if (only_near && cache.near_n[c] == 0) return;
     const int tid = threadIdx.x;
     const CurveRec cv = curves[c];
     if (!(cv.rho > 0.0) || cv.s == 0.0 || !(cv.q_eff > 0.0)) return;  // no caustic cache
@@ -363,13 +366,14 @@ lcbg_status launch_curve_prep(const CurveRec* d_curves, int n_curves, const FsCa
     return check_launch();
 }
 
-lcbg_status launch_branch_cache(const CurveRec* d_curves, int n_curves, const FsCache& cache, hipStream_t st)
+lcbg_status launch_branch_cache(const CurveRec* d_curves, int n_curves, const FsCache& cache, hipStream_t st,
+                                bool only_near)
 {
     if (n_curves <= 0) return LCBG_OK;
     hipLaunchKernelGGL(k_branch_track, dim3(static_cast<unsigned>(n_curves)), dim3(kWave), 0, st, d_curves,
-                       cache);
+                       cache, only_near);
     hipLaunchKernelGGL(k_branch_order, dim3(static_cast<unsigned>(n_curves)), dim3(kOrderBlock), 0,
-                       st, d_curves, cache);
+                       st, d_curves, cache, only_near);
     return check_launch();
 }
 

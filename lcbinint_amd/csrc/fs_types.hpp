@@ -130,6 +130,9 @@ struct FsEpochs {
     double* dint;       // [epoch][6] tangents of the magnification (gradient runs)
     double* err;        // [epoch] error estimate of the method that settled the epoch
                         // (FiniteSourceResult::error_estimate, finite_source_magnifier.hpp:47-54)
+    const double* src_y;// [epoch] the impact parameter in place of the curve's u0, or null
+                        // (curves with t0 = 0, tE = 1, alpha = 0 whose source positions
+                        // come from a trajectory: times are x, src_y is y)
 };
 
 // Scratch of the resident waves of each inverse-ray kernel.

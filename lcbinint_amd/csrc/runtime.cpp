@@ -140,6 +140,7 @@ extern "C" lcbg_status lcbg_batch_destroy(lcbg_batch* batch)
 {
     if (!batch) return LCBG_OK;
     if (batch->dyn_child) lcbg_batch_destroy(batch->dyn_child);
+    if (batch->dyn) dyn_destroy(batch->dyn);
     if (batch->graph) (void)hipGraphExecDestroy(batch->graph);
     if (batch->capture_stream) (void)hipStreamDestroy(batch->capture_stream);
     if (batch->fs) fs_destroy(batch->fs);

@@ -42,6 +42,8 @@ private:
 
 struct FsState;
 void fs_destroy(FsState* fs);
+struct DynState;
+void dyn_destroy(DynState* d);
 
 }  // namespace lcbg
 
@@ -56,7 +58,10 @@ struct lcbg_batch {
     bool caches_stale = true;            // per-curve caustic caches need a rebuild
     std::vector<lcbg_curve> h_params;    // the curves as passed (for host trajectories)
     bool dynamic_images = false;         // point-source sum and seeds of the LensModel path
-    lcbg_batch* dyn_child = nullptr;     // per-epoch pseudo-curves of the dynamic path
+    bool one_epoch_curves = false;       // every curve holds at most one epoch (pseudo-curves)
+    lcbg_batch* dyn_child = nullptr;     // pseudo-curves of the dynamic path
+    lcbg::DynState* dyn = nullptr;       // device buffers of the dynamic path
+    const double* src_y = nullptr;       // per-epoch impact parameters replacing u0 (device), or null
     lcbg_schedule schedule{};            // launch schedule (0 fields: defaults)
     int64_t work_budget = 0;             // wave steps per inverse-ray epoch (0: default)
     hipGraphExec_t graph = nullptr;      // captured finite-source run

@@ -79,6 +79,13 @@ class FiniteSourceLightCurves:
         self.curve_of = torch.repeat_interleave(
             torch.arange(self.n_curves, device=device), torch.from_numpy(np.diff(offsets)).to(device))
         self.last_method = None
+        self._dev_epochs = None
+
+    def device_epochs(self, dev):
+        """The offsets and times as device tensors (uploaded once per device)."""
+        if self._dev_epochs is None or self._dev_epochs[0].device != dev:
+            self._dev_epochs = (torch.from_numpy(self.offsets).to(dev), torch.from_numpy(self.times).to(dev))
+        return self._dev_epochs
 
     def __call__(self, params: torch.Tensor) -> torch.Tensor:
         if params.shape != (self.n_curves, 8):
@@ -100,16 +107,28 @@ class _DynamicLightCurveFn(torch.autograd.Function):
                              ld_d=0.0) for r in host]
         dyn = [capi.Dynamics(piEN=r[8], piEE=r[9], g1=r[10], g2=r[11], **fixed)
                for r, fixed in zip(host, owner.fixed)]
-        # host-synchronous path: the curves are read on the host
-        owner.batch.update_curves(curves, torch.cuda.current_stream(params.device).cuda_stream
-                                  if params.is_cuda else 0)
-        mag, method, _, dmag = capi.light_curves_dynamic_grad(owner.batch, dyn, owner.orbit_mode,
-                                                              owner.offsets, owner.times)
-        owner.last_method = method
-        d = torch.from_numpy(dmag).to(params.device)
+        dev = params.device if params.is_cuda else torch.device("cuda")
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        owner.batch.update_curves(curves, stream)
+        d_off, d_t = owner.device_epochs(dev)
+        n = len(owner.times)
+        mag = torch.empty(n, dtype=torch.float64, device=dev)
+        method = torch.empty(n, dtype=torch.int32, device=dev)
+        d = torch.empty((n, len(DYNAMIC_PARAM_NAMES)), dtype=torch.float64, device=dev)
+        dyn_arr = (capi.Dynamics * len(dyn))(*dyn)
+        # trajectories, magnifications and derivatives on the device, in order
+        # on the caller's stream
+        capi.check(capi.lcbg_batch_light_curves_dynamic_device(
+            owner.batch.handle, dyn_arr, owner.orbit_mode, d_off.data_ptr(), d_t.data_ptr(), n,
+            mag.data_ptr(), method.data_ptr(), None, d.data_ptr(), stream),
+            "lcbg_batch_light_curves_dynamic_device")
+        if not bool(torch.isfinite(mag).all()):
+            # a non-finite magnification is a numerical error (lens_model.cpp:228-231)
+            raise capi.LcbgError(2, "lcbg_batch_light_curves_dynamic_device (non-finite magnification)")
+        owner.last_method = method.cpu().numpy()
         ctx.save_for_backward(d)
         ctx.owner = owner
-        return torch.from_numpy(mag).to(params.device)
+        return mag.to(params.device)
 
     @staticmethod
     def backward(ctx, grad_mag):
@@ -128,9 +147,9 @@ class DynamicLightCurves:
     DYNAMIC_PARAM_NAMES: t0, tE, u0, alpha, s, q, rho, u_LD, piEN, piEE, g1, g2).
     `fixed` holds per curve the parameters that are not differentiated (ra,
     dec, tfix, g3, lom_szs, lom_ar).  Values and derivatives come from
-    lcbg_batch_light_curves_dynamic_grad (host trajectories, device
-    magnifications); parallax needs the Earth table
-    (capi.load_earth_ephemeris)."""
+    lcbg_batch_light_curves_dynamic_device (trajectories, magnifications and
+    their derivatives on the device, on the caller's current stream); parallax
+    needs the Earth table (capi.load_earth_ephemeris)."""
 
     def __init__(self, offsets, times, fixed, orbit_mode=capi.ORBIT_CIRCULAR, nbin=None,
                  options: capi.Options | None = None):
@@ -149,6 +168,13 @@ class DynamicLightCurves:
         self.curve_of = torch.repeat_interleave(torch.arange(self.n_curves),
                                                 torch.from_numpy(np.diff(self.offsets)))
         self.last_method = None
+        self._dev_epochs = None
+
+    def device_epochs(self, dev):
+        """The offsets and times as device tensors (uploaded once per device)."""
+        if self._dev_epochs is None or self._dev_epochs[0].device != dev:
+            self._dev_epochs = (torch.from_numpy(self.offsets).to(dev), torch.from_numpy(self.times).to(dev))
+        return self._dev_epochs
 
     def __call__(self, params: torch.Tensor) -> torch.Tensor:
         if params.shape != (self.n_curves, len(DYNAMIC_PARAM_NAMES)):
