@@ -218,6 +218,33 @@ lcbg_status lcbg_batch_dynamic_sources_device(lcbg_batch* batch, const lcbg_dyna
                                               const double* times, int64_t n_epochs, double* x, double* y,
                                               double* s, double* jac, void* stream);
 
+/* The source trajectory of the geometry helper source_trajectory
+ * (python/lcbinint_pybind.cpp:1819-1880, source_for_geometry): per epoch the
+ * source position, with parallax as Trajectory::source_position and, with
+ * orbital motion, the sky-frame position rotated into the binary frame of the
+ * epoch (always through (x cos a + y sin a, -x sin a + y cos a), which differs
+ * from LensModel's (time - t0) / tE without parallax by rounding only).  Host
+ * arrays; evaluated on the device by the trajectory kernel on a one-curve
+ * handle kept per host thread (allocated on its first call). */
+lcbg_status lcbg_source_trajectory(const lcbg_curve* curve, const lcbg_dynamics* dyn,
+                                   int32_t orbital_motion_mode, const double* times, int64_t n, double* x,
+                                   double* y);
+
+/* Critical curves (map_to_source_plane = 0) or caustics (1) of the geometry
+ * helpers critical_curves / caustics (python/lcbinint_pybind.cpp:1882-1926):
+ * the critical-curve quartic at the n_points phases 2 pi i / n_points (the
+ * polynomial and solve of the caustic cache), its four roots or their images
+ * through the lens equation, tracked into four branches as
+ * py_append_tracked_points (:621-658) does (first phase ordered by atan2,
+ * then the permutation of least total squared step).  vbm != 0 takes the
+ * mass ratio as 1/q (:1911-1912).  x, y: host arrays of 4 * n_points, branch
+ * b at [b * n_points, (b + 1) * n_points).  n_points >= 8, else
+ * LCBG_INVALID_ARGUMENT (the reference raises invalid_argument).  The roots
+ * are solved one thread per phase and tracked by one wave on the device; the
+ * call allocates and frees its device buffers. */
+lcbg_status lcbg_geometry_branches(double s, double q, int32_t n_points, int32_t map_to_source_plane,
+                                   int32_t vbm, double* x, double* y);
+
 /* Light curves of the batch's curves with annual parallax and/or orbital
  * motion (dyn: n_curves records), the LensModel path of
  * evaluate_binary_light_curve_numpy (python/lcbinint_pybind.cpp:1159-1173):

@@ -13,7 +13,7 @@ has not been built, every API name raises ImportError when it is used; the
 build module stays importable (``python -m lcbinint_amd.build``).
 """
 _API = ("LightCurve", "LightCurveFunc", "LightCurveInfo", "Options", "LimbDarkening",
-        "OrbitalMotionMode", "FiniteSourceMethod", "EventCoordinates")
+        "OrbitalMotionMode", "FiniteSourceMethod", "EventCoordinates", "GeometryCurve", "GeometryBranches")
 
 # torch first, so that the extension binds to torch's HIP runtime (one HIP
 # runtime per process; see capi.py)
@@ -26,6 +26,8 @@ try:
     from ._lcbinint import (  # noqa: F401
         EventCoordinates,
         FiniteSourceMethod,
+        GeometryBranches,
+        GeometryCurve,
         LightCurve,
         LightCurveFunc,
         LightCurveInfo,

@@ -540,6 +540,7 @@ void traj_consts(const TrajIn<T>& P, const lcbg_dynamics& d, int orbit_mode, Tra
     K.sin_t = to_u(f_sin(P.alpha), tag);
     K.parallax = parallax ? 1 : 0;
     K.mode = orbit_mode;
+    K.sky_rotation = 0;
     K.offset = 0.0;
     for (int i = 0; i < 3; ++i) K.event[i] = K.east[i] = K.north[i] = 0.0;
     const U zero = to_u(T(0.0), tag);

@@ -1016,7 +1016,7 @@ lcbg_status dyn_earth(DynState& d)
 // (lens_model.cpp:117-120).
 lcbg_status dyn_trajectory(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t orbit_mode, const int64_t* off,
                            const double* times, bool with_jac, double* x, double* y, double* s, double* jac,
-                           hipStream_t stream)
+                           hipStream_t stream, bool sky_rotation = false)
 {
     DynState& d = *batch->dyn;
     const int32_t nc = batch->n_curves;
@@ -1031,6 +1031,8 @@ lcbg_status dyn_trajectory(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t 
         trajectory_constants(batch->h_params[static_cast<std::size_t>(c)], dyn[c], orbit_mode,
                              with_jac ? nullptr : &kv[static_cast<std::size_t>(c)],
                              with_jac ? &kd[static_cast<std::size_t>(c)] : nullptr);
+    for (auto& k : kv) k.sky_rotation = sky_rotation ? 1 : 0;
+    for (auto& k : kd) k.sky_rotation = sky_rotation ? 1 : 0;
     const void* kh = with_jac ? static_cast<const void*>(kd.data()) : static_cast<const void*>(kv.data());
     const std::size_t kbytes = with_jac ? kd.size() * sizeof(kd[0]) : kv.size() * sizeof(kv[0]);
     st = map_hip(hipMemcpyAsync(d.consts, kh, kbytes, hipMemcpyHostToDevice, stream));
@@ -1555,4 +1557,78 @@ extern "C" lcbg_status lcbg_batch_epoch_results_device(lcbg_batch* batch, int64_
     ep.err = err_view(fs);
     return launch_epoch_results(batch->d_curves, fs->opt, ep, batch->options.finite_source_tol,
                                 batch->options.finite_source_reltol, results, static_cast<hipStream_t>(stream));
+}
+
+namespace {
+// One-curve batch of the source-trajectory helper, kept per host thread.
+thread_local lcbg_batch* g_geometry_batch = nullptr;
+thread_local int g_geometry_device = -1;
+}  // namespace
+
+extern "C" lcbg_status lcbg_source_trajectory(const lcbg_curve* curve, const lcbg_dynamics* dyn,
+                                              int32_t orbital_motion_mode, const double* times, int64_t n,
+                                              double* x, double* y)
+{
+    if (!curve || !dyn || n < 0 || (n > 0 && (!times || !x || !y))) return LCBG_INVALID_ARGUMENT;
+    if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER)
+        return LCBG_UNSUPPORTED;
+    if (n == 0) return LCBG_OK;
+    int dev = 0;
+    lcbg_status st = map_hip(hipGetDevice(&dev));
+    if (st != LCBG_OK) return st;
+    if (g_geometry_batch && g_geometry_device != dev) {
+        // a handle of another device is left to the process end
+        g_geometry_batch = nullptr;
+    }
+    if (!g_geometry_batch) {
+        const lcbg_options opt = lcbg_default_options();
+        st = lcbg_batch_create(curve, 1, &opt, &g_geometry_batch);
+        if (st != LCBG_OK) return st;
+        g_geometry_device = dev;
+    } else {
+        st = lcbg_batch_update_curves(g_geometry_batch, curve, nullptr);
+        if (st != LCBG_OK) return st;
+    }
+    lcbg_batch* b = g_geometry_batch;
+    st = dyn_reserve(b, n);
+    if (st != LCBG_OK) return st;
+    DynState& d = *b->dyn;
+    const int64_t off[2] = {0, n};
+    st = map_hip(hipMemcpy(d.off, off, sizeof(off), hipMemcpyHostToDevice));
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(d.t, times, n * sizeof(double), hipMemcpyHostToDevice));
+    if (st == LCBG_OK)
+        st = dyn_trajectory(b, dyn, orbital_motion_mode, d.off, d.t, false, d.x, d.y, d.s, nullptr, nullptr, true);
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(x, d.x, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(y, d.y, n * sizeof(double), hipMemcpyDeviceToHost));
+    return st;
+}
+
+extern "C" lcbg_status lcbg_geometry_branches(double s, double q, int32_t n_points, int32_t map_to_source_plane,
+                                              int32_t vbm, double* x, double* y)
+{
+    if (n_points < 8 || !x || !y) return LCBG_INVALID_ARGUMENT;
+    const double q_eff = (vbm != 0 && q != 0.0) ? 1.0 / q : q;
+    const std::size_t n = static_cast<std::size_t>(n_points);
+    std::vector<Pt2> host(n);
+    for (std::size_t i = 0; i < n; ++i) {
+        // as geometry_branches writes the phase (python/lcbinint_pybind.cpp:1917-1918)
+        const double phase = 2.0 * kPiHost * static_cast<double>(i) / static_cast<double>(n_points);
+        host[i] = Pt2{std::cos(phase), std::sin(phase)};
+    }
+    Pt2* mem = nullptr;  // phases, points [n][4], branches [4][n]
+    lcbg_status st = map_hip(hipMalloc(reinterpret_cast<void**>(&mem), 9 * n * sizeof(Pt2)));
+    if (st != LCBG_OK) return st;
+    st = map_hip(hipMemcpy(mem, host.data(), n * sizeof(Pt2), hipMemcpyHostToDevice));
+    if (st == LCBG_OK)
+        st = launch_geometry_branches(s, q_eff, mem, n_points, map_to_source_plane != 0, mem + n, mem + 5 * n,
+                                      nullptr);
+    std::vector<Pt2> br(4 * n);
+    if (st == LCBG_OK) st = map_hip(hipMemcpy(br.data(), mem + 5 * n, 4 * n * sizeof(Pt2), hipMemcpyDeviceToHost));
+    (void)hipFree(mem);
+    if (st != LCBG_OK) return st;
+    for (std::size_t k = 0; k < 4 * n; ++k) {
+        x[k] = br[k].x;
+        y[k] = br[k].y;
+    }
+    return LCBG_OK;
 }

@@ -120,6 +120,12 @@ lcbg_status launch_epoch_results(const CurveRec* d_curves, const FsOptions& opt,
 // Zeroes the queue counters in-stream with a kernel (graph-capturable).
 lcbg_status launch_reset_counters(unsigned* counters, int n, hipStream_t st);
 
+// Critical curves (to_source false) or caustics of one lens at n phases
+// (trig: cos, sin of 2 pi i / n): pts [n][4] the points, br [4][n] the tracked
+// branches.
+lcbg_status launch_geometry_branches(double s, double q_eff, const Pt2* trig, int n, bool to_source, Pt2* pts,
+                                     Pt2* br, hipStream_t st);
+
 // Device trajectories (traj.hip).  consts: n_curves TrajConst<double>, or
 // TrajConst<TDual<9>> with with_jac (then jac [epoch][3][9] is written);
 // per epoch the source position xs, ys in the binary frame and the

@@ -182,13 +182,96 @@ __global__ void __launch_bounds__(kPrepBlock) k_curve_prep(const CurveRec* __res
     for (int k = tid; k < kSeedPhases * 4; k += kPrepBlock) assign[k] = 255;
 }
 
-// Branch tracking (append_tracked_caustic_points :569-611), one wave per curve
-// so that every curve's sequential chain of phases runs at once: 64 phases at a
-// time, lane j loads phase base + j and keeps the points chosen for it (stored
-// once the chunk is done), a step reads its phase's points from lane i - base,
-// lanes 0-23 price the 24 permutations.  Then the branch grid cells and the
-// longest non-wrap segment (:4065-4095).  Stage 1 needs only what
-// k_curve_prep writes (bounding box, n), so it may run beside this kernel.
+// The four points of each of `bins` phases (pts [phase][4]) tracked into four
+// branches (br [branch][bins]) by append_tracked_caustic_points
+// (finite_source_magnifier.cpp:569-611): the first phase ordered by atan2,
+// every later one by the permutation of least total squared step, the first
+// such permutation in std::next_permutation order, the identity when no cost
+// is below +inf.  One wave: 64 phases per coalesced load held in lanes, each
+// phase's points taken by readlane and picked by register selects, the 24
+// permutations priced on lanes 0-23, argmin by DPP.
+__device__ __forceinline__ void track_branches(const Pt* pts, int bins, Pt* br, int lane)
+{
+    Pt prev[4];
+    {
+        // first phase: order by atan2 (std::sort of 4 elements = insertion sort)
+        Pt p[4];
+        double key[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            p[r] = pts[r];
+            key[r] = atan2(p[r].y, p[r].x);
+        }
+        for (int i = 1; i < 4; ++i) {
+            const Pt vp = p[i];
+            const double vk = key[i];
+            int j = i;
+            while (j > 0 && vk < key[j - 1]) {
+                p[j] = p[j - 1];
+                key[j] = key[j - 1];
+                --j;
+            }
+            p[j] = vp;
+            key[j] = vk;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) prev[r] = p[r];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) wave_store(&br[r * bins], prev[r]);
+    }
+    const int pl = lane < 24 ? lane : 0;
+    const int p0 = c_perm[pl][0], p1 = c_perm[pl][1], p2 = c_perm[pl][2], p3 = c_perm[pl][3];
+    for (int base = 0; base < bins; base += kWave) {
+        Pt ch[4], keep[4];
+        const int ph = base + lane;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            ch[r] = ph < bins ? pts[4 * ph + r] : Pt{0.0, 0.0};
+            keep[r] = Pt{0.0, 0.0};
+        }
+        const int i0 = base > 0 ? base : 1;
+        const int i1 = base + kWave < bins ? base + kWave : bins;
+        for (int i = i0; i < i1; ++i) {
+            const int j = i - base;
+            const Pt c0{read_lane(ch[0].x, j), read_lane(ch[0].y, j)};
+            const Pt c1{read_lane(ch[1].x, j), read_lane(ch[1].y, j)};
+            const Pt c2{read_lane(ch[2].x, j), read_lane(ch[2].y, j)};
+            const Pt c3{read_lane(ch[3].x, j), read_lane(ch[3].y, j)};
+            double cost = 0.0;
+            cost += dist2(prev[0], pick4(c0, c1, c2, c3, p0));
+            cost += dist2(prev[1], pick4(c0, c1, c2, c3, p1));
+            cost += dist2(prev[2], pick4(c0, c1, c2, c3, p2));
+            cost += dist2(prev[3], pick4(c0, c1, c2, c3, p3));
+            if (lane >= 24) cost = __builtin_inf();
+            int idx = lane;
+            argmin_rows01(cost, idx);
+            // the reference keeps the identity when no cost is below +inf
+            if (!(cost < __builtin_inf())) idx = 0;
+            const int b0 = __builtin_amdgcn_readlane(p0, idx), b1 = __builtin_amdgcn_readlane(p1, idx);
+            const int b2 = __builtin_amdgcn_readlane(p2, idx), b3 = __builtin_amdgcn_readlane(p3, idx);
+            prev[0] = pick4(c0, c1, c2, c3, b0);
+            prev[1] = pick4(c0, c1, c2, c3, b1);
+            prev[2] = pick4(c0, c1, c2, c3, b2);
+            prev[3] = pick4(c0, c1, c2, c3, b3);
+            const bool mine = lane == j;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                keep[r].x = mine ? prev[r].x : keep[r].x;
+                keep[r].y = mine ? prev[r].y : keep[r].y;
+            }
+        }
+        if (ph >= i0 && ph < i1) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) br[r * bins + ph] = keep[r];
+        }
+    }
+}
+
+// Branch tracking of the caustic cache, one wave per curve so that every
+// curve's sequential chain of phases runs at once (track_branches).  Then the
+// branch grid cells and the longest non-wrap segment (:4065-4095).  Stage 1
+// needs only what k_curve_prep writes (bounding box, n), so it may run beside
+// this kernel.
 __global__ void __launch_bounds__(kWave) k_branch_track(const CurveRec* __restrict__ curves, FsCache cache,
                                                         bool only_near)
 {
@@ -202,81 +285,7 @@ __global__ void __launch_bounds__(kWave) k_branch_track(const CurveRec* __restri
     Pt* br = cache.br + static_cast<size_t>(c) * bins * 4;
     int* br_cell = cache.br_cell + static_cast<size_t>(c) * bins * 4;
     CausticScalars* sc = cache.sc + c;
-    {
-        Pt prev[4];
-        {
-            // first phase: order by atan2 (std::sort of 4 elements = insertion sort)
-            Pt p[4];
-            double key[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                p[r] = pts[r];
-                key[r] = atan2(p[r].y, p[r].x);
-            }
-            for (int i = 1; i < 4; ++i) {
-                const Pt vp = p[i];
-                const double vk = key[i];
-                int j = i;
-                while (j > 0 && vk < key[j - 1]) {
-                    p[j] = p[j - 1];
-                    key[j] = key[j - 1];
-                    --j;
-                }
-                p[j] = vp;
-                key[j] = vk;
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) prev[r] = p[r];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) wave_store(&br[r * bins], prev[r]);
-        }
-        const int pl = lane < 24 ? lane : 0;
-        const int p0 = c_perm[pl][0], p1 = c_perm[pl][1], p2 = c_perm[pl][2], p3 = c_perm[pl][3];
-        for (int base = 0; base < bins; base += kWave) {
-            Pt ch[4], keep[4];
-            const int ph = base + lane;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                ch[r] = ph < bins ? pts[4 * ph + r] : Pt{0.0, 0.0};
-                keep[r] = Pt{0.0, 0.0};
-            }
-            const int i0 = base > 0 ? base : 1;
-            const int i1 = base + kWave < bins ? base + kWave : bins;
-            for (int i = i0; i < i1; ++i) {
-                const int j = i - base;
-                const Pt c0{read_lane(ch[0].x, j), read_lane(ch[0].y, j)};
-                const Pt c1{read_lane(ch[1].x, j), read_lane(ch[1].y, j)};
-                const Pt c2{read_lane(ch[2].x, j), read_lane(ch[2].y, j)};
-                const Pt c3{read_lane(ch[3].x, j), read_lane(ch[3].y, j)};
-                double cost = 0.0;
-                cost += dist2(prev[0], pick4(c0, c1, c2, c3, p0));
-                cost += dist2(prev[1], pick4(c0, c1, c2, c3, p1));
-                cost += dist2(prev[2], pick4(c0, c1, c2, c3, p2));
-                cost += dist2(prev[3], pick4(c0, c1, c2, c3, p3));
-                if (lane >= 24) cost = __builtin_inf();
-                int idx = lane;
-                argmin_rows01(cost, idx);
-                // the reference keeps the identity when no cost is below +inf
-                if (!(cost < __builtin_inf())) idx = 0;
-                const int b0 = __builtin_amdgcn_readlane(p0, idx), b1 = __builtin_amdgcn_readlane(p1, idx);
-                const int b2 = __builtin_amdgcn_readlane(p2, idx), b3 = __builtin_amdgcn_readlane(p3, idx);
-                prev[0] = pick4(c0, c1, c2, c3, b0);
-                prev[1] = pick4(c0, c1, c2, c3, b1);
-                prev[2] = pick4(c0, c1, c2, c3, b2);
-                prev[3] = pick4(c0, c1, c2, c3, b3);
-                const bool mine = lane == j;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    keep[r].x = mine ? prev[r].x : keep[r].x;
-                    keep[r].y = mine ? prev[r].y : keep[r].y;
-                }
-            }
-            if (ph >= i0 && ph < i1) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) br[r * bins + ph] = keep[r];
-            }
-        }
-    }
+    track_branches(pts, bins, br, lane);
     __syncthreads();
     const double min_x = sc->min_x, min_y = sc->min_y, step_x = sc->step_x, step_y = sc->step_y;
     double seg2 = 0.0;
@@ -374,6 +383,39 @@ lcbg_status launch_branch_cache(const CurveRec* d_curves, int n_curves, const Fs
                        cache, only_near);
     hipLaunchKernelGGL(k_branch_order, dim3(static_cast<unsigned>(n_curves)), dim3(kOrderBlock), 0,
                        st, d_curves, cache, only_near);
+    return check_launch();
+}
+
+// The geometry helpers' critical curves and caustics (geometry_branches,
+// python/lcbinint_pybind.cpp:1882-1926): the critical-curve quartic at n
+// phases, one thread each (the same polynomial and solve as the caustic
+// cache), its roots or their images through the lens equation; then the four
+// branches tracked by one wave as the caustic cache tracks them.
+__global__ void __launch_bounds__(kPrepBlock) k_geometry_points(double s, double q_eff, const Pt* __restrict__ trig,
+                                                                int n, bool to_source, Pt* pts)
+{
+    const int i = static_cast<int>(blockIdx.x) * kPrepBlock + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    const Lens L = make_lens(s, q_eff);
+    const Pt ph = trig[i];
+    cx roots[4];
+    critical_roots(L, cx{ph.x, ph.y}, roots);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pts[4 * i + r] = to_source ? lens_equation_complex(L, roots[r]) : Pt{roots[r].re, roots[r].im};
+}
+
+__global__ void __launch_bounds__(kWave) k_geometry_track(const Pt* __restrict__ pts, int n, Pt* br)
+{
+    track_branches(pts, n, br, lane_id());
+}
+
+lcbg_status launch_geometry_branches(double s, double q_eff, const Pt2* trig, int n, bool to_source, Pt2* pts,
+                                     Pt2* br, hipStream_t st)
+{
+    if (n <= 0) return LCBG_OK;
+    const unsigned blocks = static_cast<unsigned>((n + kPrepBlock - 1) / kPrepBlock);
+    hipLaunchKernelGGL(k_geometry_points, dim3(blocks), dim3(kPrepBlock), 0, st, s, q_eff, trig, n, to_source, pts);
+    hipLaunchKernelGGL(k_geometry_track, dim3(1), dim3(kWave), 0, st, pts, n, br);
     return check_launch();
 }
 

@@ -16,7 +16,10 @@
 #include <pybind11/stl.h>
 
 #include <cmath>
+#include <cstring>
 #include <limits>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -149,13 +152,35 @@ struct PyEvent {
     double ra = 0.0, dec = 0.0, tfix = 0.0, obs_lat = 0.0, obs_lon = 0.0;
 };
 
+// GeometryCurve and GeometryBranches (python/lcbinint_pybind.cpp:45-54)
+struct PyGeometryCurve {
+    std::vector<double> times, x, y;
+};
+
+struct PyGeometryBranches {
+    std::vector<std::vector<double>> x, y;
+};
+
+// The one-curve handle of the LensModel path, kept by the LightCurve across
+// calls (its device buffers and caches are reused; the curve is updated).
+struct DynHandle {
+    std::mutex mu;
+    lcbg_batch* batch = nullptr;
+    lcbg_options options{};
+    ~DynHandle()
+    {
+        if (batch) lcbg_batch_destroy(batch);
+    }
+};
+
 class PyLightCurve {
 public:
     PyLightCurve(std::string lens, std::string source, const PyEvent& event, const PyOptions& options,
                  const PyLimbDarkening& ld, int orbital_motion_mode, bool parallax,
                  bool terrestrial_parallax)
         : lens_(std::move(lens)), source_(std::move(source)), event_(event), options_(options), ld_(ld),
-          orbit_(orbital_motion_mode), parallax_(parallax), terrestrial_(terrestrial_parallax)
+          orbit_(orbital_motion_mode), parallax_(parallax), terrestrial_(terrestrial_parallax),
+          dyn_(std::make_shared<DynHandle>())
     {
         if (lens_ != "binary_lens" && lens_ != "triple_lens")
             throw std::invalid_argument("lens must be 'binary_lens' or 'triple_lens'");
@@ -214,12 +239,12 @@ public:
                 st = lcbg_light_curve_results(&cv, &opts, t.data(), n, rec.data(), &first);
             } else {
                 const lcbg_dynamics dyn{pN, pE, event_.ra, event_.dec, event_.tfix, g1, g2, g3, lom_szs, lom_ar};
+                std::lock_guard<std::mutex> lock(dyn_->mu);
                 lcbg_batch* b = nullptr;
-                st = lcbg_batch_create(&cv, 1, &opts, &b);
+                st = dyn_batch(cv, opts, &b);
                 const int64_t off[2] = {0, n};
                 if (st == LCBG_OK)
                     st = lcbg_batch_light_curves_dynamic_results(b, &dyn, orbit_, off, t.data(), n, rec.data());
-                lcbg_batch_destroy(b);
             }
         }
         raise_status(st);
@@ -274,14 +299,14 @@ public:
             // LensModel path (python/lcbinint_pybind.cpp:1159-1173)
             const lcbg_dynamics dyn{pN, pE, event_.ra, event_.dec, event_.tfix, g1, g2, g3, lom_szs, lom_ar};
             py::gil_scoped_release release;
+            std::lock_guard<std::mutex> lock(dyn_->mu);
             lcbg_batch* b = nullptr;
-            st = lcbg_batch_create(&cv, 1, &opts, &b);
+            st = dyn_batch(cv, opts, &b);
             const int64_t off[2] = {0, static_cast<int64_t>(n)};
             std::vector<int32_t> meth(static_cast<std::size_t>(n));
             if (st == LCBG_OK)
                 st = lcbg_batch_light_curves_dynamic(b, &dyn, orbit_, off, t.data(), static_cast<int64_t>(n),
                                                      out.mutable_data(), meth.data());
-            lcbg_batch_destroy(b);
             if (st == LCBG_OK) {
                 const double* v = out.data();
                 for (py::ssize_t i = 0; i < n; ++i)
@@ -303,6 +328,126 @@ public:
                         dict_get(p, "lom_ar", 1.0));
     }
 
+    // source_for_geometry / dynamic_source_trajectory (python/lcbinint_pybind.cpp:1819-1880)
+    PyGeometryCurve trajectory(const std::vector<double>& times, double t0, double tE, double u0, double alpha,
+                               double s, double q, double piEN, double piEE, double g1, double g2, double g3,
+                               double lom_szs, double lom_ar) const
+    {
+        // the device trajectory is the VBM one (the legacy frame's formulas are not built)
+        if (options_.vbm_compatible == 0) throw std::runtime_error("unsupported");
+        PyGeometryCurve out;
+        out.times = times;
+        out.x.resize(times.size());
+        out.y.resize(times.size());
+        const lcbg_curve cv{t0, tE, u0, alpha, s, q, 0.0, ld_.c, ld_.d};
+        const lcbg_dynamics dyn{piEN, piEE, event_.ra, event_.dec, event_.tfix, g1, g2, g3, lom_szs, lom_ar};
+        lcbg_status st;
+        {
+            py::gil_scoped_release release;
+            st = lcbg_source_trajectory(&cv, &dyn, orbit_, times.data(), static_cast<int64_t>(times.size()),
+                                        out.x.data(), out.y.data());
+        }
+        raise_status(st);
+        return out;
+    }
+
+    void require_parallax() const
+    {
+        if (!parallax_)
+            throw std::invalid_argument(
+                "piEN/piEE require LightCurve(parallax=True); the current trajectory model is static");
+    }
+
+    void require_binary_positional_api() const
+    {
+        if (lens_ != "binary_lens")
+            throw std::invalid_argument("triple_lens requires a parameter dictionary with q2, sep2, and ang");
+    }
+
+    PyGeometryCurve source_trajectory_from_dict(const std::vector<double>& times, const py::dict& p) const
+    {
+        if (lens_ != "binary_lens") throw std::runtime_error("unsupported");
+        return trajectory(times, dict_get(p, "t0", 0.0), dict_get(p, "tE", 1.0), dict_get(p, "u0", 0.0),
+                          dict_get(p, "alpha", 0.0), dict_get(p, "s", 1.0), dict_get(p, "q", 1.0),
+                          dict_get(p, "piEN", 0.0), dict_get(p, "piEE", 0.0), dict_get(p, "g1", 0.0),
+                          dict_get(p, "g2", 0.0), dict_get(p, "g3", 0.0), dict_get(p, "lom_szs", 0.0),
+                          dict_get(p, "lom_ar", 1.0));
+    }
+
+    PyGeometryCurve source_trajectory_static(const std::vector<double>& times, double t0, double tE, double u0,
+                                             double alpha, double s, double q, double g1, double g2, double g3,
+                                             double lom_szs, double lom_ar) const
+    {
+        require_binary_positional_api();
+        return trajectory(times, t0, tE, u0, alpha, s, q, 0.0, 0.0, g1, g2, g3, lom_szs, lom_ar);
+    }
+
+    PyGeometryCurve source_trajectory(const std::vector<double>& times, double t0, double tE, double u0,
+                                      double alpha, double s, double q, double piEN, double piEE, double g1,
+                                      double g2, double g3, double lom_szs, double lom_ar) const
+    {
+        require_binary_positional_api();
+        require_parallax();
+        return trajectory(times, t0, tE, u0, alpha, s, q, piEN, piEE, g1, g2, g3, lom_szs, lom_ar);
+    }
+
+    // geometry_branches (python/lcbinint_pybind.cpp:1882-1926)
+    PyGeometryBranches branches(double s, double q, int n_points, double time, double t0, double tE, double u0,
+                                double alpha, double g1, double g2, double g3, double lom_szs, double lom_ar,
+                                bool to_source) const
+    {
+        if (n_points < 8) throw std::invalid_argument("n_points must be at least 8");
+        double separation = s;
+        lcbg_status st = LCBG_OK;
+        if (orbit_ != STATIC && std::isfinite(time)) {
+            // the orbit's separation at `time` (host orbit constants, as the reference's orbit cache)
+            const lcbg_curve cv{t0, tE, u0, alpha, s, q, 0.0, ld_.c, ld_.d};
+            const lcbg_dynamics dyn{0.0, 0.0, event_.ra, event_.dec, event_.tfix, g1, g2, g3, lom_szs, lom_ar};
+            double xys[3];
+            st = lcbg_dynamic_sources(&cv, &dyn, orbit_, &time, 1, xys);
+            separation = xys[2];
+        }
+        raise_status(st);
+        const std::size_t n = static_cast<std::size_t>(n_points);
+        std::vector<double> x(4 * n), y(4 * n);
+        {
+            py::gil_scoped_release release;
+            st = lcbg_geometry_branches(separation, q, n_points, to_source ? 1 : 0, options_.vbm_compatible, x.data(),
+                                        y.data());
+        }
+        raise_status(st);
+        PyGeometryBranches out;
+        for (std::size_t b = 0; b < 4; ++b) {
+            out.x.emplace_back(x.begin() + b * n, x.begin() + (b + 1) * n);
+            out.y.emplace_back(y.begin() + b * n, y.begin() + (b + 1) * n);
+        }
+        return out;
+    }
+
+    PyGeometryBranches caustics(double s, double q, int n_points, double time, double t0, double tE, double u0,
+                                double alpha, double g1, double g2, double g3, double lom_szs, double lom_ar) const
+    {
+        require_binary_positional_api();
+        return branches(s, q, n_points, time, t0, tE, u0, alpha, g1, g2, g3, lom_szs, lom_ar, true);
+    }
+
+    PyGeometryBranches critical_curves(double s, double q, int n_points, double time, double t0, double tE,
+                                       double u0, double alpha, double g1, double g2, double g3, double lom_szs,
+                                       double lom_ar) const
+    {
+        require_binary_positional_api();
+        return branches(s, q, n_points, time, t0, tE, u0, alpha, g1, g2, g3, lom_szs, lom_ar, false);
+    }
+
+    // caustics_from_dict (python/lcbinint_pybind.cpp:2539-2556): the lens at
+    // its reference separation
+    PyGeometryBranches caustics_from_dict(const py::dict& p, int n_points) const
+    {
+        if (lens_ != "binary_lens") throw std::runtime_error("unsupported");
+        return branches(dict_get(p, "s", 1.0), dict_get(p, "q", 1.0), n_points, kNaN, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0,
+                        0.0, 0.0, 1.0, true);
+    }
+
     const PyEvent& event() const { return event_; }
 
     const std::string& lens() const { return lens_; }
@@ -320,6 +465,26 @@ private:
     PyLimbDarkening ld_;
     int orbit_;
     bool parallax_, terrestrial_;
+    std::shared_ptr<DynHandle> dyn_;
+
+    // the kept one-curve handle, holding `cv` under `opts`; dyn_->mu held
+    lcbg_status dyn_batch(const lcbg_curve& cv, const lcbg_options& opts, lcbg_batch** out) const
+    {
+        DynHandle& h = *dyn_;
+        if (h.batch && std::memcmp(&h.options, &opts, sizeof(lcbg_options)) != 0) {
+            lcbg_batch_destroy(h.batch);
+            h.batch = nullptr;
+        }
+        lcbg_status st = LCBG_OK;
+        if (!h.batch) {
+            st = lcbg_batch_create(&cv, 1, &opts, &h.batch);
+            h.options = opts;
+        } else {
+            st = lcbg_batch_update_curves(h.batch, &cv, nullptr);
+        }
+        *out = h.batch;
+        return st;
+    }
 };
 
 }  // namespace
@@ -478,6 +643,15 @@ PYBIND11_MODULE(_lcbinint, m)
         .def_readwrite("obs_lat", &PyEvent::obs_lat)
         .def_readwrite("obs_lon", &PyEvent::obs_lon);
 
+    py::class_<PyGeometryCurve>(m, "GeometryCurve")
+        .def_readonly("times", &PyGeometryCurve::times)
+        .def_readonly("x", &PyGeometryCurve::x)
+        .def_readonly("y", &PyGeometryCurve::y);
+
+    py::class_<PyGeometryBranches>(m, "GeometryBranches")
+        .def_readonly("x", &PyGeometryBranches::x)
+        .def_readonly("y", &PyGeometryBranches::y);
+
     py::class_<PyLightCurve>(m, "LightCurve")
         .def(py::init([](const std::string& lens, const std::string& source, const PyEvent& event,
                          const PyOptions& options, const PyLimbDarkening& ld, OrbitalMotionMode orbital_motion_mode,
@@ -509,7 +683,27 @@ PYBIND11_MODULE(_lcbinint, m)
              py::arg("tE") = 1.0, py::arg("u0") = 0.0, py::arg("alpha") = 0.0, py::arg("s") = 1.0,
              py::arg("q") = 1.0, py::arg("rho") = 0.0, py::arg("piEN") = 0.0, py::arg("piEE") = 0.0,
              py::arg("g1") = 0.0, py::arg("g2") = 0.0, py::arg("g3") = 0.0, py::arg("lom_szs") = 0.0,
-             py::arg("lom_ar") = 1.0);
+             py::arg("lom_ar") = 1.0)
+        // geometry helpers (python/lcbinint_pybind.cpp:3292-3364)
+        .def("source_trajectory", &PyLightCurve::source_trajectory_from_dict, py::arg("times"), py::arg("params"))
+        .def("source_trajectory", &PyLightCurve::source_trajectory_static, py::arg("times"), py::kw_only(),
+             py::arg("t0") = 0.0, py::arg("tE") = 1.0, py::arg("u0") = 0.0, py::arg("alpha") = 0.0,
+             py::arg("s") = 1.0, py::arg("q") = 1.0, py::arg("g1") = 0.0, py::arg("g2") = 0.0,
+             py::arg("g3") = 0.0, py::arg("lom_szs") = 0.0, py::arg("lom_ar") = 1.0)
+        .def("source_trajectory", &PyLightCurve::source_trajectory, py::arg("times"), py::kw_only(),
+             py::arg("t0") = 0.0, py::arg("tE") = 1.0, py::arg("u0") = 0.0, py::arg("alpha") = 0.0,
+             py::arg("s") = 1.0, py::arg("q") = 1.0, py::arg("piEN") = 0.0, py::arg("piEE") = 0.0,
+             py::arg("g1") = 0.0, py::arg("g2") = 0.0, py::arg("g3") = 0.0, py::arg("lom_szs") = 0.0,
+             py::arg("lom_ar") = 1.0)
+        .def("caustics", &PyLightCurve::caustics_from_dict, py::arg("params"), py::arg("n_points") = 1000)
+        .def("caustics", &PyLightCurve::caustics, py::kw_only(), py::arg("s") = 1.0, py::arg("q") = 1.0,
+             py::arg("n_points") = 1400, py::arg("time") = kNaN, py::arg("t0") = 0.0, py::arg("tE") = 1.0,
+             py::arg("u0") = 0.0, py::arg("alpha") = 0.0, py::arg("g1") = 0.0, py::arg("g2") = 0.0,
+             py::arg("g3") = 0.0, py::arg("lom_szs") = 0.0, py::arg("lom_ar") = 1.0)
+        .def("critical_curves", &PyLightCurve::critical_curves, py::kw_only(), py::arg("s") = 1.0,
+             py::arg("q") = 1.0, py::arg("n_points") = 1400, py::arg("time") = kNaN, py::arg("t0") = 0.0,
+             py::arg("tE") = 1.0, py::arg("u0") = 0.0, py::arg("alpha") = 0.0, py::arg("g1") = 0.0,
+             py::arg("g2") = 0.0, py::arg("g3") = 0.0, py::arg("lom_szs") = 0.0, py::arg("lom_ar") = 1.0);
 
     // python/lcbinint_pybind.cpp:3405
     m.attr("LightCurveFunc") = m.attr("LightCurve");

@@ -163,6 +163,8 @@ template <class T>
 struct TrajConst {
     T t0, tE, u0, cos_t, sin_t;
     int parallax, mode;
+    int sky_rotation;  // with orbital motion, rotate the sky-frame source (x, y) even without
+                       // parallax (the geometry helpers' source_for_geometry)
     // parallax
     double offset;                        // JD origin added to reduced dates
     double event[3], east[3], north[3];   // sky basis of the event
@@ -282,7 +284,7 @@ __device__ inline bool traj_epoch(const TrajConst<T>& K, const EarthRowD* rows, 
     const bool ok = isfinite(tval(s_t)) && isfinite(tval(ang));
     if (K.mode != kTrajStatic) {
         T tau, b;
-        if (!K.parallax) {
+        if (!K.parallax && !K.sky_rotation) {
             tau = (time - K.t0) / K.tE;
             b = K.u0;
         } else {

@@ -139,6 +139,8 @@ class Ref:
         self._ir = _f(L, "ref_binary_inverse_ray", D, P(RefSettings), D, D, D, D, D, P(I32))
         self._caus = _f(L, "ref_caustic_points_at_phase", I32, D, D, D, V)
         self._crits = _f(L, "ref_critical_sources_at_phase", I32, D, D, D, V)
+        self._critp = _f(L, "ref_critical_points_at_phase", I32, D, D, D, V)
+        self._geo_src = _f(L, "ref_source_for_geometry", I32, V, I32, V, ctypes.c_long, V)
         self._seeds = _f(L, "ref_image_seeds", I32, P(RefSettings), D, D, D, D, D, V, I32)
         self._cart = _f(L, "ref_cartesian_area", D, P(RefSettings), D, D, D, D, D, V, I32, V)
         self._polar = _f(L, "ref_polar_area", D, P(RefSettings), D, D, D, D, D, V, I32)
@@ -242,6 +244,22 @@ class Ref:
         out = np.zeros(8)
         k = self._caus(s, q, phase, ptr(out))
         return out[:2 * k].reshape(-1, 2)
+
+    def critical_points(self, s, q, phase):
+        """The four roots of the critical-curve quartic at one phase, (k, 2)."""
+        out = np.zeros(8)
+        k = self._critp(s, q, phase, ptr(out))
+        return out[:2 * k].reshape(-1, 2)
+
+    def source_for_geometry(self, times, params, orbit_mode=0):
+        """The geometry helpers' source positions, (n, 2); params as
+        light_curve_dynamic takes them."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        p = np.array([params.get(k, 1.0 if k in ("lom_ar", "tE") else 0.0) for k in self.DYNAMIC_KEYS],
+                     dtype=np.float64)
+        out = np.empty((len(t), 2))
+        self._geo_src(ptr(p), int(orbit_mode), ptr(t), len(t), ptr(out))
+        return out
 
     def critical_sources(self, s, q, phase):
         out = np.zeros(8)

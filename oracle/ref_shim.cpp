@@ -313,6 +313,22 @@ int ref_caustic_points_at_phase(double s, double q, double phase, double* out)
     return static_cast<int>(pts.size());
 }
 
+// The roots of the critical-curve quartic at one phase (x, y interleaved), as
+// py_critical_points_at_phase (python/lcbinint_pybind.cpp:585-604) and the
+// first half of caustic_points_at_phase compute them.
+int ref_critical_points_at_phase(double s, double q, double phase, double* out)
+{
+    const lcbinint::Complex ph = std::polar(1.0, phase);
+    lcbinint::math::PolynomialRootSolver solver;
+    const auto roots = solver.solve(fsm::critical_curve_polynomial_coefficients(s, q, ph));
+    if (roots.status != lcbinint::math::RootSolverStatus::ok) return 0;
+    for (std::size_t i = 0; i < roots.roots.size(); ++i) {
+        out[2 * i] = roots.roots[i].real();
+        out[2 * i + 1] = roots.roots[i].imag();
+    }
+    return static_cast<int>(roots.roots.size());
+}
+
 // critical_sources_at_phase (finite_source_magnifier.cpp:1622-1641).
 int ref_critical_sources_at_phase(double s, double q, double phase, double* out)
 {
@@ -484,6 +500,61 @@ int ref_light_curve_dynamic(const RefSettings* rs, const double* times, long n, 
         out[i] = fs.magnification;
         if (method) method[i] = static_cast<int>(fs.decision.method);
         if (!std::isfinite(fs.magnification)) return 2;
+    }
+    return 0;
+}
+
+// The geometry helpers' source_for_geometry (python/lcbinint_pybind.cpp:1840-1880),
+// VBM frame, over the core's Trajectory and orbital_state (the binding's orbit
+// cache, :803-925, evaluates orbital_state's expressions).  p as in
+// ref_light_curve_dynamic; xy: x, y interleaved.
+int ref_source_for_geometry(const double* p, int orbit_mode, const double* times, long n, double* xy)
+{
+    using namespace lcbinint::model;
+    LensParameters lp;
+    lp.t0 = p[0];
+    lp.tE = p[1];
+    lp.umin = p[2];
+    lp.theta = p[3];
+    lp.sep = p[4];
+    lp.q = p[5];
+    lp.rho = p[6];
+    lp.piEN = p[7];
+    lp.piEE = p[8];
+    lp.ra = p[9];
+    lp.dec = p[10];
+    lp.tfix = p[11];
+    lp.g1 = p[12];
+    lp.g2 = p[13];
+    lp.g3 = p[14];
+    lp.lom_szs = p[15];
+    lp.lom_ar = p[16];
+    lp.orbital_motion_mode = static_cast<lcbi_orbital_motion_mode>(orbit_mode);
+    const Trajectory trajectory(lp);
+    for (long i = 0; i < n; ++i) {
+        const double time = times[i];
+        SourcePosition source;
+        if (lp.piEN == 0.0 && lp.piEE == 0.0) {
+            const double tau = (time - lp.t0) / lp.tE;
+            const double beta = lp.umin;
+            const double cos_theta = std::cos(lp.theta);
+            const double sin_theta = std::sin(lp.theta);
+            source.x = tau * cos_theta - beta * sin_theta;
+            source.y = tau * sin_theta + beta * cos_theta;
+        } else {
+            source = trajectory.source_position(time, true, LCBI_XALLARAP_NONE);
+        }
+        if (lp.orbital_motion_mode != LCBI_ORBIT_STATIC) {
+            const auto orbit = orbital_state(lp, time);
+            const double cos_theta = std::cos(lp.theta);
+            const double sin_theta = std::sin(lp.theta);
+            const double tau = source.x * cos_theta + source.y * sin_theta;
+            const double beta = -source.x * sin_theta + source.y * cos_theta;
+            source = {tau * std::cos(orbit.angle) - beta * std::sin(orbit.angle),
+                      beta * std::cos(orbit.angle) + tau * std::sin(orbit.angle)};
+        }
+        xy[2 * i] = source.x;
+        xy[2 * i + 1] = source.y;
     }
     return 0;
 }

@@ -291,3 +291,29 @@ def test_device_entry_matches_host_entry(gpu, mode):
     assert np.array_equal(aa.cpu().numpy().view(np.int64), g_aa.view(np.int64))
     assert np.array_equal(dm.cpu().numpy().view(np.int64), g_dm.view(np.int64))
     b.close()
+
+
+def test_lightcurve_keeps_its_handle_across_calls(gpu):
+    """LightCurve reuses one device handle for the LensModel path across calls
+    (curves updated in place); every call equals a fresh LightCurve's."""
+    import lcbinint
+    from lcbinint_amd import capi
+    capi.load_earth_ephemeris(TABLE)
+    t0 = 10100.0
+    times = np.linspace(t0 - 10, t0 + 10, 121)
+
+    def make():
+        return lcbinint.LightCurve(options=lcbinint.Options(coordinates="vbm", nbin=40),
+                                   limb_darkening=lcbinint.LimbDarkening.linear(0.4),
+                                   event=lcbinint.EventCoordinates(ra=270.0, dec=-30.0, tfix=t0),
+                                   orbital_motion_mode=lcbinint.OrbitalMotionMode.CIRCULAR, parallax=True)
+
+    kept = make()
+    for k, s in enumerate((1.1, 0.9, 1.25)):
+        params = dict(t0=t0, tE=30.0, u0=0.03 + 0.01 * k, alpha=0.7, s=s, q=0.02, rho=3e-3, piEN=0.2,
+                      piEE=-0.1, g1=0.3 / s / 365.25, g2=-0.5 / 365.25)
+        a = kept(times, params)
+        b = make()(times, params)
+        assert np.array_equal(a.view(np.int64), b.view(np.int64)), k
+        info = kept.info(times, params)
+        assert np.array_equal(np.asarray(info.magnifications).view(np.int64), a.view(np.int64)), k
