@@ -1236,6 +1236,227 @@ __device__ __forceinline__ bool hook_poll(Ctx& C, ImageHook& hk)
     return true;
 }
 
+// State of the gap-repair examination (image_area stage 2) that gap_batch
+// advances.
+struct GapState {
+    int row, cs;
+    double dxmax, dxmin;
+    int yi, nyi, gaps;
+    double max_jump;
+};
+
+// Several gap-repair scans per wave step.  The examination of rows
+// row .. nyi - 2 (image_area stage 2, :2578-2644) lists the next scans: start
+// cell, row and threshold come from the row records, which stay as they are
+// while the scans find nothing.  The first row of each listed scan is then
+// evaluated in one wave step, kc lanes per scan: its start cell and the one
+// after it (the right scan) and the kc - 2 cells before it (the left scan),
+// each lane forming its coordinate with the additions the one-scan path makes
+// (start + incr, or start + (-incr) repeated).  The scans are resolved in order
+// with the row rules of cart_scan_rows.  A scan whose first row holds no weight
+// ends there, with no area, no row pushed and yi back where it was: it is
+// committed here, with the records, counts and examination state that the
+// one-scan path leaves behind.  The first scan that finds weight, runs past its
+// lanes or meets a limit ends the batch and is left to the one-scan path.
+// Returns the scans committed, or -1 with C.overflow set.
+__device__ __forceinline__ int gap_batch(Ctx& C, const Cart& K, Rows& R, GapState& G, const int* watch, int expect)
+{
+    const double incr = K.incr;
+    const double g = 1.1 * incr;
+    const int last = G.nyi - 2;  // the last row is left to the one-scan path
+    if (G.row < 0 || G.row > last) return 0;
+    const int base_row = G.row;
+    const int span_rows = 11;  // records of rows base_row .. base_row + 11 in lanes 0..59
+    const int nrec = (base_row + span_rows <= G.nyi - 1 ? span_rows : G.nyi - 1 - base_row) + 1;
+    double rec = 0.0;
+    if (C.lane < 5 * nrec) rec = R.r[base_row * 5 + C.lane];
+    auto fld = [&](int row, int f) { return uni(read_lane(rec, (row - base_row) * 5 + f)); };
+    // the scans, lane j holding scan j's start, row y, threshold, dy, the
+    // start record's xmin / xmax, and its (row, case)
+    constexpr int kMaxScans = 8;
+    double v_sx = 0.0, v_sy = 0.0, v_thr = 0.0, v_dy = 0.0, v_mn = 0.0, v_mx = 0.0;
+    int v_rc = 0;
+    int nd = 0;
+    int r = base_row, k = G.cs;
+    double dxmax = G.dxmax, dxmin = G.dxmin;
+    double mj = G.max_jump;
+    for (int it = 0; it < 64 && nd < kMaxScans; ++it) {
+        if (k >= 4) {
+            if (r + 1 > last || r + 2 - base_row >= nrec) break;
+            ++r;
+            dxmax = fld(r + 1, XMAX) - fld(r, XMAX);
+            dxmin = fld(r + 1, XMIN) - fld(r, XMIN);
+            k = fld(r + 1, AX) > 0.0 ? 0 : 4;
+            if (k == 0) mj = dmax(mj, dmax(fabs(dxmax), fabs(dxmin)) / incr);
+            continue;
+        }
+        const int kk = k++;
+        bool have = false;
+        double x = 0.0, y = 0.0, d = 0.0, a = 0.0, b = 0.0;
+        if (kk == 0 && dxmax > g) {
+            x = fld(r + 1, XMAX);
+            y = fld(r, YY);
+            a = fld(r, XMAX);
+            b = fld(r + 1, XMAX);
+            d = -fld(r, DYS);
+            have = true;
+        } else if (kk == 1 && dxmin > g) {
+            x = fld(r + 1, XMIN) - incr;
+            y = fld(r + 1, YY);
+            a = fld(r, XMIN);
+            b = fld(r + 1, XMIN);
+            d = fld(r, DYS);
+            have = true;
+        } else if (kk == 2 && dxmin < -g) {
+            x = fld(r, XMIN) - incr;
+            y = fld(r, YY);
+            a = fld(r + 1, XMIN);
+            b = fld(r, XMIN);
+            d = -fld(r, DYS);
+            have = true;
+        } else if (kk == 3 && dxmax < -g) {
+            x = fld(r + 1, XMAX) + incr;
+            y = fld(r + 1, YY);
+            a = fld(r + 1, XMAX);
+            b = fld(r, XMAX);
+            d = fld(r, DYS);
+            have = true;
+        }
+        if (!have) continue;
+        const bool mine = C.lane == nd;
+        v_sx = mine ? x : v_sx;
+        v_sy = mine ? y : v_sy;
+        v_thr = mine ? a - (-incr) : v_thr;  // thr = prev_xmin - (-incr), prev_xmin = the start record's xmin
+        v_dy = mine ? d : v_dy;
+        v_mn = mine ? a : v_mn;
+        v_mx = mine ? b : v_mx;
+        v_rc = mine ? (r * 4 + kk) : v_rc;
+        ++nd;
+    }
+    if (nd == 0) return 0;
+    // lanes per scan: the widest left scan (start down to the threshold) of
+    // the listed scans, plus the right scan's two cells
+    double span_d = 0.0;
+    if (C.lane < nd) span_d = (v_sx - v_thr) * K.inv_incr;
+    const double span_max = uni(wave_max(C.lane < nd ? span_d : 0.0));
+    const int need = static_cast<int>(span_max) + 4;
+    const int kc = need <= 8 ? 8 : (need <= 16 ? 16 : 32);
+    const int ns = nd < kWave / kc ? nd : kWave / kc;
+    // each lane's cell: c >= 0 the right scan's start + c incr, c < 0 the left
+    // scan's start + (-incr) |c| times
+    const int my_j = C.lane / kc;
+    const int my_t = C.lane - my_j * kc;
+    const int c = my_t - (kc - 2);
+    const double sx = shfl(v_sx, my_j);
+    const double sy = shfl(v_sy, my_j);
+    double xk = sx;
+    if (c > 0) xk = xk + incr;
+    for (int i = 0; i < -c; ++i) xk = xk + (-incr);
+    const double thr = shfl(v_thr, my_j);
+    double w = 0.0;
+    bool in = false;
+    const bool live = my_j < ns;
+    if (live) in = cart_cell(C, K, xk, sy, &w);
+    const uint64_t inm = ballot(in);
+    const uint64_t nzm = ballot(in && w != 0.0);
+    const uint64_t belowm = ballot(live && !(xk >= thr));  // left of the threshold (cond false)
+    const uint64_t kmask = kc >= 64 ? ~0ull : ((1ull << kc) - 1ull);
+    // resolve in order
+    int n = 0;
+    long long visited = 0;
+    int xmn_lane = -1;               // the last scan's xmin cell
+    int xmx_lane = -1, xmx_plus = 0;  // the last xmax a scan set: its cell, and whether it is cell + incr
+    const int p = kc - 2;  // the start's lane within a scan
+    for (int j = 0; j < ns; ++j) {
+        const uint64_t bits = (inm >> (j * kc)) & kmask;
+        const uint64_t nzb = (nzm >> (j * kc)) & kmask;
+        const uint64_t blw = (belowm >> (j * kc)) & kmask;
+        // right scan: p, p + 1
+        const uint64_t outs = ~bits & kmask & ~((1ull << p) - 1ull);
+        if (!outs) break;
+        const int s_r = first_set(outs);
+        bool nzs = (nzb & ((1ull << s_r) - 1ull) & ~((1ull << p) - 1ull)) != 0;
+        int xmx_t = -1, xmx_inc = 0;
+        if (s_r > p) {
+            xmx_t = s_r;  // xmax = the first out cell
+            xmx_inc = 0;
+        }
+        int stop_t = -1;
+        for (int t = p - 1; t >= 0; --t) {
+            if ((bits >> t) & 1ull) {
+                if (!nzs) {
+                    xmx_t = t;  // xmax = cell + incr
+                    xmx_inc = 1;
+                }
+                if ((nzb >> t) & 1ull) nzs = true;
+            } else if (nzs || ((blw >> t) & 1ull)) {
+                stop_t = t;
+                break;
+            }
+        }
+        if (stop_t < 0 || nzs) break;
+        if (G.yi + 1 >= C.row_cap) break;
+        const bool prev_in = stop_t + 1 <= p - 1 && ((bits >> (stop_t + 1)) & 1ull);
+        xmn_lane = j * kc + (prev_in ? stop_t : p - 1);
+        if (xmx_t >= 0) {
+            xmx_lane = j * kc + xmx_t;
+            xmx_plus = xmx_inc;
+        }
+        visited += (s_r - p + 1) + (p - stop_t);
+        n = j + 1;
+    }
+    if (n == 0) return 0;
+    // commit: each scan wrote its start record at yi and its one row at
+    // yi + 1, then yi came back; the records hold the last scan's writes
+    if (!rows_ensure(C, R, G.yi + 1)) return -1;
+    const double l_mn = uni(read_lane(v_mn, n - 1));
+    const double l_mx = uni(read_lane(v_mx, n - 1));
+    const double l_dy = uni(read_lane(v_dy, n - 1));
+    const double l_y = uni(read_lane(v_sy, n - 1));
+    const int l_rc = __builtin_amdgcn_readlane(v_rc, n - 1);
+    const double x_mn = uni(read_lane(xk, xmn_lane));
+    double x_mx = 0.0;
+    if (xmx_lane >= 0) {
+        const double cell = uni(read_lane(xk, xmx_lane));
+        x_mx = xmx_plus ? cell - (-incr) : cell;
+    }
+    wave_sync();
+    {
+        int idx = -1;
+        double v = 0.0;
+        const int f = C.lane;
+        if (f == 0) { idx = G.yi * 5 + XMIN; v = l_mn; }
+        else if (f == 1) { idx = G.yi * 5 + XMAX; v = l_mx; }
+        else if (f == 2) { idx = G.yi * 5 + DYS; v = l_dy; }
+        else if (f == 3) { idx = (G.yi + 1) * 5 + XMIN; v = x_mn; }
+        else if (f == 4 && xmx_lane >= 0) { idx = (G.yi + 1) * 5 + XMAX; v = x_mx; }
+        else if (f == 5) { idx = (G.yi + 1) * 5 + AX; v = 0.0; }
+        else if (f == 6) { idx = (G.yi + 1) * 5 + YY; v = l_y; }
+        else if (f == 7) { idx = (G.yi + 1) * 5 + DYS; v = -l_dy; }
+        if (idx >= 0) R.r[idx] = v;
+    }
+    wave_sync();
+    G.gaps += n;
+    G.max_jump = mj;
+    C.n_rows = uni(C.n_rows + n);
+    if (K.jac_sign == 0)
+        C.n_cart = uni(C.n_cart + visited);
+    else
+        C.n_cart_guarded = uni(C.n_cart_guarded + visited);
+    C.steps = uni(C.steps + 2 * n);
+    // the examination continues after the last committed scan
+    const int lrow = l_rc >> 2;
+    G.dxmax = fld(lrow + 1, XMAX) - fld(lrow, XMAX);
+    G.dxmin = fld(lrow + 1, XMIN) - fld(lrow, XMIN);
+    G.row = lrow;
+    G.cs = (l_rc & 3) + 1;
+    if (watch && uni(__hip_atomic_load(watch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) != expect) {
+        C.overflow = true;
+        return -1;
+    }
+    return n;
+}
+
 // The scans of one image (:2520-2680): up from the seed, down from the row
 // below it, then the gap-repair scans of rows 0..nyi-1, four cases per row,
 // all through one call site of cart_scan.  A failure leaves C.overflow set.
@@ -1257,6 +1478,7 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
     int nyi = 0;
     double areai = 0.0, areabound = 0.0;
     const double g = 1.1 * incr;
+    int gap_cool = 0, gap_fails = 0;  // backs off gap_batch where it commits nothing
     for (;;) {
         if (!hook_poll(C, hk)) return out;
         Pt start{0.0, 0.0};
@@ -1279,6 +1501,25 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
             ++yi;
         } else {
             bool have = false;
+            if (gap_cool <= 0 && row >= 0 && row <= nyi - 2) {
+                GapState G{row, cs, dxmax, dxmin, yi, nyi, gaps, max_jump};
+                const int nb = gap_batch(C, K, R, G, hk.watch, hk.expect);
+                if (nb < 0) return out;
+                if (nb > 0) {
+                    row = G.row;
+                    cs = G.cs;
+                    dxmax = G.dxmax;
+                    dxmin = G.dxmin;
+                    gaps = G.gaps;
+                    max_jump = G.max_jump;
+                    gap_fails = 0;
+                    continue;
+                }
+                ++gap_fails;
+                gap_cool = gap_fails < 4 ? (1 << gap_fails) : 16;
+            } else if (gap_cool > 0) {
+                --gap_cool;
+            }
             while (!have) {
                 if (cs >= 4) {
                     if (row >= 0 && row == nyi - 1 && areabound > 0.0 && yi > nyi) nyi = yi;
