@@ -53,7 +53,19 @@ typedef struct lcbg_curve {
 } lcbg_curve;
 
 /* Finite-source options: the fields of lcbi_options
- * (include/lcbinint/lcbinint.h:85-104) that the plain method reads. */
+ * (include/lcbinint/lcbinint.h:85-104) that the finite-source path reads.
+ * adaptive_source_bins != 0 enables the topology retry of
+ * fixed_inverse_ray_binary (src/lcbinint/magnification/
+ * finite_source_magnifier.cpp:1108-1121, 3166-3196: a Cartesian epoch whose
+ * scan shows >= 16 seeds, >= 100 overlaps, >= 1000 gap repairs and a row-edge
+ * jump of >= 1000 cells is scanned again at min(max_source_bins,
+ * max(50, 2 source_bins)) bins, for source_bins < 50 and rho < 1e-2); with a
+ * positive finite_source_tol or finite_source_reltol it is the adaptive mode:
+ * a bounding-box margin of max(point_source_threshold, 60) rho, point-source
+ * and hexadecapole thresholds derived from the tolerances (:4679-4790), and
+ * the local refinement of the Cartesian area (:3071-3289).  The gradient
+ * entry points and lcbg_light_curve_device_noalloc reject
+ * adaptive_source_bins != 0 (LCBG_UNSUPPORTED). */
 typedef struct lcbg_options {
     int source_bins;              /* nbin, cells per source radius (default 50) */
     int caustic_bins;             /* critical-curve phases (default 1400) */
@@ -64,8 +76,10 @@ typedef struct lcbg_options {
     double point_source_threshold;/* bbox margin in rho (default 20) */
     double hexadecapole_threshold;/* caustic-distance threshold in rho (default 3) */
     double adaptive_hex_threshold;/* hexadecapole tolerance (default 1e-3) */
-    double finite_source_tol;     /* absolute tolerance of the converged flag (default 0) */
-    double finite_source_reltol;  /* relative tolerance of the converged flag (default 0) */
+    double finite_source_tol;     /* absolute tolerance (default 0): converged flag, adaptive mode */
+    double finite_source_reltol;  /* relative tolerance (default 0): converged flag, adaptive mode */
+    int adaptive_source_bins;     /* 0 fixed source_bins (default), else retry / adaptive refinement */
+    int max_source_bins;          /* finest retry lattice and refinement depth bound (default 400) */
 } lcbg_options;
 
 /* Per-epoch result record, the fields of lcbi_result
@@ -77,10 +91,13 @@ typedef struct lcbg_options {
  * estimate (FiniteSourceResult::error_estimate: 0 for point source and polar,
  * the derivative or hexadecapole relative error times max(|A|, 1), the
  * Cartesian area error indicator of finite_source_magnifier.cpp:1065-1106 with
- * the high-magnification floor of :2745-2756), the refinement level (always
- * 0: the plain method does not refine) and the converged flag
- * (:3290-3294: a Cartesian epoch converged iff its error estimate is at most
- * 0.999 (tol + reltol max(|A|, 1)); other methods iff A is finite). */
+ * the high-magnification floor of :2745-2756; in the adaptive mode the
+ * refinement's estimate of :3284-3289), the refinement level (1 after a
+ * topology retry, the deepest split of the local refinement; 0 without
+ * adaptive_source_bins) and the converged flag (:3290-3294: a Cartesian epoch
+ * converged iff its error estimate is at most 0.999 (tol + reltol max(|A|, 1))
+ * and, in the adaptive mode, it was scanned at 32 or more bins; other methods
+ * iff A is finite). */
 typedef struct lcbg_result {
     double magnification;
     double point_source_magnification;

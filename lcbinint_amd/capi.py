@@ -46,7 +46,7 @@ class Curve(ctypes.Structure):
 
 
 class Options(ctypes.Structure):
-    """lcbg_options: the finite-source options the plain method reads."""
+    """lcbg_options: the finite-source options (plain method, retry, adaptive mode)."""
     _fields_ = [
         ("source_bins", ctypes.c_int), ("caustic_bins", ctypes.c_int),
         ("mode", ctypes.c_int), ("polar_source_bins", ctypes.c_int),
@@ -56,6 +56,8 @@ class Options(ctypes.Structure):
         ("adaptive_hex_threshold", ctypes.c_double),
         ("finite_source_tol", ctypes.c_double),
         ("finite_source_reltol", ctypes.c_double),
+        ("adaptive_source_bins", ctypes.c_int),
+        ("max_source_bins", ctypes.c_int),
     ]
 
 

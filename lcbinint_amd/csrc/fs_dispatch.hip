@@ -58,7 +58,7 @@ __device__ __forceinline__ bool stage1_epoch(const CurveRec& cv, const CausticSc
     // rho == 0: binary_mag0 only (python/lcbinint_pybind.cpp:1129-1139);
     // finite_mode <= 0: point-source magnification (:4671-4674)
     const double rho = cv.rho;
-    const double margin = opt.kinji * rho;
+    const double margin = bbox_margin(opt, rho);
     if (!(rho > 0.0) || opt.mode <= 0 || sc.n == 0 || src.x < sc.min_x - margin || src.x > sc.max_x + margin ||
         src.y < sc.min_y - margin || src.y > sc.max_y + margin) {
         ep.mag[e] = a_ps;
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(kWave) k_fs_near_dist(const CurveRec* __restri
         const Pt src = ep.src[e];
         const double rho = curves[c].rho;
 
-        const double margin = opt.kinji * rho;
+        const double margin = bbox_margin(opt, rho);
         const int ix0 = cell_coord(src.x - margin, sc.min_x, sc.step_x);
         const int ix1 = cell_coord(src.x + margin, sc.min_x, sc.step_x);
         const int iy0 = cell_coord(src.y - margin, sc.min_y, sc.step_y);
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(kNearCurveBlock) k_fs_near_dist_curve(const Cu
         const int64_t begin = ep.offsets[c];
         const int count = cache.near_n[c];
         const double rho = curves[c].rho;
-        const double margin = opt.kinji * rho;
+        const double margin = bbox_margin(opt, rho);
         // The near test reads the distance only through d < hex_threshold rho
         // and, for rho >= 1e-3, the safety factor max(1, 30 (hex_threshold /
         // max(d / rho, hex_threshold))^3); both are the same for every
@@ -504,7 +504,10 @@ __global__ void __launch_bounds__(kNearTestBlock) k_fs_near_test(const CurveRec*
                     const double t = opt.hex_threshold / dmax(dist_ratio, opt.hex_threshold);
                     safety = dmax(1.0, 30.0 * t * t * t);
                 }
-                const double requested = opt.adaptive_hex > 0.0 ? opt.adaptive_hex : 1.0e-3;
+                // the adaptive mode asks the tolerances of the point-source
+                // magnification (:4731-4738)
+                const double requested = opt.adaptive ? opt.reltol + opt.tol / dmax(fabs(a_ps), 1.0)
+                                         : opt.adaptive_hex > 0.0 ? opt.adaptive_hex : 1.0e-3;
                 const Pt src = ep.src[e];
                 const Geom g = make_geom(cv.s, cv.q_eff, src.x, src.y);
                 cx roots[5];
@@ -601,7 +604,21 @@ __global__ void __launch_bounds__(kWave) k_fs_hex(const CurveRec* __restrict__ c
             const double hex = a4rho4 / 3.0 * (1.0 - 11.0 * gamma / 35.0 - 7.0 * lambda / 39.0);
             const double hmag = a_ps + quad + hex;
             const double rel = fabs(hex) / dmax(fabs(hmag), 1.0e-10);
-            if (rel <= opt.adaptive_hex) {
+            double thr = opt.adaptive_hex;
+            if (opt.adaptive) {
+                // the tolerances of the hexadecapole magnification over the
+                // caustic-distance safety factor (:4771-4790)
+                const double requested = opt.reltol + opt.tol / dmax(fabs(hmag), 1.0);
+                const double refined = ep.dist[e];
+                double safety = 1.0;
+                if (cv.rho >= 1.0e-3 && isfinite(refined) && cv.rho > 0.0) {
+                    const double t = opt.hex_threshold / dmax(refined / cv.rho, opt.hex_threshold);
+                    safety = dmax(1.0, 30.0 * t * t * t);
+                }
+                const double limit = requested / safety;
+                thr = thr < limit ? thr : limit;
+            }
+            if (rel <= thr) {
                 ep.mag[e] = hmag;
                 ep.method[e] = LCBG_METHOD_HEXADECAPOLE;
                 ep.err[e] = rel * dmax(fabs(hmag), 1.0);  // :4792-4802
@@ -671,9 +688,13 @@ __global__ void __launch_bounds__(kScanBlock) k_epoch_results(const CurveRec* __
         }
         const double err = ep.err[e];
         bool converged = isfinite(mag);
-        // fixed_inverse_ray_binary (finite_source_magnifier.cpp:3292-3294)
+        // adaptive Cartesian epochs: the level k_ir_cart_adapt recorded
+        const int lv =
+            (ep.level && opt.adaptive_source_bins != 0 && method == LCBG_METHOD_INVERSE_RAY_CARTESIAN) ? ep.level[e] : 0;
+        // fixed_inverse_ray_binary (finite_source_magnifier.cpp:3290-3294)
         if (method == LCBG_METHOD_INVERSE_RAY_CARTESIAN)
-            converged = converged && err <= 0.999 * (tol + reltol * dmax(fabs(mag), 1.0));
+            converged = converged && err <= 0.999 * (tol + reltol * dmax(fabs(mag), 1.0)) &&
+                        !(lv & kLevelUnconverged);
         lcbg_result r;
         r.magnification = mag;
         r.point_source_magnification = ep.a_ps[e];
@@ -683,7 +704,7 @@ __global__ void __launch_bounds__(kScanBlock) k_epoch_results(const CurveRec* __
         r.image_count = images;
         r.method = method;
         r.error_estimate = err;
-        r.refinement_level = 0;
+        r.refinement_level = lv & (kLevelUnconverged - 1);
         r.converged = converged ? 1 : 0;
         out[e] = r;
     }

@@ -94,7 +94,7 @@ lcbg_status ensure_epochs(FsState* fs, int64_t n)
                               carve_size(cap * kTangentsHost * sizeof(double)) +
                               carve_size(cap * sizeof(int64_t)) * 3 + carve_size(cap * sizeof(double)) +
                               carve_size(cap * sizeof(int)) * 4 + carve_size(cap * sizeof(double)) +
-                              carve_size(cap * kGradSlots * sizeof(double));
+                              carve_size(cap * kGradSlots * sizeof(double)) + carve_size(cap * sizeof(int32_t));
     lcbg_status st = map_hip(hipMalloc(&fs->epoch_mem, bytes));
     if (st != LCBG_OK) {
         fs->epoch_mem = nullptr;
@@ -251,6 +251,16 @@ void grad_item_view(FsState* fs, GradWork& w)
     w.points = carve<double>(p, cap * kGradSlots);
 }
 
+// Per-epoch refinement levels of adaptive runs, after the gradient item fields.
+int32_t* level_view(FsState* fs)
+{
+    GradWork w{};
+    grad_item_view(fs, w);
+    char* p = reinterpret_cast<char*>(w.points);
+    (void)carve<double>(p, fs->epoch_cap * kGradSlots);
+    return carve<int32_t>(p, fs->epoch_cap);
+}
+
 // The status a run's outcome maps to: a method outside this library is
 // unsupported, a team wait that ran out of time a device failure, and any
 // other abandoned epoch (its magnification is NaN) a numerical error, as a
@@ -275,6 +285,7 @@ struct ScratchPool {
     GradWork grad{};       // pools of the gradient quadrature (first gradient run)
     PolarScratch grad_slots{}, grad_large{};  // its enumeration slots
     void* grad_mem = nullptr;
+    void* adapt_mem = nullptr;  // adaptive Cartesian slots (first adaptive run)
 };
 
 // Runs and 64-cell chunks the gradient's inverse-ray quadrature may hold in one
@@ -320,6 +331,55 @@ lcbg_status ensure_grad_pool(ScratchPool& pool)
         if (st != LCBG_OK) return st;
     }
     return map_hip(hipDeviceSynchronize());
+}
+
+// The adaptive Cartesian kernel's workspace for refinement budgets up to
+// `max_bins` (max of the curves' source_bins and max_source_bins) and tapes of
+// up to max(64, max_source_bins) cells; grown (after a device-wide wait) when
+// a run needs more.  Its queue lives in LDS when it fits in 64 KiB.
+lcbg_status ensure_adapt_pool(ScratchPool& pool, int max_source_bins, int max_bins)
+{
+    const int tape_cap = max_source_bins > 64 ? max_source_bins : 64;
+    const int splits = 4 * max_bins > 64 ? 4 * max_bins : 64;
+    const int cell_cap = tape_cap + 4 * splits;
+    const int heap_cap = tape_cap + 3 * splits + 1;
+    AdaptScratch& a = pool.scr.adapt;
+    if (pool.adapt_mem && a.tape_cap >= tape_cap && a.cell_cap >= cell_cap && a.heap_cap >= heap_cap)
+        return LCBG_OK;
+    if (pool.adapt_mem) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(pool.adapt_mem);
+        pool.adapt_mem = nullptr;
+    }
+    const bool lds = static_cast<std::size_t>(heap_cap) * 12 <= 65536;
+    const int large = pool.scr.cart_large.n_slots < 128 ? pool.scr.cart_large.n_slots : 128;
+    int small = pool.cus * adapt_resident_per_cu(lds ? static_cast<unsigned>(heap_cap) * 12u : 0u);
+    if (small > pool.scr.cart.n_slots) small = pool.scr.cart.n_slots;
+    const std::size_t n = static_cast<std::size_t>(large + small);
+    const std::size_t tc = static_cast<std::size_t>(tape_cap), cc = static_cast<std::size_t>(cell_cap),
+                      hc = lds ? 0 : static_cast<std::size_t>(heap_cap);
+    const std::size_t bytes = carve_size(n * tc * 3 * sizeof(double)) + carve_size(n * tc * 2 * sizeof(long long)) +
+                              carve_size(n * cc * 4 * sizeof(double)) + carve_size(n * cc * sizeof(int)) +
+                              carve_size(n * hc * sizeof(double)) + carve_size(n * hc * sizeof(int));
+    lcbg_status st = map_hip(hipMalloc(&pool.adapt_mem, bytes));
+    if (st != LCBG_OK) {
+        pool.adapt_mem = nullptr;
+        return st;
+    }
+    char* p = static_cast<char*>(pool.adapt_mem);
+    a.tape_cap = tape_cap;
+    a.cell_cap = cell_cap;
+    a.heap_cap = heap_cap;
+    a.heap_lds = lds ? 1 : 0;
+    a.tape = carve<double>(p, n * tc * 3);
+    a.key = carve<long long>(p, n * tc * 2);
+    a.cell = carve<double>(p, n * cc * 4);
+    a.depth = carve<int>(p, n * cc);
+    a.heap_key = carve<double>(p, n * hc);
+    a.heap_val = carve<int>(p, n * hc);
+    pool.scr.adapt_large = large;
+    pool.scr.adapt_small = small;
+    return LCBG_OK;
 }
 
 constexpr int kMaxDevices = 64;
@@ -487,6 +547,12 @@ lcbg_status fs_prepare(lcbg_batch* b)
     fs->opt.kinji = o.point_source_threshold;
     fs->opt.hex_threshold = o.hexadecapole_threshold;
     fs->opt.adaptive_hex = o.adaptive_hex_threshold;
+    fs->opt.adaptive_source_bins = o.adaptive_source_bins;
+    fs->opt.max_source_bins = o.max_source_bins;
+    fs->opt.tol = o.finite_source_tol;
+    fs->opt.reltol = o.finite_source_reltol;
+    fs->opt.adaptive = (o.adaptive_source_bins != 0 && (o.finite_source_tol > 0.0 || o.finite_source_reltol > 0.0))
+                           ? 1 : 0;
     fs->opt.ps_images = b->dynamic_images ? 1 : 0;
     fs->opt.heavy_budget = kHeavyBudget;
     fs->opt.lead_items = kLeadItems;
@@ -576,6 +642,8 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
                    double* mag, int32_t* method, hipStream_t stream, float* stage_ms,
                    const GradArgs* ga = nullptr, bool capture = false)
 {
+    // the gradient differentiates the fixed-lattice quadrature only
+    if (ga && b->options.adaptive_source_bins != 0) return LCBG_UNSUPPORTED;
     lcbg_status st = fs_prepare(b);
     if (st != LCBG_OK) return st;
     FsState* fs = b->fs;
@@ -590,6 +658,7 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     ep.near_curve = near_curve_view(fs);
     ep.near_chunks = near_chunks_view(fs);
     ep.err = err_view(fs);
+    ep.level = level_view(fs);
     ep.src_y = b->src_y;
     // the pseudo-curve batch of the dynamic path runs different epochs under
     // the same count, so it keeps no keys
@@ -653,6 +722,13 @@ lcbg_status fs_run(lcbg_batch* b, const int64_t* offsets, const double* times, i
     std::lock_guard<std::mutex> lock(pool.mu);
     st = ensure_scratch(pool);
     if (st == LCBG_OK && ga) st = ensure_grad_pool(pool);
+    if (st == LCBG_OK && b->options.adaptive_source_bins != 0) {
+        int max_bins = b->options.max_source_bins > b->options.source_bins ? b->options.max_source_bins
+                                                                             : b->options.source_bins;
+        for (const CurveRec& r : b->h_curves)
+            if (r.source_bins > max_bins) max_bins = r.source_bins;
+        st = ensure_adapt_pool(pool, b->options.max_source_bins, max_bins);
+    }
     if (st != LCBG_OK) return st;
     // LCBG_STAGE_SYNC=1: wait after every stage and report it on stderr
     // (a diagnostic in the spirit of the reference's LCBININT_AREA_DIAGNOSTICS)
@@ -1555,6 +1631,7 @@ extern "C" lcbg_status lcbg_batch_epoch_results_device(lcbg_batch* batch, int64_
     if (!fs || !fs->epoch_mem || fs->last_n != n_epochs) return LCBG_INVALID_ARGUMENT;
     FsEpochs ep = epoch_view(fs, n_epochs, nullptr, nullptr, fs->last_mag, fs->last_method);
     ep.err = err_view(fs);
+    ep.level = level_view(fs);
     return launch_epoch_results(batch->d_curves, fs->opt, ep, batch->options.finite_source_tol,
                                 batch->options.finite_source_reltol, results, static_cast<hipStream_t>(stream));
 }

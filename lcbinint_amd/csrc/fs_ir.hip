@@ -17,6 +17,7 @@
 #include "fs_launch.hpp"
 #include "fs_math.hpp"
 #include "fs_wave.hpp"
+#include "pq_heap.hpp"
 
 namespace lcbg {
 
@@ -788,6 +789,79 @@ __device__ __forceinline__ bool cart_cell(const Ctx& C, const Cart& K, double x,
     return in;
 }
 
+// ------------------------------------------------------ local refinement tape
+//
+// LocalRefinementTape (finite_source_magnifier.cpp:977-1004): while the
+// adaptive mode scans an epoch, every evaluated cell whose mapped distance lies
+// within the boundary band |d^2 - rho^2| <= max(4 rho incr, 4 incr^2) of the
+// source edge (and, for guarded cells, on the seed's side of the critical
+// curve) is recorded with its weight (:1952-1961), in scan order, once per
+// lattice key (llround(x / incr), llround(y / incr)), up to `cap` cells.  Once
+// full the tape takes nothing more, so the scans record only until then.
+struct Tape {
+    double* x;
+    double* y;
+    double* w;
+    long long* kx;
+    long long* ky;
+    int n, cap;
+    double band2, step;
+};
+
+__device__ __forceinline__ bool tape_open(const Tape* T) { return T && T->n < T->cap; }
+
+// Appends the cells of the lanes in `rec` in lane order (the scan's visiting
+// order within one step), each unless its key is already on the tape.
+__device__ void tape_add(Ctx& C, Tape& T, bool rec, double x, double y, double w)
+{
+    uint64_t m = ballot(rec);
+    while (m && T.n < T.cap) {
+        const int j = first_set(m);
+        m &= m - 1ull;
+        const double cx = read_lane(x, j);
+        const double cw = read_lane(w, j);
+        const long long kx = llround(cx / T.step);
+        const long long ky = llround(y / T.step);
+        bool dup = false;
+        for (int base = 0; base < T.n && !dup; base += kWave) {
+            const int i = base + C.lane;
+            dup = ballot(i < T.n && T.kx[i] == kx && T.ky[i] == ky) != 0;
+        }
+        if (dup) continue;
+        wave_sync();
+        wave_store(&T.x[T.n], cx);
+        wave_store(&T.y[T.n], y);
+        wave_store(&T.w[T.n], cw);
+        wave_store(&T.kx[T.n], kx);
+        wave_store(&T.ky[T.n], ky);
+        wave_sync();
+        T.n = uni(T.n + 1);
+    }
+}
+
+// cart_cell, also returning the mapped distance and whether the cell may enter
+// the tape as an outside cell (jac_ok of :1927-1937).
+__device__ __forceinline__ bool cart_cell_md(const Ctx& C, const Cart& K, double x, double y, double* w,
+                                             double* md_out, bool* jok_out)
+{
+    double md, dz2;
+    bool jok = true;
+    if (K.jac_sign == 0) {
+        md = mapped_dist2(C.L, x, y, C.src);
+        dz2 = md;
+    } else {
+        double jac;
+        md = mapped_dist2_jac(C.L, x, y, C.src, &jac);
+        jok = sign_of(jac) != -K.jac_sign;
+        dz2 = jok ? md : C.r2 + 1.0;
+    }
+    const bool in = dz2 <= C.r2;
+    *w = in ? (C.uniform ? 1.0 : ld_brightness(md * C.inv_r2, C.ld_c, C.ld_d)) : 0.0;
+    *md_out = md;
+    *jok_out = jok;
+    return in;
+}
+
 // cartesian_image_area_impl (finite_source_magnifier.cpp:1892-2014).  The
 // scan keeps the current row's xmin/xmax and the previous row's xmin in
 // registers (written through to the row records), and each lane keeps its
@@ -809,9 +883,13 @@ __device__ __forceinline__ bool cart_cell(const Ctx& C, const Cart& K, double x,
 // The scan counts its cells (guard), wave steps and rows in locals; cart_scan
 // adds them to the epoch's counts when it returns.  `room` is the epoch's
 // remaining step budget.
+//
+// With kTape, the cells each step visits in the reference's order (up to and
+// including the cell that ends the run) go through the tape while it is open.
+template <bool kTape>
 __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi,
                                                  double prev_xmin_in, const int* watch, int expect,
-                                                 long long& guard, int& steps, int& rows, int room)
+                                                 long long& guard, int& steps, int& rows, int room, Tape* T)
 {
     const double incr = K.incr;
     double acc = 0.0;     // this lane's share of the row's brightness sum (countx)
@@ -841,6 +919,8 @@ __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R,
         int stash_n = 0;      // left-scan cells held in lanes [64 - stash_n, 64)
         double sx = 0.0, sw = 0.0;
         bool s_in = false;
+        double s_md = 0.0;    // their mapped distances and guard results (tape)
+        bool s_jok = false;
         int right_cells = 0;
         for (bool first = true;; first = false) {
             if (++steps > room) {
@@ -854,15 +934,31 @@ __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R,
             const double xk = combined ? seq_lanes2(x, incr, x0 + (-incr), -incr, split) : seq_lanes(x, incr);
             double w = 0.0;
             bool in = false;
-            if (combined || C.lane < n_eval) in = cart_cell(C, K, xk, y, &w);
+            double md = 0.0;
+            bool jok = false;
+            if constexpr (kTape) {
+                if (combined || C.lane < n_eval) in = cart_cell_md(C, K, xk, y, &w, &md, &jok);
+            } else {
+                if (combined || C.lane < n_eval) in = cart_cell(C, K, xk, y, &w);
+            }
             if (combined) {
                 stash_n = kWave - split;
                 sx = xk;
                 sw = w;
                 s_in = in;
+                if constexpr (kTape) {
+                    s_md = md;
+                    s_jok = jok;
+                }
             }
             const bool valid = C.lane < n_eval;
             const int s = first_set(ballot(!in && valid));
+            if constexpr (kTape) {
+                if (tape_open(T)) {
+                    const bool vis = valid && (s < 0 || C.lane <= s);
+                    tape_add(C, *T, vis && jok && fabs(md - C.r2) <= T->band2, xk, y, w);
+                }
+            }
             if (s < 0) {
                 const double wv = valid ? w : 0.0;
                 acc += wv;
@@ -904,12 +1000,18 @@ __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R,
             int lo, n_eval;
             double xk, w;
             bool in;
+            double md = 0.0;
+            bool jok = false;
             if (use_stash) {
                 lo = kWave - stash_n;
                 n_eval = stash_n;
                 xk = sx;
                 w = sw;
                 in = s_in;
+                if constexpr (kTape) {
+                    md = s_md;
+                    jok = s_jok;
+                }
                 use_stash = false;
             } else {
                 lo = 0;
@@ -917,7 +1019,11 @@ __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R,
                 xk = seq_lanes(x, -incr);
                 w = 0.0;
                 in = false;
-                if (C.lane < n_eval) in = cart_cell(C, K, xk, y, &w);
+                if constexpr (kTape) {
+                    if (C.lane < n_eval) in = cart_cell_md(C, K, xk, y, &w, &md, &jok);
+                } else {
+                    if (C.lane < n_eval) in = cart_cell(C, K, xk, y, &w);
+                }
             }
             const int end = lo + n_eval;
             const bool valid = C.lane >= lo && C.lane < end;
@@ -929,6 +1035,12 @@ __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R,
             const bool cond = (yi != 0) && (xk >= thr);
             const bool stop = valid && !in && (cb || (g >= 0 && C.lane > g) || yi == 0 || !cond);
             const int s = first_set(ballot(stop));
+            if constexpr (kTape) {
+                if (tape_open(T)) {
+                    const bool vis = valid && (s < 0 || C.lane <= s);
+                    tape_add(C, *T, vis && jok && fabs(md - C.r2) <= T->band2, xk, y, w);
+                }
+            }
             const int limit = s >= 0 ? s : end;
             const double wl = (C.lane < limit && inv) ? w : 0.0;
             acc += wl;
@@ -1025,14 +1137,16 @@ __device__ __forceinline__ double cart_scan_rows(Ctx& C, const Cart& K, Rows& R,
     }
 }
 
+template <bool kTape>
 __device__ __forceinline__ double cart_scan(Ctx& C, const Cart& K, Rows& R, Pt seed, double dy, int& yi,
-                                            double prev_xmin, const int* watch, int expect)
+                                            double prev_xmin, const int* watch, int expect, Tape* T)
 {
     long long guard = 0;
     int steps = 0, rows = 0;
     const long long left = C.budget - C.steps;
     const int room = left < 0 ? 0 : static_cast<int>(left);
-    const double r = cart_scan_rows(C, K, R, seed, dy, yi, prev_xmin, watch, expect, guard, steps, rows, room);
+    const double r = cart_scan_rows<kTape>(C, K, R, seed, dy, yi, prev_xmin, watch, expect, guard, steps, rows,
+                                           room, T);
     C.steps = uni(C.steps + steps);
     if (K.jac_sign == 0)
         C.n_cart = uni(C.n_cart + guard);
@@ -1460,7 +1574,11 @@ __device__ __forceinline__ int gap_batch(Ctx& C, const Cart& K, Rows& R, GapStat
 // The scans of one image (:2520-2680): up from the seed, down from the row
 // below it, then the gap-repair scans of rows 0..nyi-1, four cases per row,
 // all through one call site of cart_scan.  A failure leaves C.overflow set.
-__device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed, ImageHook& hk)
+// With kTape the scans record into T; gap-repair scans run one at a time
+// while it is open, so that they record in the reference's order.
+template <bool kTape = false>
+__device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed, ImageHook& hk,
+                                               Tape* T = nullptr)
 {
     ImageRun out{__builtin_nan(""), 0, 0, 0.0};
     const double incr = K.incr;
@@ -1501,7 +1619,7 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
             ++yi;
         } else {
             bool have = false;
-            if (gap_cool <= 0 && row >= 0 && row <= nyi - 2) {
+            if (gap_cool <= 0 && row >= 0 && row <= nyi - 2 && !(kTape && tape_open(T))) {
                 GapState G{row, cs, dxmax, dxmin, yi, nyi, gaps, max_jump};
                 const int nb = gap_batch(C, K, R, G, hk.watch, hk.expect);
                 if (nb < 0) return out;
@@ -1600,7 +1718,7 @@ __device__ __forceinline__ ImageRun image_area(Ctx& C, Cart& K, Rows& R, Pt seed
             }
             if (!have) break;
         }
-        const double r = cart_scan(C, K, R, start, sdy, yi, prev_xmin, hk.watch, hk.expect);
+        const double r = cart_scan<kTape>(C, K, R, start, sdy, yi, prev_xmin, hk.watch, hk.expect, T);
         if (C.overflow) return out;
         if (stage == 0) {
             areai = r;
@@ -1755,7 +1873,8 @@ __device__ double cart_error_estimate(const Ctx& C, int source_bins, double mag)
 
 // The whole Cartesian area of one epoch by one wave, images in seed order
 // (the epoch-at-a-time form of k_ir_cart's loop, for single-epoch kernels).
-__device__ double cartesian_mag(Ctx& C, int source_bins, const CartScratch& scr, int slot)
+template <bool kTape = false>
+__device__ double cartesian_mag(Ctx& C, int source_bins, const CartScratch& scr, int slot, Tape* T = nullptr)
 {
     const int n_img = C.n_seeds;
     if (n_img == 0 || !(C.rho > 0.0)) return __builtin_nan("");
@@ -1774,7 +1893,7 @@ __device__ double cartesian_mag(Ctx& C, int source_bins, const CartScratch& scr,
     double area = 0.0;
     for (int i = 0; i < n_img; ++i) {
         if (flags[i] & 1) continue;
-        const ImageRun img = image_area(C, K, R, C.seeds[i], hk);
+        const ImageRun img = image_area<kTape>(C, K, R, C.seeds[i], hk, T);
         if (C.overflow) return __builtin_nan("");
         account_image(C, K, i, R, img, source_bins, area, areaimage, flags, nullptr, 0);
     }
@@ -2675,6 +2794,322 @@ __global__ void __launch_bounds__(kTeamWaves * kWave, 3) k_ir_cart(const CurveRe
     }
 }
 
+// ------------------------------------------------ adaptive Cartesian epochs
+//
+// fixed_inverse_ray_binary's Cartesian branch in adaptive mode
+// (finite_source_magnifier.cpp:3071-3296): the area scan records the
+// refinement tape; an epoch whose scan shows the global topology pattern is
+// scanned again at a finer lattice (needs_global_topology_retry, :1108-1121;
+// :3166-3196); then the tape's cells are refined, largest error first, until
+// the summed error estimate meets the tolerance.  One wave per epoch: the
+// queue, the sums and the decisions are the wave's uniform control program;
+// the twenty weights of each split (four child centres, four corners each) are
+// evaluated one per lane.
+
+// Store of the refinement queue: every lane writes the uniform value.
+struct WaveStore {
+    template <class T>
+    __device__ void operator()(T* p, T v) const { *p = uni(v); }
+};
+
+// cell_weight (:3090-3099): brightness of the source point the image point
+// maps to, zero outside the disk.
+__device__ __forceinline__ double refine_weight(const Ctx& C, double x, double y)
+{
+    const double md = mapped_dist2(C.L, x, y, C.src);
+    if (md > C.r2) return 0.0;
+    return C.uniform ? 1.0 : ld_brightness(md / C.r2, C.ld_c, C.ld_d);
+}
+
+// estimate_local_cell_error (:3100-3128) from the four corner weights
+// (corner order (-,-), (+,-), (-,+), (+,+)).
+__device__ __forceinline__ double corner_error(double w0, double w1, double w2, double w3, double coarse, double frac)
+{
+    double sum = 0.0;
+    double mn = __builtin_inf();
+    double mx = 0.0;
+    int nz = 0;
+    const double w[4] = {w0, w1, w2, w3};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        sum += w[k];
+        mn = w[k] < mn ? w[k] : mn;
+        mx = mx < w[k] ? w[k] : mx;
+        nz += w[k] > 0.0 ? 1 : 0;
+    }
+    const double avg = 0.25 * sum;
+    const double center = fabs(avg - coarse);
+    const double variation = 0.5 * (mx - mn);
+    const double boundary = (nz > 0 && nz < 4) ? 0.25 * mx : 0.0;
+    double m = center;
+    if (m < variation) m = variation;
+    if (m < boundary) m = boundary;
+    return frac * m;
+}
+
+struct Refined {
+    double mag, err;
+    int level;
+    bool full;  // the cell pool ran out (the epoch fails)
+};
+
+// The local refinement loop (:3201-3289) over the tape's n cells of side
+// `incr`, after an area scan that gave `mag` with error estimate `base_err`
+// (diagnostics.estimated_error) at `active` bins per radius; `nbin` is the
+// curve's own setting.  Cells are kept in the slot's pool, the queue in
+// (hk, hv).
+__device__ Refined refine_cells(const Ctx& C, const FsOptions& opt, const Tape& T, double incr, int nbin, int active,
+                                double mag, double base_err, int level, double* cx, double* cy, double* ccoarse,
+                                double* cerr, int* cdepth, int cell_cap, double* hk, int* hv)
+{
+    Refined out{mag, base_err, level, false};
+    const int n = T.n;
+    const double rho = C.rho;
+    const double scale = source_flux(rho, C.ld_c, C.ld_d) / (rho * rho) *
+                         static_cast<double>(active) * static_cast<double>(active);
+    const double safety = nbin < 32 ? 8.0 : rho >= 8.0e-3 ? 2.5 : rho >= 3.0e-3 ? 1.8 : 1.25;
+    auto guarded = [&](double m) { return (opt.tol + opt.reltol * dmax(fabs(m), 1.0)) / safety; };
+    const int max_bins = active > opt.max_source_bins ? active : opt.max_source_bins;
+    int max_depth = 0;
+    for (int b = active > 1 ? active : 1; b < max_bins && max_depth < 8; b *= 2) ++max_depth;
+    const int max_splits = 4 * max_bins > 64 ? 4 * max_bins : 64;
+    if (n + 4 * max_splits > cell_cap) {
+        out.full = true;
+        return out;
+    }
+    // the tape's cells: corner weights four lanes per cell, sixteen cells a step
+    {
+        const double half = 0.5 * incr;
+        for (int base = 0; base < n; base += kWave / 4) {
+            const int i = base + C.lane / 4;
+            const int k = C.lane & 3;
+            double w = 0.0;
+            if (i < n) {
+                const double x = T.x[i], y = T.y[i];
+                const double px = (k & 1) ? x + half : x - half;
+                const double py = (k & 2) ? y + half : y - half;
+                w = refine_weight(C, px, py);
+            }
+            const double w1 = shfl(w, (C.lane & ~3) + 1);
+            const double w2 = shfl(w, (C.lane & ~3) + 2);
+            const double w3 = shfl(w, (C.lane & ~3) + 3);
+            if (i < n && k == 0) {
+                cx[i] = T.x[i];
+                cy[i] = T.y[i];
+                ccoarse[i] = T.w[i];
+                cerr[i] = corner_error(w, w1, w2, w3, T.w[i], 1.0);
+                cdepth[i] = 0;
+            }
+        }
+    }
+    wave_sync();
+    PqHeap<WaveStore> H{hk, hv, 0, WaveStore{}};
+    double lec = 0.0;  // local_error_counts
+    for (int i = 0; i < n; ++i) {
+        const double e = uni(cerr[i]);
+        lec += e;
+        H.push(e, i);
+        H.n = uni(H.n);
+    }
+    wave_sync();
+    double corr = 0.0;  // local_correction_counts
+    int splits = 0;
+    int pool = n;
+    while (H.n > 0 && lec > guarded(mag + corr / scale) * scale && splits < max_splits) {
+        double pk;
+        const int pi = uni(H.pop(&pk));
+        H.n = uni(H.n);
+        wave_sync();
+        const double perr = uni(cerr[pi]);
+        const int pdepth = uni(cdepth[pi]);
+        if (perr <= 0.0 || pdepth >= max_depth) break;
+        const double px = uni(cx[pi]), py = uni(cy[pi]), pcoarse = uni(ccoarse[pi]);
+        const double size = ldexp(incr, -pdepth);        // 0.5 x size, depth times
+        const double pfrac = ldexp(1.0, -2 * pdepth);    // 0.25 x area_fraction, depth times
+        const double offset = 0.25 * size;
+        const double csize = 0.5 * size;
+        const double cfrac = 0.25 * pfrac;
+        const double half = 0.5 * csize;
+        // lanes 0-3: child centres; lane 4 + 4 j + k: corner k of child j
+        double w = 0.0;
+        if (C.lane < 20) {
+            const int j = C.lane < 4 ? C.lane : (C.lane - 4) >> 2;
+            const double ccx = (j & 1) ? px + offset : px - offset;
+            const double ccy = (j & 2) ? py + offset : py - offset;
+            double x = ccx, y = ccy;
+            if (C.lane >= 4) {
+                const int k = (C.lane - 4) & 3;
+                x = (k & 1) ? ccx + half : ccx - half;
+                y = (k & 2) ? ccy + half : ccy - half;
+            }
+            w = refine_weight(C, x, y);
+        }
+        double coarse[4], err[4], ccxs[4], ccys[4];
+        double contribution = 0.0, child_error = 0.0;
+        int worst = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            coarse[j] = read_lane(w, j);
+            err[j] = corner_error(read_lane(w, 4 + 4 * j), read_lane(w, 5 + 4 * j), read_lane(w, 6 + 4 * j),
+                                  read_lane(w, 7 + 4 * j), coarse[j], cfrac);
+            ccxs[j] = (j & 1) ? px + offset : px - offset;
+            ccys[j] = (j & 2) ? py + offset : py - offset;
+            contribution += cfrac * coarse[j];
+            child_error += err[j];
+            if (err[j] > err[worst]) worst = j;
+        }
+        corr += contribution - pfrac * pcoarse;
+        lec += child_error - perr;
+        out.level = out.level > pdepth + 1 ? out.level : pdepth + 1;
+        ++splits;
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            wave_store(&cx[pool + j], ccxs[j]);
+            wave_store(&cy[pool + j], ccys[j]);
+            wave_store(&ccoarse[pool + j], coarse[j]);
+            wave_store(&cerr[pool + j], err[j]);
+            wave_store(&cdepth[pool + j], pdepth + 1);
+        }
+        wave_sync();
+        const bool all = lec <= guarded(mag + corr / scale) * scale || err[worst] <= 0.0 || pdepth + 1 >= max_depth;
+        for (int j = 0; j < 4; ++j)
+            if (all || j != worst) H.push(err[j], pool + j);
+        if (!all) H.push(err[worst], pool + worst);
+        H.n = uni(H.n);
+        wave_sync();
+        pool += 4;
+    }
+    if (out.level > 0) out.mag = mag + corr / scale;
+    double e = safety * lec / scale;
+    if (out.level == 0 && C.ld_c == 0.0 && C.ld_d == 0.0) e = e < base_err ? base_err : e;
+    out.err = e;
+    return out;
+}
+
+// Adaptive Cartesian epochs of one queue (the pass's queue of k_ir_cart), one
+// wave per workgroup and slot; slot b of the kernel uses the Cartesian slot b
+// of `scr` and the adaptive slot slot0 + b of `ad`.
+__global__ void __launch_bounds__(kWave) k_ir_cart_adapt(const CurveRec* __restrict__ curves, FsOptions opt,
+                                                          FsEpochs ep, IrLists L, CartScratch scr,
+                                                          AdaptScratch ad, int slot0, int pass)
+{
+    extern __shared__ double adapt_lds[];
+    const int slot = static_cast<int>(blockIdx.x);
+    const int aslot = slot0 + slot;
+    const bool large = pass != kPassSmall;
+    const int* queue = pass == kPassSmall ? L.cart_q : pass == kPassBig ? L.cart_big : L.cart_retry;
+    const int qword = pass == kPassSmall ? kQCartCount : pass == kPassBig ? kQCartBigCount : kQCartRetryCount;
+    unsigned* cursor = &ep.counters[qword + 1];
+    const int count = uni(static_cast<int>(ep.counters[qword]));
+    const size_t tc = static_cast<size_t>(ad.tape_cap), cc = static_cast<size_t>(ad.cell_cap);
+    double* tape = ad.tape + static_cast<size_t>(aslot) * tc * 3;
+    long long* key = ad.key + static_cast<size_t>(aslot) * tc * 2;
+    double* cell = ad.cell + static_cast<size_t>(aslot) * cc * 4;
+    int* depth = ad.depth + static_cast<size_t>(aslot) * cc;
+    double* hk = ad.heap_lds ? adapt_lds : ad.heap_key + static_cast<size_t>(aslot) * ad.heap_cap;
+    int* hv = ad.heap_lds ? reinterpret_cast<int*>(adapt_lds + ad.heap_cap)
+                          : ad.heap_val + static_cast<size_t>(aslot) * ad.heap_cap;
+    Ctx C;
+    for (;;) {
+        const int qi = next_item(cursor);
+        if (qi >= count) break;
+        const int item = uni(queue[qi]);
+        const int64_t e = L.ir_sorted[item];
+        const long long t_begin = static_cast<long long>(wall_clock64());
+        const CurveRec cv = curves[ep.curve_of[e]];
+        init_ctx(C, cv, ep.src[e], L.seed_pool + uni(L.seed_off[item]));
+        C.budget = opt.step_budget;
+        C.n_seeds = uni(L.seed_n[item]);
+        C.row_cap = scr.row_cap;
+        const int nbin = curve_nbin(cv, opt);
+        const bool adaptive = opt.adaptive != 0;
+        const double rho = C.rho;
+        Tape T;
+        T.x = tape;
+        T.y = tape + tc;
+        T.w = tape + 2 * tc;
+        T.kx = key;
+        T.ky = key + tc;
+        T.cap = adaptive ? (opt.max_source_bins > 64 ? opt.max_source_bins : 64) : 0;
+        auto start_tape = [&](int bins) {
+            const double incr = rho / static_cast<double>(bins > 1 ? bins : 1);
+            T.n = 0;
+            T.step = incr;
+            const double a = 4.0 * rho * incr, b = 4.0 * incr * incr;
+            T.band2 = a < b ? b : a;
+        };
+        start_tape(nbin);
+        double mag = C.n_seeds > 0 ? cartesian_mag<true>(C, nbin, scr, slot, &T) : __builtin_nan("");
+        double err = isfinite(mag) ? cart_error_estimate(C, nbin, mag) : __builtin_nan("");
+        int active = nbin > 1 ? nbin : 1;
+        int level = 0;
+        // needs_global_topology_retry (:1108-1121) on the first scan's diagnostics
+        if (!C.overflow && isfinite(mag) && opt.adaptive_source_bins != 0 && nbin < 50 &&
+            opt.max_source_bins > nbin && rho < 1.0e-2 && C.n_seeds >= 16 && C.d_overlaps >= 100 &&
+            C.d_gaps >= 1000 && C.d_max_jump >= 1000.0) {
+            const int twice = 2 * (nbin > 1 ? nbin : 1);
+            const int want = twice > 50 ? twice : 50;
+            const int rbins = want < opt.max_source_bins ? want : opt.max_source_bins;
+            if (rbins > nbin) {
+                C.d_processed = 0;
+                C.d_fold = 0;
+                C.d_boundary_rows = 0;
+                C.d_gaps = 0;
+                C.d_overlaps = 0;
+                C.d_max_jump = 0.0;
+                start_tape(rbins);
+                const double m2 = cartesian_mag<true>(C, rbins, scr, slot, &T);
+                // the reference's retry is finite whenever the first scan is;
+                // a retry that fails here (capacity, budget) fails the epoch
+                mag = m2;
+                err = isfinite(m2) ? cart_error_estimate(C, rbins, m2) : __builtin_nan("");
+                active = rbins;
+                level = 1;
+            }
+        }
+        bool full = false;
+        if (!C.overflow && isfinite(mag) && adaptive && T.n > 0) {
+            const Refined r = refine_cells(C, opt, T, T.step, nbin, active, mag, err, level, cell, cell + cc,
+                                           cell + 2 * cc, cell + 3 * cc, depth, ad.cell_cap, hk, hv);
+            full = r.full;
+            mag = r.mag;
+            err = r.err;
+            level = r.level;
+        }
+        if (C.cap_overflow && !large) {
+            const int pos = wave_fetch_inc(&ep.counters[kQCartRetryCount]);
+            wave_store(&L.cart_retry[pos], item);
+        } else {
+            const bool bad = C.overflow || full;
+            wave_store(&ep.mag[e], bad ? __builtin_nan("") : mag);
+            wave_store(&ep.err[e], bad ? __builtin_nan("") : err);
+            wave_store(&ep.level[e], level | ((adaptive && active < 32) ? kLevelUnconverged : 0));
+            if (ep.cost) {
+                const int64_t n = ep.n;
+                wave_store(&ep.cost[5 * n + e], static_cast<int64_t>(C.d_processed));
+                wave_store(&ep.cost[6 * n + e], static_cast<int64_t>(C.d_fold));
+                wave_store(&ep.cost[7 * n + e], static_cast<int64_t>(C.d_boundary_rows));
+                wave_store(&ep.cost[8 * n + e], static_cast<int64_t>(C.d_gaps));
+                wave_store(&ep.cost[9 * n + e], static_cast<int64_t>(C.d_overlaps));
+                wave_store(&ep.cost[10 * n + e], static_cast<int64_t>(__double_as_longlong(C.d_max_jump)));
+            }
+            if (C.overflow) atomicOr(&ep.counters[kQFlags], C.cap_overflow ? kErrRowCap : kErrBudget);
+            if (full) atomicOr(&ep.counters[kQFlags], kErrAdaptCap);
+        }
+        flush_work(C, ep);
+        const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
+        add_total(&work_totals(ep)[6], ticks);
+        if (ep.cost) {
+            wave_store(&ep.cost[ep.n + e], ep.cost[ep.n + e] + static_cast<int64_t>(ticks));
+            wave_store(&ep.cost[3 * ep.n + e], ep.cost[3 * ep.n + e] + static_cast<int64_t>(C.n_rows));
+            wave_store(&ep.cost[4 * ep.n + e], ep.cost[4 * ep.n + e] + static_cast<int64_t>(C.steps));
+        }
+        if (ep.sched) wave_store(&ep.sched[e], ep.sched[e] + static_cast<float>(ticks));
+        wave_sync();
+    }
+}
+
 // Polar flood fills of the queued epochs, in the same three passes.
 __global__ void __launch_bounds__(kWave) k_ir_polar(const CurveRec* __restrict__ curves,
                                                      FsOptions opt, FsEpochs ep, IrLists L,
@@ -2788,10 +3223,25 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     lcbg_status status = check_launch();
     if (status == LCBG_OK) status = fork(ss.fork_lead, st, ss.big);
     if (status != LCBG_OK) return status;
+    // adaptive runs scan their Cartesian epochs with k_ir_cart_adapt, one wave
+    // per slot, in the same passes
+    const bool adapt = opt.adaptive_source_bins != 0;
+    const unsigned adapt_lds = scr.adapt.heap_lds ? static_cast<unsigned>(scr.adapt.heap_cap) * 12u : 0u;
+    auto cart_pass = [&](int pass, hipStream_t s) {
+        const bool small = pass == kPassSmall;
+        const CartScratch& cs = small ? scr.cart : scr.cart_large;
+        if (adapt) {
+            int n = small ? scr.adapt_small : scr.adapt_large;
+            n = n < cs.n_slots ? n : cs.n_slots;  // the schedule may leave fewer Cartesian slots
+            hipLaunchKernelGGL(k_ir_cart_adapt, dim3(static_cast<unsigned>(n)), dim3(kWave), adapt_lds, s,
+                               d_curves, opt, ep, lists, cs, scr.adapt, small ? scr.adapt_large : 0, pass);
+        } else {
+            hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(cs.n_slots / kTeamWaves)),
+                               dim3(kTeamWaves * kWave), 0, s, d_curves, opt, ep, lists, cs, pass);
+        }
+    };
     mark(2 * kIrCartBig, ss.big);
-    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots / kTeamWaves)),
-                       dim3(kTeamWaves * kWave), 0,
-                       ss.big, d_curves, opt, ep, lists, scr.cart_large, int{kPassBig});
+    cart_pass(kPassBig, ss.big);
     mark(2 * kIrCartBig + 1, ss.big);
     status = check_launch();
     // the leading polar epochs on large slots (polar stream)
@@ -2828,9 +3278,7 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     status = check_launch();
     if (status != LCBG_OK) return status;
     mark(2 * kIrCartSmall, st);
-    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart.n_slots / kTeamWaves)),
-                       dim3(kTeamWaves * kWave), 0, st,
-                       d_curves, opt, ep, lists, scr.cart, int{kPassSmall});
+    cart_pass(kPassSmall, st);
     mark(2 * kIrCartSmall + 1, st);
     status = check_launch();
     if (status == LCBG_OK) status = fork(ss.join_big, ss.big, st);
@@ -2839,15 +3287,24 @@ lcbg_status launch_fs_ir(const CurveRec* d_curves, const FsCache& cache, const F
     if (status != LCBG_OK) return status;
     // the epochs that outgrew a small slot, on the (now free) large slots
     mark(2 * kIrCartRetry, st);
-    hipLaunchKernelGGL(k_ir_cart, dim3(static_cast<unsigned>(scr.cart_large.n_slots / kTeamWaves)),
-                       dim3(kTeamWaves * kWave), 0,
-                       st, d_curves, opt, ep, lists, scr.cart_large, int{kPassRetry});
+    cart_pass(kPassRetry, st);
     mark(2 * kIrCartRetry + 1, st);
     mark(2 * kIrPolarRetry, st);
     hipLaunchKernelGGL(k_ir_polar, dim3(static_cast<unsigned>(scr.polar_large.n_slots)), dim3(kWave), 0,
                        st, d_curves, opt, ep, lists, scr.polar_large, int{kPassRetry});
     mark(2 * kIrPolarRetry + 1, st);
     return check_launch();
+}
+
+// Resident workgroups per CU of the adaptive Cartesian kernel with `lds`
+// bytes of dynamic LDS.
+int adapt_resident_per_cu(unsigned lds)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(k_ir_cart_adapt), kWave,
+                                                     lds) != hipSuccess || n < 1)
+        n = 1;
+    return n;
 }
 
 int ir_resident_per_cu(int kind)

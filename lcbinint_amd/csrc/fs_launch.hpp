@@ -108,6 +108,7 @@ lcbg_status launch_grad_rest(const CurveRec* d_curves, int n_curves, const FsOpt
 // Workgroups of one inverse-ray kernel resident per CU (kind 0 seeds,
 // 1 Cartesian, 2 polar), from its register and LDS use.
 int ir_resident_per_cu(int kind);
+int adapt_resident_per_cu(unsigned lds);
 
 // Records the first epoch of mag[0, n) that is not finite in
 // counters[kFirstFailWord] (n - e; 0: none).

@@ -224,6 +224,8 @@ extern "C" lcbg_status lcbg_light_curve_device_noalloc(const lcbg_curve* curve, 
     const lcbg_options o = options ? *options : lcbg_default_options();
     const int bins = o.caustic_bins > 32 ? o.caustic_bins : 32;
     if (bins > kNaMaxBins) return LCBG_INVALID_ARGUMENT;
+    // the retry and the refinement need workspace this array does not hold
+    if (o.adaptive_source_bins != 0) return LCBG_UNSUPPORTED;
     if (n == 0) return LCBG_OK;
     const hipStream_t st_ = static_cast<hipStream_t>(stream);
     int dev = 0;

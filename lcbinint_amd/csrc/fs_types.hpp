@@ -103,7 +103,29 @@ struct FsOptions {
     // wave steps one inverse-ray epoch may take before it is abandoned
     // (lcbg_batch_set_work_budget; default kStepBudget)
     long long step_budget;
+    // adaptive inverse ray (FiniteSourceSettings::adaptive_source_bins,
+    // max_source_bins, finite_source_tol / reltol): adaptive_source_bins != 0
+    // allows the topology retry at a finer lattice; `adaptive` (it and a
+    // positive tolerance) also widens the bounding-box margin, derives the
+    // point-source and hexadecapole thresholds from the tolerances, and refines
+    // the Cartesian areas locally (finite_source_magnifier.cpp:3071-3296, 4679-4790)
+    int adaptive_source_bins;
+    int max_source_bins;
+    int adaptive;
+    double tol, reltol;
 };
+
+// Bounding-box margin of the dispatcher (finite_source_magnifier.cpp:4678-4685).
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline double bbox_margin(const FsOptions& o, double rho)
+{
+    const double m = o.kinji * rho;
+    if (!o.adaptive) return m;
+    const double wide = 60.0 * rho;
+    return m < wide ? wide : m;
+}
 
 struct FsEpochs {
     int64_t n;
@@ -130,6 +152,8 @@ struct FsEpochs {
     double* dint;       // [epoch][6] tangents of the magnification (gradient runs)
     double* err;        // [epoch] error estimate of the method that settled the epoch
                         // (FiniteSourceResult::error_estimate, finite_source_magnifier.hpp:47-54)
+    int32_t* level;     // [epoch] adaptive Cartesian epochs: refinement level, plus
+                        // kLevelUnconverged when fewer than 32 bins were active
     const double* src_y;// [epoch] the impact parameter in place of the curve's u0, or null
                         // (curves with t0 = 0, tE = 1, alpha = 0 whose source positions
                         // come from a trajectory: times are x, src_y is y)
@@ -171,6 +195,22 @@ struct CartScratch {
     int* row_key;      // [slot][row_ints_per_slot(row_cap)]: nodes, heads, dirty
 };
 
+// Workspace of the adaptive Cartesian kernel per slot: the refinement tape
+// (cells recorded by the scans), the pool of refinement cells (the tape's, then
+// four per split) and, when it does not fit in LDS, the refinement queue.
+struct AdaptScratch {
+    int tape_cap;      // max(64, max_source_bins)
+    int cell_cap;      // tape_cap + 4 splits of the largest split budget
+    int heap_cap;      // tape_cap + 3 splits + 1
+    int heap_lds;      // 1: the queue lives in the kernel's dynamic LDS
+    double* tape;      // [slot][tape_cap*3]: x, y, weight
+    long long* key;    // [slot][tape_cap*2]: lattice keys
+    double* cell;      // [slot][cell_cap*4]: centre x, y, coarse weight, error weight
+    int* depth;        // [slot][cell_cap]
+    double* heap_key;  // [slot][heap_cap] (heap_lds == 0)
+    int* heap_val;     // [slot][heap_cap]
+};
+
 struct PolarScratch {
     int n_slots;
     int run_cap;
@@ -183,6 +223,12 @@ struct IrScratch {
     CartScratch cart, cart_large;
     PolarScratch polar, polar_large;
     int farm_slots;  // resident waves of the probe farm
+    // adaptive Cartesian kernel: its slots are the first adapt_large slots of
+    // cart_large (adaptive slots [0, adapt_large)) and the first adapt_small
+    // slots of cart (adaptive slots from adapt_large on); tape == null until
+    // an adaptive run allocates them
+    AdaptScratch adapt;
+    int adapt_large, adapt_small;
 };
 
 // Work lists of the inverse-ray stage, in the epoch workspace.  Items index
@@ -271,6 +317,10 @@ constexpr unsigned kErrBudget = 16u;
 constexpr unsigned kErrSeedPool = 32u;
 constexpr unsigned kErrTeam = 64u;  // a Cartesian team wait ran out of time
 constexpr unsigned kErrGradPool = 128u;  // the gradient's run or chunk pool was full
+constexpr unsigned kErrAdaptCap = 256u;  // the adaptive refinement's cell pool was full
+// FsEpochs::level flag: an adaptive epoch scanned with fewer than 32 bins,
+// never converged (finite_source_magnifier.cpp:3292-3294)
+constexpr int kLevelUnconverged = 256;
 
 // Work budget of one inverse-ray epoch in wave steps (64-cell chunks, probe
 // merges, polar runs); an epoch that exceeds it ends with an error flag, so

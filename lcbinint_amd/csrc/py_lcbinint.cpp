@@ -194,9 +194,7 @@ public:
     {
         const bool supported = lens_ == "binary_lens" && source_ == "single" && !terrestrial_ &&
                                options_.vbm_compatible != 0 && options_.center_of_mass == 0 &&
-                               options_.xallarap == "none" &&
-                               (options_.adaptive_source_bins == 0 ||
-                                (options_.finite_source_tol <= 0.0 && options_.finite_source_reltol <= 0.0));
+                               options_.xallarap == "none";
         if (!supported) throw std::runtime_error("unsupported");
         if (!std::isfinite(tE) || tE == 0.0 || !std::isfinite(q) || !std::isfinite(s))
             throw std::runtime_error("numerical error");
@@ -215,6 +213,8 @@ public:
         lcbg_options opts = options_.o;
         opts.finite_source_tol = options_.finite_source_tol;
         opts.finite_source_reltol = options_.finite_source_reltol;
+        opts.adaptive_source_bins = options_.adaptive_source_bins;
+        opts.max_source_bins = options_.max_source_bins;
         return opts;
     }
 

@@ -100,6 +100,8 @@ extern "C" lcbg_options lcbg_default_options(void)
     o.adaptive_hex_threshold = 0.001;
     o.finite_source_tol = 0.0;
     o.finite_source_reltol = 0.0;
+    o.adaptive_source_bins = 0;
+    o.max_source_bins = 400;
     return o;
 }
 

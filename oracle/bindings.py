@@ -30,11 +30,12 @@ class RefSettings(ctypes.Structure):
         ("grid_ratio", D), ("polar_source_bins", ctypes.c_int), ("polar_grid_ratio", D),
         ("finite_mode", ctypes.c_int), ("kinji_threshold", D), ("hex_threshold", D),
         ("adaptive_hex_threshold", D), ("limb_darkening_c", D), ("limb_darkening_d", D),
+        ("adaptive_source_bins", ctypes.c_int), ("max_source_bins", ctypes.c_int),
     ]
 
 
 def default_settings(**kw):
-    s = RefSettings(50, 1400, 4.0, 0, 0.0, 4, 20.0, 3.0, 0.001, 0.0, 0.0)
+    s = RefSettings(50, 1400, 4.0, 0, 0.0, 4, 20.0, 3.0, 0.001, 0.0, 0.0, 0, 400)
     for k, v in kw.items():
         setattr(s, k, v)
     return s

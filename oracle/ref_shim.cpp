@@ -46,6 +46,8 @@ struct RefSettings {
     double adaptive_hex_threshold;
     double limb_darkening_c;
     double limb_darkening_d;
+    int adaptive_source_bins;
+    int max_source_bins;
 };
 
 FiniteSourceSettings to_settings(const RefSettings& s)
@@ -62,6 +64,8 @@ FiniteSourceSettings to_settings(const RefSettings& s)
     out.adaptive_hex_threshold = s.adaptive_hex_threshold;
     out.limb_darkening_c = s.limb_darkening_c;
     out.limb_darkening_d = s.limb_darkening_d;
+    out.adaptive_source_bins = s.adaptive_source_bins;
+    out.max_source_bins = s.max_source_bins;
     return out;
 }
 
