@@ -451,8 +451,11 @@ lcbg_status lcbg_batch_finite_source_profile(lcbg_batch* batch, const int64_t* o
  * (finite_source_magnifier.cpp:1006-1015): cost[5n + e] processed images,
  * cost[6n + e] fold-guarded images, cost[7n + e] counted rows (boundary_rows),
  * cost[8n + e] gap repairs, cost[9n + e] overlap hits and cost[10n + e] the
- * bits of the double max_jump_cells.  cost is a device pointer with 11
- * n_epochs entries; other epochs are left untouched.  NULL turns it off.
+ * bits of the double max_jump_cells.  Gradient runs add, per inverse-ray
+ * epoch, the ticks of the cell enumeration of its quadrature to cost[11n + e],
+ * its row runs to cost[12n + e], its cells to cost[13n + e] and its wave steps
+ * to cost[14n + e].  cost is a device pointer with 15 n_epochs entries; other
+ * epochs are left untouched.  NULL turns it off.
  * lcbg_wall_clock_rate_khz() gives the tick rate of the current device. */
 lcbg_status lcbg_batch_set_epoch_cost(lcbg_batch* batch, int64_t* cost);
 int64_t lcbg_wall_clock_rate_khz(void);

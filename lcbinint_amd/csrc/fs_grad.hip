@@ -337,6 +337,80 @@ __device__ __forceinline__ bool aa_visited(const AaFill& F, int i, int j)
     return false;
 }
 
+// The runs already recorded in row j, one per lane (lane k < n holds run k),
+// gathered by one walk of the row's hash chain; `many` when the row has more
+// runs than lanes (the checks then walk the chain per cell).
+struct RowRuns {
+    int l, r;
+    int n;
+    bool many;
+};
+
+__device__ __forceinline__ int lane_int(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+
+__device__ RowRuns row_runs(const AaFill& F, int j)
+{
+    RowRuns V{0, -1, 0, false};
+    const int lane = lane_id();
+    for (int p = uni(F.head[j & (kPhiHash - 1)]); p >= 0;) {
+        const int row = uni(F.runs[4 * p + 0]);
+        const int l = uni(F.runs[4 * p + 1]);
+        const int r = uni(F.runs[4 * p + 2]);
+        const int next = uni(F.runs[4 * p + 3]);
+        if (row == j) {
+            if (V.n < kWave) {
+                if (lane == V.n) {
+                    V.l = l;
+                    V.r = r;
+                }
+                V.n = V.n + 1;
+            } else {
+                V.many = true;
+            }
+        }
+        p = next;
+    }
+    return V;
+}
+
+__device__ __forceinline__ void row_runs_add(RowRuns& V, int l, int r)
+{
+    if (V.n < kWave) {
+        if (lane_id() == V.n) {
+            V.l = l;
+            V.r = r;
+        }
+        V.n = V.n + 1;
+    } else {
+        V.many = true;
+    }
+}
+
+__device__ __forceinline__ bool row_visited(const AaFill& F, const RowRuns& V, int i, int j)
+{
+    if (V.many) return aa_visited(F, i, j);
+    bool v = false;
+    for (int k = 0; k < V.n; ++k) v = v || (i >= lane_int(V.l, k) && i <= lane_int(V.r, k));
+    return v;
+}
+
+// The first cell at or after i (uniform) that no recorded run of the row holds.
+__device__ __forceinline__ int skip_runs(const RowRuns& V, int i)
+{
+    if (V.many) return i;
+    for (bool moved = true; moved;) {
+        moved = false;
+        for (int k = 0; k < V.n; ++k) {
+            const int l = lane_int(V.l, k), r = lane_int(V.r, k);
+            if (i >= l && i <= r) {
+                i = r + 1;
+                moved = true;
+            }
+        }
+    }
+    return i;
+}
+
 __device__ __forceinline__ bool aa_step(AaFill& F)
 {
     F.steps = uni(F.steps + 1);
@@ -344,35 +418,28 @@ __device__ __forceinline__ bool aa_step(AaFill& F)
     return !F.overflow;
 }
 
-// Extends cell (i, j) to its run along the row and records it.
-__device__ void aa_add_run(const GradCtx& G, AaFill& F, int i, int j)
+// Cells k = 0, 1, ... of the row from `from` in direction `dir` that are in
+// the fill and unvisited, 64 per wave step: the count of such cells before the
+// first that is not.
+__device__ int extend_cells(const GradCtx& G, AaFill& F, const RowRuns& V, int from, int dir, int j)
 {
-    int left = i;
+    int n = 0;
     for (;;) {
-        if (!aa_step(F)) return;
-        const int c = left - 1 - G.lane;
-        const bool ok = !aa_visited(F, c, j) && aa_in(G, c, j);
+        if (!aa_step(F)) return n;
+        const int c = from + dir * (n + G.lane);
+        const bool ok = !row_visited(F, V, c, j) && aa_in(G, c, j);
         const int stop = first_set(~ballot(ok));
         if (stop < 0) {
-            left -= kWave;
+            n += kWave;
             continue;
         }
-        left -= stop;
-        break;
+        return n + stop;
     }
-    int right = i;
-    for (;;) {
-        if (!aa_step(F)) return;
-        const int c = right + 1 + G.lane;
-        const bool ok = !aa_visited(F, c, j) && aa_in(G, c, j);
-        const int stop = first_set(~ballot(ok));
-        if (stop < 0) {
-            right += kWave;
-            continue;
-        }
-        right += stop;
-        break;
-    }
+}
+
+// Records run [left, right] of row j.
+__device__ void record_run(AaFill& F, RowRuns& V, int left, int right, int j)
+{
     if (F.n_runs >= F.run_cap) {
         F.overflow = true;
         return;
@@ -389,7 +456,93 @@ __device__ void aa_add_run(const GradCtx& G, AaFill& F, int i, int j)
     }
     F.n_runs = uni(F.n_runs + 1);
     F.cells = uni(F.cells + static_cast<long long>(right - left + 1));
+    row_runs_add(V, left, right);
     wave_sync();
+}
+
+// Extends cell (i, j) to its run along the row and records it.
+__device__ void aa_add_run(const GradCtx& G, AaFill& F, int i, int j)
+{
+    RowRuns V = row_runs(F, j);
+    const int left = i - extend_cells(G, F, V, i - 1, -1, j);
+    const int right = i + extend_cells(G, F, V, i + 1, 1, j);
+    if (F.overflow) return;
+    record_run(F, V, left, right, j);
+}
+
+// The runs of row j that touch cells [lo, hi] (8-connected neighbours of a
+// run of the next or previous row).  A row with one recorded run overlapping
+// [lo, hi] (most often the row the growing run was found from) first has the
+// cells of [lo, hi] outside that run tested in one gathered step, which
+// usually finds none in the fill and ends the row.  Otherwise the range is evaluated in
+// chunks of 64 consecutive cells, stretches held by recorded runs skipped
+// without evaluation, the first chunk reaching kLookLeft cells left of lo; a
+// run found in a chunk takes its cells from that chunk's evaluation and is
+// extended past it only where it reaches the chunk's edge.  A run that starts
+// right of lo has an evaluated neighbour on its left outside the fill (a
+// visited neighbour would have been taken into its run when that run was
+// extended, the fill being a fixed set), so it needs no left extension.
+constexpr int kLookLeft = 8;
+
+__device__ void grow_row(const GradCtx& G, AaFill& F, int lo, int hi, int j)
+{
+    RowRuns V = row_runs(F, j);
+    // the row's recorded runs that overlap [lo, hi]: one of them (m == 1) at k1
+    int m = 0, k1 = 0;
+    for (int k = 0; k < V.n && !V.many; ++k) {
+        if (lane_int(V.l, k) <= hi && lane_int(V.r, k) >= lo) {
+            ++m;
+            k1 = k;
+        }
+    }
+    if (!V.many && m == 1) {
+        const int l1 = lane_int(V.l, k1), r1 = lane_int(V.r, k1);
+        const int lend = l1 - 1 < hi ? l1 - 1 : hi;
+        const int nl = lend >= lo ? lend - lo + 1 : 0;    // [lo, min(l1 - 1, hi)]
+        const int rs = r1 + 1 > lo ? r1 + 1 : lo;
+        const int nr = hi >= rs ? hi - rs + 1 : 0;         // [max(r1 + 1, lo), hi]
+        if (nl + nr <= kWave) {
+            if (!aa_step(F)) return;
+            bool ok = false;
+            if (G.lane < nl + nr) ok = aa_in(G, G.lane < nl ? lo + G.lane : rs + (G.lane - nl), j);
+            if (!ballot(ok)) return;
+        }
+    }
+    int c = lo;
+    bool first = true;
+    while (!F.overflow) {
+        c = skip_runs(V, c);
+        if (c > hi || !aa_step(F)) break;
+        const int b = (first && c == lo) ? lo - kLookLeft : c;  // cell of lane 0
+        first = false;
+        const int cc = b + G.lane;
+        const bool ok = !row_visited(F, V, cc, j) && aa_in(G, cc, j);
+        const uint64_t okm = ballot(ok);
+        const int from = c - b, to = hi - b;  // lanes that may start a run
+        uint64_t pending = okm & (to >= kWave - 1 ? ~0ull : ((2ull << to) - 1ull)) & ~((1ull << from) - 1ull);
+        int next = b + kWave;
+        while (pending && !F.overflow) {
+            const int sb = first_set(pending);
+            const int s = b + sb;
+            int left = s;
+            if (s == lo) {
+                const uint64_t below = ~okm & ((1ull << sb) - 1ull);  // cells left of s outside
+                left = below ? b + (63 - __builtin_clzll(below)) + 1 : b - extend_cells(G, F, V, b - 1, -1, j);
+            }
+            const uint64_t rest = ~(okm >> sb);  // zero bits: the run's cells in this chunk
+            const int len = rest ? first_set(rest) : kWave - sb;
+            int right = s + len - 1;
+            if (sb + len == kWave) {
+                right = right + extend_cells(G, F, V, b + kWave, 1, j);
+                next = right + 1;
+            }
+            if (F.overflow) break;
+            record_run(F, V, left, right, j);
+            const int done = sb + len;  // bits [0, done] are settled (done: not in the fill)
+            pending = done >= kWave - 1 ? 0ull : pending & ~((2ull << done) - 1ull);
+        }
+        c = next;
+    }
 }
 
 // The cells of one epoch's antialiased quadrature, flood-filled from the
@@ -463,25 +616,11 @@ __device__ Dual aa_enumerate(const GradCtx& G, AaFill& F, const Pt* seeds, int n
     // neighbours included (8-connected), so that a thin diagonal image stays
     // one component while its fringe cells' weights go to zero
     for (int q = 0; q < F.n_runs && !F.overflow; ++q) {
-        const int row = F.runs[4 * q + 0];
-        const int L = F.runs[4 * q + 1] - 1;
-        const int R = F.runs[4 * q + 2] + 1;
-        for (int side = 0; side < 2; ++side) {
-            const int nb = side == 0 ? row - 1 : row + 1;
-            int c = L;
-            while (c <= R && aa_step(F)) {
-                const int cc = c + G.lane;
-                bool ok = false;
-                if (cc <= R) ok = !aa_visited(F, cc, nb) && aa_in(G, cc, nb);
-                const uint64_t m = ballot(ok);
-                if (!m) {
-                    c += kWave;
-                    continue;
-                }
-                aa_add_run(G, F, c + first_set(m), nb);
-                c = F.runs[4 * (F.n_runs - 1) + 2] + 1;
-            }
-        }
+        const int row = uni(F.runs[4 * q + 0]);
+        const int L = uni(F.runs[4 * q + 1]) - 1;
+        const int R = uni(F.runs[4 * q + 2]) + 1;
+        grow_row(G, F, L, R, row - 1);
+        grow_row(G, F, L, R, row + 1);
     }
     wave_sync();
     for (int q = G.lane; q < F.n_runs; q += kWave) F.head[F.runs[4 * q] & (kPhiHash - 1)] = -1;
@@ -575,6 +714,7 @@ __global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ 
         if (qi >= count) break;
         const int item = retry ? uni(Lst.grad_retry[qi]) : qi;
         const int64_t e = Lst.ir_sorted[item];
+        const long long t_begin = static_cast<long long>(wall_clock64());
         const int n_seeds = uni(Lst.seed_n[item]);
         const CurveRec cv = curves[ep.curve_of[e]];
         const GradCtx G = grad_ctx(cv, ep.src[e], opt);
@@ -587,6 +727,16 @@ __global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ 
             failed = F.overflow;
         }
         const bool cap = F.overflow && F.n_runs >= F.run_cap;
+        if (ep.cost) {
+            // enumeration ticks, runs, cells and wave steps (added: a capacity
+            // retry adds its own)
+            const int64_t n = ep.n;
+            const long long ticks = static_cast<long long>(wall_clock64()) - t_begin;
+            wave_store(&ep.cost[11 * n + e], ep.cost[11 * n + e] + static_cast<int64_t>(ticks));
+            wave_store(&ep.cost[12 * n + e], ep.cost[12 * n + e] + static_cast<int64_t>(F.n_runs));
+            wave_store(&ep.cost[13 * n + e], ep.cost[13 * n + e] + static_cast<int64_t>(F.cells));
+            wave_store(&ep.cost[14 * n + e], ep.cost[14 * n + e] + static_cast<int64_t>(F.steps));
+        }
         if (cap && !retry) {
             const int pos = wave_fetch_inc(&ep.counters[kQGradRetryCount]);
             wave_store(&Lst.grad_retry[pos], item);

@@ -118,7 +118,7 @@ def test_cartesian_diagnostics_match_reference(gpu, ref):
     o.mode = 1
     batch = gpu.Batch([gpu.Curve(**cv)], o)
     n = len(times)
-    cost = torch.zeros(11 * n, dtype=torch.int64, device="cuda")
+    cost = torch.zeros(15 * n, dtype=torch.int64, device="cuda")
     gpu.check(gpu.lcbg_batch_set_epoch_cost(batch.handle, cost.data_ptr()), "set_epoch_cost")
     d_off = torch.tensor([0, n], dtype=torch.int64, device="cuda")
     d_t = torch.from_numpy(times).cuda()
