@@ -175,7 +175,18 @@ lcbg_status lcbg_light_curve_point_source(const lcbg_curve* curve, const double*
  * degrees, reference time tfix, 0 = t0) of EventCoordinates
  * (python/lcbinint_pybind.cpp:95-101), and the orbital motion parameters of
  * binary_params_from_dict (:145-179): VBM (w1, w2, w3) = (g1, g2, g3) in
- * 1/day, and for the Keplerian orbit lom_szs, lom_ar (default 1). */
+ * 1/day, and for the Keplerian orbit lom_szs, lom_ar (default 1).
+ * The fields after lom_ar are the reference's source-trajectory extensions
+ * (src/lcbinint/model/trajectory.cpp:287-401), zero when unused:
+ * terrestrial parallax from an observatory at (obs_lat, obs_lon) degrees,
+ * applied with annual parallax when either is nonzero (lcbi_params obs_lat,
+ * obs_lon; LightCurve(terrestrial_parallax=True)); xallarap, the source's
+ * own orbit, with xallarap_type = lcbi_xallarap_param_type (0 none, 1
+ * angular velocity: xi_1, xi_2, omega_xa, inc_xa, phi_xa; 2 orbital
+ * elements: piEN_xa, piEE_xa, period_xa, ecc_xa, peri_xa, inc_xa).
+ * Amplitudes of the other xallarap form than xallarap_type make the curve
+ * unsupported, as has_unsupported_dynamic_effects does
+ * (src/lcbinint/model/lens_model.cpp:15-26). */
 typedef struct lcbg_dynamics {
     double piEN;
     double piEE;
@@ -187,6 +198,20 @@ typedef struct lcbg_dynamics {
     double g3;
     double lom_szs;
     double lom_ar;
+    double obs_lat;
+    double obs_lon;
+    int32_t xallarap_type;
+    int32_t reserved;
+    double xi_1;
+    double xi_2;
+    double omega_xa;
+    double inc_xa;
+    double phi_xa;
+    double piEN_xa;
+    double piEE_xa;
+    double period_xa;
+    double ecc_xa;
+    double peri_xa;
 } lcbg_dynamics;
 
 /* Orbital motion models, the values of lcbi_orbital_motion_mode
@@ -206,7 +231,9 @@ lcbg_status lcbg_load_earth_ephemeris(const char* path);
  * as LensModel::magnification (src/lcbinint/model/lens_model.cpp:84-104,
  * 123-178) derives them: Trajectory::source_position with annual parallax
  * (trajectory.cpp:436-484), orbital_state (orbital_motion.cpp:183-199) and
- * the rotation into the instantaneous binary frame (VBM convention). */
+ * the rotation into the instantaneous binary frame (VBM convention).  Curves
+ * with terrestrial parallax or xallarap: LCBG_UNSUPPORTED (the device entry
+ * points cover them). */
 lcbg_status lcbg_dynamic_sources(const lcbg_curve* curve, const lcbg_dynamics* dyn,
                                  int32_t orbital_motion_mode, const double* times, int64_t n,
                                  double* xys);

@@ -253,9 +253,15 @@ def work_counts(batch: Batch):
                      "area_ticks", "rows"), list(out)))
 
 class Dynamics(ctypes.Structure):
-    """lcbg_dynamics: annual parallax, event coordinates and orbital motion."""
-    _fields_ = [(k, ctypes.c_double) for k in ("piEN", "piEE", "ra", "dec", "tfix", "g1", "g2", "g3",
-                                               "lom_szs", "lom_ar")]
+    """lcbg_dynamics: annual and terrestrial parallax, event coordinates, orbital
+    motion and xallarap."""
+    _fields_ = ([(k, ctypes.c_double) for k in ("piEN", "piEE", "ra", "dec", "tfix", "g1", "g2", "g3",
+                                                "lom_szs", "lom_ar", "obs_lat", "obs_lon")] +
+                [("xallarap_type", ctypes.c_int32), ("reserved", ctypes.c_int32)] +
+                [(k, ctypes.c_double) for k in ("xi_1", "xi_2", "omega_xa", "inc_xa", "phi_xa", "piEN_xa",
+                                                "piEE_xa", "period_xa", "ecc_xa", "peri_xa")])
+
+    XALLARAP_NONE, XALLARAP_ANGULAR_VELOCITY, XALLARAP_ORBITAL_ELEMENTS = 0, 1, 2
 
     def __init__(self, **kw):
         kw.setdefault("lom_ar", 1.0)
@@ -466,6 +472,23 @@ def light_curve_results(curve: Curve, times, options: Options | None = None):
     st = lcbg_light_curve_results(ctypes.byref(curve), ctypes.byref(options) if options is not None else None,
                                   t.ctypes.data_as(c_double_p), len(t), out.ctypes.data, ctypes.byref(first))
     return out, st, first.value
+
+
+lcbg_batch_light_curves_dynamic_results = _sig(
+    "lcbg_batch_light_curves_dynamic_results", ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(Dynamics),
+    ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p)
+
+
+def light_curves_dynamic_results(batch: "Batch", dyn, orbit_mode: int, offsets, times):
+    """(records, status) of the batch's curves on the LensModel path: records is
+    a structured array of RESULT_DTYPE (host), status the lcbg_status."""
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    d = (Dynamics * len(dyn))(*dyn)
+    out = np.zeros(len(t), dtype=RESULT_DTYPE)
+    st = lcbg_batch_light_curves_dynamic_results(batch.handle, d, int(orbit_mode), off.ctypes.data,
+                                                 t.ctypes.data, len(t), out.ctypes.data)
+    return out, st
 
 
 def last_run_status(batch: "Batch"):

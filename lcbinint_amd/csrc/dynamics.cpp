@@ -527,6 +527,28 @@ inline TDual<kTrajParams> to_u(const Dual<kTrajParams>& x, TDual<kTrajParams>*)
     return r;
 }
 
+// keplerian_position of the xallarap orbit (trajectory.cpp:318-333), host libm.
+void xallarap_position(double time, double t0, double period, double ecc, double peri, double* x, double* y)
+{
+    const double mean = 2.0 * kPiHost / period * (time - t0) + peri;
+    double E = mean + ecc * std::sin(mean);
+    for (int i = 0; i < 10; ++i) {
+        const double f = E - ecc * std::sin(E) - mean;
+        const double fp = 1.0 - ecc * std::cos(E);
+        const double delta = f / fp;
+        E -= delta;
+        if (std::abs(delta) < 1.0e-13) break;
+    }
+    const double cos_e = std::cos(E);
+    const double sin_e = std::sin(E);
+    const double r = 1.0 - ecc * cos_e;
+    const double root = std::sqrt(std::max(0.0, 1.0 - ecc * ecc));
+    const double cos_f = (cos_e - ecc) / r;
+    const double sin_f = root * sin_e / r;
+    *x = r * cos_f;
+    *y = r * sin_f;
+}
+
 template <class T, class U>
 void traj_consts(const TrajIn<T>& P, const lcbg_dynamics& d, int orbit_mode, TrajConst<U>* out)
 {
@@ -580,6 +602,48 @@ void traj_consts(const TrajIn<T>& P, const lcbg_dynamics& d, int orbit_mode, Tra
     for (int i = 0; i < 3; ++i) {
         K.xa[i] = to_u(orbit.xa[i], tag);
         K.ya[i] = to_u(orbit.ya[i], tag);
+    }
+    // terrestrial parallax: with annual parallax and an observatory position
+    K.terrestrial = (parallax && (d.obs_lat != 0.0 || d.obs_lon != 0.0)) ? 1 : 0;
+    K.obs_lon = K.obs_rc = K.obs_rs = 0.0;
+    if (K.terrestrial) {
+        const double lat = d.obs_lat * kDeg;
+        K.obs_lon = d.obs_lon * kDeg;
+        K.obs_rc = kTrajEarthRadiusAu * std::cos(lat);
+        K.obs_rs = kTrajEarthRadiusAu * std::sin(lat);
+    }
+    // xallarap of the form the options name, when its amplitudes are set
+    K.xallarap = kXaNone;
+    K.xa_sinc = K.xa_phi = K.xa_cphi = K.xa_sphi = K.xa_omega = K.xa_xi1 = K.xa_xi2 = 0.0;
+    K.xa_w = K.xa_peri = K.xa_ecc = K.xa_root = K.xa_p0x = K.xa_p0y = K.xa_vx = K.xa_vy = 0.0;
+    K.xa_pin = K.xa_pie = 0.0;
+    if (d.xallarap_type == kXaAngular && (d.xi_1 != 0.0 || d.xi_2 != 0.0)) {
+        K.xallarap = kXaAngular;
+        K.xa_sinc = std::sin(d.inc_xa);
+        K.xa_phi = d.phi_xa;
+        K.xa_cphi = std::cos(d.phi_xa);
+        K.xa_sphi = std::sin(d.phi_xa);
+        K.xa_omega = d.omega_xa;
+        K.xa_xi1 = d.xi_1;
+        K.xa_xi2 = d.xi_2;
+    } else if (d.xallarap_type == kXaKepler && (d.piEN_xa != 0.0 || d.piEE_xa != 0.0) && d.period_xa > 0.0) {
+        // the orbit at t0 and its velocity there by a forward difference of
+        // period 1e-7 (apply_xallarap_orbital_elements, trajectory.cpp:335-361)
+        const double t0 = val(P.t0);
+        K.xallarap = kXaKepler;
+        K.xa_sinc = std::sin(d.inc_xa);
+        K.xa_w = 2.0 * kPiHost / d.period_xa;
+        K.xa_peri = d.peri_xa;
+        K.xa_ecc = d.ecc_xa;
+        K.xa_root = std::sqrt(std::max(0.0, 1.0 - d.ecc_xa * d.ecc_xa));
+        xallarap_position(t0, t0, d.period_xa, d.ecc_xa, d.peri_xa, &K.xa_p0x, &K.xa_p0y);
+        const double dt_small = d.period_xa * 1.0e-7;
+        double px = 0.0, py = 0.0;
+        xallarap_position(t0 + dt_small, t0, d.period_xa, d.ecc_xa, d.peri_xa, &px, &py);
+        K.xa_vx = (px - K.xa_p0x) / dt_small;
+        K.xa_vy = (py - K.xa_p0y) / dt_small;
+        K.xa_pin = d.piEN_xa;
+        K.xa_pie = d.piEE_xa;
     }
 }
 
@@ -668,12 +732,24 @@ extern "C" lcbg_status lcbg_load_earth_ephemeris(const char* path)
     return g_earth.size() >= 2 ? LCBG_OK : LCBG_INVALID_ARGUMENT;
 }
 
+namespace {
+// Terrestrial parallax or xallarap: effects the host computation leaves to
+// the device trajectory.
+bool device_only_effects(const lcbg_dynamics& d)
+{
+    const bool parallax = d.piEN != 0.0 || d.piEE != 0.0;
+    return (parallax && (d.obs_lat != 0.0 || d.obs_lon != 0.0)) || d.xi_1 != 0.0 || d.xi_2 != 0.0 ||
+           d.piEN_xa != 0.0 || d.piEE_xa != 0.0;
+}
+}  // namespace
+
 extern "C" lcbg_status lcbg_dynamic_sources(const lcbg_curve* curve, const lcbg_dynamics* dyn,
                                             int32_t orbital_motion_mode, const double* times,
                                             int64_t n, double* xys)
 {
     if (!curve || !dyn || (n > 0 && (!times || !xys)) || n < 0) return LCBG_INVALID_ARGUMENT;
-    if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER)
+    if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER ||
+        device_only_effects(*dyn))
         return LCBG_UNSUPPORTED;
     return lcbg::dynamic_sources(*curve, *dyn, orbital_motion_mode, times, n, xys) ? LCBG_OK
                                                                                    : LCBG_NUMERICAL_ERROR;
@@ -684,7 +760,8 @@ extern "C" lcbg_status lcbg_dynamic_sources_jacobian(const lcbg_curve* curve, co
                                                      int64_t n, double* xys, double* jac)
 {
     if (!curve || !dyn || (n > 0 && (!times || !xys || !jac)) || n < 0) return LCBG_INVALID_ARGUMENT;
-    if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER)
+    if (orbital_motion_mode < LCBG_ORBIT_STATIC || orbital_motion_mode > LCBG_ORBIT_KEPLER ||
+        device_only_effects(*dyn))
         return LCBG_UNSUPPORTED;
     return lcbg::dynamic_sources_jacobian(*curve, *dyn, orbital_motion_mode, times, n, xys, jac)
                ? LCBG_OK

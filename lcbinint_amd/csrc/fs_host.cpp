@@ -1096,6 +1096,15 @@ lcbg_status dyn_trajectory(lcbg_batch* batch, const lcbg_dynamics* dyn, int32_t 
 {
     DynState& d = *batch->dyn;
     const int32_t nc = batch->n_curves;
+    // xallarap amplitudes of the form the options do not name are
+    // unsupported (has_unsupported_dynamic_effects, lens_model.cpp:15-26)
+    for (int32_t c = 0; c < nc; ++c) {
+        const lcbg_dynamics& q = dyn[c];
+        if ((q.xallarap_type != 1 && (q.xi_1 != 0.0 || q.xi_2 != 0.0)) ||
+            (q.xallarap_type != 2 && (q.piEN_xa != 0.0 || q.piEE_xa != 0.0)) || q.xallarap_type < 0 ||
+            q.xallarap_type > 2)
+            return LCBG_UNSUPPORTED;
+    }
     lcbg_status st = LCBG_OK;
     bool any_parallax = false;
     for (int32_t c = 0; c < nc; ++c) any_parallax |= dyn[c].piEN != 0.0 || dyn[c].piEE != 0.0;

@@ -8,9 +8,9 @@
 // LightCurve :3114-3181, dict keys :145-179).  The light curve is evaluated by
 // the HIP library with the GIL released: the static binary lens through
 // lcbg_light_curve, annual parallax and circular / Keplerian orbital motion
-// through lcbg_batch_light_curves_dynamic (the LensModel path); other models
-// (triple lens, binary source, terrestrial parallax, xallarap, legacy
-// coordinates) raise as unsupported.
+// through lcbg_batch_light_curves_dynamic (the LensModel path), with terrestrial
+// parallax and xallarap there too; a binary source blends two such curves; the
+// triple lens and the legacy coordinates raise as unsupported.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -188,13 +188,13 @@ public:
             throw std::invalid_argument("source must be 'single' or 'binary'");
     }
 
-    // the binary lens in VBM coordinates, static or with annual parallax and
-    // orbital motion; other models are outside this library
+    // the binary lens in VBM coordinates (static, annual and terrestrial
+    // parallax, orbital motion, xallarap; single or binary source); triple
+    // lenses and the legacy frame are outside this library
     void check_supported(double tE, double s, double q) const
     {
-        const bool supported = lens_ == "binary_lens" && source_ == "single" && !terrestrial_ &&
-                               options_.vbm_compatible != 0 && options_.center_of_mass == 0 &&
-                               options_.xallarap == "none";
+        const bool supported =
+            lens_ == "binary_lens" && options_.vbm_compatible != 0 && options_.center_of_mass == 0;
         if (!supported) throw std::runtime_error("unsupported");
         if (!std::isfinite(tE) || tE == 0.0 || !std::isfinite(q) || !std::isfinite(s))
             throw std::runtime_error("numerical error");
@@ -218,27 +218,148 @@ public:
         return opts;
     }
 
-    // evaluate_binary_light_curve_info (python/lcbinint_pybind.cpp:1176-1290)
-    PyLightCurveInfo info(py::array_t<double, py::array::c_style | py::array::forcecast> times, double t0,
-                          double tE, double u0, double alpha, double s, double q, double rho, double piEN,
-                          double piEE, double g1, double g2, double g3, double lom_szs, double lom_ar) const
+    // The parameters of binary_params_from_dict (python/lcbinint_pybind.cpp:145-179).
+    struct CurveParams {
+        double t0 = 0.0, tE = 1.0, u0 = 0.0, alpha = 0.0, s = 1.0, q = 1.0, rho = 0.0;
+        double piEN = 0.0, piEE = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0, lom_szs = 0.0, lom_ar = 1.0;
+        double u0_2 = 0.0, t0_2 = 0.0, flux_ratio = 0.0;
+        double xi_1 = 0.0, xi_2 = 0.0, omega_xa = 0.0, inc_xa = 0.0, phi_xa = 0.0;
+        double piEN_xa = 0.0, piEE_xa = 0.0, period_xa = 0.0, ecc_xa = 0.0, peri_xa = 0.0;
+    };
+
+    static CurveParams params_from_dict(const py::dict& d)
     {
-        std::vector<double> t = times_of(times);
-        check_supported(tE, s, q);
-        const lcbg_curve cv{t0, tE, u0, alpha, s, q, rho, ld_.c, ld_.d};
+        CurveParams p;
+        const std::pair<const char*, double*> keys[] = {
+            {"t0", &p.t0}, {"tE", &p.tE}, {"u0", &p.u0}, {"alpha", &p.alpha}, {"s", &p.s}, {"q", &p.q},
+            {"rho", &p.rho}, {"piEN", &p.piEN}, {"piEE", &p.piEE}, {"g1", &p.g1}, {"g2", &p.g2}, {"g3", &p.g3},
+            {"lom_szs", &p.lom_szs}, {"lom_ar", &p.lom_ar}, {"u0_2", &p.u0_2}, {"t0_2", &p.t0_2},
+            {"flux_ratio", &p.flux_ratio}, {"xi_1", &p.xi_1}, {"xi_2", &p.xi_2}, {"omega_xa", &p.omega_xa},
+            {"inc_xa", &p.inc_xa}, {"phi_xa", &p.phi_xa}, {"piEN_xa", &p.piEN_xa}, {"piEE_xa", &p.piEE_xa},
+            {"period_xa", &p.period_xa}, {"ecc_xa", &p.ecc_xa}, {"peri_xa", &p.peri_xa}};
+        for (const auto& k : keys) *k.second = dict_get(d, k.first, *k.second);
+        return p;
+    }
+
+    static CurveParams positional(double t0, double tE, double u0, double alpha, double s, double q, double rho,
+                                  double piEN, double piEE, double g1, double g2, double g3, double lom_szs,
+                                  double lom_ar)
+    {
+        CurveParams p;
+        p.t0 = t0;
+        p.tE = tE;
+        p.u0 = u0;
+        p.alpha = alpha;
+        p.s = s;
+        p.q = q;
+        p.rho = rho;
+        p.piEN = piEN;
+        p.piEE = piEE;
+        p.g1 = g1;
+        p.g2 = g2;
+        p.g3 = g3;
+        p.lom_szs = lom_szs;
+        p.lom_ar = lom_ar;
+        return p;
+    }
+
+    // make_c_params_for_single_source (python/lcbinint_pybind.cpp:2720-2748):
+    // parallax only with LightCurve(parallax=True), the observatory only with
+    // terrestrial_parallax=True, xallarap in the options' form
+    lcbg_dynamics dynamics_of(const CurveParams& p) const
+    {
+        lcbg_dynamics d{};
+        d.piEN = parallax_ ? p.piEN : 0.0;
+        d.piEE = parallax_ ? p.piEE : 0.0;
+        d.ra = event_.ra;
+        d.dec = event_.dec;
+        d.tfix = event_.tfix;
+        d.g1 = p.g1;
+        d.g2 = p.g2;
+        d.g3 = p.g3;
+        d.lom_szs = p.lom_szs;
+        d.lom_ar = p.lom_ar;
+        d.obs_lat = terrestrial_ ? event_.obs_lat : 0.0;
+        d.obs_lon = terrestrial_ ? event_.obs_lon : 0.0;
+        d.xallarap_type = options_.xallarap == "angular_velocity" ? 1 : options_.xallarap == "orbital_elements" ? 2 : 0;
+        d.xi_1 = p.xi_1;
+        d.xi_2 = p.xi_2;
+        d.omega_xa = p.omega_xa;
+        d.inc_xa = p.inc_xa;
+        d.phi_xa = p.phi_xa;
+        d.piEN_xa = p.piEN_xa;
+        d.piEE_xa = p.piEE_xa;
+        d.period_xa = p.period_xa;
+        d.ecc_xa = p.ecc_xa;
+        d.peri_xa = p.peri_xa;
+        return d;
+    }
+
+    // direct_static_binary of evaluate_binary_light_curve_numpy
+    // (python/lcbinint_pybind.cpp:1103-1116): otherwise the LensModel path
+    bool model_path(const lcbg_dynamics& d) const
+    {
+        return orbit_ != 0 || d.piEN != 0.0 || d.piEE != 0.0 || d.xi_1 != 0.0 || d.xi_2 != 0.0 ||
+               d.piEN_xa != 0.0 || d.piEE_xa != 0.0;
+    }
+
+    // One source's light curve (the static VBM branch or the LensModel branch).
+    std::vector<double> source_curve(const std::vector<double>& t, const CurveParams& p) const
+    {
+        check_supported(p.tE, p.s, p.q);
+        const lcbg_curve cv{p.t0, p.tE, p.u0, p.alpha, p.s, p.q, p.rho, ld_.c, ld_.d};
         const lcbg_options opts = run_options();
-        const double pN = parallax_ ? piEN : 0.0;
-        const double pE = parallax_ ? piEE : 0.0;
+        const lcbg_dynamics dyn = dynamics_of(p);
+        const int64_t n = static_cast<int64_t>(t.size());
+        std::vector<double> out(t.size());
+        lcbg_status st;
+        {
+            py::gil_scoped_release release;
+            if (!model_path(dyn)) {
+                st = lcbg_light_curve(&cv, &opts, t.data(), n, out.data(), nullptr);
+            } else {
+                std::lock_guard<std::mutex> lock(dyn_->mu);
+                lcbg_batch* b = nullptr;
+                st = dyn_batch(cv, opts, &b);
+                const int64_t off[2] = {0, n};
+                std::vector<int32_t> meth(t.size());
+                if (st == LCBG_OK)
+                    st = lcbg_batch_light_curves_dynamic(b, &dyn, orbit_, off, t.data(), n, out.data(), meth.data());
+                if (st == LCBG_OK)
+                    for (double v : out)
+                        if (!std::isfinite(v)) st = LCBG_NUMERICAL_ERROR;
+            }
+        }
+        raise_status(st);
+        return out;
+    }
+
+    // The second source of a binary source: the same model at (u0_2, t0_2)
+    // (python/lcbinint_pybind.cpp:2331-2353).
+    static CurveParams second_source(const CurveParams& p)
+    {
+        CurveParams p2 = p;
+        p2.u0 = p.u0_2;
+        p2.t0 = p.t0_2;
+        return p2;
+    }
+
+    // evaluate_binary_light_curve_info (python/lcbinint_pybind.cpp:1176-1290)
+    PyLightCurveInfo records(const std::vector<double>& t, const CurveParams& p) const
+    {
+        check_supported(p.tE, p.s, p.q);
+        const lcbg_curve cv{p.t0, p.tE, p.u0, p.alpha, p.s, p.q, p.rho, ld_.c, ld_.d};
+        const lcbg_options opts = run_options();
+        const lcbg_dynamics dyn = dynamics_of(p);
         const int64_t n = static_cast<int64_t>(t.size());
         std::vector<lcbg_result> rec(t.size());
         lcbg_status st;
         {
             py::gil_scoped_release release;
-            if (orbit_ == 0 && pN == 0.0 && pE == 0.0) {
+            if (!model_path(dyn)) {
                 int64_t first = -1;
                 st = lcbg_light_curve_results(&cv, &opts, t.data(), n, rec.data(), &first);
             } else {
-                const lcbg_dynamics dyn{pN, pE, event_.ra, event_.dec, event_.tfix, g1, g2, g3, lom_szs, lom_ar};
                 std::lock_guard<std::mutex> lock(dyn_->mu);
                 lcbg_batch* b = nullptr;
                 st = dyn_batch(cv, opts, &b);
@@ -266,14 +387,41 @@ public:
         return out;
     }
 
-    PyLightCurveInfo info_from_dict(py::array_t<double, py::array::c_style | py::array::forcecast> times,
-                                    const py::dict& p) const
+    PyLightCurveInfo info(py::array_t<double, py::array::c_style | py::array::forcecast> times, double t0,
+                          double tE, double u0, double alpha, double s, double q, double rho, double piEN,
+                          double piEE, double g1, double g2, double g3, double lom_szs, double lom_ar) const
     {
-        return info(times, dict_get(p, "t0", 0.0), dict_get(p, "tE", 1.0), dict_get(p, "u0", 0.0),
-                    dict_get(p, "alpha", 0.0), dict_get(p, "s", 1.0), dict_get(p, "q", 1.0),
-                    dict_get(p, "rho", 0.0), parallax_ ? dict_get(p, "piEN", 0.0) : 0.0,
-                    parallax_ ? dict_get(p, "piEE", 0.0) : 0.0, dict_get(p, "g1", 0.0), dict_get(p, "g2", 0.0),
-                    dict_get(p, "g3", 0.0), dict_get(p, "lom_szs", 0.0), dict_get(p, "lom_ar", 1.0));
+        return records(times_of(times), positional(t0, tE, u0, alpha, s, q, rho, piEN, piEE, g1, g2, g3, lom_szs,
+                                                   lom_ar));
+    }
+
+    // info_from_dict (python/lcbinint_pybind.cpp:2455-2479): a binary source
+    // blends the two magnification columns, the other fields are the first
+    // source's
+    PyLightCurveInfo info_from_dict(py::array_t<double, py::array::c_style | py::array::forcecast> times,
+                                    const py::dict& d) const
+    {
+        const std::vector<double> t = times_of(times);
+        const CurveParams p = params_from_dict(d);
+        PyLightCurveInfo out = records(t, p);
+        if (source_ == "binary") {
+            const std::vector<double> v2 = source_curve(t, second_source(p));
+            const double fr = p.flux_ratio;
+            const double inv = 1.0 / (1.0 + fr);
+            for (std::size_t i = 0; i < out.magnifications.size(); ++i)
+                out.magnifications[i] = (out.magnifications[i] + fr * v2[i]) * inv;
+            for (std::size_t i = 0; i < out.finite_source_magnifications.size(); ++i)
+                out.finite_source_magnifications[i] = (out.finite_source_magnifications[i] + fr * v2[i]) * inv;
+        }
+        return out;
+    }
+
+    static py::array_t<double> as_array(const std::vector<double>& v)
+    {
+        const auto n = static_cast<py::ssize_t>(v.size());
+        auto out = py::array_t<double>({n}, {static_cast<py::ssize_t>(sizeof(double))});
+        std::copy(v.begin(), v.end(), out.mutable_data());
+        return out;
     }
 
     py::array_t<double> evaluate(py::array_t<double, py::array::c_style | py::array::forcecast> times,
@@ -281,51 +429,38 @@ public:
                                  double rho, double piEN, double piEE, double g1, double g2,
                                  double g3, double lom_szs, double lom_ar) const
     {
-        std::vector<double> t = times_of(times);
-        const auto n = static_cast<py::ssize_t>(t.size());
-        auto out = py::array_t<double>({n}, {static_cast<py::ssize_t>(sizeof(double))});
-        check_supported(tE, s, q);
-        lcbg_curve cv{t0, tE, u0, alpha, s, q, rho, ld_.c, ld_.d};
-        const lcbg_options opts = run_options();
-        const double pN = parallax_ ? piEN : 0.0;
-        const double pE = parallax_ ? piEE : 0.0;
-        lcbg_status st;
-        if (orbit_ == 0 && pN == 0.0 && pE == 0.0) {
-            // direct static binary (python/lcbinint_pybind.cpp:1103-1158)
-            py::gil_scoped_release release;
-            st = lcbg_light_curve(&cv, &opts, t.data(), static_cast<int64_t>(n),
-                                  out.mutable_data(), nullptr);
-        } else {
-            // LensModel path (python/lcbinint_pybind.cpp:1159-1173)
-            const lcbg_dynamics dyn{pN, pE, event_.ra, event_.dec, event_.tfix, g1, g2, g3, lom_szs, lom_ar};
-            py::gil_scoped_release release;
-            std::lock_guard<std::mutex> lock(dyn_->mu);
-            lcbg_batch* b = nullptr;
-            st = dyn_batch(cv, opts, &b);
-            const int64_t off[2] = {0, static_cast<int64_t>(n)};
-            std::vector<int32_t> meth(static_cast<std::size_t>(n));
-            if (st == LCBG_OK)
-                st = lcbg_batch_light_curves_dynamic(b, &dyn, orbit_, off, t.data(), static_cast<int64_t>(n),
-                                                     out.mutable_data(), meth.data());
-            if (st == LCBG_OK) {
-                const double* v = out.data();
-                for (py::ssize_t i = 0; i < n; ++i)
-                    if (!std::isfinite(v[i])) st = LCBG_NUMERICAL_ERROR;
-            }
-        }
-        raise_status(st);
-        return out;
+        return as_array(source_curve(times_of(times), positional(t0, tE, u0, alpha, s, q, rho, piEN, piEE, g1, g2,
+                                                                 g3, lom_szs, lom_ar)));
     }
 
+    // light_curve_from_dict (python/lcbinint_pybind.cpp:2325-2356): a binary
+    // source is (A1 + f A2) / (1 + f), computed as (A1 + f A2) (1 / (1 + f))
     py::array_t<double> from_dict(py::array_t<double, py::array::c_style | py::array::forcecast> times,
-                                  const py::dict& p) const
+                                  const py::dict& d) const
     {
-        return evaluate(times, dict_get(p, "t0", 0.0), dict_get(p, "tE", 1.0), dict_get(p, "u0", 0.0),
-                        dict_get(p, "alpha", 0.0), dict_get(p, "s", 1.0), dict_get(p, "q", 1.0),
-                        dict_get(p, "rho", 0.0), parallax_ ? dict_get(p, "piEN", 0.0) : 0.0,
-                        parallax_ ? dict_get(p, "piEE", 0.0) : 0.0, dict_get(p, "g1", 0.0),
-                        dict_get(p, "g2", 0.0), dict_get(p, "g3", 0.0), dict_get(p, "lom_szs", 0.0),
-                        dict_get(p, "lom_ar", 1.0));
+        const std::vector<double> t = times_of(times);
+        const CurveParams p = params_from_dict(d);
+        std::vector<double> v = source_curve(t, p);
+        if (source_ == "binary") {
+            const std::vector<double> v2 = source_curve(t, second_source(p));
+            const double fr = p.flux_ratio;
+            const double inv = 1.0 / (1.0 + fr);
+            for (std::size_t i = 0; i < v.size(); ++i) v[i] = (v[i] + fr * v2[i]) * inv;
+        }
+        return as_array(v);
+    }
+
+    // magnification_from_dict (python/lcbinint_pybind.cpp:2643-2655): one
+    // epoch; a binary source is (A1 + f A2) / (1 + f)
+    double magnification_from_dict(double time, const py::dict& d) const
+    {
+        const std::vector<double> t{time};
+        const CurveParams p = params_from_dict(d);
+        const double a1 = source_curve(t, p)[0];
+        if (source_ != "binary") return a1;
+        const double a2 = source_curve(t, second_source(p))[0];
+        const double fr = p.flux_ratio;
+        return (a1 + fr * a2) / (1.0 + fr);
     }
 
     // source_for_geometry / dynamic_source_trajectory (python/lcbinint_pybind.cpp:1819-1880)
@@ -678,6 +813,7 @@ PYBIND11_MODULE(_lcbinint, m)
              py::arg("piEE") = 0.0, py::arg("g1") = 0.0, py::arg("g2") = 0.0, py::arg("g3") = 0.0,
              py::arg("lom_szs") = 0.0, py::arg("lom_ar") = 1.0)
         .def("light_curve", &PyLightCurve::from_dict, py::arg("times"), py::arg("params"))
+        .def("magnification", &PyLightCurve::magnification_from_dict, py::arg("time"), py::arg("params"))
         .def("info", &PyLightCurve::info_from_dict, py::arg("times"), py::arg("params"))
         .def("info", &PyLightCurve::info, py::arg("times"), py::kw_only(), py::arg("t0") = 0.0,
              py::arg("tE") = 1.0, py::arg("u0") = 0.0, py::arg("alpha") = 0.0, py::arg("s") = 1.0,

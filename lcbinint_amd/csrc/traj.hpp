@@ -173,7 +173,21 @@ struct TrajConst {
     T sep, angle, t_ref;
     T w_orb, phi0, c_inc, s_true, c_om, s_om;            // circular
     T n, ecc, t_peri, a, one_minus_e2, xa[3], ya[3];     // Keplerian
+    // terrestrial parallax (apply_terrestrial_parallax, trajectory.cpp:363-401):
+    // the observatory's longitude (rad) and R cos(lat), R sin(lat) (AU)
+    int terrestrial;
+    double obs_lon, obs_rc, obs_rs;
+    // xallarap (trajectory.cpp:287-361): kXaAngular or kXaKepler, or 0
+    int xallarap;
+    double xa_sinc;                                       // sin(inc_xa)
+    double xa_phi, xa_cphi, xa_sphi, xa_omega, xa_xi1, xa_xi2;   // angular velocity
+    double xa_w, xa_peri, xa_ecc, xa_root, xa_p0x, xa_p0y, xa_vx, xa_vy, xa_pin, xa_pie;  // orbital elements
 };
+
+enum : int { kXaNone = 0, kXaAngular = 1, kXaKepler = 2 };
+constexpr double kTrajEarthRadiusAu = 4.2635212e-5;
+constexpr double kTrajSiderealDegPerDay = 360.98564736629;
+constexpr double kTrajDegree = 3.14159265358979323846 / 180.0;
 
 // interpolate_earth_state (trajectory.cpp:185-209): rows sorted by time;
 // NaN outside the table.
@@ -226,6 +240,60 @@ __device__ inline T traj_kepler_e(const T& m, const T& e)
     return E;
 }
 
+// solve_kepler of the xallarap orbit (trajectory.cpp:303-316): E from M.
+template <class T>
+__device__ inline T traj_kepler_xa(const T& m, double e)
+{
+    T E = m + e * t_sin(m);
+    for (int i = 0; i < 10; ++i) {
+        const T f = E - e * t_sin(E) - m;
+        const T fp = 1.0 - e * t_cos(E);
+        const T d = f / fp;
+        E = E - d;
+        if (fabs(tval(d)) < 1.0e-13) break;
+    }
+    return E;
+}
+
+// The terrestrial parallax and xallarap terms added to (tn, beta) after the
+// annual parallax, in the reference's order (Trajectory::source_position,
+// trajectory.cpp:436-448); the per-epoch cos and sin come from the device libm.
+template <class T>
+__device__ inline void traj_extra(const TrajConst<T>& K, double time, T& tn, T& beta)
+{
+    if (K.terrestrial) {
+        const double jd = time + (time < 2450000.0 ? 2450000.0 : 0.0);
+        const double gmst = (280.46061837 + kTrajSiderealDegPerDay * (jd - 2451545.0)) * kTrajDegree;
+        const double ha = gmst + K.obs_lon;
+        const double r[3] = {K.obs_rc * cos(ha), K.obs_rc * sin(ha), K.obs_rs};
+        const double pn = -(r[0] * K.north[0] + r[1] * K.north[1] + r[2] * K.north[2]);
+        const double pe = -(r[0] * K.east[0] + r[1] * K.east[1] + r[2] * K.east[2]);
+        tn = tn + (K.piEN * pn + K.piEE * pe);
+        beta = beta + (-K.piEE * pn + K.piEN * pe);
+    }
+    if (K.xallarap == kXaAngular) {
+        const T phit = K.xa_omega * (time - K.t0);
+        const T arg = K.xa_phi + phit;
+        const T disp0 = K.xa_sinc * ((-K.xa_cphi + t_cos(arg)) + phit * K.xa_sphi);
+        const T disp1 = (-phit * K.xa_cphi - K.xa_sphi) + t_sin(arg);
+        tn = tn + (K.xa_xi1 * disp0 + K.xa_xi2 * disp1);
+        beta = beta + (K.xa_xi2 * disp0 - K.xa_xi1 * disp1);
+    } else if (K.xallarap == kXaKepler) {
+        const T elapsed = time - K.t0;
+        const T E = traj_kepler_xa(K.xa_w * elapsed + K.xa_peri, K.xa_ecc);
+        const T ce = t_cos(E);
+        const T se = t_sin(E);
+        const T r = 1.0 - K.xa_ecc * ce;
+        const T cf = (ce - K.xa_ecc) / r;
+        const T sf = K.xa_root * se / r;
+        const T dev_x = (r * cf - K.xa_p0x) - K.xa_vx * elapsed;
+        const T dev_y = (r * sf - K.xa_p0y) - K.xa_vy * elapsed;
+        const T disp0 = K.xa_sinc * dev_x;
+        tn = tn + (K.xa_pin * disp0 + K.xa_pie * dev_y);
+        beta = beta + (K.xa_pie * disp0 - K.xa_pin * dev_y);
+    }
+}
+
 // One epoch: the source position (x, y) in the instantaneous binary frame and
 // the separation s.  Returns false if the orbit is not finite
 // (lens_model.cpp:117-120).
@@ -255,6 +323,7 @@ __device__ inline bool traj_epoch(const TrajConst<T>& K, const EarthRowD* rows, 
         tn = tn + dtau;
         beta = beta + dbeta;
     }
+    traj_extra(K, time, tn, beta);
     T x = tn * K.cos_t - beta * K.sin_t;
     T y = tn * K.sin_t + beta * K.cos_t;
     T s_t, ang;

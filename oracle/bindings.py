@@ -41,6 +41,35 @@ def default_settings(**kw):
     return s
 
 
+class LcbiParams(ctypes.Structure):
+    """lcbi_params of the reference's C API (include/lcbinint/lcbinint.h:20-77)."""
+    _fields_ = ([(k, D) for k in ("t0", "tE", "umin", "q", "sep", "theta", "rho", "omega", "piEN", "piEE",
+                                  "piEN_xa", "piEE_xa", "ra_xa", "dec_xa", "period_xa", "ecc_xa", "peri_xa",
+                                  "v_sep", "q2", "sep2", "ang", "ra", "dec", "earth_axis", "tfix", "obs_lat",
+                                  "obs_lon", "limb_darkening_c", "limb_darkening_d")] +
+                [("orbital_motion_mode", ctypes.c_int)] +
+                [(k, D) for k in ("g1", "g2", "g3", "lom_szs", "lom_ar", "xi_1", "xi_2", "omega_xa", "inc_xa",
+                                  "phi_xa")])
+
+
+class LcbiOptions(ctypes.Structure):
+    """lcbi_options of the reference's C API (include/lcbinint/lcbinint.h:85-104)."""
+    _fields_ = ([(k, ctypes.c_int) for k in ("parallax_mode", "orbit_pair", "center_of_mass", "caustic_bins",
+                                             "source_bins", "mode", "vbm_compatible", "xallarap_param_type")] +
+                [("grid_ratio", D), ("polar_source_bins", ctypes.c_int), ("polar_grid_ratio", D),
+                 ("point_source_threshold", D), ("hexadecapole_threshold", D), ("adaptive_hex_threshold", D),
+                 ("adaptive_source_bins", ctypes.c_int), ("max_source_bins", ctypes.c_int),
+                 ("finite_source_tol", D), ("finite_source_reltol", D)])
+
+
+class LcbiResult(ctypes.Structure):
+    """lcbi_result of the reference's C API (include/lcbinint/lcbinint.h:106-116)."""
+    _fields_ = [("magnification", D), ("point_source_magnification", D), ("finite_source_magnification", D),
+                ("source_x", D), ("source_y", D), ("image_count", ctypes.c_int),
+                ("finite_source_error_estimate", D), ("finite_source_refinement_level", ctypes.c_int),
+                ("finite_source_converged", ctypes.c_int)]
+
+
 class Oracle:
     def __init__(self, path):
         L = ctypes.CDLL(str(path))
@@ -226,6 +255,16 @@ class Ref:
     DYNAMIC_KEYS = ("t0", "tE", "u0", "alpha", "s", "q", "rho", "piEN", "piEE", "ra", "dec", "tfix",
                     "g1", "g2", "g3", "lom_szs", "lom_ar")
 
+    def lens_model_at(self, settings, s, q, x, y, rho):
+        """(magnification, method) of the LensModel path's finite-source step at
+        source (x, y) in the binary frame and separation s."""
+        fn = self.lib.ref_lens_model_at
+        fn.restype = D
+        fn.argtypes = [P(RefSettings), D, D, D, D, D, P(I32)]
+        m = I32()
+        a = fn(ctypes.byref(settings), s, q, x, y, rho, ctypes.byref(m))
+        return a, m.value
+
     def light_curve_dynamic(self, settings, times, params, orbit_mode=0):
         """LensModel path (parallax and/or orbital motion); params: dict with
         DYNAMIC_KEYS (missing keys: 0, lom_ar 1).  Returns (mag, method,
@@ -291,6 +330,44 @@ class Ref:
         m = I32()
         a = self._ir(ctypes.byref(settings), s, q, x, y, rho, ctypes.byref(m))
         return a, m.value
+
+
+class RefCApi:
+    """The reference's own C API (src/lcbinint/lcbinint.cpp): lcbi_magnification_array
+    runs LensModel::magnification over the epochs, the LensModel branch of the
+    Python loop (python/lcbinint_pybind.cpp:1159-1173)."""
+
+    def __init__(self, path):
+        L = ctypes.CDLL(str(path))
+        self._dp = L.lcbi_default_params
+        self._dp.restype = LcbiParams
+        self._do = L.lcbi_default_options
+        self._do.restype = LcbiOptions
+        self._arr = _f(L, "lcbi_magnification_array", ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                       ctypes.POINTER(LcbiParams), ctypes.POINTER(LcbiOptions), ctypes.c_void_p)
+
+    def params(self, **kw):
+        p = self._dp()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    def options(self, **kw):
+        o = self._do()
+        o.vbm_compatible = 1
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def curve(self, times, params, options):
+        """Records of every epoch (dict of arrays of the lcbi_result fields) and
+        the status (0, or that of the first failing epoch, where it stops)."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        res = (LcbiResult * len(t))()
+        st = self._arr(t.ctypes.data_as(ctypes.c_void_p), len(t), ctypes.byref(params), ctypes.byref(options),
+                       ctypes.cast(res, ctypes.c_void_p))
+        out = {name: np.array([getattr(r, name) for r in res]) for name, _ in LcbiResult._fields_}
+        return out, st
 
 
 def load_oracle(path):

@@ -508,6 +508,30 @@ int ref_light_curve_dynamic(const RefSettings* rs, const double* times, long n, 
     return 0;
 }
 
+// LensModel::magnification's binary finite-source step (lens_model.cpp:186-234,
+// as ref_light_curve_dynamic runs it) at a given source position (x, y) in the
+// binary frame and separation s: the sum over binary_images and the
+// dispatcher with those images as seed hints.  Returns the magnification and
+// the method.
+double ref_lens_model_at(const RefSettings* rs, double s, double q, double x, double y, double rho, int* method)
+{
+    PointSourceMagnifier pm;
+    FiniteSourceMagnifier fm(to_settings(*rs));
+    const double effective_q = q != 0.0 ? 1.0 / q : q;
+    const SourcePosition sm {x, y};
+    const auto images = pm.binary_images(s, effective_q, sm);
+    double a_ps = 0.0;
+    std::vector<SourcePosition> seeds;
+    seeds.reserve(images.size());
+    for (const auto& image : images) {
+        a_ps += 1.0 / std::abs(image.jacobian_determinant);
+        seeds.push_back(image.position);
+    }
+    const auto fs = fm.binary_mag(s, effective_q, sm, std::abs(rho), a_ps, &seeds, true);
+    if (method) *method = static_cast<int>(fs.decision.method);
+    return fs.magnification;
+}
+
 // The geometry helpers' source_for_geometry (python/lcbinint_pybind.cpp:1840-1880),
 // VBM frame, over the core's Trajectory and orbital_state (the binding's orbit
 // cache, :803-925, evaluates orbital_state's expressions).  p as in

@@ -34,9 +34,9 @@ def test_point_source_curve_and_errors(gpu, ref):
     assert np.array_equal(got, exp)
     with pytest.raises(ValueError):
         lc(np.zeros((3, 3)), {})
-    # a binary source is outside this library (orbital motion is inside it)
+    # a triple lens is outside this library (binary sources and orbital motion are inside it)
     with pytest.raises(RuntimeError, match="unsupported"):
-        lcbinint.LightCurve(source="binary")(t, {"u0": 0.1})
+        lcbinint.LightCurve(lens="triple_lens")(t, {"u0": 0.1, "q2": 0.1, "sep2": 1.0, "ang": 0.5})
     with pytest.raises(RuntimeError, match="numerical error"):
         lc(t, {"tE": 0.0})
 
