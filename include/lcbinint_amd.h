@@ -214,6 +214,43 @@ typedef struct lcbg_dynamics {
     double peri_xa;
 } lcbg_dynamics;
 
+/* A triple-lens light curve in VBM coordinates: the trajectory of t0, tE, u0,
+ * alpha and the geometry of make_triple_lens_geometry_vbm
+ * (src/lcbinint/model/triple_lens_geometry.cpp:36-50): separation s and mass
+ * ratio q of the first pair, the third lens at distance sep2 from the first at
+ * angle ang, mass ratio q2 (> 0) to the first lens (lcbi_params q, sep, q2,
+ * sep2, ang; LightCurve(lens="triple_lens")). */
+typedef struct lcbg_triple {
+    double t0;
+    double tE;
+    double u0;
+    double alpha;
+    double s;
+    double q;
+    double q2;
+    double sep2;
+    double ang;
+} lcbg_triple;
+
+/* Triple-lens point-source light curves on the device: PointSourceMagnifier::
+ * triple_mag0 (src/lcbinint/magnification/point_source_magnifier.cpp:861-882,
+ * 929-994) on the static trajectory of LensModel (src/lcbinint/model/
+ * lens_model.cpp:90-99, 126-140), one thread per epoch: the degree-10 lens
+ * polynomial, its Skowron-Gould roots, 2D Newton polish, physical selection by
+ * residual.  curves is a host array of n_curves records; offsets (n_curves + 1),
+ * times, mag and image_count (may be NULL) are device arrays; synchronous on
+ * `stream`.  LCBG_UNSUPPORTED if a polynomial has fewer than three effective
+ * degrees (its epochs are NaN). */
+lcbg_status lcbg_triple_point_source_device(const lcbg_triple* curves, int32_t n_curves,
+                                            const int64_t* offsets, const double* times, double* mag,
+                                            int32_t* image_count, void* stream);
+
+/* Host-pointer form for one curve (copies in, runs, copies out): the drop-in
+ * path behind LightCurve(lens="triple_lens")(times, params) with rho == 0.
+ * LCBG_NUMERICAL_ERROR if a magnification is not finite. */
+lcbg_status lcbg_triple_light_curve_point_source(const lcbg_triple* curve, const double* times, int64_t n,
+                                                 double* mag, int32_t* image_count);
+
 /* Orbital motion models, the values of lcbi_orbital_motion_mode
  * (include/lcbinint/lcbinint.h:15-19). */
 enum { LCBG_ORBIT_STATIC = 0, LCBG_ORBIT_CIRCULAR = 1, LCBG_ORBIT_KEPLER = 2 };

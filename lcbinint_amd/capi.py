@@ -45,6 +45,12 @@ class Curve(ctypes.Structure):
                 ("t0", "tE", "u0", "alpha", "s", "q", "rho", "ld_c", "ld_d")]
 
 
+class Triple(ctypes.Structure):
+    """lcbg_triple: a triple-lens light curve (VBM geometry)."""
+    _fields_ = [(name, ctypes.c_double) for name in
+                ("t0", "tE", "u0", "alpha", "s", "q", "q2", "sep2", "ang")]
+
+
 class Options(ctypes.Structure):
     """lcbg_options: the finite-source options (plain method, retry, adaptive mode)."""
     _fields_ = [
@@ -472,6 +478,24 @@ def light_curve_results(curve: Curve, times, options: Options | None = None):
     st = lcbg_light_curve_results(ctypes.byref(curve), ctypes.byref(options) if options is not None else None,
                                   t.ctypes.data_as(c_double_p), len(t), out.ctypes.data, ctypes.byref(first))
     return out, st, first.value
+
+
+lcbg_triple_light_curve_point_source = _sig(
+    "lcbg_triple_light_curve_point_source", ctypes.c_int, ctypes.POINTER(Triple), ctypes.c_void_p,
+    ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p)
+lcbg_triple_point_source_device = _sig(
+    "lcbg_triple_point_source_device", ctypes.c_int, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+
+
+def triple_light_curve_point_source(curve: Triple, times):
+    """(magnifications, image counts, status) of one triple-lens curve, host arrays."""
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    mag = np.empty(len(t))
+    cnt = np.empty(len(t), dtype=np.int32)
+    st = lcbg_triple_light_curve_point_source(ctypes.byref(curve), t.ctypes.data, len(t), mag.ctypes.data,
+                                              cnt.ctypes.data)
+    return mag, cnt, st
 
 
 lcbg_batch_light_curves_dynamic_results = _sig(

@@ -193,8 +193,7 @@ public:
     // lenses and the legacy frame are outside this library
     void check_supported(double tE, double s, double q) const
     {
-        const bool supported =
-            lens_ == "binary_lens" && options_.vbm_compatible != 0 && options_.center_of_mass == 0;
+        const bool supported = options_.vbm_compatible != 0 && options_.center_of_mass == 0;
         if (!supported) throw std::runtime_error("unsupported");
         if (!std::isfinite(tE) || tE == 0.0 || !std::isfinite(q) || !std::isfinite(s))
             throw std::runtime_error("numerical error");
@@ -225,6 +224,7 @@ public:
         double u0_2 = 0.0, t0_2 = 0.0, flux_ratio = 0.0;
         double xi_1 = 0.0, xi_2 = 0.0, omega_xa = 0.0, inc_xa = 0.0, phi_xa = 0.0;
         double piEN_xa = 0.0, piEE_xa = 0.0, period_xa = 0.0, ecc_xa = 0.0, peri_xa = 0.0;
+        double q2 = 0.0, sep2 = 0.0, ang = 0.0;
     };
 
     static CurveParams params_from_dict(const py::dict& d)
@@ -236,7 +236,8 @@ public:
             {"lom_szs", &p.lom_szs}, {"lom_ar", &p.lom_ar}, {"u0_2", &p.u0_2}, {"t0_2", &p.t0_2},
             {"flux_ratio", &p.flux_ratio}, {"xi_1", &p.xi_1}, {"xi_2", &p.xi_2}, {"omega_xa", &p.omega_xa},
             {"inc_xa", &p.inc_xa}, {"phi_xa", &p.phi_xa}, {"piEN_xa", &p.piEN_xa}, {"piEE_xa", &p.piEE_xa},
-            {"period_xa", &p.period_xa}, {"ecc_xa", &p.ecc_xa}, {"peri_xa", &p.peri_xa}};
+            {"period_xa", &p.period_xa}, {"ecc_xa", &p.ecc_xa}, {"peri_xa", &p.peri_xa}, {"q2", &p.q2},
+            {"sep2", &p.sep2}, {"ang", &p.ang}};
         for (const auto& k : keys) *k.second = dict_get(d, k.first, *k.second);
         return p;
     }
@@ -303,9 +304,42 @@ public:
                d.piEN_xa != 0.0 || d.piEE_xa != 0.0;
     }
 
+    // The triple lens (make_c_params_for_single_source :2722-2731 and
+    // LensModel's triple branch, lens_model.cpp:123-150): the point-source
+    // magnification on the static trajectory; a finite source, parallax,
+    // xallarap or orbital motion with three lenses is outside this library.
+    void check_triple(const CurveParams& p) const
+    {
+        if (!(p.q2 > 0.0)) throw py::value_error("triple_lens requires q2 > 0");
+        const lcbg_dynamics d = dynamics_of(p);
+        if (parallax_ || orbit_ != 0 || p.rho != 0.0 || d.xi_1 != 0.0 || d.xi_2 != 0.0 || d.piEN_xa != 0.0 ||
+            d.piEE_xa != 0.0)
+            throw std::runtime_error("unsupported");
+    }
+
+    std::vector<double> triple_curve(const std::vector<double>& t, const CurveParams& p,
+                                     std::vector<int32_t>* counts = nullptr) const
+    {
+        check_supported(p.tE, p.s, p.q);
+        check_triple(p);
+        const lcbg_triple tr{p.t0, p.tE, p.u0, p.alpha, p.s, p.q, p.q2, p.sep2, p.ang};
+        std::vector<double> out(t.size());
+        std::vector<int32_t> cnt(t.size());
+        lcbg_status st;
+        {
+            py::gil_scoped_release release;
+            st = lcbg_triple_light_curve_point_source(&tr, t.data(), static_cast<int64_t>(t.size()), out.data(),
+                                                      cnt.data());
+        }
+        raise_status(st);
+        if (counts) *counts = cnt;
+        return out;
+    }
+
     // One source's light curve (the static VBM branch or the LensModel branch).
     std::vector<double> source_curve(const std::vector<double>& t, const CurveParams& p) const
     {
+        if (lens_ == "triple_lens") return triple_curve(t, p);
         check_supported(p.tE, p.s, p.q);
         const lcbg_curve cv{p.t0, p.tE, p.u0, p.alpha, p.s, p.q, p.rho, ld_.c, ld_.d};
         const lcbg_options opts = run_options();
@@ -347,6 +381,30 @@ public:
     // evaluate_binary_light_curve_info (python/lcbinint_pybind.cpp:1176-1290)
     PyLightCurveInfo records(const std::vector<double>& t, const CurveParams& p) const
     {
+        if (lens_ == "triple_lens") {
+            // point-source records: the magnification, the image count and
+            // the source position of the static trajectory
+            std::vector<int32_t> cnt;
+            const std::vector<double> a = triple_curve(t, p, &cnt);
+            const double ca = std::cos(p.alpha), sa = std::sin(p.alpha);
+            PyLightCurveInfo out;
+            out.times = t;
+            for (std::size_t i = 0; i < t.size(); ++i) {
+                const double tn = (t[i] - p.t0) / p.tE;
+                out.magnifications.push_back(a[i]);
+                out.point_source_magnifications.push_back(a[i]);
+                out.finite_source_magnifications.push_back(a[i]);
+                out.finite_source_error_estimates.push_back(0.0);
+                out.source_x.push_back(tn * ca - p.u0 * sa);
+                out.source_y.push_back(tn * sa + p.u0 * ca);
+                out.image_counts.push_back(cnt[i]);
+                out.finite_source_methods.push_back(LCBG_METHOD_POINT_SOURCE);
+                out.finite_source_method_names.push_back(method_name(LCBG_METHOD_POINT_SOURCE));
+                out.finite_source_refinement_levels.push_back(0);
+                out.finite_source_converged.push_back(true);
+            }
+            return out;
+        }
         check_supported(p.tE, p.s, p.q);
         const lcbg_curve cv{p.t0, p.tE, p.u0, p.alpha, p.s, p.q, p.rho, ld_.c, ld_.d};
         const lcbg_options opts = run_options();

@@ -34,9 +34,9 @@ def test_point_source_curve_and_errors(gpu, ref):
     assert np.array_equal(got, exp)
     with pytest.raises(ValueError):
         lc(np.zeros((3, 3)), {})
-    # a triple lens is outside this library (binary sources and orbital motion are inside it)
+    # a finite source on a triple lens is outside this library (its point source is inside it)
     with pytest.raises(RuntimeError, match="unsupported"):
-        lcbinint.LightCurve(lens="triple_lens")(t, {"u0": 0.1, "q2": 0.1, "sep2": 1.0, "ang": 0.5})
+        lcbinint.LightCurve(lens="triple_lens")(t, {"u0": 0.1, "q2": 0.1, "sep2": 1.0, "ang": 0.5, "rho": 1e-3})
     with pytest.raises(RuntimeError, match="numerical error"):
         lc(t, {"tE": 0.0})
 
