@@ -696,13 +696,142 @@ __device__ __forceinline__ int wave_exclusive_scan(int v, int* total)
     return x - v;
 }
 
+// The weight sums of chunk c (cells 64c .. 64c + 63 of the pool, all of one
+// item) into W.partial: each lane finds its cell's run by binary search over
+// the item's runs and evaluates aa_weight.  Chunks of an item that failed after
+// its chunks were counted are skipped.
+__device__ void grad_chunk(const CurveRec* __restrict__ curves, const FsOptions& opt, const FsEpochs& ep,
+                           const IrLists& Lst, const GradWork& W, int c)
+{
+    const int item = uni(W.chunk_item[c]);
+    const int run_off = uni(W.run_off[item]);
+    if (run_off < 0) return;
+    const int run_n = uni(W.run_n[item]);
+    const int cells = uni(W.cells[item]);
+    const int t = (c - uni(W.chunk_off[item])) * kWave + lane_id();
+    const int64_t e = Lst.ir_sorted[item];
+    const CurveRec cv = curves[ep.curve_of[e]];
+    GradCtx G = grad_ctx(cv, ep.src[e], opt);
+    Dual w = dconst(0.0);
+    if (t < cells) {
+        // the last run whose first cell index is <= t
+        int lo = 0, hi = run_n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (W.run_cells[run_off + mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int* run = W.runs + 4 * static_cast<size_t>(run_off + lo);
+        const int i = run[1] + (t - W.run_cells[run_off + lo]);
+        w = aa_weight(G, i, run[0]);
+    }
+    double sums[kGradSlots];
+    sums[0] = wave_sum(w.v);
+#pragma unroll
+    for (int k = 0; k < kTangents; ++k) sums[1 + k] = wave_sum(w.d[k]);
+#pragma unroll
+    for (int k = 0; k < kGradSlots; ++k) wave_store(&W.partial[static_cast<size_t>(c) * kGradSlots + k], sums[k]);
+}
+
+__device__ __forceinline__ unsigned agent_load(const unsigned* p)
+{
+    return uni(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__device__ __forceinline__ int agent_load(const int* p)
+{
+    return uni(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Bounded waits of the task consumers, in wall-clock ticks (100 MHz): for a
+// reserved task to be published (its producer writes it right after the
+// reservation) and for new tasks while items are still being enumerated.
+constexpr long long kTaskPublishWait = 10000000;  // 0.1 s
+constexpr long long kTaskIdleWait = 200000000;    // 2 s
+
+// The enumeration publishes an item's chunks as tasks of kGradTaskChunks
+// chunks once its runs are in W (the item's data, a release fence, then the
+// ready flags), and counts the item as done afterwards.
+__device__ void publish_tasks(const FsEpochs& ep, const GradWork& W, int chunk_base, int n_chunks)
+{
+    if (n_chunks > 0) {
+        const int n_tasks = (n_chunks + kGradTaskChunks - 1) / kGradTaskChunks;
+        const int t0 = wave_fetch_add(&ep.counters[kQGradTasks], static_cast<unsigned>(n_tasks));
+        const long long cap = static_cast<long long>(W.task_cap);
+        const int lim = static_cast<int>(t0 >= cap ? 0 : (t0 + n_tasks <= cap ? n_tasks : cap - t0));
+        for (int k = lane_id(); k < lim; k += kWave) {
+            const int first = k * kGradTaskChunks;
+            W.tasks[2 * static_cast<size_t>(t0 + k)] = chunk_base + first;
+            W.tasks[2 * static_cast<size_t>(t0 + k) + 1] =
+                n_chunks - first < kGradTaskChunks ? n_chunks - first : kGradTaskChunks;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        for (int k = lane_id(); k < lim; k += kWave)
+            __hip_atomic_store(&W.task_ready[t0 + k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const unsigned inc = lane_id() == 0 ? 1u : 0u;
+    __hip_atomic_fetch_add(&ep.counters[kQGradItemsDone], inc, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    wave_sync();
+}
+
+// Once the item queue is empty, a wave of the enumeration evaluates published
+// tasks, flagging each evaluated chunk in W.chunk_done, until every item has
+// been enumerated.  A wave that waits past a bound leaves too; the chunks no
+// wave has evaluated are k_grad_cells' (which evaluates every unflagged chunk
+// the same way), so the results do not depend on the timing.
+__device__ void take_tasks(const CurveRec* __restrict__ curves, const FsOptions& opt, const FsEpochs& ep,
+                           const IrLists& Lst, const GradWork& W, int count)
+{
+    unsigned* cursor = &ep.counters[kQGradTaskCursor];
+    const int n_chunks = uni(static_cast<int>(W.chunk_cap));
+    int t = wave_fetch_inc(cursor);
+    long long idle_from = static_cast<long long>(wall_clock64());
+    for (;;) {
+        const unsigned reserved = agent_load(&ep.counters[kQGradTasks]);
+        if (static_cast<unsigned>(t) < reserved) {
+            if (static_cast<unsigned>(t) >= W.task_cap) return;
+            const long long w0 = static_cast<long long>(wall_clock64());
+            bool ready = agent_load(&W.task_ready[t]) != 0;
+            while (!ready && static_cast<long long>(wall_clock64()) - w0 < kTaskPublishWait) {
+                __builtin_amdgcn_s_sleep(2);
+                ready = agent_load(&W.task_ready[t]) != 0;
+            }
+            if (!ready) return;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const int c0 = uni(W.tasks[2 * static_cast<size_t>(t)]);
+            const int nc = uni(W.tasks[2 * static_cast<size_t>(t) + 1]);
+            for (int c = c0; c < c0 + nc && c >= 0 && c < n_chunks; ++c) {
+                if (uni(W.chunk_item[c]) >= count) break;
+                grad_chunk(curves, opt, ep, Lst, W, c);
+                wave_store(&W.chunk_done[c], uint8_t{1});
+            }
+            wave_sync();
+            if (static_cast<int>(agent_load(&ep.counters[kQGradItemsDone])) >= count) return;
+            t = wave_fetch_inc(cursor);
+            idle_from = static_cast<long long>(wall_clock64());
+            continue;
+        }
+        // once every item is enumerated, the tasks left go to k_grad_cells,
+        // which evaluates them at a higher occupancy
+        const unsigned done = uni(__hip_atomic_load(&ep.counters[kQGradItemsDone], __ATOMIC_ACQUIRE,
+                                                    __HIP_MEMORY_SCOPE_AGENT));
+        if (static_cast<int>(done) >= count) return;
+        if (static_cast<long long>(wall_clock64()) - idle_from > kTaskIdleWait) return;
+        __builtin_amdgcn_s_sleep(8);
+    }
+}
+
 // Cell enumeration of every inverse-ray item, one wave per item, persistent
-// waves over the items in cost order; the runs go to the pools of W.
+// waves over the items in cost order; the runs go to the pools of W.  In the
+// first pass a wave that finds the item queue empty takes chunk tasks while
+// the other waves still enumerate (the enumeration's tail is a few large
+// items, which leave most of the device idle).
 __global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ curves,
                                                     FsOptions opt, FsEpochs ep, IrLists Lst,
                                                     PolarScratch scr, int pass, GradWork W)
 {
     const bool retry = pass == kPassRetry;
+    const bool tasks = !retry && W.task_cap > 0;
     AaFill F;
     F.head = scr.phi_head + static_cast<size_t>(blockIdx.x) * kPhiHash;
     F.runs = scr.runs + static_cast<size_t>(blockIdx.x) * scr.run_cap * 4;
@@ -741,6 +870,7 @@ __global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ 
             const int pos = wave_fetch_inc(&ep.counters[kQGradRetryCount]);
             wave_store(&Lst.grad_retry[pos], item);
             wave_sync();
+            if (tasks) publish_tasks(ep, W, 0, 0);
             continue;
         }
         if (F.overflow) atomicOr(&ep.counters[kQFlags], cap ? kErrPolarCap : kErrBudget);
@@ -797,46 +927,36 @@ __global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ 
         for (int k = 0; k < kTangents; ++k)
             wave_store(&W.points[static_cast<size_t>(item) * kGradSlots + 1 + k], pts.d[k]);
         wave_sync();
+        if (tasks) publish_tasks(ep, W, chunk_base, failed ? 0 : n_chunks);
     }
+    if (tasks) take_tasks(curves, opt, ep, Lst, W, count);
 }
 
-// Weights of every chunk of 64 enumerated cells, one wave per chunk
-// (persistent waves): each lane finds its cell's run by binary search over the
-// item's runs, evaluates aa_weight, and the wave's sums go to W.partial.
+// Weights of every chunk of 64 enumerated cells not evaluated by a task of the
+// enumeration, one wave per chunk, the waves taking 64 consecutive chunks at a
+// time; the chunk flags and the task flags are cleared for the next run.
 __global__ void __launch_bounds__(kWave) k_grad_cells(const CurveRec* __restrict__ curves, FsOptions opt,
                                                        FsEpochs ep, IrLists Lst, GradWork W)
 {
     const unsigned total = ep.counters[kQGradChunks];
     const int n_chunks = uni(static_cast<int>(total < W.chunk_cap ? total : W.chunk_cap));
-    for (int c = static_cast<int>(blockIdx.x); c < n_chunks; c += static_cast<int>(gridDim.x)) {
-        const int item = uni(W.chunk_item[c]);
-        const int run_off = uni(W.run_off[item]);
-        if (run_off < 0) continue;  // the item failed after its chunks were counted
-        const int run_n = uni(W.run_n[item]);
-        const int cells = uni(W.cells[item]);
-        const int t = (c - uni(W.chunk_off[item])) * kWave + lane_id();
-        const int64_t e = Lst.ir_sorted[item];
-        const CurveRec cv = curves[ep.curve_of[e]];
-        GradCtx G = grad_ctx(cv, ep.src[e], opt);
-        Dual w = dconst(0.0);
-        if (t < cells) {
-            // the last run whose first cell index is <= t
-            int lo = 0, hi = run_n - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (W.run_cells[run_off + mid] <= t) lo = mid;
-                else hi = mid - 1;
-            }
-            const int* run = W.runs + 4 * static_cast<size_t>(run_off + lo);
-            const int i = run[1] + (t - W.run_cells[run_off + lo]);
-            w = aa_weight(G, i, run[0]);
+    const unsigned reserved = ep.counters[kQGradTasks];
+    const int n_tasks = uni(static_cast<int>(reserved < W.task_cap ? reserved : W.task_cap));
+    const int stride = static_cast<int>(gridDim.x) * kWave;
+    for (int t = static_cast<int>(blockIdx.x) * kWave + lane_id(); t < n_tasks; t += stride) W.task_ready[t] = 0;
+    for (int base = static_cast<int>(blockIdx.x) * kWave; base < n_chunks; base += stride) {
+        const int c = base + lane_id();
+        bool todo = false;
+        if (c < n_chunks) {
+            todo = W.chunk_done[c] == 0;
+            if (!todo) W.chunk_done[c] = 0;
         }
-        double sums[kGradSlots];
-        sums[0] = wave_sum(w.v);
-#pragma unroll
-        for (int k = 0; k < kTangents; ++k) sums[1 + k] = wave_sum(w.d[k]);
-#pragma unroll
-        for (int k = 0; k < kGradSlots; ++k) wave_store(&W.partial[static_cast<size_t>(c) * kGradSlots + k], sums[k]);
+        uint64_t m = ballot(todo);
+        while (m) {
+            const int k = first_set(m);
+            m &= m - 1;
+            grad_chunk(curves, opt, ep, Lst, W, base + k);
+        }
     }
 }
 
@@ -952,6 +1072,7 @@ lcbg_status launch_grad_enumerate(const CurveRec* d_curves, const FsOptions& opt
                                   const IrLists& lists, const PolarScratch& slots,
                                   const PolarScratch& large_slots, const GradWork& work, hipStream_t st)
 {
+    // one wave per slot: waves that only take tasks would slow the enumeration
     hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(slots.n_slots)), dim3(kWave), 0, st, d_curves,
                        opt, ep, lists, slots, int{kPassSmall}, work);
     hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(large_slots.n_slots)), dim3(kWave), 0, st,

@@ -292,6 +292,7 @@ struct ScratchPool {
 // run over all its items: 2^24 each (1.3 GB).
 constexpr unsigned kGradRunCap = 1u << 24;
 constexpr unsigned kGradChunkCap = 1u << 24;
+constexpr unsigned kGradTaskCap = 1u << 21;  // chunk tasks per run (more go to k_grad_cells)
 
 lcbg_status ensure_grad_pool(ScratchPool& pool)
 {
@@ -306,8 +307,10 @@ lcbg_status ensure_grad_pool(ScratchPool& pool)
         const std::size_t n = static_cast<std::size_t>(ps.n_slots);
         return carve_size(n * kPhiHash * sizeof(int)) + carve_size(n * ps.run_cap * 4 * sizeof(int));
     };
+    const std::size_t tc = kGradTaskCap;
     const std::size_t bytes = carve_size(r * 4 * sizeof(int)) + carve_size(r * sizeof(int)) +
                               carve_size(c * sizeof(int)) + carve_size(c * kGradSlots * sizeof(double)) +
+                              carve_size(tc * 2 * sizeof(int)) + carve_size(tc * sizeof(int)) + carve_size(c) +
                               slot_bytes(pool.grad_slots) + slot_bytes(pool.grad_large);
     lcbg_status st = map_hip(hipMalloc(&pool.grad_mem, bytes));
     if (st != LCBG_OK) {
@@ -320,8 +323,16 @@ lcbg_status ensure_grad_pool(ScratchPool& pool)
     w.run_cells = carve<int>(p, r);
     w.chunk_item = carve<int>(p, c);
     w.partial = carve<double>(p, c * kGradSlots);
+    w.tasks = carve<int>(p, tc * 2);
+    w.task_ready = carve<int>(p, tc);
+    w.chunk_done = carve<uint8_t>(p, c);
     w.run_cap = kGradRunCap;
     w.chunk_cap = kGradChunkCap;
+    w.task_cap = kGradTaskCap;
+    // no task published, no chunk evaluated (each run leaves them so)
+    st = map_hip(hipMemset(w.task_ready, 0, tc * sizeof(int)));
+    if (st == LCBG_OK) st = map_hip(hipMemset(w.chunk_done, 0, c));
+    if (st != LCBG_OK) return st;
     for (PolarScratch* ps : {&pool.grad_slots, &pool.grad_large}) {
         const std::size_t n = static_cast<std::size_t>(ps->n_slots);
         ps->phi_head = carve<int>(p, n * kPhiHash);

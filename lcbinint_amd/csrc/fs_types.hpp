@@ -264,7 +264,12 @@ constexpr int kBucketOffset = 44;
 constexpr int kFirstFailWord = kBucketOffset + 2 * kCostBuckets;
 constexpr int kQGradRuns = kFirstFailWord + 1;
 constexpr int kQGradChunks = kFirstFailWord + 2;
-constexpr int kCounterWords = kFirstFailWord + 3;
+// the chunk tasks the enumeration publishes (reserved, taken) and the items it
+// has finished (GradWork::tasks)
+constexpr int kQGradTasks = kFirstFailWord + 3;
+constexpr int kQGradTaskCursor = kFirstFailWord + 4;
+constexpr int kQGradItemsDone = kFirstFailWord + 5;
+constexpr int kCounterWords = kFirstFailWord + 6;
 enum : int {
     kQNearCount = 0, kQNearCursor = 1, kQIrCount = 2, kQIrCursor = 3, kQFlags = 4,
     kQCartCount = 5, kQCartCursor = 6, kQPolarCount = 7, kQPolarCursor = 8,
@@ -297,7 +302,15 @@ struct GradWork {
     int* cells;
     double* h;         // lattice cell side
     double* points;    // [item][kGradSlots] unresolved point images
+    // chunk tasks: an enumerated item's chunks in groups of kGradTaskChunks,
+    // published for the enumeration waves that have run out of items
+    int* tasks;        // [task_cap][2]: first chunk, chunk count
+    int* task_ready;   // [task_cap] 1 once published, back to 0 once evaluated
+    uint8_t* chunk_done; // [chunk_cap] evaluated by a task (k_grad_cells skips it
+                         // and clears the flag)
+    unsigned task_cap;
 };
+constexpr int kGradTaskChunks = 16;
 // queued epochs of one curve per stage-2 distance work item
 constexpr int kNearChunk = 128;
 // leading inverse-ray items searched first and scanned on large slots at once
