@@ -128,7 +128,7 @@ def test_cartesian_diagnostics_match_reference(gpu, ref):
                       torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     batch.close()
-    c = cost.cpu().numpy().reshape(11, n)
+    c = cost.cpu().numpy().reshape(15, n)
     meth = d_k.cpu().numpy()
     st = default_settings(source_bins=50, finite_mode=1, limb_darkening_c=cv["ld_c"])
     ca, sa = np.cos(cv["alpha"]), np.sin(cv["alpha"])
