@@ -57,12 +57,12 @@ FLOPS_POLAR_CELL = 37.0
 # (profiles/k_ir_batched_fs.md): KiB per dispatch x 1024, FETCH_SIZE scaled by
 # 2.0 and WRITE_SIZE by 1.0, the factors that bring each counter's reading for
 # a 256 MiB device copy to 256 MiB (bench.py --copy-calibration).
-K_IR_CART_TRAFFIC_BYTES = (2.0 * 1047639.438 + 1.0 * 1849287.133) * 1024 / 3
+K_IR_CART_TRAFFIC_BYTES = (2.0 * 1051500.850 + 1.0 * 1858051.950) * 1024 / 3
 # FP64 lane operations the hardware issues per k_ir_cart launch on the same
 # workload (rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64, 64 lanes per
 # wave instruction, FMA as 2; profiles/k_ir_batched_fs.md): an upper bound on
 # the executed FP64 operations, reported beside the algorithmic rate.
-K_IR_CART_HW_FP64_FLOP = 1.699648e+11
+K_IR_CART_HW_FP64_FLOP = 1.491016e+11
 COPY_BYTES = 1 << 28
 
 
