@@ -41,9 +41,16 @@ sys.path.insert(0, str(ROOT))
 # gradient enumeration).  With HIP's default of four hardware queues per
 # process two of them share a queue and run one after the other: on the value
 # + gradient workload the main seed search then waits for the large-slot
-# Cartesian pass (239 ms per step instead of 208 ms).  Set before the HIP
-# runtime starts; a value the caller exported is kept.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# Cartesian pass (239 ms per step instead of 208 ms).  A replay of the captured
+# graph needs more: with eight queues it takes 180 ms per headline step
+# against 145 ms eager, with sixteen 145 ms.  Raised to sixteen (a larger value
+# the caller exported is kept) before the HIP runtime starts.
+try:
+    _hw_queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+except ValueError:
+    _hw_queues = 0
+if _hw_queues < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "LD finite-source mags/s and value+grad/s, batched vs plain GPU, 1e-4 rel err"
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (vendor datasheet)
