@@ -264,19 +264,45 @@ __device__ __forceinline__ void sqrt_gauss(double t, double* F, double* dF)
 // edge, g + 8 (sigma |grad g|_1 + sigma^2 sqrt(floor)) > 0 with
 // sigma = kSmear h (the 1-norm bounds the 2-norm from above, and the sum bounds
 // the width of aa_weight).
+//
+// The test only bounds the set of cells with nonzero weight (its margin is
+// eight widths), so it is evaluated with the hardware reciprocal and reciprocal
+// square root refined once (relative error ~1e-16 instead of IEEE rounding): a
+// cell whose test value lies within that error of zero has a weight below
+// erfc(8) either way.  The cells' weights (aa_weight) are exact.
+__device__ __forceinline__ double rcp_refined(double d)
+{
+    const double r = __builtin_amdgcn_rcp(d);
+    const double e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 __device__ __forceinline__ bool aa_in(const GradCtx& G, int i, int j)
 {
     const double rho = G.rho.v;
     const double x = (static_cast<double>(i) + 0.5) * G.h;
     const double y = (static_cast<double>(j) + 0.5) * G.h;
-    const MapOut<double> m = lens_map_k(G.L, x, y);
-    const double dx = m.wx - G.yx.v;
-    const double dy = m.wy - G.yy.v;
+    const Lens& L = G.L;
+    const double xa = x - L.a;
+    const double y2 = y * y;
+    const double i1 = rcp_refined(xa * xa + y2);
+    const double i2 = rcp_refined(x * x + y2);
+    const double wx = x - L.m1 * xa * i1 - L.m2 * x * i2 - L.a * L.m1;
+    const double wy = y - L.m1 * y * i1 - L.m2 * y * i2;
+    const double i1s = i1 * i1;
+    const double i2s = i2 * i2;
+    const double kr = L.m1 * (xa * xa - y2) * i1s + L.m2 * (x * x - y2) * i2s;
+    const double ki = 2.0 * (L.m1 * xa * y * i1s + L.m2 * x * y * i2s);
+    const double dx = wx - G.yx.v;
+    const double dy = wy - G.yy.v;
     const double g = rho * rho - (dx * dx + dy * dy);
-    const double gx = -2.0 * (dx * (1.0 + m.kr) + dy * m.ki);
-    const double gy = -2.0 * (dx * m.ki + dy * (1.0 - m.kr));
+    const double gx = -2.0 * (dx * (1.0 + kr) + dy * ki);
+    const double gy = -2.0 * (dx * ki + dy * (1.0 - kr));
     const double sig = kSmear * G.h;
-    const double curv = sig * sig * __builtin_sqrt(width_floor(m));
+    const double k = kr * kr + ki * ki;
+    const double fl = 256.0 * k * rcp_refined(1.0 + k * k);
+    const double root = fl > 0.0 ? fl * __builtin_amdgcn_rsq(fl) : 0.0;
+    const double curv = sig * sig * root;
     return g + 8.0 * (sig * (dabs(gx) + dabs(gy)) + curv) > 0.0;
 }
 
