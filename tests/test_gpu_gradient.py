@@ -81,6 +81,25 @@ def test_value_path_unchanged(gpu):
     assert np.median(rel) < 2e-3 and rel.max() < 1e-2
 
 
+def test_gradient_runs_repeat_bitwise(gpu):
+    """The inverse-ray quadrature's 64-cell chunks are evaluated either by
+    enumeration waves that have run out of epochs (while the largest epochs
+    are still being enumerated) or by k_grad_cells afterwards, depending on
+    timing; a chunk's sums do not depend on which, and each epoch sums its
+    chunks in a fixed order.  Three runs over the full 256 x 1000 workload,
+    the curves uploaded again before each, agree bitwise."""
+    from lcbinint_amd.workloads import differentiation_batch
+    curves, offsets, times = differentiation_batch()
+    run = Runner(curves, offsets, times)
+    first = run.grad(curves)
+    for _ in range(2):
+        again = run.grad(curves)
+        for a, b, name in zip(first, again, ("mag", "method", "mag_aa", "dmag")):
+            assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8)), name
+    assert np.all(np.isfinite(first[2])) and np.all(np.isfinite(first[3]))
+    print(f"{len(times)} epochs, {int((first[1] >= 2).sum())} inverse-ray: three runs bitwise equal")
+
+
 def test_rho_to_zero_limit(gpu):
     """Inverse-ray and hexadecapole gradients at rho = 1e-5 against the
     point-source gradient of the same epochs (rho = 0)."""
