@@ -958,25 +958,31 @@ __global__ void __launch_bounds__(kWave) k_grad_ir(const CurveRec* __restrict__ 
     if (tasks) take_tasks(curves, opt, ep, Lst, W, count);
 }
 
+// No chunk task published and no chunk evaluated: every flag of the task and
+// chunk pools cleared at the start of a gradient run (so that a run that ended
+// early leaves nothing behind for the next).
+__global__ void __launch_bounds__(256) k_grad_clear(GradWork W)
+{
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t t0 = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t t = t0; t < W.task_cap; t += stride) W.task_ready[t] = 0;
+    // the chunk flags 16 at a time (the pool's size is a multiple of 16)
+    uint4* flags = reinterpret_cast<uint4*>(W.chunk_done);
+    for (size_t t = t0; t < W.chunk_cap / 16; t += stride) flags[t] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // Weights of every chunk of 64 enumerated cells not evaluated by a task of the
 // enumeration, one wave per chunk, the waves taking 64 consecutive chunks at a
-// time; the chunk flags and the task flags are cleared for the next run.
+// time.
 __global__ void __launch_bounds__(kWave) k_grad_cells(const CurveRec* __restrict__ curves, FsOptions opt,
                                                        FsEpochs ep, IrLists Lst, GradWork W)
 {
     const unsigned total = ep.counters[kQGradChunks];
     const int n_chunks = uni(static_cast<int>(total < W.chunk_cap ? total : W.chunk_cap));
-    const unsigned reserved = ep.counters[kQGradTasks];
-    const int n_tasks = uni(static_cast<int>(reserved < W.task_cap ? reserved : W.task_cap));
     const int stride = static_cast<int>(gridDim.x) * kWave;
-    for (int t = static_cast<int>(blockIdx.x) * kWave + lane_id(); t < n_tasks; t += stride) W.task_ready[t] = 0;
     for (int base = static_cast<int>(blockIdx.x) * kWave; base < n_chunks; base += stride) {
         const int c = base + lane_id();
-        bool todo = false;
-        if (c < n_chunks) {
-            todo = W.chunk_done[c] == 0;
-            if (!todo) W.chunk_done[c] = 0;
-        }
+        const bool todo = c < n_chunks && W.chunk_done[c] == 0;
         uint64_t m = ballot(todo);
         while (m) {
             const int k = first_set(m);
@@ -1098,6 +1104,7 @@ lcbg_status launch_grad_enumerate(const CurveRec* d_curves, const FsOptions& opt
                                   const IrLists& lists, const PolarScratch& slots,
                                   const PolarScratch& large_slots, const GradWork& work, hipStream_t st)
 {
+    hipLaunchKernelGGL(k_grad_clear, dim3(1024), dim3(256), 0, st, work);
     // one wave per slot: waves that only take tasks would slow the enumeration
     hipLaunchKernelGGL(k_grad_ir, dim3(static_cast<unsigned>(slots.n_slots)), dim3(kWave), 0, st, d_curves,
                        opt, ep, lists, slots, int{kPassSmall}, work);

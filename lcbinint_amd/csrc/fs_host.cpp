@@ -329,10 +329,7 @@ lcbg_status ensure_grad_pool(ScratchPool& pool)
     w.run_cap = kGradRunCap;
     w.chunk_cap = kGradChunkCap;
     w.task_cap = kGradTaskCap;
-    // no task published, no chunk evaluated (each run leaves them so)
-    st = map_hip(hipMemset(w.task_ready, 0, tc * sizeof(int)));
-    if (st == LCBG_OK) st = map_hip(hipMemset(w.chunk_done, 0, c));
-    if (st != LCBG_OK) return st;
+    static_assert(kGradChunkCap % 16 == 0, "chunk flags are cleared 16 at a time");
     for (PolarScratch* ps : {&pool.grad_slots, &pool.grad_large}) {
         const std::size_t n = static_cast<std::size_t>(ps->n_slots);
         ps->phi_head = carve<int>(p, n * kPhiHash);

@@ -305,9 +305,9 @@ struct GradWork {
     // chunk tasks: an enumerated item's chunks in groups of kGradTaskChunks,
     // published for the enumeration waves that have run out of items
     int* tasks;        // [task_cap][2]: first chunk, chunk count
-    int* task_ready;   // [task_cap] 1 once published, back to 0 once evaluated
-    uint8_t* chunk_done; // [chunk_cap] evaluated by a task (k_grad_cells skips it
-                         // and clears the flag)
+    int* task_ready;   // [task_cap] 1 once published (cleared at the start of a run)
+    uint8_t* chunk_done; // [chunk_cap] evaluated by a task, k_grad_cells skips it
+                         // (cleared at the start of a run)
     unsigned task_cap;
 };
 constexpr int kGradTaskChunks = 16;
