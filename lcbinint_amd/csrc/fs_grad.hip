@@ -809,7 +809,7 @@ __device__ void take_tasks(const CurveRec* __restrict__ curves, const FsOptions&
                            const IrLists& Lst, const GradWork& W, int count)
 {
     unsigned* cursor = &ep.counters[kQGradTaskCursor];
-    const int n_chunks = uni(static_cast<int>(W.chunk_cap));
+    const int chunk_cap = uni(static_cast<int>(W.chunk_cap));
     int t = wave_fetch_inc(cursor);
     long long idle_from = static_cast<long long>(wall_clock64());
     for (;;) {
@@ -826,8 +826,8 @@ __device__ void take_tasks(const CurveRec* __restrict__ curves, const FsOptions&
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             const int c0 = uni(W.tasks[2 * static_cast<size_t>(t)]);
             const int nc = uni(W.tasks[2 * static_cast<size_t>(t) + 1]);
-            for (int c = c0; c < c0 + nc && c >= 0 && c < n_chunks; ++c) {
-                if (uni(W.chunk_item[c]) >= count) break;
+            for (int c = c0; c < c0 + nc && c >= 0 && c < chunk_cap; ++c) {
+                if (uni(W.chunk_item[c]) >= count) break;  // not an item of this run
                 grad_chunk(curves, opt, ep, Lst, W, c);
                 wave_store(&W.chunk_done[c], uint8_t{1});
             }
