@@ -1019,9 +1019,9 @@ void dyn_destroy(DynState* d)
 namespace {
 // Epochs per device pass of the dynamic path with orbital motion: each epoch
 // is a curve of its own (its own separation, its own caustic cache of about
-// 340 KB), so 262144 of them hold about 89 GB; fewer, larger passes share
+// 340 KB), so 524288 of them hold about 178 GB; fewer, larger passes share
 // the inverse-ray stage's tail (its longest epoch) among more epochs.
-constexpr int32_t kDynChunk = 262144;
+constexpr int32_t kDynChunk = 524288;
 constexpr int kDynGradParams = 12;
 
 lcbg_status dyn_reserve(lcbg_batch* b, int64_t n_epochs)
